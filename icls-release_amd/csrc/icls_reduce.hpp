@@ -1,0 +1,45 @@
+// icls_reduce.hpp -- deterministic block reductions (64-wide wavefronts).
+// A reduction is two launches: blocks fold their values in a fixed tree and write one
+// partial each; a single block then folds the partials in a fixed order.  The result
+// is bit-identical from run to run (it differs from the reference's serial Fortran
+// sum() by reassociation only).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace icls {
+
+struct OpSum { __device__ double operator()(double a, double b) const { return a + b; } };
+struct OpMax { __device__ double operator()(double a, double b) const { return fmax(a, b); } };
+struct OpMin { __device__ double operator()(double a, double b) const { return fmin(a, b); } };
+
+// All threads of the block must call this; nthreads <= 1024, multiple of 64.
+template <class Op>
+__device__ __forceinline__ double block_reduce(double v, Op op, double* smem /* >= 16 doubles */) {
+    for (int off = 32; off > 0; off >>= 1) v = op(v, __shfl_down(v, off));
+    const int tid = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
+    const int nthr = blockDim.x * blockDim.y * blockDim.z;
+    const int lane = tid & 63, wid = tid >> 6, nw = (nthr + 63) >> 6;
+    __syncthreads();
+    if (lane == 0) smem[wid] = v;
+    __syncthreads();
+    double r = 0.;
+    if (tid == 0) {
+        r = smem[0];
+        for (int w = 1; w < nw; w++) r = op(r, smem[w]);
+    }
+    __syncthreads();
+    return r;  // valid in thread 0
+}
+
+// Fold n partials (in order of index) into out[0] with one 256-thread block.
+template <class Op>
+__global__ void k_reduce_final(const double* __restrict__ part, int n, double init, double* out) {
+    __shared__ double smem[16];
+    double v = init;
+    Op op;
+    for (int q = threadIdx.x; q < n; q += blockDim.x) v = op(v, part[q]);
+    double r = block_reduce(v, op, smem);
+    if (threadIdx.x == 0) out[0] = r;
+}
+
+}  // namespace icls

@@ -1,0 +1,745 @@
+// icls_runtime.hip -- host side of libicls: the device context, parameter checks,
+// constant tables, host<->device field transfer in the reference's Fortran layout,
+// and the C-ABI entry points that stand in for the reference's subroutines.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "icls_internal.hpp"
+
+namespace icls {
+
+static Ctx* g_ctx = nullptr;
+static char g_err[1024] = "";
+
+void set_error(const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+// ---------------------------------------------------------------- profiling
+struct Profiler {
+    struct Rec { std::string name; hipEvent_t a, b; };
+    std::vector<Rec> open;
+    std::vector<hipEvent_t> pool;
+    std::map<std::string, std::pair<double, long>> acc;
+    hipEvent_t get() {
+        if (!pool.empty()) { hipEvent_t e = pool.back(); pool.pop_back(); return e; }
+        hipEvent_t e;
+        hipEventCreate(&e);
+        return e;
+    }
+    void flush(hipStream_t s) {
+        hipStreamSynchronize(s);
+        for (auto& r : open) {
+            float ms = 0.f;
+            hipEventElapsedTime(&ms, r.a, r.b);
+            auto& x = acc[r.name];
+            x.first += ms;
+            x.second += 1;
+            pool.push_back(r.a);
+            pool.push_back(r.b);
+        }
+        open.clear();
+    }
+    ~Profiler() {
+        for (auto& r : open) { hipEventDestroy(r.a); hipEventDestroy(r.b); }
+        for (auto e : pool) hipEventDestroy(e);
+    }
+};
+
+void prof_begin(Ctx& c, const char* name) {
+    Profiler* p = c.prof;
+    Profiler::Rec r{name, p->get(), p->get()};
+    hipEventRecord(r.a, c.stream);
+    p->open.push_back(r);
+    if (p->open.size() > 4096) {
+        // keep the event count bounded on long runs
+        Profiler::Rec last = p->open.back();
+        p->open.pop_back();
+        p->flush(c.stream);
+        p->open.push_back(last);
+    }
+}
+void prof_end(Ctx& c, const char* name) {
+    Profiler* p = c.prof;
+    for (auto it = p->open.rbegin(); it != p->open.rend(); ++it)
+        if (it->name == name) { hipEventRecord(it->b, c.stream); return; }
+}
+
+// ---------------------------------------------------------------- tables
+// pseudo_inv_A = (A^T A)^-1 A^T for the quadratic basis on the 27-point stencil
+// (init.f90:1619-1650); Gauss-Jordan on the 10x10 normal matrix in long double,
+// rounded once to double (the reference calls LAPACK dgetrf/dgetri).
+static void pseudo_inverse(double dx, double dy, double dz, double* out) {
+    long double A[27][10], M[10][20];
+    int cnt = 0;
+    for (int dzn = -1; dzn <= 1; dzn++)
+        for (int dyn = -1; dyn <= 1; dyn++)
+            for (int dxn = -1; dxn <= 1; dxn++) {
+                double a2 = dx * dxn, a3 = dy * dyn, a4 = dz * dzn;
+                A[cnt][0] = 1.0L; A[cnt][1] = a2; A[cnt][2] = a3; A[cnt][3] = a4;
+                A[cnt][4] = 0.5L * (long double)a2 * a2; A[cnt][5] = 0.5L * (long double)a3 * a3;
+                A[cnt][6] = 0.5L * (long double)a4 * a4;
+                A[cnt][7] = (long double)a2 * a3; A[cnt][8] = (long double)a2 * a4; A[cnt][9] = (long double)a3 * a4;
+                cnt++;
+            }
+    for (int r = 0; r < 10; r++)
+        for (int c = 0; c < 10; c++) {
+            long double s = 0;
+            for (int q = 0; q < 27; q++) s += A[q][r] * A[q][c];
+            M[r][c] = s;
+            M[r][10 + c] = (r == c) ? 1.0L : 0.0L;
+        }
+    for (int col = 0; col < 10; col++) {
+        int piv = col;
+        for (int r = col + 1; r < 10; r++) if (fabsl(M[r][col]) > fabsl(M[piv][col])) piv = r;
+        if (piv != col) for (int c = 0; c < 20; c++) { long double t = M[col][c]; M[col][c] = M[piv][c]; M[piv][c] = t; }
+        long double d = M[col][col];
+        for (int c = 0; c < 20; c++) M[col][c] /= d;
+        for (int r = 0; r < 10; r++) if (r != col) {
+            long double f = M[r][col];
+            if (f != 0) for (int c = 0; c < 20; c++) M[r][c] -= f * M[col][c];
+        }
+    }
+    for (int r = 0; r < 10; r++)
+        for (int q = 0; q < 27; q++) {
+            long double s = 0;
+            for (int c = 0; c < 10; c++) s += M[r][10 + c] * A[q][c];
+            out[r + 10 * q] = (double)s;
+        }
+}
+
+// eigenvalues of the periodic second difference in half-complex order
+// (initsolver.f90:58-82)
+static void eigenvalues(int n, double di, double pi, std::vector<double>& rt) {
+    rt.assign(n + 1, 0.);
+    for (int i = 3; i <= n; i += 2) {
+        double s = sin((double)(i - 1) * pi / (2. * n));
+        rt[i - 1] = -4. * di * di * (s * s);
+        rt[i] = rt[i - 1];
+    }
+    rt[1] = 0.;
+    rt[n] = -4. * di * di;
+}
+
+static bool pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+
+// ---------------------------------------------------------------- context
+static void free_ctx(Ctx* c) {
+    if (!c) return;
+    std::vector<void*> ptrs = {c->unew, c->vnew, c->wnew, c->dudt, c->dvdt, c->dwdt, c->pnew, c->pold, c->p,
+                               c->p_jump, c->duold, c->dvold, c->dwold, c->p_x, c->p_y, c->p_z, c->p_xold,
+                               c->p_yold, c->p_zold, c->miu, c->rho, c->curv, c->curv2, c->ucc, c->vcc, c->wcc,
+                               c->u_l, c->v_l, c->w_l, c->dheav, c->ucc_l, c->vcc_l, c->wcc_l, c->normal_m,
+                               c->ls_a, c->ls_b, c->ls_c, c->mask, c->thomas_d, c->dscal, c->dpart, c->pinvA,
+                               c->xyrt, c->tw_x, c->tw_y, c->dls, c->stash, c->scratch};
+    for (auto* p : ptrs) if (p) hipFree(p);
+    for (auto* p : c->lvset) if (p) hipFree(p);
+    if (c->hscal) hipHostFree(c->hscal);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c->prof;
+    delete c;
+}
+
+static int alloc_field(Ctx& c, double** p) {
+    HIP_OK(hipMalloc((void**)p, sizeof(double) * c.nfield));
+    HIP_OK(hipMemset(*p, 0, sizeof(double) * c.nfield));
+    return ICLS_OK;
+}
+
+#define TRY(x) do { int rc_ = (x); if (rc_) return rc_; } while (0)
+
+static int check_params(const icls_params* p) {
+    if (!p) { set_error("null parameter block"); return ICLS_ERR_PARAM; }
+    if (!pow2(p->itot) || !pow2(p->jtot) || p->itot < 8 || p->jtot < 8 || p->itot > 2048 || p->jtot > 2048) {
+        set_error("itot and jtot must be powers of two in [8, 2048] (got %d, %d)", p->itot, p->jtot);
+        return ICLS_ERR_PARAM;
+    }
+    if (p->ktot < 8) { set_error("ktot must be >= 8 (got %d)", p->ktot); return ICLS_ERR_PARAM; }
+    if (p->lmax < 1 || p->lmax > ICLS_MAX_DROPS) { set_error("lmax out of range (%d)", p->lmax); return ICLS_ERR_PARAM; }
+    if (!(p->lx > 0 && p->ly > 0 && p->lz > 0 && p->timestep > 0)) { set_error("domain and dt must be positive"); return ICLS_ERR_PARAM; }
+    if (p->ls_adv_space != ICLS_ADV_HOUC5 && p->ls_adv_space != ICLS_ADV_WENO5) { set_error("unknown ls_adv_space"); return ICLS_ERR_PARAM; }
+    if (p->reinit_step < 1 || p->inflate_step < 1) { set_error("reinit_step and inflate_step must be >= 1"); return ICLS_ERR_PARAM; }
+    if (p->re_initialization && p->reinit_iter_max != 1) {
+        set_error("reinit_iter_max other than 1 is not supported by this build");
+        return ICLS_ERR_UNSUPPORTED;
+    }
+    if (p->mode != ICLS_MODE_FP64) { set_error("precision mode %d not available in this build", p->mode); return ICLS_ERR_UNSUPPORTED; }
+    return ICLS_OK;
+}
+
+static int build_ctx(const icls_params* prm) {
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) { set_error("no GPU visible"); return ICLS_ERR_NODEVICE; }
+    Ctx* c = new Ctx();
+    g_ctx = c;
+    Ctx& C = *c;
+    C.P = *prm;
+    HIP_OK(hipGetDevice(&C.device));
+    HIP_OK(hipStreamCreateWithFlags(&C.stream, hipStreamNonBlocking));
+    Grid& g = C.g;
+    g.nx = prm->itot; g.ny = prm->jtot; g.nz = prm->ktot;
+    g.sy = g.nx; g.sz = (long)g.nx * g.ny;
+    // param.f90:37-38
+    g.dxi = prm->itot / prm->lx; g.dyi = prm->jtot / prm->ly; g.dzi = prm->ktot / prm->lz;
+    g.dx = 1. / g.dxi; g.dy = 1. / g.dyi; g.dz = 1. / g.dzi;
+    C.lmax = prm->lmax;
+    C.nfield = g.sz * (long)(g.nz + 6);
+    C.NB2 = 6. * g.dz;
+    C.rho0 = prm->rho1 < prm->rho2 ? prm->rho1 : prm->rho2;
+    C.visc = 1. / prm->Re;
+    C.pi = acos(-1.);
+    C.dt = prm->timestep;
+    double** fields[] = {&C.unew, &C.vnew, &C.wnew, &C.dudt, &C.dvdt, &C.dwdt, &C.pnew, &C.pold, &C.p, &C.p_jump,
+                         &C.duold, &C.dvold, &C.dwold, &C.p_x, &C.p_y, &C.p_z, &C.p_xold, &C.p_yold, &C.p_zold,
+                         &C.miu, &C.rho, &C.curv, &C.curv2, &C.ucc, &C.vcc, &C.wcc, &C.u_l, &C.v_l, &C.w_l,
+                         &C.dheav, &C.ucc_l, &C.vcc_l, &C.wcc_l, &C.normal_m, &C.ls_a, &C.ls_b, &C.ls_c,
+                         &C.thomas_d, &C.scratch};
+    for (auto** f : fields) TRY(alloc_field(C, f));
+    C.lvset.assign(C.lmax, nullptr);
+    for (int l = 0; l < C.lmax; l++) TRY(alloc_field(C, &C.lvset[l]));
+    HIP_OK(hipMalloc((void**)&C.mask, C.nfield));
+    HIP_OK(hipMemset(C.mask, 0, C.nfield));
+    HIP_OK(hipMalloc((void**)&C.stash, sizeof(double) * 2 * g.sz));
+    const int nscal = DS_COUNT_FIXED + 3 * C.lmax + 16;
+    HIP_OK(hipMalloc((void**)&C.dscal, sizeof(double) * nscal));
+    HIP_OK(hipMemset(C.dscal, 0, sizeof(double) * nscal));
+    HIP_OK(hipHostMalloc((void**)&C.hscal, sizeof(double) * nscal));
+    memset(C.hscal, 0, sizeof(double) * nscal);
+    C.nscal = nscal;
+    // reduction partials: enough for lmax per-block sums of any sweep
+    // (the chkdt sweep covers 1..i1 x 1..j1 x 1..k1, one row and column more than the others)
+    long nb = (long)((g.nx + 2 + 63) / 64 + 1) * ((g.ny + 2 + 3) / 4 + 1) * ((g.nz + 2 + 7) / 8 + 1);
+    C.npart_cap = nb * (C.lmax > 3 ? C.lmax : 3) + 4096;
+    HIP_OK(hipMalloc((void**)&C.dpart, sizeof(double) * C.npart_cap));
+    HIP_OK(hipMalloc((void**)&C.dls, sizeof(double*) * C.lmax));
+    HIP_OK(hipMemcpy(C.dls, C.lvset.data(), sizeof(double*) * C.lmax, hipMemcpyHostToDevice));
+    // constant tables
+    C.host_pinvA.assign(270, 0.);
+    pseudo_inverse(g.dx, g.dy, g.dz, C.host_pinvA.data());
+    HIP_OK(hipMalloc((void**)&C.pinvA, sizeof(double) * 270));
+    HIP_OK(hipMemcpy(C.pinvA, C.host_pinvA.data(), sizeof(double) * 270, hipMemcpyHostToDevice));
+    std::vector<double> xrt, yrt;
+    eigenvalues(g.nx, g.dxi, C.pi, xrt);
+    eigenvalues(g.ny, g.dyi, C.pi, yrt);
+    std::vector<double> xy((size_t)g.nx * g.ny);
+    for (int j = 1; j <= g.ny; j++)
+        for (int i = 1; i <= g.nx; i++) xy[(size_t)(i - 1) + (size_t)g.nx * (j - 1)] = xrt[i] + yrt[j];
+    HIP_OK(hipMalloc((void**)&C.xyrt, sizeof(double) * xy.size()));
+    HIP_OK(hipMemcpy(C.xyrt, xy.data(), sizeof(double) * xy.size(), hipMemcpyHostToDevice));
+    for (int dir = 0; dir < 2; dir++) {
+        int n = dir == 0 ? g.nx : g.ny;
+        std::vector<double> tw((size_t)n);   // n/2 complex exp(+2 pi i t/n)
+        for (int t = 0; t < n / 2; t++) {
+            double ang = 2. * C.pi * (double)t / (double)n;
+            tw[2 * t] = cos(ang);
+            tw[2 * t + 1] = sin(ang);
+        }
+        double** dst = dir == 0 ? &C.tw_x : &C.tw_y;
+        HIP_OK(hipMalloc((void**)dst, sizeof(double) * tw.size()));
+        HIP_OK(hipMemcpy(*dst, tw.data(), sizeof(double) * tw.size(), hipMemcpyHostToDevice));
+    }
+    HIP_OK(hipDeviceSynchronize());
+    return ICLS_OK;
+}
+
+static Ctx* need_ctx() {
+    if (!g_ctx) set_error("icls_init has not been called");
+    return g_ctx;
+}
+
+#define CTX_OR_FAIL()            \
+    Ctx* cp = need_ctx();        \
+    if (!cp) return ICLS_ERR_STATE; \
+    Ctx& c = *cp
+
+// ---------------------------------------------------------------- field layout conversion
+static double* field_ptr(Ctx& c, int field, int l) {
+    switch (field) {
+    case ICLS_F_UNEW: return c.unew;  case ICLS_F_VNEW: return c.vnew;  case ICLS_F_WNEW: return c.wnew;
+    case ICLS_F_DUDT: return c.dudt;  case ICLS_F_DVDT: return c.dvdt;  case ICLS_F_DWDT: return c.dwdt;
+    case ICLS_F_PNEW: return c.pnew;  case ICLS_F_POLD: return c.pold;  case ICLS_F_P: return c.p;
+    case ICLS_F_PJUMP: return c.p_jump;
+    case ICLS_F_DUOLD: return c.duold; case ICLS_F_DVOLD: return c.dvold; case ICLS_F_DWOLD: return c.dwold;
+    case ICLS_F_PX: return c.p_x; case ICLS_F_PY: return c.p_y; case ICLS_F_PZ: return c.p_z;
+    case ICLS_F_PXOLD: return c.p_xold; case ICLS_F_PYOLD: return c.p_yold; case ICLS_F_PZOLD: return c.p_zold;
+    case ICLS_F_MIU: return c.miu; case ICLS_F_RHO: return c.rho; case ICLS_F_CURV: return c.curv2;
+    case ICLS_F_UCC: return c.ucc; case ICLS_F_VCC: return c.vcc; case ICLS_F_WCC: return c.wcc;
+    case ICLS_F_LVSET: return (l >= 0 && l < c.lmax) ? c.lvset[l] : nullptr;
+    case ICLS_F_NORMAL_M: return c.normal_m;
+    default: return nullptr;
+    }
+}
+
+static int shape_of(int field) {
+    switch (field) {
+    case ICLS_F_DUOLD: case ICLS_F_DVOLD: case ICLS_F_DWOLD: case ICLS_F_UCC: case ICLS_F_VCC: case ICLS_F_WCC:
+    case ICLS_F_RHO_U: case ICLS_F_RHO_V: case ICLS_F_RHO_W: case ICLS_F_NORMAL_M: return ICLS_SHAPE_I;
+    case ICLS_F_LVSET: return ICLS_SHAPE_LS;
+    default: return ICLS_SHAPE_C;
+    }
+}
+
+// host Fortran extents and the offsets mapping host (ih,jh,kh) to (Fortran i,j,k)
+static void host_dims(const Ctx& c, int shape, int& hx, int& hy, int& hz, int& lb) {
+    const Grid& g = c.g;
+    if (shape == ICLS_SHAPE_C) { hx = g.nx + 2; hy = g.ny + 2; hz = g.nz + 2; lb = 0; }
+    else if (shape == ICLS_SHAPE_LS) { hx = g.nx + 6; hy = g.ny + 6; hz = g.nz + 6; lb = -2; }
+    else { hx = g.nx; hy = g.ny; hz = g.nz; lb = 1; }
+}
+
+}  // namespace icls
+
+using namespace icls;
+
+extern "C" {
+
+const char* icls_last_error(void) { return g_err; }
+int32_t icls_version(void) { return 100; }
+
+int32_t icls_init(const icls_params* p) {
+    int rc = check_params(p);
+    if (rc) return rc;
+    if (g_ctx) { free_ctx(g_ctx); g_ctx = nullptr; }
+    rc = build_ctx(p);
+    if (rc) { free_ctx(g_ctx); g_ctx = nullptr; }
+    return rc;
+}
+
+int32_t icls_finalize(void) {
+    if (g_ctx) { hipStreamSynchronize(g_ctx->stream); free_ctx(g_ctx); g_ctx = nullptr; }
+    return ICLS_OK;
+}
+
+int32_t icls_field_shape(int32_t field, int32_t* shape, int64_t* count) {
+    CTX_OR_FAIL();
+    if (field < 0 || field >= ICLS_F_COUNT) { set_error("bad field id %d", field); return ICLS_ERR_PARAM; }
+    int sh = shape_of(field), hx, hy, hz, lb;
+    host_dims(c, sh, hx, hy, hz, lb);
+    if (shape) *shape = sh;
+    if (count) *count = (int64_t)hx * hy * hz;
+    return ICLS_OK;
+}
+
+int32_t icls_sync(void) {
+    CTX_OR_FAIL();
+    HIP_OK(hipStreamSynchronize(c.stream));
+    return ICLS_OK;
+}
+
+int32_t icls_upload(int32_t field, int32_t l, const double* host) {
+    CTX_OR_FAIL();
+    double* d = field_ptr(c, field, l);
+    if (!d || !host) { set_error("field %d (l=%d) cannot be uploaded", field, l); return ICLS_ERR_PARAM; }
+    int sh = shape_of(field), hx, hy, hz, lb;
+    host_dims(c, sh, hx, hy, hz, lb);
+    const Grid& g = c.g;
+    std::vector<double> stage((size_t)c.nfield);
+    HIP_OK(hipMemcpyAsync(stage.data(), d, sizeof(double) * c.nfield, hipMemcpyDeviceToHost, c.stream));
+    HIP_OK(hipStreamSynchronize(c.stream));
+    for (int kh = 0; kh < hz; kh++) {
+        int k = kh + lb;   // Fortran k
+        for (int j0 = 0; j0 < g.ny; j0++)
+            for (int i0 = 0; i0 < g.nx; i0++) {
+                int ih = i0 + 1 - lb, jh = j0 + 1 - lb;
+                stage[(size_t)gidx(g, i0, j0, k)] = host[(size_t)ih + (size_t)hx * ((size_t)jh + (size_t)hy * kh)];
+            }
+    }
+    HIP_OK(hipMemcpyAsync(d, stage.data(), sizeof(double) * c.nfield, hipMemcpyHostToDevice, c.stream));
+    HIP_OK(hipStreamSynchronize(c.stream));
+    return ICLS_OK;
+}
+
+int32_t icls_download(int32_t field, int32_t l, double* host) {
+    CTX_OR_FAIL();
+    if (!host) { set_error("null host buffer"); return ICLS_ERR_PARAM; }
+    double* d = field_ptr(c, field, l);
+    if (field == ICLS_F_JIEMIAN) { TRY(launch_jiemian(c, c.scratch)); d = c.scratch; }
+    if (field == ICLS_F_RHO_U || field == ICLS_F_RHO_V || field == ICLS_F_RHO_W) d = c.rho;
+    if (!d) { set_error("field %d (l=%d) cannot be downloaded", field, l); return ICLS_ERR_PARAM; }
+    int sh = shape_of(field), hx, hy, hz, lb;
+    host_dims(c, sh, hx, hy, hz, lb);
+    const Grid& g = c.g;
+    std::vector<double> stage((size_t)c.nfield);
+    HIP_OK(hipMemcpyAsync(stage.data(), d, sizeof(double) * c.nfield, hipMemcpyDeviceToHost, c.stream));
+    HIP_OK(hipStreamSynchronize(c.stream));
+    const bool face = field == ICLS_F_RHO_U || field == ICLS_F_RHO_V || field == ICLS_F_RHO_W;
+    const bool req = c.P.rho1 == c.P.rho2;
+    for (int kh = 0; kh < hz; kh++) {
+        int k = kh + lb;
+        for (int jh = 0; jh < hy; jh++) {
+            int j0 = wrap(jh + lb - 1, g.ny);
+            for (int ih = 0; ih < hx; ih++) {
+                int i0 = wrap(ih + lb - 1, g.nx);
+                double v = stage[(size_t)gidx(g, i0, j0, k)];
+                if (face) {
+                    // arithmetic face means (misc.f90:151-172)
+                    if (req) v = c.P.rho1;
+                    else if (field == ICLS_F_RHO_U) v = (v + stage[(size_t)gidx(g, wrap(i0 + 1, g.nx), j0, k)]) / 2.;
+                    else if (field == ICLS_F_RHO_V) v = (v + stage[(size_t)gidx(g, i0, wrap(j0 + 1, g.ny), k)]) / 2.;
+                    else v = (v + stage[(size_t)gidx(g, i0, j0, k + 1)]) / 2.;
+                }
+                host[(size_t)ih + (size_t)hx * ((size_t)jh + (size_t)hy * kh)] = v;
+            }
+        }
+    }
+    return ICLS_OK;
+}
+
+static int pull_scalars(Ctx& c) {
+    HIP_OK(hipMemcpyAsync(c.hscal, c.dscal, sizeof(double) * c.nscal, hipMemcpyDeviceToHost, c.stream));
+    HIP_OK(hipStreamSynchronize(c.stream));
+    return ICLS_OK;
+}
+
+int32_t icls_get_scalar(int32_t which, double* value) {
+    CTX_OR_FAIL();
+    if (!value) return ICLS_ERR_PARAM;
+    if (which == ICLS_S_TIME) { *value = c.time; return ICLS_OK; }
+    if (which == ICLS_S_DT) { *value = c.dt; return ICLS_OK; }
+    TRY(pull_scalars(c));
+    switch (which) {
+    case ICLS_S_RHO_AVR: *value = c.hscal[DS_RHO_AVR]; break;
+    case ICLS_S_V_BULK: *value = c.hscal[DS_V_BULK] / (c.g.nx * c.g.ny * c.g.nz); break;
+    case ICLS_S_MAXCHK: *value = c.hscal[DS_MAXCHK]; break;
+    case ICLS_S_REINIT_RES: *value = c.hscal[DS_REINIT_RES]; break;
+    case ICLS_S_NORM_OLD: *value = c.hscal[DS_NORM_OLD]; break;
+    default: set_error("bad scalar id %d", which); return ICLS_ERR_PARAM;
+    }
+    return ICLS_OK;
+}
+
+int32_t icls_set_scalar(int32_t which, double value) {
+    CTX_OR_FAIL();
+    int slot = -1;
+    switch (which) {
+    case ICLS_S_TIME: c.time = value; return ICLS_OK;
+    case ICLS_S_DT: c.dt = value; return ICLS_OK;
+    case ICLS_S_RHO_AVR: slot = DS_RHO_AVR; break;
+    case ICLS_S_NORM_OLD: slot = DS_NORM_OLD; break;
+    default: set_error("scalar %d is not settable", which); return ICLS_ERR_PARAM;
+    }
+    HIP_OK(hipMemcpyAsync(c.dscal + slot, &value, sizeof(double), hipMemcpyHostToDevice, c.stream));
+    HIP_OK(hipStreamSynchronize(c.stream));
+    return ICLS_OK;
+}
+
+int32_t icls_get_volume(int32_t l, double* volume, double* init_volume) {
+    CTX_OR_FAIL();
+    if (l < 0 || l >= c.lmax) { set_error("bad level-set index %d", l); return ICLS_ERR_PARAM; }
+    TRY(pull_scalars(c));
+    if (volume) *volume = c.hscal[DS_COUNT_FIXED + l];
+    if (init_volume) *init_volume = c.hscal[DS_COUNT_FIXED + c.lmax + l];
+    return ICLS_OK;
+}
+
+// ---------------------------------------------------------------- level set
+static int ls_main_impl(Ctx& c, int istep) {
+    for (int l = 0; l < c.lmax; l++) {
+        double* ph = c.lvset[l];
+        // tube_construction (levelset.f90:87-197)
+        TRY(launch_tube(c, ph, c.ls_a));
+        TRY(launch_bc_ls(c, c.ls_a));
+        // ls_advection, SSP-RK3 (levelset.f90:235-243 / 268-276)
+        const int sch = c.P.ls_adv_space;
+        TRY(launch_adv_stage(c, sch, 1, c.ls_a, c.ls_a, c.ls_b, c.ucc, c.vcc, c.wcc, 0, nullptr, nullptr));
+        TRY(launch_bc_ls(c, c.ls_b));
+        TRY(launch_adv_stage(c, sch, 2, c.ls_a, c.ls_b, c.ls_c, c.ucc, c.vcc, c.wcc, 0, nullptr, nullptr));
+        TRY(launch_bc_ls(c, c.ls_c));
+        TRY(launch_adv_stage(c, sch, 3, c.ls_a, c.ls_c, ph, c.ucc, c.vcc, c.wcc, 0, nullptr, nullptr));
+        TRY(launch_bc_ls(c, ph));
+        // ls_mass_correction, predicated on the device (levelset.f90:377-582)
+        if (c.P.mass_correction) {
+            const double* flag = c.dscal + DS_INFL_FLAG;
+            TRY(launch_inflation_prepare(c, l, istep));
+            TRY(launch_curvature(c, ph, c.curv2, c.normal_m, flag));
+            TRY(launch_inflation_velocity(c, l));
+            TRY(launch_adv_stage(c, ICLS_ADV_HOUC5, 1, ph, ph, c.ls_a, c.ucc_l, c.vcc_l, c.wcc_l, 0, nullptr, flag));
+            TRY(launch_bc_ls_pred(c, c.ls_a, flag));
+            TRY(launch_adv_stage(c, ICLS_ADV_HOUC5, 2, ph, c.ls_a, c.ls_b, c.ucc_l, c.vcc_l, c.wcc_l, 0, nullptr, flag));
+            TRY(launch_bc_ls_pred(c, c.ls_b, flag));
+            TRY(launch_adv_stage(c, ICLS_ADV_HOUC5, 3, ph, c.ls_b, ph, c.ucc_l, c.vcc_l, c.wcc_l, 0, nullptr, flag));
+            TRY(launch_bc_ls_pred(c, ph, flag));
+        }
+        // ls_reinit_main, WENO5 + RK3 (levelset.f90:592-882)
+        if (c.P.re_initialization && istep % c.P.reinit_step == 0) {
+            TRY(launch_reinit_stage(c, 1, ph, ph, c.ls_a));
+            TRY(launch_bc_ls(c, c.ls_a));
+            TRY(launch_reinit_stage(c, 2, ph, c.ls_a, c.ls_b));
+            TRY(launch_bc_ls(c, c.ls_b));
+            TRY(launch_reinit_stage(c, 3, ph, c.ls_b, c.ls_c));
+            TRY(launch_reinit_residual(c, ph, c.ls_c));
+            HIP_OK(hipMemcpyAsync(ph, c.ls_c, sizeof(double) * c.nfield, hipMemcpyDeviceToDevice, c.stream));
+        }
+    }
+    // update_materials (misc.f90:79-290)
+    return launch_materials(c);
+}
+
+int32_t icls_ls_main(int32_t istep) {
+    CTX_OR_FAIL();
+    return ls_main_impl(c, istep);
+}
+
+// ---------------------------------------------------------------- Navier-Stokes
+int32_t icls_average_stream_vel(double* v_avr) {
+    CTX_OR_FAIL();
+    TRY(launch_sum_interior(c, c.vnew, DS_V_BULK));
+    // v_avr = v_avr/(itot*jtot*kmax) (misc.f90:652) on the device scalar
+    if (v_avr) {
+        TRY(pull_scalars(c));
+        *v_avr = c.hscal[DS_V_BULK] / (c.g.nx * c.g.ny * c.g.nz);
+    }
+    return ICLS_OK;
+}
+
+int32_t icls_ab2(void) {
+    CTX_OR_FAIL();
+    TRY(launch_momentum(c));
+    return launch_bc_uvw(c, c.dudt, c.dvdt, c.dwdt);
+}
+
+int32_t icls_fillps(void) {
+    CTX_OR_FAIL();
+    return launch_fillps(c);
+}
+
+static void rotate_jumps(Ctx& c) {
+    // p_xold = p_x (gfm.f90:65-67): the new jumps overwrite the old buffers
+    std::swap(c.p_x, c.p_xold);
+    std::swap(c.p_y, c.p_yold);
+    std::swap(c.p_z, c.p_zold);
+}
+
+static int gfm_impl(Ctx& c, bool fuse_fillps, bool store_pjump) {
+    rotate_jumps(c);
+    for (int l = 0; l < c.lmax; l++) {
+        TRY(launch_curvature(c, c.lvset[l], c.curv, nullptr, nullptr));
+        TRY(launch_recurvature(c, c.lvset[l], c.curv, c.curv2));
+        TRY(launch_gfm(c, l, c.curv2, fuse_fillps && c.lmax == 1, store_pjump || c.lmax > 1));
+    }
+    if (fuse_fillps && c.lmax > 1) {
+        TRY(launch_fillps(c));
+        TRY(launch_add_pjump(c));
+    }
+    return ICLS_OK;
+}
+
+int32_t icls_gfm4p(void) {
+    CTX_OR_FAIL();
+    return gfm_impl(c, false, true);
+}
+
+int32_t icls_add_pjump(void) {
+    CTX_OR_FAIL();
+    return launch_add_pjump(c);
+}
+
+static int solver_impl(Ctx& c) {
+    TRY(launch_fft_x(c, c.p, 0));
+    TRY(launch_fft_y(c, c.p, 0));
+    TRY(launch_thomas(c, c.p));
+    TRY(launch_fft_y(c, c.p, 1));
+    TRY(launch_fft_x(c, c.p, 1));
+    return ICLS_OK;
+}
+
+int32_t icls_solver2d(void) {
+    CTX_OR_FAIL();
+    return solver_impl(c);
+}
+
+int32_t icls_boundp(int32_t field) {
+    CTX_OR_FAIL();
+    double* d = field_ptr(c, field, 0);
+    if (!d) { set_error("bad field for boundp"); return ICLS_ERR_PARAM; }
+    return launch_bc_p(c, d);
+}
+
+int32_t icls_correc(void) {
+    CTX_OR_FAIL();
+    return launch_correc(c);
+}
+
+int32_t icls_bounduvw(int32_t which) {
+    CTX_OR_FAIL();
+    if (which == 0) return launch_bc_uvw(c, c.unew, c.vnew, c.wnew);
+    return launch_bc_uvw(c, c.dudt, c.dvdt, c.dwdt);
+}
+
+int32_t icls_update_pressure(int32_t first_step) {
+    CTX_OR_FAIL();
+    TRY(launch_pressure_norm(c));
+    return launch_pressure_rotate(c, first_step);
+}
+
+int32_t icls_average_vel_at_center(void) {
+    CTX_OR_FAIL();
+    return launch_avg_vel(c);
+}
+
+int32_t icls_check_nan(void) {
+    CTX_OR_FAIL();
+    TRY(launch_nan_check(c));
+    TRY(pull_scalars(c));
+    if (c.hscal[DS_NAN_FLAG] != 0.) { set_error("Got NaN... Terminate brutally. :("); return ICLS_ERR_NAN; }
+    return ICLS_OK;
+}
+
+int32_t icls_chkdiv(int32_t istep, double* divtot, double* divmax, int32_t* ijk) {
+    CTX_OR_FAIL();
+    TRY(launch_chkdiv(c));
+    TRY(pull_scalars(c));
+    double dm = c.hscal[DS_DIV_MAX];
+    long loc = (long)c.hscal[DS_DIV_MAX + 16];
+    if (divtot) *divtot = c.hscal[DS_DIV_TOT];
+    if (divmax) *divmax = dm;
+    if (ijk) {
+        if (dm > 0.) {
+            ijk[0] = (int)(loc % c.g.nx) + 1;
+            ijk[1] = (int)((loc / c.g.nx) % c.g.ny) + 1;
+            ijk[2] = (int)(loc / ((long)c.g.nx * c.g.ny)) + 1;
+        } else { ijk[0] = ijk[1] = ijk[2] = 0; }
+    }
+    if (dm > 1e-12 && istep != 0) {
+        set_error("Magnitude of the maximal divergence > 1e-12 (%.6e)", dm);
+        return ICLS_ERR_DIVERGENCE;
+    }
+    return ICLS_OK;
+}
+
+// chkdt (chkdt.f90:11-105) from the device maxima
+int32_t icls_chkdt(double* dtmax) {
+    CTX_OR_FAIL();
+    TRY(launch_chkdt_umax(c));
+    TRY(pull_scalars(c));
+    const Grid& g = c.g;
+    const icls_params& P = c.P;
+    double u_max = c.hscal[DS_UMAX], v_max = c.hscal[DS_UMAX + 1], w_max = c.hscal[DS_UMAX + 2];
+    double speed_conv = u_max / g.dx + v_max / g.dy + w_max / g.dz;
+    double ra = c.hscal[DS_RHO_AVR];
+    double gs[6] = {fabs(P.iFr * (1. - ra / P.rho1) * P.g_unit[0]), fabs(P.iFr * (1. - ra / P.rho2) * P.g_unit[0]),
+                    fabs(P.iFr * (1. - ra / P.rho1) * P.g_unit[1]), fabs(P.iFr * (1. - ra / P.rho2) * P.g_unit[1]),
+                    fabs(P.iFr * (1. - ra / P.rho1) * P.g_unit[2]), fabs(P.iFr * (1. - ra / P.rho2) * P.g_unit[2])};
+    double g_max = gs[0];
+    for (int q = 1; q < 6; q++) if (gs[q] > g_max) g_max = gs[q];
+    double h_min = fmin(g.dx, fmin(g.dy, g.dz));
+    double speed_grav = sqrt(g_max / h_min);
+    double v1 = fmax(P.miu1 / P.rho1, P.miu2 / P.rho2) * c.visc;
+    double v2 = 2. / (g.dx * g.dx) + 2. / (g.dy * g.dy) + 2. / (g.dz * g.dz);
+    double speed_visc = v1 * v2;
+    double curv_max = 2. / h_min;
+    double speed_surf = sqrt(curv_max / (P.We * c.rho0 * (h_min * h_min)));
+    double s1 = speed_conv + speed_visc;
+    double speed = (s1 + sqrt(s1 * s1 + 4. * (speed_grav * speed_grav) + 4. * (speed_surf * speed_surf))) / 2.;
+    if (dtmax) *dtmax = 0.5 / speed;
+    return ICLS_OK;
+}
+
+int32_t icls_max_vel_at_center(double* vmax) {
+    CTX_OR_FAIL();
+    TRY(launch_max_vel(c));
+    TRY(pull_scalars(c));
+    if (vmax) *vmax = c.hscal[DS_VMAX];
+    return ICLS_OK;
+}
+
+// ---------------------------------------------------------------- init_program
+static int init_derived_impl(Ctx& c) {
+    c.dt = c.P.timestep;                                   // init.f90:68
+    TRY(launch_bc_uvw(c, c.unew, c.vnew, c.wnew));         // init.f90:72-76
+    TRY(launch_bc_uvw(c, c.dudt, c.dvdt, c.dwdt));
+    for (int l = 0; l < c.lmax; l++) TRY(launch_boundmls(c, c.lvset[l]));
+    TRY(launch_bc_p(c, c.pnew));
+    TRY(launch_avg_vel(c));                                // init.f90:79
+    TRY(launch_materials(c));                              // init.f90:85
+    // init_volume = volume (init.f90:86)
+    HIP_OK(hipMemcpyAsync(c.dscal + DS_COUNT_FIXED + c.lmax, c.dscal + DS_COUNT_FIXED, sizeof(double) * c.lmax,
+                          hipMemcpyDeviceToDevice, c.stream));
+    TRY(launch_momentum_rhs_only(c));                      // init.f90:90
+    // pold = pnew, p_xold = p_x ... (init.f90:107-110)
+    HIP_OK(hipMemcpyAsync(c.pold, c.pnew, sizeof(double) * c.nfield, hipMemcpyDeviceToDevice, c.stream));
+    HIP_OK(hipMemcpyAsync(c.p_xold, c.p_x, sizeof(double) * c.nfield, hipMemcpyDeviceToDevice, c.stream));
+    HIP_OK(hipMemcpyAsync(c.p_yold, c.p_y, sizeof(double) * c.nfield, hipMemcpyDeviceToDevice, c.stream));
+    HIP_OK(hipMemcpyAsync(c.p_zold, c.p_z, sizeof(double) * c.nfield, hipMemcpyDeviceToDevice, c.stream));
+    HIP_OK(hipStreamSynchronize(c.stream));
+    c.initialised = true;
+    return ICLS_OK;
+}
+
+int32_t icls_init_derived(void) {
+    CTX_OR_FAIL();
+    return init_derived_impl(c);
+}
+
+int32_t icls_init_program(void) {
+    CTX_OR_FAIL();
+    c.time = 0.;
+    // init_vel_p (init.f90:148-210): fields start at zero; 'cou' sets v
+    double* zf[] = {c.unew, c.vnew, c.wnew, c.dudt, c.dvdt, c.dwdt, c.pnew};
+    for (auto* f : zf) TRY(launch_fill(c, f, 0.));
+    TRY(launch_init_velocity(c));
+    for (int l = 0; l < c.lmax; l++) TRY(launch_init_levelset(c, l, c.lvset[l]));
+    return init_derived_impl(c);
+}
+
+// ---------------------------------------------------------------- one time step
+int32_t icls_step(int32_t istep, int32_t begin) {
+    CTX_OR_FAIL();
+    if (!c.initialised) { set_error("icls_init_program has not run"); return ICLS_ERR_STATE; }
+    c.time = c.time + c.dt;                                // main.f90:107
+    TRY(ls_main_impl(c, istep));                           // main.f90:113
+    TRY(launch_sum_interior(c, c.vnew, DS_V_BULK));        // main.f90:124
+    TRY(launch_momentum(c));                               // main.f90:130
+    TRY(launch_bc_uvw(c, c.dudt, c.dvdt, c.dwdt));
+    TRY(gfm_impl(c, true, false));                         // main.f90:139-143
+    TRY(solver_impl(c));                                   // main.f90:149
+    TRY(launch_bc_p(c, c.p));                              // main.f90:155
+    TRY(launch_correc(c));                                 // main.f90:159-160
+    TRY(launch_bc_uvw(c, c.unew, c.vnew, c.wnew));
+    TRY(launch_pressure_norm(c));                          // main.f90:164-185
+    TRY(launch_pressure_rotate(c, istep == begin + 1));
+    TRY(launch_avg_vel(c));                                // main.f90:190
+    TRY(launch_nan_check(c));                              // main.f90:207 (sticky flag)
+    return ICLS_OK;
+}
+
+// ---------------------------------------------------------------- profiling API
+int32_t icls_profile_enable(int32_t on) {
+    CTX_OR_FAIL();
+    if (on && !c.prof) c.prof = new Profiler();
+    if (!on && c.prof) { c.prof->flush(c.stream); delete c.prof; c.prof = nullptr; }
+    return ICLS_OK;
+}
+
+int32_t icls_profile_reset(void) {
+    CTX_OR_FAIL();
+    if (c.prof) { c.prof->flush(c.stream); c.prof->acc.clear(); }
+    return ICLS_OK;
+}
+
+int32_t icls_kernel_times(int32_t n, char* names, double* total_ms, int64_t* counts, int32_t* n_out) {
+    CTX_OR_FAIL();
+    if (!c.prof) { if (n_out) *n_out = 0; return ICLS_OK; }
+    c.prof->flush(c.stream);
+    int q = 0;
+    for (auto& kv : c.prof->acc) {
+        if (q >= n) break;
+        if (names) { strncpy(names + 64 * q, kv.first.c_str(), 63); names[64 * q + 63] = 0; }
+        if (total_ms) total_ms[q] = kv.second.first;
+        if (counts) counts[q] = kv.second.second;
+        q++;
+    }
+    if (n_out) *n_out = q;
+    return ICLS_OK;
+}
+
+}  // extern "C"

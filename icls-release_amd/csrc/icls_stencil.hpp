@@ -1,0 +1,97 @@
+// icls_stencil.hpp -- pointwise and line-stencil device functions of the ICLS step.
+// Each follows the reference expression cited beside it, in the same evaluation
+// order; the library is compiled with -ffp-contract=off so no product is fused into
+// an add and FP64 results match the reference's operation sequence.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace icls {
+
+// HOUC5 upwind derivative along a line f[-3..3] (hyperPDE.f90:427-433, 441-447, 455-461).
+template <typename T>
+__device__ __forceinline__ T houc5(T vel, const T* f) {
+    if (vel >= T(0))
+        return T(1.) / T(60.) * (-T(2.) * f[-3] + T(15.) * f[-2] - T(60.) * f[-1] + T(20.) * f[0] + T(30.) * f[1] - T(3.) * f[2]);
+    return T(1.) / T(60.) * (T(2.) * f[3] - T(15.) * f[2] + T(60.) * f[1] - T(20.) * f[0] - T(30.) * f[-1] + T(3.) * f[-2]);
+}
+
+// HOUC3 next to the walls, k = 2 and k = kmax-1 (hyperPDE.f90:465-471)
+template <typename T>
+__device__ __forceinline__ T houc3(T vel, const T* f) {
+    if (vel >= T(0)) return (T(1.) / T(6.)) * f[-2] - f[-1] + (T(1.) / T(2.)) * f[0] + (T(1.) / T(3.)) * f[1];
+    return -(T(1.) / T(6.)) * f[2] + f[1] - (T(1.) / T(2.)) * f[0] - (T(1.) / T(3.)) * f[-1];
+}
+
+// first-order upwind at k = 1 and k = kmax (hyperPDE.f90:477-481)
+template <typename T>
+__device__ __forceinline__ T upwind1(T vel, const T* f) {
+    if (vel >= T(0)) return f[0] - f[-1];
+    return f[1] - f[0];
+}
+
+// Jiang-Shu WENO5 reconstruction from five differences (hyperPDE.f90:551-576)
+template <typename T>
+__device__ __forceinline__ T weno5(T v1, T v2, T v3, T v4, T v5) {
+    const T eps_weno = T(1.e-6), sigma_1 = T(0.1), sigma_2 = T(0.6), sigma_3 = T(0.3);
+    T t1 = v1 - T(2.) * v2 + v3, t2 = v1 - T(4.) * v2 + T(3.) * v3;
+    T S1 = (T(13.) / T(12.)) * (t1 * t1) + (T(1.) / T(4.)) * (t2 * t2);
+    T t3 = v2 - T(2.) * v3 + v4, t4 = v2 - v4;
+    T S2 = (T(13.) / T(12.)) * (t3 * t3) + (T(1.) / T(4.)) * (t4 * t4);
+    T t5 = v3 - T(2.) * v4 + v5, t6 = T(3.) * v3 - T(4.) * v4 + v5;
+    T S3 = (T(13.) / T(12.)) * (t5 * t5) + (T(1.) / T(4.)) * (t6 * t6);
+    T e1 = eps_weno + S1, e2 = eps_weno + S2, e3 = eps_weno + S3;
+    T a1 = sigma_1 / (e1 * e1);
+    T a2 = sigma_2 / (e2 * e2);
+    T a3 = sigma_3 / (e3 * e3);
+    T w1 = a1 / (a1 + a2 + a3);
+    T w2 = a2 / (a1 + a2 + a3);
+    T w3 = a3 / (a1 + a2 + a3);
+    return w1 * ((T(1.) / T(3.)) * v1 - (T(7.) / T(6.)) * v2 + (T(11.) / T(6.)) * v3) +
+           w2 * ((-T(1.) / T(6.)) * v2 + (T(5.) / T(6.)) * v3 + (T(1.) / T(3.)) * v4) +
+           w3 * ((T(1.) / T(3.)) * v3 + (T(5.) / T(6.)) * v4 - (T(1.) / T(6.)) * v5);
+}
+
+// left-biased derivative (hyperPDE.f90:545-549)
+template <typename T>
+__device__ __forceinline__ T weno_minus(const T* f, T h) {
+    return weno5<T>((f[-2] - f[-3]) / h, (f[-1] - f[-2]) / h, (f[0] - f[-1]) / h, (f[1] - f[0]) / h, (f[2] - f[1]) / h);
+}
+// right-biased derivative (hyperPDE.f90:579-583)
+template <typename T>
+__device__ __forceinline__ T weno_plus(const T* f, T h) {
+    return weno5<T>((f[3] - f[2]) / h, (f[2] - f[1]) / h, (f[1] - f[0]) / h, (f[0] - f[-1]) / h, (f[-1] - f[-2]) / h);
+}
+
+// Heaviside 'trig' (misc.f90:439-459); pi = acos(-1.)
+template <typename T>
+__device__ __forceinline__ T heaviside_trig(T phi, T eps, T pi) {
+    if (phi < -eps) return T(0);
+    if (phi < eps) return T(.5) + T(.5) * phi / eps + T(.5) / pi * sin(pi * phi / eps);
+    return T(1);
+}
+
+// Dirac 'trig' (misc.f90:518-538)
+template <typename T>
+__device__ __forceinline__ T dirac_trig(T phi, T eps, T pi) {
+    if (phi < -eps) return T(0);
+    if (phi < eps) return T(.5) / eps + T(.5) / eps * cos(pi * phi / eps);
+    return T(0);
+}
+
+// icurvature, 3D branch (interface.f90:126-179); ph/cv point at the centre value
+template <typename T>
+__device__ __forceinline__ T icurvature(int nb, const T* ph, const T* cv) {
+    T sp = T(0);
+    if (cv[0] < T(0)) sp = T(-1.);
+    if (cv[0] > T(0)) sp = T(1.);
+    T eps_0 = sp * ph[0];
+    T eps_nb = sp * ph[nb];
+    if (cv[0] * cv[nb] > T(0)) {
+        T a1 = eps_nb * cv[0];
+        T a2 = eps_0 * cv[nb];
+        return (a1 - a2) / (eps_nb - eps_0);
+    }
+    return T(0);
+}
+
+}  // namespace icls

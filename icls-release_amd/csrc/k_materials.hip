@@ -1,0 +1,264 @@
+// k_materials.hip -- update_materials (Heaviside, viscosity, density, volume), the
+// mass-correction "inflation" sums and velocities, scalar bookkeeping kernels and the
+// generic interior sums.
+#include "icls_internal.hpp"
+#include "icls_reduce.hpp"
+#include "icls_stencil.hpp"
+
+namespace icls {
+
+static constexpr int BX = 64, BY = 4;
+static constexpr int KCH = 8;   // z layers folded per block in the reductions
+
+// number of blocks of a (BX x BY) x (z chunks of KCH) sweep over k = k_lo .. k_hi
+static inline dim3 red_grid(const Grid& g, int k_lo, int k_hi) {
+    int nk = k_hi - k_lo + 1;
+    return dim3((g.nx + BX - 1) / BX, (g.ny + BY - 1) / BY, (nk + KCH - 1) / KCH);
+}
+static inline int red_nblocks(dim3 d) { return (int)(d.x * d.y * d.z); }
+__device__ __forceinline__ int block_linear() { return blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z); }
+
+// ---------------------------------------------------------------- update_materials
+// misc.f90:100-147 for every cell of the C-halo range (x/y by wrap, z = 0..k1):
+//   H_l  = Heaviside('0','trig',l,1.5 dz),  H = min_l H_l   (jiemian and h_rho)
+//   miu  = miu1 H + miu2 (1 - H),  rho = rho1 H + rho2 (1 - H)
+// and the partial interior sums of every H_l for volume(l) (misc.f90:128-134).
+__global__ void k_materials(Grid g, int lmax, const double* const* __restrict__ ls, double eps, double pi, double miu1,
+                            double miu2, double rho1, double rho2, double* __restrict__ miu, double* __restrict__ rho,
+                            double* __restrict__ jiemian, double* __restrict__ part, int nblocks) {
+    __shared__ double smem[16];
+    int i0 = blockIdx.x * BX + threadIdx.x, j0 = blockIdx.y * BY + threadIdx.y;
+    bool valid = i0 < g.nx && j0 < g.ny;
+    // up to 4 level sets summed in registers per pass; more are folded pass by pass
+    for (int l0 = 0; l0 < lmax; l0 += 4) {
+        double acc[4] = {0., 0., 0., 0.};
+        for (int kk = 0; kk < KCH; kk++) {
+            int k = blockIdx.z * KCH + kk;   // k = 0 .. nz+1
+            if (!valid || k > g.nz + 1) continue;
+            long q = gidx(g, i0, j0, k);
+            double H = 0.;
+            if (l0 == 0) {
+                H = heaviside_trig<double>(ls[0][q], eps, pi);
+                for (int l = 1; l < lmax; l++) H = fmin(H, heaviside_trig<double>(ls[l][q], eps, pi));
+                miu[q] = miu1 * H + miu2 * (1. - H);
+                rho[q] = rho1 * H + rho2 * (1. - H);
+                if (jiemian) jiemian[q] = H;
+            }
+            if (k >= 1 && k <= g.nz) {
+                for (int t = 0; t < 4 && l0 + t < lmax; t++)
+                    acc[t] += (l0 == 0 && t == 0 && lmax == 1) ? H : heaviside_trig<double>(ls[l0 + t][q], eps, pi);
+            }
+        }
+        for (int t = 0; t < 4 && l0 + t < lmax; t++) {
+            double r = block_reduce(acc[t], OpSum(), smem);
+            if (threadIdx.x == 0 && threadIdx.y == 0) part[(long)(l0 + t) * nblocks + block_linear()] = r;
+        }
+    }
+}
+
+// volume(l) = 1 - sum/N (misc.f90:134), rho_avr (misc.f90:147)
+__global__ void k_volume_final(int lmax, double ncell, double rho1, double rho2, const double* __restrict__ sums,
+                               double* dscal) {
+    if (threadIdx.x != 0) return;
+    double sv = 0.;
+    for (int l = 0; l < lmax; l++) {
+        double v = 1. - sums[l] / ncell;
+        dscal[DS_COUNT_FIXED + l] = v;
+        sv = sv + v;
+    }
+    dscal[DS_RHO_AVR] = rho1 * (1. - sv) + rho2 * sv;
+}
+
+int launch_materials(Ctx& c) {
+    ProfScope ps(c, "k_materials");
+    dim3 blk(BX, BY, 1);
+    dim3 grd = red_grid(c.g, 0, c.g.nz + 1);
+    int nb = red_nblocks(grd);
+    if ((long)nb * c.lmax > c.npart_cap) { set_error("reduction scratch too small"); return ICLS_ERR_STATE; }
+    hipLaunchKernelGGL(k_materials, grd, blk, 0, c.stream, c.g, c.lmax, (const double* const*)c.dls, 1.5 * c.g.dz, c.pi,
+                       c.P.miu1, c.P.miu2, c.P.rho1, c.P.rho2, c.miu, c.rho, (double*)nullptr, c.dpart, nb);
+    HIP_OK(hipGetLastError());
+    double* sums = c.dscal + DS_COUNT_FIXED + 2 * c.lmax;   // lmax scratch sums after volume/init_volume
+    for (int l = 0; l < c.lmax; l++)
+        hipLaunchKernelGGL(k_reduce_final<OpSum>, dim3(1), dim3(256), 0, c.stream, c.dpart + (long)l * nb, nb, 0.,
+                           sums + l);
+    double ncell = 1. * c.g.nx * c.g.ny * c.g.nz;
+    hipLaunchKernelGGL(k_volume_final, dim3(1), dim3(64), 0, c.stream, c.lmax, ncell, c.P.rho1, c.P.rho2,
+                       (const double*)sums, c.dscal);
+    HIP_OK(hipGetLastError());
+    return ICLS_OK;
+}
+
+// jiemian = min_l H_l over the C-halo range (misc.f90:115-120), on request only
+__global__ void k_jiemian(Grid g, int lmax, const double* const* __restrict__ ls, double eps, double pi,
+                          double* __restrict__ out) {
+    int i0 = blockIdx.x * BX + threadIdx.x, j0 = blockIdx.y * BY + threadIdx.y, k = blockIdx.z;
+    if (i0 >= g.nx || j0 >= g.ny) return;
+    long q = gidx(g, i0, j0, k);
+    double H = heaviside_trig<double>(ls[0][q], eps, pi);
+    for (int l = 1; l < lmax; l++) H = fmin(H, heaviside_trig<double>(ls[l][q], eps, pi));
+    out[q] = H;
+}
+
+int launch_jiemian(Ctx& c, double* out) {
+    dim3 blk(BX, BY, 1);
+    hipLaunchKernelGGL(k_jiemian, grid3(c.g, c.g.nz + 2, blk), blk, 0, c.stream, c.g, c.lmax, (const double* const*)c.dls,
+                       1.5 * c.g.dz, c.pi, out);
+    HIP_OK(hipGetLastError());
+    return ICLS_OK;
+}
+
+// ---------------------------------------------------------------- interior sums
+template <int KIND>
+__global__ void k_sum_interior(Grid g, const double* __restrict__ f, double* __restrict__ part) {
+    __shared__ double smem[16];
+    int i0 = blockIdx.x * BX + threadIdx.x, j0 = blockIdx.y * BY + threadIdx.y;
+    double acc = 0.;
+    if (i0 < g.nx && j0 < g.ny)
+        for (int kk = 0; kk < KCH; kk++) {
+            int k = 1 + blockIdx.z * KCH + kk;
+            if (k <= g.nz) acc += f[gidx(g, i0, j0, k)];
+        }
+    double r = block_reduce(acc, OpSum(), smem);
+    if (threadIdx.x == 0 && threadIdx.y == 0) part[block_linear()] = r;
+}
+
+// dscal[slot] = sum over the interior of f
+int launch_sum_interior(Ctx& c, const double* f, int slot) {
+    dim3 blk(BX, BY, 1);
+    dim3 grd = red_grid(c.g, 1, c.g.nz);
+    int nb = red_nblocks(grd);
+    hipLaunchKernelGGL(k_sum_interior<0>, grd, blk, 0, c.stream, c.g, f, c.dpart);
+    hipLaunchKernelGGL(k_reduce_final<OpSum>, dim3(1), dim3(256), 0, c.stream, (const double*)c.dpart, nb, 0.,
+                       c.dscal + slot);
+    HIP_OK(hipGetLastError());
+    return ICLS_OK;
+}
+
+// ---------------------------------------------------------------- mass correction
+// ls_mass_correction trigger (levelset.f90:384-395): maxchk_glb = 0, then inflate
+// when mod(istep,inflate_step) == 0 and |(V - V0)/V0| > 1e-5.
+__global__ void k_infl_flag(int l, int lmax, int due, double* dscal) {
+    if (threadIdx.x != 0) return;
+    dscal[DS_MAXCHK] = 0.;
+    double V = dscal[DS_COUNT_FIXED + l], V0 = dscal[DS_COUNT_FIXED + lmax + l];
+    double massloss = (V - V0) / V0;
+    dscal[DS_INFL_FLAG] = (due && fabs(massloss) > 1e-5) ? 1. : 0.;
+}
+
+// partial sums of delta*kappa*|grad phi| (levelset.f90:499) and of the Heaviside
+// (levelset.f90:505) over the interior; dheav over the C-halo range
+__global__ void k_infl_sums(Grid g, const double* __restrict__ flag, const double* __restrict__ phi,
+                            const double* __restrict__ kappa, const double* __restrict__ nm, double s1, double pi,
+                            double* __restrict__ dheav, double* __restrict__ part, int nblocks) {
+    if (flag[0] == 0.) return;
+    __shared__ double smem[16];
+    int i0 = blockIdx.x * BX + threadIdx.x, j0 = blockIdx.y * BY + threadIdx.y;
+    bool valid = i0 < g.nx && j0 < g.ny;
+    double sa = 0., sv = 0.;
+    for (int kk = 0; kk < KCH; kk++) {
+        int k = blockIdx.z * KCH + kk;
+        if (!valid || k > g.nz + 1) continue;
+        long q = gidx(g, i0, j0, k);
+        double h = heaviside_trig<double>(phi[q], s1, pi);
+        dheav[q] = h;
+        if (k >= 1 && k <= g.nz) {
+            double del = dirac_trig<double>(phi[q], s1, pi);
+            sa += del * kappa[q] * nm[q];
+            sv += h;
+        }
+    }
+    double ra = block_reduce(sa, OpSum(), smem);
+    double rv = block_reduce(sv, OpSum(), smem);
+    if (threadIdx.x == 0 && threadIdx.y == 0) { part[block_linear()] = ra; part[nblocks + block_linear()] = rv; }
+}
+
+// levelset.f90:499-521 after the two global sums
+__global__ void k_infl_scalars(int l, int lmax, double ncell, double dx, double dy, double dz, double lxyz,
+                               double inflate_dt, double s1, const double* __restrict__ sums, double* dscal) {
+    if (threadIdx.x != 0 || dscal[DS_INFL_FLAG] == 0.) return;
+    double area = sums[0] * dx * dy * dz;
+    double vol = 1. - sums[1] / ncell;
+    dscal[DS_COUNT_FIXED + l] = vol;
+    double loss_rate = lxyz * (vol - dscal[DS_COUNT_FIXED + lmax + l]) / inflate_dt;
+    double temp = loss_rate / area;
+    dscal[DS_AREA] = area;
+    dscal[DS_INFL_TEMP] = temp;
+    dscal[DS_MAXCHK] = temp / s1;
+}
+
+// u_l = -temp*(dheav(i+1)-dheav(i))/dx etc. (levelset.f90:527-535) after boundp(dheav)
+__global__ void k_infl_face_vel(Grid g, const double* __restrict__ dscal, const double* __restrict__ dheav,
+                                double* __restrict__ ul, double* __restrict__ vl, double* __restrict__ wl) {
+    if (dscal[DS_INFL_FLAG] == 0.) return;
+    int i0 = blockIdx.x * BX + threadIdx.x, j0 = blockIdx.y * BY + threadIdx.y, k = blockIdx.z + 1;
+    if (i0 >= g.nx || j0 >= g.ny) return;
+    const double temp = dscal[DS_INFL_TEMP];
+    long q = gidx(g, i0, j0, k);
+    double h = dheav[q];
+    ul[q] = -temp * (dheav[gidx(g, wrap(i0 + 1, g.nx), j0, k)] - h) / g.dx;
+    vl[q] = -temp * (dheav[gidx(g, i0, wrap(j0 + 1, g.ny), k)] - h) / g.dy;
+    wl[q] = -temp * (dheav[gidx(g, i0, j0, k + 1)] - h) / g.dz;
+}
+
+// u_lc = (u_l(i-1)+u_l(i))/2*kappa (levelset.f90:541-549)
+__global__ void k_infl_center_vel(Grid g, const double* __restrict__ dscal, const double* __restrict__ kappa,
+                                  const double* __restrict__ ul, const double* __restrict__ vl,
+                                  const double* __restrict__ wl, double* __restrict__ uc, double* __restrict__ vc,
+                                  double* __restrict__ wc) {
+    if (dscal[DS_INFL_FLAG] == 0.) return;
+    int i0 = blockIdx.x * BX + threadIdx.x, j0 = blockIdx.y * BY + threadIdx.y, k = blockIdx.z + 1;
+    if (i0 >= g.nx || j0 >= g.ny) return;
+    long q = gidx(g, i0, j0, k);
+    double kap = kappa[q];
+    uc[q] = (ul[gidx(g, wrap(i0 - 1, g.nx), j0, k)] + ul[q]) / 2. * kap;
+    vc[q] = (vl[gidx(g, i0, wrap(j0 - 1, g.ny), k)] + vl[q]) / 2. * kap;
+    wc[q] = (wl[gidx(g, i0, j0, k - 1)] + wl[q]) / 2. * kap;
+}
+
+int launch_inflation_prepare(Ctx& c, int l, int istep) {
+    ProfScope ps(c, "k_infl_flag");
+    int due = (c.P.inflate_step > 0 && istep % c.P.inflate_step == 0) ? 1 : 0;
+    hipLaunchKernelGGL(k_infl_flag, dim3(1), dim3(64), 0, c.stream, l, c.lmax, due, c.dscal);
+    HIP_OK(hipGetLastError());
+    return ICLS_OK;
+}
+
+// Everything of inflation (levelset.f90:407-549) up to the cell-centre correction
+// velocities; the caller has filled c.curv2 (kappa) and c.normal_m from the current
+// level set with get_curvature predicated on the same flag.
+
+int launch_inflation_velocity(Ctx& c, int l) {
+    ProfScope ps(c, "k_infl");
+    const double* flag = c.dscal + DS_INFL_FLAG;
+    const double s1 = 1.5 * c.g.dz;
+    dim3 blk(BX, BY, 1);
+    dim3 grd = red_grid(c.g, 0, c.g.nz + 1);
+    int nb = red_nblocks(grd);
+    hipLaunchKernelGGL(k_infl_sums, grd, blk, 0, c.stream, c.g, flag, (const double*)c.lvset[l], (const double*)c.curv2,
+                       (const double*)c.normal_m, s1, c.pi, c.dheav, c.dpart, nb);
+    double* sums = c.dscal + DS_COUNT_FIXED + 2 * c.lmax;
+    hipLaunchKernelGGL(k_reduce_final<OpSum>, dim3(1), dim3(256), 0, c.stream, (const double*)c.dpart, nb, 0., sums);
+    hipLaunchKernelGGL(k_reduce_final<OpSum>, dim3(1), dim3(256), 0, c.stream, (const double*)(c.dpart + nb), nb, 0.,
+                       sums + 1);
+    double ncell = 1. * c.g.nx * c.g.ny * c.g.nz;
+    double lxyz = c.P.lx * c.P.ly * c.P.lz;
+    double inflate_dt = c.P.inflate_step * c.dt;
+    hipLaunchKernelGGL(k_infl_scalars, dim3(1), dim3(64), 0, c.stream, l, c.lmax, ncell, c.g.dx, c.g.dy, c.g.dz, lxyz,
+                       inflate_dt, s1, (const double*)sums, c.dscal);
+    HIP_OK(hipGetLastError());
+    int rc = launch_bc_p_pred(c, c.dheav, flag);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_infl_face_vel, grid3(c.g, c.g.nz, blk), blk, 0, c.stream, c.g, (const double*)c.dscal,
+                       (const double*)c.dheav, c.u_l, c.v_l, c.w_l);
+    HIP_OK(hipGetLastError());
+    rc = launch_bc_uvw_pred(c, c.u_l, c.v_l, c.w_l, flag);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_infl_center_vel, grid3(c.g, c.g.nz, blk), blk, 0, c.stream, c.g, (const double*)c.dscal,
+                       (const double*)c.curv2, (const double*)c.u_l, (const double*)c.v_l, (const double*)c.w_l,
+                       c.ucc_l, c.vcc_l, c.wcc_l);
+    HIP_OK(hipGetLastError());
+    return ICLS_OK;
+}
+
+}  // namespace icls

@@ -1,0 +1,225 @@
+// k_poisson.hip -- solver2d (solver.f90:68-196): real FFT along x, real FFT along y,
+// Thomas elimination along z per Fourier mode, inverse FFTs.  On one GPU the
+// 2DECOMP pencil transposes (solver.f90:82-83, 98, 113, 165, 179, 192-193) are
+// identities: every pass works in place on the same i-fastest array.
+//
+// The transforms follow VRFFTF / VRFFTB semantics (libs/fft/fft.f:1073-1145,
+// 374-450): half-complex packing R(1) = Re F0, R(2k) = Re Fk, R(2k+1) = Im Fk,
+// R(N) = Re F(N/2), scaled by sqrt(1/N) both ways.  A length-N real line is
+// transformed as a length-N/2 complex Stockham FFT in LDS plus the even/odd split.
+#include "icls_internal.hpp"
+
+namespace icls {
+
+struct cplx { double re, im; };
+__device__ __forceinline__ cplx cmul(cplx a, cplx b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+__device__ __forceinline__ cplx cadd(cplx a, cplx b) { return {a.re + b.re, a.im + b.im}; }
+__device__ __forceinline__ cplx csub(cplx a, cplx b) { return {a.re - b.re, a.im - b.im}; }
+__device__ __forceinline__ cplx cconj(cplx a) { return {a.re, -a.im}; }
+
+// In-LDS Stockham radix-2 FFT of M points for one line; TPL threads per line.
+// tw[t] = exp(+2 pi i t / N) for t < N/2 (N = 2M); sign = -1 forward, +1 inverse.
+__device__ __forceinline__ cplx* stockham(cplx* x, cplx* y, int M, int lane, int TPL, const cplx* tw, int sign) {
+    for (int Ns = 1; Ns < M; Ns <<= 1) {
+        for (int j = lane; j < M / 2; j += TPL) {
+            int kk = j & (Ns - 1);
+            cplx a = x[j], b = x[j + M / 2];
+            cplx w = tw[kk * (M / Ns)];      // exp(+2 pi i kk / (2 Ns))
+            if (sign < 0) w.im = -w.im;
+            b = cmul(b, w);
+            int d = ((j - kk) << 1) + kk;
+            y[d] = cadd(a, b);
+            y[d + Ns] = csub(a, b);
+        }
+        __syncthreads();
+        cplx* t = x; x = y; y = t;
+    }
+    return x;
+}
+
+// DIR 0: lines along x (contiguous), blocks of LPB consecutive j at one k.
+// DIR 1: lines along y (stride nx), blocks of LPB consecutive i at one k.
+template <int DIR, int INV>
+__global__ void k_rfft(Grid g, int N, int LPB, int TPL, const double* __restrict__ twg, double* __restrict__ f) {
+    extern __shared__ cplx smem[];
+    const int M = N / 2;
+    cplx* bufa = smem;
+    cplx* bufb = smem + (long)LPB * M;
+    cplx* tw = smem + 2L * LPB * M;            // N/2 twiddles
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    const int k = blockIdx.y + 1;
+    const int line0 = blockIdx.x * LPB;
+    for (int t = tid; t < M; t += nthr) tw[t] = {twg[2 * t], twg[2 * t + 1]};
+    const double scale = sqrt(1. / (double)N);
+    // ---- load
+    if (!INV) {
+        for (int e = tid; e < LPB * N; e += nthr) {
+            int ln, n;
+            if (DIR == 0) { ln = e / N; n = e - ln * N; } else { n = e / LPB; ln = e - n * LPB; }
+            long q = DIR == 0 ? gidx(g, n, line0 + ln, k) : gidx(g, line0 + ln, n, k);
+            double v = f[q];
+            cplx* z = bufa + (long)ln * M + (n >> 1);
+            if (n & 1) z->im = v; else z->re = v;
+        }
+    } else {
+        // half-complex -> X_k, then Z_k = (X_k + conj X_{M-k}) + i w^k (X_k - conj X_{M-k})
+        for (int e = tid; e < LPB * N; e += nthr) {
+            int ln, n;
+            if (DIR == 0) { ln = e / N; n = e - ln * N; } else { n = e / LPB; ln = e - n * LPB; }
+            long q = DIR == 0 ? gidx(g, n, line0 + ln, k) : gidx(g, line0 + ln, n, k);
+            double v = f[q];
+            // stage the raw half-complex values in bufb (as doubles)
+            ((double*)(bufb + (long)ln * M))[n] = v;
+        }
+        __syncthreads();
+        for (int e = tid; e < LPB * M; e += nthr) {
+            int ln = e / M, kk = e - ln * M;
+            const double* r = (const double*)(bufb + (long)ln * M);
+            cplx X, Y;   // X_k and X_{M-k}
+            if (kk == 0) { X = {r[0], 0.}; Y = {r[N - 1], 0.}; }
+            else { X = {r[2 * kk - 1], r[2 * kk]}; Y = {r[2 * (M - kk) - 1], r[2 * (M - kk)]}; }
+            cplx cy = cconj(Y);
+            cplx s = cadd(X, cy), dlt = csub(X, cy);
+            cplx w = tw[kk];
+            cplx t = cmul(w, dlt);          // w^k (X_k - conj X_{M-k})
+            cplx it = {-t.im, t.re};        // i * t
+            bufa[(long)ln * M + kk] = cadd(s, it);
+        }
+    }
+    __syncthreads();
+    // ---- transform each line
+    const int ln = tid / TPL, lane = tid - ln * TPL;
+    cplx* res = nullptr;
+    if (ln < LPB) res = stockham(bufa + (long)ln * M, bufb + (long)ln * M, M, lane, TPL, tw, INV ? 1 : -1);
+    else {
+        // threads beyond LPB*TPL still take part in the barriers
+        for (int Ns = 1; Ns < M; Ns <<= 1) __syncthreads();
+    }
+    // every line ends in the same buffer
+    int nst = 0;
+    for (int Ns = 1; Ns < M; Ns <<= 1) nst++;
+    cplx* outb = (nst & 1) ? bufb : bufa;
+    (void)res;
+    // ---- store
+    if (!INV) {
+        // X_k = (Z_k + conj Z_{M-k})/2 + w^-k (Z_k - conj Z_{M-k})/(2i), k = 0..M
+        for (int e = tid; e < LPB * N; e += nthr) {
+            int lnn, n;
+            if (DIR == 0) { lnn = e / N; n = e - lnn * N; } else { n = e / LPB; lnn = e - n * LPB; }
+            const cplx* Z = outb + (long)lnn * M;
+            double v;
+            if (n == 0) v = Z[0].re + Z[0].im;
+            else if (n == N - 1) v = Z[0].re - Z[0].im;
+            else {
+                int kk = (n + 1) >> 1;
+                cplx a = Z[kk], b = cconj(Z[M - kk]);
+                cplx ev = {(a.re + b.re) * 0.5, (a.im + b.im) * 0.5};
+                cplx od = {(a.im - b.im) * 0.5, -(a.re - b.re) * 0.5};   // (a - b)/(2i)
+                cplx w = tw[kk];
+                w.im = -w.im;
+                cplx X = cadd(ev, cmul(w, od));
+                v = (n & 1) ? X.re : X.im;
+            }
+            long q = DIR == 0 ? gidx(g, n, line0 + lnn, k) : gidx(g, line0 + lnn, n, k);
+            f[q] = scale * v;
+        }
+    } else {
+        for (int e = tid; e < LPB * N; e += nthr) {
+            int lnn, n;
+            if (DIR == 0) { lnn = e / N; n = e - lnn * N; } else { n = e / LPB; lnn = e - n * LPB; }
+            cplx z = outb[(long)lnn * M + (n >> 1)];
+            double v = (n & 1) ? z.im : z.re;
+            long q = DIR == 0 ? gidx(g, n, line0 + lnn, k) : gidx(g, line0 + lnn, n, k);
+            f[q] = scale * v;
+        }
+    }
+}
+
+static int rfft_launch(Ctx& c, double* f, int dir, int inverse) {
+    const int N = dir == 0 ? c.g.nx : c.g.ny;
+    const int nlines = dir == 0 ? c.g.ny : c.g.nx;
+    const int M = N / 2;
+    int LPB = dir == 0 ? 4 : 16;
+    if (LPB > nlines) LPB = nlines;
+    while (nlines % LPB) LPB >>= 1;
+    while (LPB > 1 && (size_t)(2L * LPB * M + M) * sizeof(cplx) > 64 * 1024) LPB >>= 1;
+    int TPL = M / 2 < 64 ? M / 2 : 64;
+    if (TPL < 1) TPL = 1;
+    while (LPB * TPL > 1024) TPL >>= 1;
+    int nthr = LPB * TPL;
+    if (nthr < 64) nthr = 64;
+    size_t shm = (size_t)(2L * LPB * M + M) * sizeof(cplx);
+    dim3 grd(nlines / LPB, c.g.nz, 1);
+    const double* tw = dir == 0 ? c.tw_x : c.tw_y;
+#define RF(D, I) hipLaunchKernelGGL((k_rfft<D, I>), grd, dim3(nthr), shm, c.stream, c.g, N, LPB, TPL, tw, f)
+    if (dir == 0 && !inverse) RF(0, 0);
+    else if (dir == 0) RF(0, 1);
+    else if (!inverse) RF(1, 0);
+    else RF(1, 1);
+#undef RF
+    HIP_OK(hipGetLastError());
+    return ICLS_OK;
+}
+
+int launch_fft_x(Ctx& c, double* f, int inverse) {
+    ProfScope ps(c, inverse ? "k_rfft_x_inv" : "k_rfft_x_fwd");
+    return rfft_launch(c, f, 0, inverse);
+}
+int launch_fft_y(Ctx& c, double* f, int inverse) {
+    ProfScope ps(c, inverse ? "k_rfft_y_inv" : "k_rfft_y_fwd");
+    return rfft_launch(c, f, 1, inverse);
+}
+
+// Thomas elimination per (i,j) column (solver.f90:119-160) with the Periodic-branch
+// matrix of initsolver.f90:36-49: a = c = dzi^2, b = -(a+c), b(1) += a, b(kmax) += c,
+// a(1) = c(kmax) = 0, diagonal shifted by xyrt(i,j).  A zero last pivot (the (0,0)
+// mode) gives p(kmax) = 0 as in the reference.
+__global__ void k_thomas(Grid g, const double* __restrict__ xyrt, double* __restrict__ p, double* __restrict__ d) {
+    int i0 = blockIdx.x * blockDim.x + threadIdx.x, j0 = blockIdx.y;
+    if (i0 >= g.nx) return;
+    const int kmax = g.nz;
+    const double a = g.dzi * g.dzi, cc = g.dzi * g.dzi;
+    const double bmid = -(a + cc);
+    const double b1 = bmid + a, bN = bmid + cc;
+    const double lam = xyrt[i0 + (long)g.nx * j0];
+    long q = gidx(g, i0, j0, 1);
+    const long sz = g.sz;
+    double z = 1. / (b1 + lam);
+    double dprev = cc * z;
+    double pprev = p[q] * z;
+    d[q] = dprev;
+    p[q] = pprev;
+    for (int k = 2; k <= kmax - 1; k++) {
+        q += sz;
+        double bb = bmid + lam;
+        z = 1. / (bb - a * dprev);
+        dprev = cc * z;
+        pprev = (p[q] - a * pprev) * z;
+        d[q] = dprev;
+        p[q] = pprev;
+    }
+    q += sz;
+    {
+        double bb = bN + lam;
+        z = bb - a * dprev;
+        if (z != 0.) pprev = (p[q] - a * pprev) / z;
+        else pprev = 0.;
+        p[q] = pprev;
+    }
+    for (int k = kmax - 1; k >= 1; k--) {
+        q -= sz;
+        pprev = p[q] - d[q] * pprev;
+        p[q] = pprev;
+    }
+}
+
+int launch_thomas(Ctx& c, double* f) {
+    ProfScope ps(c, "k_thomas");
+    int bx = c.g.nx < 64 ? c.g.nx : 64;
+    hipLaunchKernelGGL(k_thomas, dim3((c.g.nx + bx - 1) / bx, c.g.ny, 1), dim3(bx), 0, c.stream, c.g,
+                       (const double*)c.xyrt, f, c.thomas_d);
+    HIP_OK(hipGetLastError());
+    return ICLS_OK;
+}
+
+}  // namespace icls

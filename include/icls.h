@@ -1,0 +1,182 @@
+/*
+ * icls.h -- C ABI of the MI355X ICLS step library (libicls.so).
+ *
+ * Every entry point replaces one Fortran subroutine of the reference's per-timestep
+ * two-phase step; the citation on each line names the subroutine it stands in for
+ * (paths relative to the reference tree).  Fortran host code binds these through
+ * iso_c_binding (see the modules under icls-release_amd/fortran/).
+ *
+ * Conventions
+ *   - Plain C types only: int32, double, pointers and sizes.  No torch types.
+ *   - Every entry returns int32: ICLS_OK (0) or one of the ICLS_ERR_* codes below.
+ *     icls_last_error() returns a human-readable message for the last failure.
+ *   - One process-wide context (the reference keeps its state in module globals,
+ *     src/common.f90:1-88).  Calls are not re-entrant and run on one HIP stream.
+ *   - Host arrays passed to icls_upload / icls_download use the reference's own
+ *     Fortran layout (column-major, i fastest) including its halo bounds:
+ *       ICLS_SHAPE_C   (0:i1,0:j1,0:k1)            src/common.f90:7-9
+ *       ICLS_SHAPE_LS  (-2:i1+2,-2:j1+2,-2:k1+2)   src/common.f90:17-18
+ *       ICLS_SHAPE_I   (1:imax,1:jmax,1:kmax)      src/common.f90:11-13
+ *     On the device x and y are periodic by index wrap (no halo storage); download
+ *     fills the x/y halos of the host array from the periodic images.
+ */
+#ifndef ICLS_H
+#define ICLS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---- */
+#define ICLS_OK              0
+#define ICLS_ERR_PARAM       1   /* bad parameter block or argument                  */
+#define ICLS_ERR_HIP         2   /* HIP runtime error (allocation, launch, copy)      */
+#define ICLS_ERR_NAN         3   /* NaN in vnew(1,1,1): src/main.f90:207-210           */
+#define ICLS_ERR_DIVERGENCE  4   /* max |div u| > 1e-12: src/chkdiv.f90:60-86          */
+#define ICLS_ERR_STATE       5   /* call out of order (no context, not initialised)    */
+#define ICLS_ERR_NODEVICE    6   /* no GPU visible                                     */
+#define ICLS_ERR_UNSUPPORTED 7   /* option the reference aborts on or this build omits */
+
+/* ---- precision modes ---- */
+#define ICLS_MODE_FP64          0   /* every field and kernel in FP64 (reference precision)   */
+#define ICLS_MODE_MIXED         1   /* FP32 level set, momentum, FFT; FP64 Thomas, residual,
+                                       pressure accumulation and volume sums                 */
+#define ICLS_MODE_MIXED_LEARNED 2   /* MIXED + MLP curvature closure on under-resolved cells  */
+
+/* ---- level-set advection schemes (src/param.f90:216, mod_param_ls) ---- */
+#define ICLS_ADV_HOUC5 0
+#define ICLS_ADV_WENO5 1
+
+/* ---- initial conditions (src/init.f90:146-210, 717-755, 901-986) ---- */
+#define ICLS_INIU_ZER 0     /* 'zer': fluid at rest        */
+#define ICLS_INIU_COU 1     /* 'cou': plane Couette profile */
+#define ICLS_INIL_DRP 0     /* 'drp'/'mdp': one level set per drop (sphere SDF) */
+
+#define ICLS_MAX_DROPS 1024
+
+/*
+ * Runtime parameter block.  Replaces the compile-time PARAMETERs of
+ * src/param.f90:12-172 and src/param.f90:199-275 (mod_param_ls).
+ * Grid spacing follows the reference's derivation: dxi = itot/lx, dx = 1/dxi
+ * (src/param.f90:37-38).
+ */
+typedef struct icls_params {
+    int32_t itot, jtot, ktot;        /* grid (param.f90:27)                              */
+    int32_t lmax;                    /* number of level sets = drops (param.f90:170)     */
+    double  lx, ly, lz;              /* domain size (param.f90:33-35)                    */
+    double  timestep;                /* dt (param.f90:47)                                */
+    double  rho1, rho2, miu1, miu2;  /* phase 1 = phi>0 (outer), phase 2 = drop          */
+    double  Re, We, iFr;             /* param.f90:69-71                                  */
+    double  g_unit[3];               /* param.f90:77                                     */
+    int32_t ls_adv_space;            /* ICLS_ADV_HOUC5 / ICLS_ADV_WENO5                  */
+    int32_t re_initialization;       /* param.f90:164                                    */
+    int32_t reinit_step;             /* param.f90:230                                    */
+    int32_t reinit_iter_max;         /* param.f90:233                                    */
+    int32_t mass_correction;         /* param.f90:165                                    */
+    int32_t inflate_step;            /* param.f90:258                                    */
+    int32_t ns_np;                   /* no-slip walls at rest (param.f90:95)             */
+    double  wall_speed;              /* |v| of the z walls when ns_np == 0; the reference
+                                        hard-codes 1.5 (bound.f90:33-34, init.f90:193-194) */
+    int32_t constPoi;                /* constant pressure gradient forcing (param.f90:96) */
+    int32_t iniu;                    /* ICLS_INIU_*                                      */
+    int32_t mode;                    /* ICLS_MODE_*                                      */
+    int32_t ioutchk;                 /* diagnostics cadence (param.f90:85)               */
+    int32_t reserved[8];
+    double  drop_xc[ICLS_MAX_DROPS]; /* drop centres and radii, one level set each       */
+    double  drop_yc[ICLS_MAX_DROPS];
+    double  drop_zc[ICLS_MAX_DROPS];
+    double  drop_r[ICLS_MAX_DROPS];
+} icls_params;
+
+/* ---- field identifiers for upload / download ---- */
+enum icls_field {
+    ICLS_F_UNEW = 0, ICLS_F_VNEW, ICLS_F_WNEW,     /* C  velocity          common.f90:7   */
+    ICLS_F_DUDT, ICLS_F_DVDT, ICLS_F_DWDT,         /* C  predictor u*      common.f90:7   */
+    ICLS_F_PNEW, ICLS_F_POLD, ICLS_F_P,            /* C  pressures         main.f90:57,60 */
+    ICLS_F_PJUMP,                                  /* C  GFM jump          main.f90:57    */
+    ICLS_F_DUOLD, ICLS_F_DVOLD, ICLS_F_DWOLD,      /* I  AB2 history       main.f90:56    */
+    ICLS_F_PX, ICLS_F_PY, ICLS_F_PZ,               /* C  face jumps        common.f90:42  */
+    ICLS_F_PXOLD, ICLS_F_PYOLD, ICLS_F_PZOLD,      /* C                    common.f90:43  */
+    ICLS_F_MIU, ICLS_F_RHO, ICLS_F_JIEMIAN,        /* C  materials         common.f90:9   */
+    ICLS_F_CURV,                                   /* C  curv_cmn          common.f90:19  */
+    ICLS_F_UCC, ICLS_F_VCC, ICLS_F_WCC,            /* I  cell-centre vel.  common.f90:11  */
+    ICLS_F_RHO_U, ICLS_F_RHO_V, ICLS_F_RHO_W,      /* I  face densities    common.f90:13  */
+    ICLS_F_LVSET,                                  /* LS level set l       common.f90:18  */
+    ICLS_F_NORMAL_M,                               /* I  |grad phi|        common.f90:25  */
+    ICLS_F_COUNT
+};
+
+#define ICLS_SHAPE_C  0
+#define ICLS_SHAPE_LS 1
+#define ICLS_SHAPE_I  2
+
+/* ---- scalar identifiers ---- */
+enum icls_scalar {
+    ICLS_S_TIME = 0,       /* time                     common.f90:71   */
+    ICLS_S_DT,             /* dt                       common.f90:71   */
+    ICLS_S_RHO_AVR,        /* rho_avr                  param.f90:62    */
+    ICLS_S_V_BULK,         /* v_bulk                   common.f90:79   */
+    ICLS_S_MAXCHK,         /* maxchk_glb               common.f90:48   */
+    ICLS_S_REINIT_RES,     /* last re-init residual    levelset.f90:831 */
+    ICLS_S_NORM_OLD,       /* pressure norm            main.f90:183    */
+    ICLS_S_COUNT
+};
+
+/* ---- lifecycle ---- */
+const char* icls_last_error(void);
+int32_t icls_version(void);
+/* Allocate the device state for this parameter block (replaces module allocation,
+ * initmpi.f90 + common.f90).  Fails with ICLS_ERR_NODEVICE without a GPU. */
+int32_t icls_init(const icls_params* p);
+int32_t icls_finalize(void);
+int32_t icls_field_shape(int32_t field, int32_t* shape, int64_t* count);
+
+/* init_program (src/init.f90:22-117) for begin == 0: initial fields, least-squares
+ * matrix, solver tables, boundary conditions, materials, init_volume, AB2 start. */
+int32_t icls_init_program(void);
+/* The part of init_program after the initial fields (init.f90:63-112), run on the
+ * fields currently on the device (after icls_upload). */
+int32_t icls_init_derived(void);
+
+/* ---- field / scalar transfer ---- */
+int32_t icls_upload(int32_t field, int32_t l, const double* host);
+int32_t icls_download(int32_t field, int32_t l, double* host);
+int32_t icls_get_scalar(int32_t which, double* value);
+int32_t icls_set_scalar(int32_t which, double value);
+int32_t icls_get_volume(int32_t l, double* volume, double* init_volume);
+int32_t icls_sync(void);
+
+/* ---- the per-step operators (src/main.f90:103-291) ---- */
+int32_t icls_ls_main(int32_t istep);                     /* levelset.f90:28-76          */
+int32_t icls_average_stream_vel(double* v_avr);          /* misc.f90:642-655            */
+int32_t icls_ab2(void);                                  /* ab2.f90:21-138              */
+int32_t icls_fillps(void);                               /* fillps.f90:13-57 (into p)   */
+int32_t icls_gfm4p(void);                                /* gfm.f90:17-267 (p_jump)     */
+int32_t icls_add_pjump(void);                            /* main.f90:142  p = p+p_jump  */
+int32_t icls_solver2d(void);                             /* solver.f90:68-196 (on p)    */
+int32_t icls_boundp(int32_t field);                      /* bound.f90:182-287           */
+int32_t icls_correc(void);                               /* correc.f90:15-44            */
+int32_t icls_bounduvw(int32_t which);                    /* bound.f90:15-177; 0=unew,1=dudt */
+int32_t icls_update_pressure(int32_t first_step);        /* main.f90:164-185            */
+int32_t icls_average_vel_at_center(void);                /* misc.f90:587-604            */
+int32_t icls_check_nan(void);                            /* main.f90:207-210            */
+int32_t icls_chkdiv(int32_t istep, double* divtot, double* divmax, int32_t* ijk); /* chkdiv.f90:10-90 */
+int32_t icls_chkdt(double* dtmax);                       /* chkdt.f90:11-105            */
+int32_t icls_max_vel_at_center(double* vmax);            /* misc.f90:610-636            */
+/* One whole time step: main.f90:105-210 in the reference order, with the kernels
+ * fused where the arithmetic per element is unchanged. */
+int32_t icls_step(int32_t istep, int32_t begin);
+
+/* ---- instrumentation ---- */
+/* Per-kernel HIP-event timing on the library's stream.  When enabled every launch
+ * is bracketed with events; icls_kernel_times copies up to n (name, total ms, count). */
+int32_t icls_profile_enable(int32_t on);
+int32_t icls_profile_reset(void);
+int32_t icls_kernel_times(int32_t n, char* names /* n*64 */, double* total_ms, int64_t* counts, int32_t* n_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ICLS_H */

@@ -1,0 +1,241 @@
+"""ctypes binding of the CPU oracle (liboracle.so) -- TEST INFRASTRUCTURE ONLY.
+
+Loaded by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg, as the
+checker.  The product library (libicls.so) never imports this module.
+
+Fields come back as numpy arrays in the reference's Fortran layout, shape
+(nx, ny, nz) with order='F', i.e. element [i, j, k] is the reference's
+field(i + lb, j + lb, k + lb) for the field's lower bound lb (0, -2 or 1).
+"""
+import ctypes
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+sys.path.insert(0, os.path.join(ROOT, "icls-release_amd"))
+from iclsamd import params as P  # noqa: E402
+
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle not built: run `make -C oracle`")
+        L = ctypes.CDLL(path)
+        vp, i32, dbl = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+        dp = ctypes.POINTER(ctypes.c_double)
+        L.oracle_create.restype = vp
+        L.oracle_create.argtypes = [ctypes.POINTER(P.IclsParams)]
+        L.oracle_destroy.argtypes = [vp]
+        for name in ["oracle_init_program", "oracle_init_derived", "oracle_ab2", "oracle_fillps",
+                     "oracle_gfm4p", "oracle_add_pjump", "oracle_solver2d", "oracle_correc",
+                     "oracle_average_vel_at_center", "oracle_update_materials", "oracle_band_count"]:
+            getattr(L, name).argtypes = [vp]
+            getattr(L, name).restype = i32
+        for name in ["oracle_ls_main", "oracle_boundp", "oracle_bounduvw", "oracle_update_pressure",
+                     "oracle_tube_construction", "oracle_ls_advection", "oracle_inflation",
+                     "oracle_get_curvature", "oracle_re_curvature"]:
+            getattr(L, name).argtypes = [vp, i32]
+            getattr(L, name).restype = i32
+        L.oracle_step.argtypes = [vp, i32, i32]
+        L.oracle_step.restype = i32
+        L.oracle_ls_mass_correction.argtypes = [vp, i32, i32]
+        L.oracle_ls_mass_correction.restype = i32
+        L.oracle_ls_reinit.argtypes = [vp, i32, i32, ctypes.POINTER(ctypes.c_int), dp]
+        L.oracle_ls_reinit.restype = i32
+        L.oracle_average_stream_vel.argtypes = [vp]
+        L.oracle_average_stream_vel.restype = dbl
+        L.oracle_chkdiv.argtypes = [vp, i32, dp, dp, ctypes.POINTER(ctypes.c_int)]
+        L.oracle_chkdiv.restype = i32
+        L.oracle_chkdt.argtypes = [vp]
+        L.oracle_chkdt.restype = dbl
+        L.oracle_max_vel_at_center.argtypes = [vp]
+        L.oracle_max_vel_at_center.restype = dbl
+        L.oracle_mom_main.argtypes = [vp, dp, dp, dp]
+        L.oracle_eval_adv.argtypes = [vp, i32, i32, dp]
+        L.oracle_eval_reinit_rhs.argtypes = [vp, i32, dp]
+        L.oracle_band_mask.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
+        L.oracle_ipot_dmin.argtypes = [vp, i32, i32]
+        L.oracle_ipot_dmin.restype = dbl
+        L.oracle_field.argtypes = [vp, i32, i32, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_long)]
+        L.oracle_field.restype = dp
+        L.oracle_scalar.argtypes = [vp, i32]
+        L.oracle_scalar.restype = dbl
+        L.oracle_set_scalar.argtypes = [vp, i32, dbl]
+        L.oracle_volume.argtypes = [vp, i32, dp]
+        L.oracle_volume.restype = dbl
+        L.oracle_pseudo_inv_A.argtypes = [vp]
+        L.oracle_pseudo_inv_A.restype = dp
+        L.oracle_rfft_forward.argtypes = [i32, dp]
+        L.oracle_rfft_backward.argtypes = [i32, dp]
+        L.oracle_pseudo_inverse.argtypes = [dbl, dbl, dbl, dp]
+        L.oracle_heaviside_trig.argtypes = [dbl, dbl]
+        L.oracle_heaviside_trig.restype = dbl
+        L.oracle_dirac_trig.argtypes = [dbl, dbl]
+        L.oracle_dirac_trig.restype = dbl
+        L.oracle_icurvature.argtypes = [i32, dp, dp]
+        L.oracle_icurvature.restype = dbl
+        L.oracle_thomas.argtypes = [i32, dbl, dbl, dp]
+        _LIB = L
+    return _LIB
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class Oracle:
+    """One oracle state for one parameter block."""
+
+    def __init__(self, p):
+        self.p = p.copy()
+        self.L = lib()
+        self.h = self.L.oracle_create(ctypes.byref(self.p))
+        if not self.h:
+            raise ValueError("oracle_create rejected the parameter block")
+
+    def close(self):
+        if self.h:
+            self.L.oracle_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # --- fields
+    def field(self, fid, l=0):
+        """Live numpy view (Fortran order) of one oracle array."""
+        shape = ctypes.c_int()
+        count = ctypes.c_long()
+        ptr = self.L.oracle_field(self.h, fid, l, ctypes.byref(shape), ctypes.byref(count))
+        if not ptr:
+            raise KeyError(fid)
+        dims = P.field_dims(self.p, fid)
+        flat = np.ctypeslib.as_array(ptr, shape=(count.value,))
+        return flat.reshape(dims, order="F")
+
+    def get(self, fid, l=0):
+        return np.array(self.field(fid, l), copy=True, order="F")
+
+    def set(self, fid, arr, l=0):
+        self.field(fid, l)[...] = arr
+
+    def scalar(self, which):
+        return self.L.oracle_scalar(self.h, which)
+
+    def set_scalar(self, which, v):
+        self.L.oracle_set_scalar(self.h, which, v)
+
+    def volume(self, l=0):
+        iv = ctypes.c_double()
+        v = self.L.oracle_volume(self.h, l, ctypes.byref(iv))
+        return v, iv.value
+
+    # --- operators
+    def init_program(self):
+        return self.L.oracle_init_program(self.h)
+
+    def init_derived(self):
+        return self.L.oracle_init_derived(self.h)
+
+    def step(self, istep, begin=0):
+        return self.L.oracle_step(self.h, istep, begin)
+
+    def run(self, nsteps, begin=0):
+        for istep in range(begin + 1, begin + nsteps + 1):
+            rc = self.step(istep, begin)
+            if rc:
+                raise RuntimeError("oracle step %d returned %d" % (istep, rc))
+
+    def __getattr__(self, name):
+        fn = getattr(lib(), "oracle_" + name)
+        h = self.__dict__["h"]
+        return lambda *a: fn(h, *a)
+
+    def chkdiv(self, istep=1):
+        dt, dm = ctypes.c_double(), ctypes.c_double()
+        ijk = (ctypes.c_int * 3)()
+        rc = self.L.oracle_chkdiv(self.h, istep, ctypes.byref(dt), ctypes.byref(dm), ijk)
+        return rc, dt.value, dm.value, tuple(ijk)
+
+    def mom_main(self):
+        n = self.p.itot * self.p.jtot * self.p.ktot
+        out = [np.zeros(n) for _ in range(3)]
+        self.L.oracle_mom_main(self.h, *[_dp(o) for o in out])
+        dims = (self.p.itot, self.p.jtot, self.p.ktot)
+        return [o.reshape(dims, order="F") for o in out]
+
+    def eval_adv(self, l=0, scheme=P.ADV_HOUC5):
+        n = self.p.itot * self.p.jtot * self.p.ktot
+        out = np.zeros(n)
+        self.L.oracle_eval_adv(self.h, l, scheme, _dp(out))
+        return out.reshape((self.p.itot, self.p.jtot, self.p.ktot), order="F")
+
+    def eval_reinit_rhs(self, l=0):
+        dims = P.field_dims(self.p, P.F_LVSET)
+        out = np.zeros(int(np.prod(dims)))
+        self.L.oracle_eval_reinit_rhs(self.h, l, _dp(out))
+        return out.reshape(dims, order="F")
+
+    def band_mask(self):
+        n = self.p.itot * self.p.jtot * self.p.ktot
+        m = np.zeros(n, dtype=np.int32)
+        self.L.oracle_band_mask(self.h, m.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+        return m.reshape((self.p.itot, self.p.jtot, self.p.ktot), order="F")
+
+    def ls_reinit(self, l=0, maxiter=1):
+        it, res = ctypes.c_int(), ctypes.c_double()
+        self.L.oracle_ls_reinit(self.h, l, maxiter, ctypes.byref(it), ctypes.byref(res))
+        return it.value, res.value
+
+    def pseudo_inv_A(self):
+        ptr = self.L.oracle_pseudo_inv_A(self.h)
+        return np.ctypeslib.as_array(ptr, shape=(270,)).reshape((10, 27), order="F").copy()
+
+
+# --- stand-alone numerics
+def rfft_forward(x):
+    r = np.array(x, dtype=np.float64, copy=True)
+    lib().oracle_rfft_forward(len(r), _dp(r))
+    return r
+
+
+def rfft_backward(x):
+    r = np.array(x, dtype=np.float64, copy=True)
+    lib().oracle_rfft_backward(len(r), _dp(r))
+    return r
+
+
+def pseudo_inverse(dx, dy, dz):
+    out = np.zeros(270)
+    lib().oracle_pseudo_inverse(dx, dy, dz, _dp(out))
+    return out.reshape((10, 27), order="F")
+
+
+def heaviside_trig(phi, eps):
+    return lib().oracle_heaviside_trig(phi, eps)
+
+
+def dirac_trig(phi, eps):
+    return lib().oracle_dirac_trig(phi, eps)
+
+
+def icurvature(nb, ph, cv):
+    ph = np.ascontiguousarray(ph, dtype=np.float64)
+    cv = np.ascontiguousarray(cv, dtype=np.float64)
+    return lib().oracle_icurvature(nb, _dp(ph), _dp(cv))
+
+
+def thomas(kmax, dzi, lam, rhs):
+    p = np.array(rhs, dtype=np.float64, copy=True)
+    lib().oracle_thomas(kmax, dzi, lam, _dp(p))
+    return p

@@ -1,0 +1,101 @@
+"""C ABI of libicls.so (CPU: no compute calls).
+
+Checks that the library loads, exports every entry point include/icls.h declares,
+that the Python and Fortran mirrors of struct icls_params have the C layout, and
+that entry points fail with status codes (never a crash) without a GPU or context.
+"""
+import ctypes
+import os
+import re
+import subprocess
+import tempfile
+
+import pytest
+
+from helpers import ROOT
+from iclsamd import _lib
+from iclsamd import params as P
+
+HEADER = os.path.join(ROOT, "include", "icls.h")
+
+
+def _declared():
+    src = open(HEADER).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(icls_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib.load()
+    declared = _declared()
+    assert len(declared) >= 30
+    missing = [s for s in declared if not hasattr(L, s)]
+    assert not missing, missing
+    assert sorted(_lib.EXPORTS) == declared
+
+
+def _c_layout():
+    code = r"""
+#include <stdio.h>
+#include <stddef.h>
+#include "icls.h"
+int main(void) {
+  printf("%zu %zu %zu %zu %zu %zu\n", sizeof(icls_params), offsetof(icls_params, g_unit),
+         offsetof(icls_params, wall_speed), offsetof(icls_params, reserved), offsetof(icls_params, drop_xc),
+         offsetof(icls_params, drop_r));
+  return 0;
+}
+"""
+    with tempfile.TemporaryDirectory() as d:
+        c = os.path.join(d, "l.c")
+        exe = os.path.join(d, "l")
+        open(c, "w").write(code)
+        subprocess.check_call(["gcc", "-I", os.path.join(ROOT, "include"), "-o", exe, c])
+        return [int(x) for x in subprocess.check_output([exe]).split()]
+
+
+def test_params_struct_layout_matches_c():
+    S = P.IclsParams
+    ours = [ctypes.sizeof(S), S.g_unit.offset, S.wall_speed.offset, S.reserved.offset, S.drop_xc.offset,
+            S.drop_r.offset]
+    assert ours == _c_layout()
+
+
+def test_calls_without_context_fail_cleanly():
+    L = _lib.load()
+    L.icls_finalize()
+    assert L.icls_ab2() == 5          # ICLS_ERR_STATE
+    assert L.icls_step(1, 0) == 5
+    assert b"icls_init" in L.icls_last_error()
+
+
+def test_bad_parameters_rejected_before_device_use():
+    L = _lib.load()
+    p = P.drop64()
+    p.itot = 48                        # not a power of two
+    assert L.icls_init(ctypes.byref(p)) == 1
+    p = P.drop64()
+    p.reinit_iter_max = 3
+    assert L.icls_init(ctypes.byref(p)) == 7
+
+
+def test_init_without_gpu_reports_nodevice():
+    try:
+        import torch
+        if torch.cuda.device_count() > 0:
+            pytest.skip("a GPU is visible")
+    except ImportError:
+        pass
+    L = _lib.load()
+    p = P.drop64()
+    assert L.icls_init(ctypes.byref(p)) == 6
+    with pytest.raises(_lib.IclsError):
+        _lib.Solver(p)
+
+
+def test_fortran_driver_links_the_library():
+    exe = os.path.join(ROOT, "icls-release_amd", "icls_fortran")
+    if not os.path.exists(exe):
+        pytest.skip("Fortran driver not built (make -C icls-release_amd fortran)")
+    out = subprocess.run(["ldd", exe], capture_output=True, text=True).stdout
+    assert "libicls.so" in out

@@ -1,0 +1,339 @@
+"""Pins of the CPU oracle (CPU only).
+
+The reference ships no tests, golden vectors or fixtures for this path, so every
+oracle function is pinned here against something independent of its own formula:
+VFFTPACK compiled from the reference's libs/fft/fft.f (oracle/_ref, when the
+reference tree is present), numpy's FFT, exact rational arithmetic, closed forms
+that the discretisation reproduces exactly (linear and quadratic fields, planar
+signed distance functions, discrete eigenfunctions), dense linear algebra and the
+reference's own runtime invariants (divergence < 1e-12).
+"""
+import ctypes
+import os
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+import pyoracle as O
+from helpers import ROOT, small_drop
+from iclsamd import params as P
+
+REF_VFFT = os.path.join(ROOT, "oracle", "_ref", "libvfftpack.so")
+
+
+# ---------------------------------------------------------------- FFT (VRFFTF / VRFFTB)
+def _vfftpack():
+    if not os.path.exists(REF_VFFT):
+        pytest.skip("VFFTPACK reference build absent (needs /root/reference; make -C oracle ref)")
+    return ctypes.CDLL(REF_VFFT)
+
+
+def _vfft(L, x, inverse):
+    n = len(x)
+    ip = ctypes.POINTER(ctypes.c_int)
+    m = ctypes.c_int(1)
+    nn = ctypes.c_int(n)
+    wsave = np.zeros(n + 15)
+    r = np.array(x, dtype=np.float64)
+    rt = np.zeros(n)
+    L.vrffti_(ctypes.byref(nn), wsave.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    fn = L.vrfftb_ if inverse else L.vrfftf_
+    fn(ctypes.byref(m), ctypes.byref(nn), r.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+       rt.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ctypes.byref(m),
+       wsave.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    return r
+
+
+def _numpy_halfcomplex(x):
+    n = len(x)
+    X = np.fft.rfft(x) / np.sqrt(n)
+    r = np.zeros(n)
+    r[0] = X[0].real
+    for k in range(1, (n + 1) // 2):
+        r[2 * k - 1] = X[k].real
+        r[2 * k] = X[k].imag
+    if n % 2 == 0:
+        r[n - 1] = X[n // 2].real
+    return r
+
+
+@pytest.mark.parametrize("n", [8, 16, 32, 64, 256, 512])
+def test_rfft_matches_vfftpack_source_build(n):
+    L = _vfftpack()
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal(n)
+    ref = _vfft(L, x, False)
+    ours = O.rfft_forward(x)
+    assert np.max(np.abs(ours - ref)) <= 1e-14 * np.max(np.abs(ref)) * np.log2(n)
+    back_ref = _vfft(L, ref, True)
+    back = O.rfft_backward(ref)
+    assert np.max(np.abs(back - back_ref)) <= 1e-14 * np.max(np.abs(back_ref)) * np.log2(n)
+
+
+@pytest.mark.parametrize("n", [8, 12, 16, 64, 256, 1024])
+def test_rfft_halfcomplex_packing_and_scaling(n):
+    rng = np.random.default_rng(1000 + n)
+    x = rng.standard_normal(n)
+    r = O.rfft_forward(x)
+    assert np.allclose(r, _numpy_halfcomplex(x), rtol=0, atol=1e-13 * np.log2(n))
+    # VRFFTB is the correctly normalised inverse of VRFFTF (fft.f:1149-1151)
+    assert np.allclose(O.rfft_backward(r), x, rtol=0, atol=1e-13 * np.log2(n))
+
+
+def test_rfft_single_modes():
+    n = 64
+    j = np.arange(n)
+    for k in [1, 5, 31]:
+        r = O.rfft_forward(np.cos(2 * np.pi * k * j / n))
+        e = np.zeros(n)
+        e[2 * k - 1] = np.sqrt(n) / 2          # Re F_k = sqrt(1/N) * N/2
+        assert np.allclose(r, e, atol=1e-13)
+        r = O.rfft_forward(np.sin(2 * np.pi * k * j / n))
+        e = np.zeros(n)
+        e[2 * k] = -np.sqrt(n) / 2             # Im F_k = -sqrt(1/N) sum x sin
+        assert np.allclose(r, e, atol=1e-13)
+    r = O.rfft_forward((-1.0) ** j)
+    e = np.zeros(n)
+    e[n - 1] = np.sqrt(n)
+    assert np.allclose(r, e, atol=1e-13)
+
+
+# ---------------------------------------------------------------- Thomas (solver.f90:119-160)
+def _tridiag(kmax, dzi, lam):
+    a = dzi * dzi
+    A = np.zeros((kmax, kmax))
+    for k in range(kmax):
+        A[k, k] = -2 * a + lam
+        if k > 0:
+            A[k, k - 1] = a
+        if k < kmax - 1:
+            A[k, k + 1] = a
+    A[0, 0] += a              # Neumann (initsolver.f90:46-49)
+    A[-1, -1] += a
+    return A
+
+
+@pytest.mark.parametrize("lam", [-1.0, -4096.0 * 0.3, -4 * 4096.0])
+def test_thomas_matches_dense_solve(lam):
+    kmax, dzi = 16, 64.0
+    rng = np.random.default_rng(7)
+    f = rng.standard_normal(kmax)
+    p = O.thomas(kmax, dzi, lam, f)
+    ref = np.linalg.solve(_tridiag(kmax, dzi, lam), f)
+    assert np.max(np.abs(p - ref)) <= 1e-12 * np.max(np.abs(ref))
+
+
+def test_thomas_zero_mode_pivot():
+    """(0,0) Fourier mode: the Neumann matrix is singular, the last pivot is exactly
+    zero for a power-of-two dzi and the reference sets p(kmax) = 0 (solver.f90:144-148)."""
+    kmax, dzi = 16, 64.0
+    rng = np.random.default_rng(8)
+    f = rng.standard_normal(kmax)
+    f -= f.mean()
+    p = O.thomas(kmax, dzi, 0.0, f)
+    assert p[-1] == 0.0
+    A = _tridiag(kmax, dzi, 0.0)
+    assert np.max(np.abs(A @ p - f)) <= 1e-9 * np.max(np.abs(f)) * dzi * dzi
+
+
+# ---------------------------------------------------------------- least squares (init.f90:1587-1650)
+def _exact_pinv(h):
+    rows = []
+    for dz in (-1, 0, 1):
+        for dy in (-1, 0, 1):
+            for dx in (-1, 0, 1):
+                x, y, z = h * dx, h * dy, h * dz
+                rows.append([Fraction(1), x, y, z, x * x / 2, y * y / 2, z * z / 2, x * y, x * z, y * z])
+    n = 10
+    M = [[sum(r[i] * r[j] for r in rows) for j in range(n)] + [Fraction(int(i == j)) for j in range(n)] for i in range(n)]
+    for c in range(n):
+        piv = next(r for r in range(c, n) if M[r][c] != 0)
+        M[c], M[piv] = M[piv], M[c]
+        d = M[c][c]
+        M[c] = [v / d for v in M[c]]
+        for r in range(n):
+            if r != c and M[r][c] != 0:
+                f = M[r][c]
+                M[r] = [a - f * b for a, b in zip(M[r], M[c])]
+    inv = [row[n:] for row in M]
+    return [[float(sum(inv[r][c] * rows[q][c] for c in range(n))) for q in range(27)] for r in range(n)]
+
+
+def test_pseudo_inverse_exact_rational():
+    h = Fraction(1, 64)
+    exact = np.array(_exact_pinv(h))
+    ours = O.pseudo_inverse(1 / 64, 1 / 64, 1 / 64)
+    # long-double Gauss-Jordan rounded once: within one ulp of the exact rational
+    assert np.all(np.abs(ours - exact) <= np.spacing(np.abs(exact)) + 1e-300)
+
+
+def test_pseudo_inverse_reproduces_quadratics():
+    dx, dy, dz = 1 / 64, 1 / 32, 1 / 16
+    A = O.pseudo_inverse(dx, dy, dz)
+    rng = np.random.default_rng(3)
+    c = rng.standard_normal(10)
+    v = []
+    for kz in (-1, 0, 1):
+        for ky in (-1, 0, 1):
+            for kx in (-1, 0, 1):
+                x, y, z = dx * kx, dy * ky, dz * kz
+                v.append(c @ [1, x, y, z, x * x / 2, y * y / 2, z * z / 2, x * y, x * z, y * z])
+    assert np.allclose(A @ np.array(v), c, rtol=1e-9, atol=1e-9)
+
+
+# ---------------------------------------------------------------- pointwise closed forms
+def test_heaviside_dirac_closed_form():
+    eps = 1.5 / 64
+    assert O.heaviside_trig(0.0, eps) == 0.5
+    assert O.heaviside_trig(-eps * 1.0001, eps) == 0.0
+    assert O.heaviside_trig(eps, eps) == 1.0
+    assert abs(O.heaviside_trig(-eps * (1 - 1e-12), eps)) < 1e-10
+    # H' = delta
+    for phi in np.linspace(-0.9 * eps, 0.9 * eps, 7):
+        h = 1e-7 * eps
+        dH = (O.heaviside_trig(phi + h, eps) - O.heaviside_trig(phi - h, eps)) / (2 * h)
+        assert abs(dH - O.dirac_trig(phi, eps)) < 1e-5 / eps
+    # integral of delta over its support is 1
+    x = np.linspace(-eps, eps, 20001)
+    d = np.array([O.dirac_trig(v, eps) for v in x])
+    assert abs(np.trapezoid(d, x) - 1.0) < 1e-6
+
+
+def test_icurvature_closed_form():
+    # constant curvature on both sides -> that curvature
+    assert abs(O.icurvature(1, [0.01, -0.004, 0.006], [-2.0, -2.0, -2.0]) + 2.0) < 1e-15
+    # curvature of opposite sign -> 0
+    assert O.icurvature(-1, [0.01, -0.004, 0.006], [1.0, -2.0, -2.0]) == 0.0
+    # linear interpolation of kappa to the zero of phi (both cells convex)
+    ph = [0.0, -0.25, 0.75]
+    cv = [0.0, 1.0, 3.0]
+    assert abs(O.icurvature(1, ph, cv) - (1.0 + 0.25 * 2.0)) < 1e-14
+
+
+# ---------------------------------------------------------------- stencils on exact fields
+def _fortran_coords(p, kind):
+    dims = P.field_dims(p, kind)
+    lb = {P.SHAPE_C: 0, P.SHAPE_LS: -2, P.SHAPE_I: 1}[P.field_shape_kind(kind)]
+    dx = p.lx / p.itot
+    dz = p.lz / p.ktot
+    i = (np.arange(dims[0]) + lb - 0.5) * dx
+    j = (np.arange(dims[1]) + lb - 0.5) * dx
+    k = (np.arange(dims[2]) + lb - 0.5) * dz
+    return np.meshgrid(i, j, k, indexing="ij")
+
+
+@pytest.mark.parametrize("scheme", [P.ADV_HOUC5, P.ADV_WENO5])
+def test_advection_exact_on_linear_field(scheme):
+    """HOUC5 and WENO5 (all smoothness indicators equal -> linear weights) are exact
+    on a linear phi; dfdt = -(u a + v b + w c) away from the periodic seam."""
+    p = small_drop(16)
+    o = O.Oracle(p)
+    o.init_program()
+    a, b, c = 0.3, -0.2, 0.1
+    X, Y, Z = _fortran_coords(p, P.F_LVSET)
+    o.set(P.F_LVSET, 1e-4 * (a * X + b * Y + c * Z))   # inside the band everywhere
+    o.tube_construction(0)
+    rng = np.random.default_rng(5)
+    u, v, w = (rng.standard_normal((16, 16, 16)) for _ in range(3))
+    o.set(P.F_UCC, u)
+    o.set(P.F_VCC, v)
+    o.set(P.F_WCC, w)
+    dfdt = o.eval_adv(0, scheme)
+    ex = -1e-4 * (u * a + v * b + w * c)
+    sl = np.s_[3:-3, 3:-3, 3:-3]   # stay clear of the x/y periodic seam of the linear field
+    assert np.max(np.abs(dfdt[sl] - ex[sl])) <= 1e-12 * np.max(np.abs(ex))
+    # z: first order / HOUC3 / HOUC5 are all exact on linear data, near the walls too
+    sl2 = np.s_[3:-3, 3:-3, :]
+    assert np.max(np.abs(dfdt[sl2] - ex[sl2])) <= 1e-12 * np.max(np.abs(ex))
+
+
+def test_reinit_planar_sdf_fixed_point():
+    """A planar signed distance |grad phi| = 1 is a steady state of the re-init
+    equation: RHS = S(phi0) (1 - |grad phi|) = 0."""
+    p = small_drop(16)
+    o = O.Oracle(p)
+    o.init_program()
+    X, Y, Z = _fortran_coords(p, P.F_LVSET)
+    nrm = np.array([0.0, 0.6, 0.8])
+    o.set(P.F_LVSET, nrm[1] * (Y - 0.5) + nrm[2] * (Z - 0.5))
+    rhs = o.eval_reinit_rhs(0)
+    sl = np.s_[6:-6, 6:-6, 6:-6]    # away from the periodic seam in y
+    assert np.max(np.abs(rhs[sl])) < 1e-12
+
+
+def test_curvature_of_quadratic_and_sphere():
+    p = small_drop(32)
+    o = O.Oracle(p)
+    o.init_program()
+    # sphere SDF: kappa -> -2/R (interface.f90:345) to second order
+    o.get_curvature(0)
+    kap = o.get(P.F_CURV)[1:-1, 1:-1, 1:-1]
+    phi = o.get(P.F_LVSET)[3:-3, 3:-3, 3:-3]
+    R = 0.2
+    band = np.abs(phi) < 0.5 / 32
+    kex = -2.0 / (R + phi[band])
+    assert np.max(np.abs(kap[band] - kex)) < 0.05 * 2 / R
+    # quadratic phi = z + q (x^2 + y^2): fit is exact, kappa from the closed form at x = y = 0
+    X, Y, Z = _fortran_coords(p, P.F_LVSET)
+    q = 0.5
+    x0 = X[3 + 16, 3 + 16, 3 + 16]
+    y0 = Y[3 + 16, 3 + 16, 3 + 16]
+    o.set(P.F_LVSET, (Z - 0.5) + q * ((X - x0) ** 2 + (Y - y0) ** 2))
+    o.get_curvature(0)
+    kap = o.get(P.F_CURV)
+    # at the stencil centre grad = (0,0,1), second derivatives 2q, 2q, 0 -> kappa = -(4q)
+    assert abs(kap[1 + 16, 1 + 16, 1 + 16] - (-4 * q)) < 1e-9
+
+
+def test_volume_brute_force_and_laplace_jump():
+    p = small_drop(32)
+    o = O.Oracle(p)
+    o.init_program()
+    phi = o.get(P.F_LVSET)[3:-3, 3:-3, 3:-3]
+    eps = 1.5 / 32
+    H = np.where(phi < -eps, 0.0, np.where(phi < eps, 0.5 + 0.5 * phi / eps + 0.5 / np.pi * np.sin(np.pi * phi / eps), 1.0))
+    v, v0 = o.volume(0)
+    assert abs(v - (1 - H.sum() / H.size)) < 1e-13
+    # smeared-Heaviside volume of a sphere: O((eps/R)^2) from the exact 4/3 pi R^3
+    assert abs(v - 4 / 3 * np.pi * 0.2 ** 3) < 0.04 * v
+    # one projection with sigma = 1: pressure jump across the interface ~ 2 sigma/R
+    o.step(1, 0)
+    pn = o.get(P.F_PNEW)[1:-1, 1:-1, 1:-1]
+    inside = pn[16 - 2:16 + 2, 16 - 2:16 + 2, 16 - 2:16 + 2].mean()
+    outside = pn[1, 1, 1:-1].mean()
+    assert abs((inside - outside) - 2 / 0.2) < 0.15 * 2 / 0.2
+
+
+def test_projection_is_divergence_free():
+    """The reference aborts when max |div u| > 1e-12 (chkdiv.f90:60-86)."""
+    p = small_drop(32)
+    o = O.Oracle(p)
+    o.init_program()
+    for s in range(1, 4):
+        o.step(s, 0)
+        rc, divtot, divmax, ijk = o.chkdiv(s)
+        assert rc == 0 and divmax < 1e-12
+
+
+def test_momentum_polynomial_fields():
+    """CNTR2 of a uniform flow is zero; delta_diff of a linear shear with constant
+    viscosity is zero (mom.f90:67-273)."""
+    p = small_drop(16)
+    p.rho1 = p.rho2 = 1.0
+    p.miu1 = p.miu2 = 1.0
+    o = O.Oracle(p)
+    o.init_program()
+    shape = P.field_dims(p, P.F_UNEW)
+    o.set(P.F_UNEW, np.full(shape, 0.7))
+    o.set(P.F_VNEW, np.full(shape, -0.3))
+    o.set(P.F_WNEW, np.full(shape, 0.2))
+    ru, rv, rw = o.mom_main()
+    assert max(np.abs(ru).max(), np.abs(rv).max(), np.abs(rw).max()) < 1e-12
+    # v = z (linear in the wall-normal direction), u = w = 0 away from the walls
+    X, Y, Z = _fortran_coords(p, P.F_VNEW)
+    o.set(P.F_UNEW, np.zeros(shape))
+    o.set(P.F_VNEW, Z)
+    o.set(P.F_WNEW, np.zeros(shape))
+    ru, rv, rw = o.mom_main()
+    assert np.abs(rv[:, :, 1:-1]).max() < 1e-9
