@@ -24,6 +24,7 @@ struct Grid {
     long sy;   // stride of j (= nx)
     long sz;   // stride of k (= nx*ny)
     double dx, dy, dz, dxi, dyi, dzi;
+    int pow2;  // dxi, dyi, dzi are powers of two: a/dx == a*dxi exactly
 };
 
 __host__ __device__ inline long gidx(const Grid& g, int i0, int j0, int k) {

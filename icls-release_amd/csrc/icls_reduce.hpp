@@ -42,4 +42,17 @@ __global__ void k_reduce_final(const double* __restrict__ part, int n, double in
     if (threadIdx.x == 0) out[0] = r;
 }
 
+// as k_reduce_final, skipped when the device switch flag[0] is zero
+template <class Op>
+__global__ void k_reduce_final_pred(const double* __restrict__ part, int n, double init, double* out,
+                                    const double* __restrict__ flag) {
+    if (flag[0] == 0.) return;
+    __shared__ double smem[16];
+    double v = init;
+    Op op;
+    for (int q = threadIdx.x; q < n; q += blockDim.x) v = op(v, part[q]);
+    double r = block_reduce(v, op, smem);
+    if (threadIdx.x == 0) out[0] = r;
+}
+
 }  // namespace icls

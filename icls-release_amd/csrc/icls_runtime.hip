@@ -30,6 +30,7 @@ struct Profiler {
     std::vector<Rec> open;
     std::vector<hipEvent_t> pool;
     std::map<std::string, std::pair<double, long>> acc;
+    std::string only;   // when set, bracket only launches of this name
     hipEvent_t get() {
         if (!pool.empty()) { hipEvent_t e = pool.back(); pool.pop_back(); return e; }
         hipEvent_t e;
@@ -57,6 +58,7 @@ struct Profiler {
 
 void prof_begin(Ctx& c, const char* name) {
     Profiler* p = c.prof;
+    if (!p->only.empty() && p->only != name) return;
     Profiler::Rec r{name, p->get(), p->get()};
     hipEventRecord(r.a, c.stream);
     p->open.push_back(r);
@@ -70,6 +72,7 @@ void prof_begin(Ctx& c, const char* name) {
 }
 void prof_end(Ctx& c, const char* name) {
     Profiler* p = c.prof;
+    if (!p->only.empty() && p->only != name) return;
     for (auto it = p->open.rbegin(); it != p->open.rend(); ++it)
         if (it->name == name) { hipEventRecord(it->b, c.stream); return; }
 }
@@ -191,6 +194,8 @@ static int build_ctx(const icls_params* prm) {
     // param.f90:37-38
     g.dxi = prm->itot / prm->lx; g.dyi = prm->jtot / prm->ly; g.dzi = prm->ktot / prm->lz;
     g.dx = 1. / g.dxi; g.dy = 1. / g.dyi; g.dz = 1. / g.dzi;
+    auto p2 = [](double v) { int e; return std::frexp(v, &e) == 0.5; };
+    g.pow2 = p2(g.dxi) && p2(g.dyi) && p2(g.dzi) && g.dx * g.dxi == 1. && g.dy * g.dyi == 1. && g.dz * g.dzi == 1.;
     C.lmax = prm->lmax;
     C.nfield = g.sz * (long)(g.nz + 6);
     C.NB2 = 6. * g.dz;
@@ -441,6 +446,18 @@ int32_t icls_get_volume(int32_t l, double* volume, double* init_volume) {
     return ICLS_OK;
 }
 
+int32_t icls_set_volume(int32_t l, double volume, double init_volume) {
+    CTX_OR_FAIL();
+    if (l < 0 || l >= c.lmax) { set_error("bad level-set index %d", l); return ICLS_ERR_PARAM; }
+    double v[1] = {volume};
+    HIP_OK(hipMemcpyAsync(c.dscal + DS_COUNT_FIXED + l, v, sizeof(double), hipMemcpyHostToDevice, c.stream));
+    HIP_OK(hipStreamSynchronize(c.stream));
+    v[0] = init_volume;
+    HIP_OK(hipMemcpyAsync(c.dscal + DS_COUNT_FIXED + c.lmax + l, v, sizeof(double), hipMemcpyHostToDevice, c.stream));
+    HIP_OK(hipStreamSynchronize(c.stream));
+    return ICLS_OK;
+}
+
 // ---------------------------------------------------------------- level set
 static int ls_main_impl(Ctx& c, int istep) {
     for (int l = 0; l < c.lmax; l++) {
@@ -475,9 +492,8 @@ static int ls_main_impl(Ctx& c, int istep) {
             TRY(launch_bc_ls(c, c.ls_a));
             TRY(launch_reinit_stage(c, 2, ph, c.ls_a, c.ls_b));
             TRY(launch_bc_ls(c, c.ls_b));
-            TRY(launch_reinit_stage(c, 3, ph, c.ls_b, c.ls_c));
-            TRY(launch_reinit_residual(c, ph, c.ls_c));
-            HIP_OK(hipMemcpyAsync(ph, c.ls_c, sizeof(double) * c.nfield, hipMemcpyDeviceToDevice, c.stream));
+            TRY(launch_reinit_stage(c, 3, ph, c.ls_b, ph));    // in place: lvset = phi_new
+            TRY(launch_reinit_residual(c, ph, ph));
         }
     }
     // update_materials (misc.f90:79-290)
@@ -717,6 +733,14 @@ int32_t icls_profile_enable(int32_t on) {
     CTX_OR_FAIL();
     if (on && !c.prof) c.prof = new Profiler();
     if (!on && c.prof) { c.prof->flush(c.stream); delete c.prof; c.prof = nullptr; }
+    return ICLS_OK;
+}
+
+int32_t icls_profile_only(const char* name) {
+    CTX_OR_FAIL();
+    if (!c.prof) { set_error("profiling is not enabled"); return ICLS_ERR_STATE; }
+    c.prof->flush(c.stream);
+    c.prof->only = name ? name : "";
     return ICLS_OK;
 }
 

@@ -51,15 +51,31 @@ __device__ __forceinline__ T weno5(T v1, T v2, T v3, T v4, T v5) {
            w3 * ((T(1.) / T(3.)) * v3 + (T(5.) / T(6.)) * v4 - (T(1.) / T(6.)) * v5);
 }
 
+// Division by a grid spacing.  When the spacing is a power of two (P2) its
+// reciprocal is exact and a/h == a*(1/h) bit for bit, so the product is used.
+template <typename T, bool P2>
+struct Sp {
+    T h, hi;
+    __device__ __forceinline__ T div(T a) const { return P2 ? a * hi : a / h; }
+};
+
 // left-biased derivative (hyperPDE.f90:545-549)
-template <typename T>
-__device__ __forceinline__ T weno_minus(const T* f, T h) {
-    return weno5<T>((f[-2] - f[-3]) / h, (f[-1] - f[-2]) / h, (f[0] - f[-1]) / h, (f[1] - f[0]) / h, (f[2] - f[1]) / h);
+template <typename T, bool P2>
+__device__ __forceinline__ T weno_minus(const T* f, Sp<T, P2> s) {
+    return weno5<T>(s.div(f[-2] - f[-3]), s.div(f[-1] - f[-2]), s.div(f[0] - f[-1]), s.div(f[1] - f[0]),
+                    s.div(f[2] - f[1]));
 }
 // right-biased derivative (hyperPDE.f90:579-583)
+template <typename T, bool P2>
+__device__ __forceinline__ T weno_plus(const T* f, Sp<T, P2> s) {
+    return weno5<T>(s.div(f[3] - f[2]), s.div(f[2] - f[1]), s.div(f[1] - f[0]), s.div(f[0] - f[-1]),
+                    s.div(f[-1] - f[-2]));
+}
+
+// seven equal values: every difference is +0, both WENO5 derivatives are exactly +0
 template <typename T>
-__device__ __forceinline__ T weno_plus(const T* f, T h) {
-    return weno5<T>((f[3] - f[2]) / h, (f[2] - f[1]) / h, (f[1] - f[0]) / h, (f[0] - f[-1]) / h, (f[-1] - f[-2]) / h);
+__device__ __forceinline__ bool flat7(const T* f) {
+    return f[0] == f[3] && f[1] == f[3] && f[2] == f[3] && f[4] == f[3] && f[5] == f[3] && f[6] == f[3];
 }
 
 // Heaviside 'trig' (misc.f90:439-459); pi = acos(-1.)

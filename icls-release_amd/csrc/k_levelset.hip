@@ -8,6 +8,7 @@
 // the reference's compacted index lists (levelset.f90:160-188); a dense masked sweep
 // touches exactly the cells RK_nb updates (levelset.f90:353-361).
 #include "icls_internal.hpp"
+#include "icls_reduce.hpp"
 #include "icls_stencil.hpp"
 
 namespace icls {
@@ -76,7 +77,8 @@ __global__ void k_bc_ls(Grid g, double* __restrict__ phi, const double* __restri
         double hi = 2. * phi[gidx(g, i0, j0, km)] - phi[gidx(g, i0, j0, km - 1)];
         phi[gidx(g, i0, j0, 0)] = lo;
         phi[gidx(g, i0, j0, km + 1)] = hi;
-        if (res_ref) r = fmax(fabs(lo - res_ref[gidx(g, i0, j0, 0)]), fabs(hi - res_ref[gidx(g, i0, j0, km + 1)]));
+        // res_ref: the two stashed wall layers of phi0, plane 0 then plane k1
+        if (res_ref) r = fmax(fabs(lo - res_ref[i0 + g.sy * j0]), fabs(hi - res_ref[g.sz + i0 + g.sy * j0]));
     }
     if (res_ref) wave_atomic_max(res, r);
 }
@@ -125,27 +127,27 @@ int launch_boundmls(Ctx& c, double* phi) {
 // dfdt of HOUC5_nb (hyperPDE.f90:397-495) or WENO5_nb (hyperPDE.f90:503-800) at one
 // band cell.  The z stencil drops to HOUC3 at k = 2, kmax-1 and to first order at
 // k = 1, kmax in both schemes.
-template <int SCHEME>
+template <int SCHEME, bool P2>
 __device__ __forceinline__ double adv_rhs(const Grid& g, const double* __restrict__ phi, int i0, int j0, int k,
                                           double vu, double vv, double vw) {
     double f[7];
-    const double dx = g.dx, dy = g.dy, dz = g.dz;
+    const Sp<double, P2> sx{g.dx, g.dxi}, sy{g.dy, g.dyi}, sz{g.dz, g.dzi};
 #pragma unroll
     for (int d = -3; d <= 3; d++) f[d + 3] = phi[gidx(g, wrap(i0 + d, g.nx), j0, k)];
     double udfdx, vdfdy, wdfdz;
     if (SCHEME == ICLS_ADV_HOUC5) {
-        udfdx = -vu * houc5<double>(vu, f + 3) / dx;
+        udfdx = sx.div(-vu * houc5<double>(vu, f + 3));
     } else {
-        double dfx = (vu >= 0.) ? weno_minus<double>(f + 3, dx) : weno_plus<double>(f + 3, dx);
+        double dfx = (vu >= 0.) ? weno_minus(f + 3, sx) : weno_plus(f + 3, sx);
         udfdx = -vu * dfx;
     }
 #pragma unroll
     for (int d = -3; d <= 3; d++) f[d + 3] = phi[gidx(g, i0, wrap(j0 + d, g.ny), k)];
     if (SCHEME == ICLS_ADV_HOUC5) {
-        vdfdy = -vv * houc5<double>(vv, f + 3) / dy;
+        vdfdy = sy.div(-vv * houc5<double>(vv, f + 3));
     } else {
         // y differences are divided by dx in the reference (hyperPDE.f90:622-626)
-        double dfy = (vv >= 0.) ? weno_minus<double>(f + 3, dx) : weno_plus<double>(f + 3, dx);
+        double dfy = (vv >= 0.) ? weno_minus(f + 3, sx) : weno_plus(f + 3, sx);
         vdfdy = -vv * dfy;
     }
     const int kmax = g.nz;
@@ -153,16 +155,16 @@ __device__ __forceinline__ double adv_rhs(const Grid& g, const double* __restric
 #pragma unroll
         for (int d = -3; d <= 3; d++) f[d + 3] = phi[gidx(g, i0, j0, k + d)];
         if (SCHEME == ICLS_ADV_HOUC5) {
-            wdfdz = -vw * houc5<double>(vw, f + 3) / dz;
+            wdfdz = sz.div(-vw * houc5<double>(vw, f + 3));
         } else {
-            double dfz = (vw >= 0.) ? weno_minus<double>(f + 3, dz) : weno_plus<double>(f + 3, dz);
+            double dfz = (vw >= 0.) ? weno_minus(f + 3, sz) : weno_plus(f + 3, sz);
             wdfdz = -vw * dfz;
         }
     } else {
 #pragma unroll
         for (int d = -2; d <= 2; d++) f[d + 3] = phi[gidx(g, i0, j0, k + d)];
         double dfz = (k == 2 || k == kmax - 1) ? houc3<double>(vw, f + 3) : upwind1<double>(vw, f + 3);
-        wdfdz = -vw * dfz / dz;
+        wdfdz = sz.div(-vw * dfz);
     }
     return udfdx + vdfdy + wdfdz;
 }
@@ -174,8 +176,9 @@ __device__ __forceinline__ double adv_rhs(const Grid& g, const double* __restric
 // The bracket is the reference's whole-array temporary, applied to every stored
 // cell including the z ghost layers.  flag (optional) predicates the launch on a
 // device-side switch (used by the mass correction).
-template <int SCHEME, int STAGE>
-__global__ void k_adv_stage(Grid g, double dt, const double* phi0, const double* __restrict__ phis, double* out, const uint8_t* __restrict__ mask, const double* __restrict__ vu,
+template <int SCHEME, int STAGE, bool P2>
+__global__ void k_adv_stage(Grid g, double dt, const double* phi0, const double* __restrict__ phis, double* out,
+                            const uint8_t* __restrict__ mask, const double* __restrict__ vu,
                             const double* __restrict__ vv, const double* __restrict__ vw, const double* __restrict__ flag) {
     if (flag && flag[0] == 0.) return;
     int i0 = blockIdx.x * BX + threadIdx.x, j0 = blockIdx.y * BY + threadIdx.y, ks = blockIdx.z;
@@ -188,7 +191,7 @@ __global__ void k_adv_stage(Grid g, double dt, const double* phi0, const double*
     else { comb = 1. / 3. * phi0[q] + 2. / 3. * phis[q]; factor = 2. / 3.; }
     double r = comb;
     if (k >= 1 && k <= g.nz && mask[q]) {
-        double dfdt = adv_rhs<SCHEME>(g, phis, i0, j0, k, vu[q], vv[q], vw[q]);
+        double dfdt = adv_rhs<SCHEME, P2>(g, phis, i0, j0, k, vu[q], vv[q], vw[q]);
         r = comb + factor * dt * dfdt;
     }
     out[q] = r;
@@ -201,9 +204,13 @@ int launch_adv_stage(Ctx& c, int scheme, int stage, const double* phi0, const do
     dim3 grd = grid3(c.g, c.g.nz + 6, blk);
 #define ADV(S, T)                                                                                                  \
     if (scheme == S && stage == T) {                                                                               \
-        ProfScope ps(c, "k_adv_stage");                                                                            \
-        hipLaunchKernelGGL((k_adv_stage<S, T>), grd, blk, 0, c.stream, c.g, c.dt, phi0, phis, out, c.mask, vu, vv, vw, \
-                           flag);                                                                                  \
+        ProfScope ps(c, T == 1 ? "k_adv_stage1" : "k_adv_stage23");                                                \
+        if (c.g.pow2)                                                                                              \
+            hipLaunchKernelGGL((k_adv_stage<S, T, true>), grd, blk, 0, c.stream, c.g, c.dt, phi0, phis, out, c.mask, \
+                               vu, vv, vw, flag);                                                                  \
+        else                                                                                                       \
+            hipLaunchKernelGGL((k_adv_stage<S, T, false>), grd, blk, 0, c.stream, c.g, c.dt, phi0, phis, out, c.mask,\
+                               vu, vv, vw, flag);                                                                  \
     }
     ADV(ICLS_ADV_HOUC5, 1) ADV(ICLS_ADV_HOUC5, 2) ADV(ICLS_ADV_HOUC5, 3)
     ADV(ICLS_ADV_WENO5, 1) ADV(ICLS_ADV_WENO5, 2) ADV(ICLS_ADV_WENO5, 3)
@@ -215,26 +222,31 @@ int launch_adv_stage(Ctx& c, int scheme, int stage, const double* phi0, const do
 // ---------------------------------------------------------------- re-initialisation
 // Six one-sided WENO5 derivatives and the Godunov-type selection of WENO5_reinit
 // (hyperPDE.f90:2662-2736, 2932-3197), evaluated where |phi| < NarrowBand_2 + dz.
+// Along an axis whose seven values are equal (the clamped far field) both WENO5
+// derivatives are exactly +0 (every difference is +0, every smoothness indicator 0,
+// the weighted sum of zeros +0), so they are set without evaluating the weights.
+template <bool P2>
 __device__ __forceinline__ double reinit_rhs(const Grid& g, const double* __restrict__ phi, int i0, int j0, int k,
                                              double s) {
     double f[7];
-    const double dx = g.dx, dz = g.dz;
+    const Sp<double, P2> sx{g.dx, g.dxi}, sz{g.dz, g.dzi};
 #pragma unroll
     for (int d = -3; d <= 3; d++) f[d + 3] = phi[gidx(g, wrap(i0 + d, g.nx), j0, k)];
-    double A = weno_minus<double>(f + 3, dx), B = weno_plus<double>(f + 3, dx);
+    double A = 0., B = 0.;
+    if (!flat7(f)) { A = weno_minus(f + 3, sx); B = weno_plus(f + 3, sx); }
 #pragma unroll
     for (int d = -3; d <= 3; d++) f[d + 3] = phi[gidx(g, i0, wrap(j0 + d, g.ny), k)];
-    double C = weno_minus<double>(f + 3, dx), D = weno_plus<double>(f + 3, dx);
-    double E, F;
+    double C = 0., D = 0.;
+    if (!flat7(f)) { C = weno_minus(f + 3, sx); D = weno_plus(f + 3, sx); }
+    double E = 0., F = 0.;
     if (k >= 3 && k <= g.nz - 2) {
 #pragma unroll
         for (int d = -3; d <= 3; d++) f[d + 3] = phi[gidx(g, i0, j0, k + d)];
-        E = weno_minus<double>(f + 3, dz);
-        F = weno_plus<double>(f + 3, dz);
+        if (!flat7(f)) { E = weno_minus(f + 3, sz); F = weno_plus(f + 3, sz); }
     } else {
         double pc = phi[gidx(g, i0, j0, k)];
-        E = (pc - phi[gidx(g, i0, j0, k - 1)]) / dz;
-        F = (phi[gidx(g, i0, j0, k + 1)] - pc) / dz;
+        E = sz.div(pc - phi[gidx(g, i0, j0, k - 1)]);
+        F = sz.div(phi[gidx(g, i0, j0, k + 1)] - pc);
     }
     double dydx, dydy, dydz;
     if ((B * s < 0.) && ((A + B) * s < 0.)) dydx = B;
@@ -257,17 +269,25 @@ __device__ __forceinline__ double reinit_rhs(const Grid& g, const double* __rest
 // RHS is zero outside the interior band (WENO5_reinit zeroes it, hyperPDE.f90:2662).
 // Stage 3 also folds |out - phi0| of every layer that boundsls does not rewrite into
 // the residual (levelset.f90:831); stage 1 resets it.
-template <int STAGE>
-__global__ void k_reinit_stage(Grid g, double NB2, double* res, const double* __restrict__ phi0,
-                               const double* __restrict__ phis, double* __restrict__ out) {
-    int i0 = blockIdx.x * BX + threadIdx.x, j0 = blockIdx.y * BY + threadIdx.y, ks = blockIdx.z;
-    if (STAGE == 1 && i0 == 0 && j0 == 0 && ks == 0) res[0] = 0.;
-    bool valid = i0 < g.nx && j0 < g.ny;
-    int k = ks - 2;
-    double d = 0.;
-    if (valid) {
-        long q = gidx(g, i0, j0, k);
-        const double dz = g.dz, dtau = 0.5 * dz;
+// Each thread marches RKZ consecutive stored layers of one (i,j) column; the stage-3
+// residual is folded per block and published with one atomic max per block (max is
+// order independent, so the result does not depend on the schedule).
+static constexpr int RKZ = 8;
+
+template <int STAGE, bool P2>
+__global__ void k_reinit_stage(Grid g, double NB2, double* res, const double* phi0, const double* __restrict__ phis,
+                               double* out, double* __restrict__ stash) {
+    __shared__ double smem[16];
+    const int i0 = blockIdx.x * BX + threadIdx.x, j0 = blockIdx.y * BY + threadIdx.y;
+    if (STAGE == 1 && i0 == 0 && j0 == 0 && blockIdx.z == 0) res[0] = 0.;
+    const bool valid = i0 < g.nx && j0 < g.ny;
+    const double dz = g.dz, dtau = 0.5 * dz;
+    double dmax = 0.;
+    for (int kk = 0; kk < RKZ; kk++) {
+        const int ks = blockIdx.z * RKZ + kk;
+        if (!valid || ks >= g.nz + 6) break;
+        const int k = ks - 2;
+        const long q = gidx(g, i0, j0, k);
         double rhs = 0.;
         if (k >= 1 && k <= g.nz) {
             double ps = phis[q];
@@ -276,38 +296,52 @@ __global__ void k_reinit_stage(Grid g, double NB2, double* res, const double* __
                 if (p0 < -dz) s = -1.;
                 else if (p0 > dz) s = 1.;
                 else s = p0 / sqrt(p0 * p0 + dz * dz);
-                rhs = reinit_rhs(g, phis, i0, j0, k, s);
+                rhs = reinit_rhs<P2>(g, phis, i0, j0, k, s);
             }
         }
         double r;
         if (STAGE == 1) r = phi0[q] + dtau * rhs;
         else if (STAGE == 2) r = 3. / 4. * phi0[q] + 1. / 4. * (phis[q] + dtau * rhs);
         else r = 1. / 3. * phi0[q] + 2. / 3. * (phis[q] + dtau * rhs);
+        if (STAGE == 3) {
+            // stage 3 may run in place (out == phi0): keep phi0 of the two wall layers
+            // that boundsls rewrites for the residual check in k_bc_ls
+            double p0 = phi0[q];
+            if (k == 0) stash[i0 + g.sy * j0] = p0;
+            else if (k == g.nz + 1) stash[g.sz + i0 + g.sy * j0] = p0;
+            else dmax = fmax(dmax, fabs(r - p0));
+        }
         out[q] = r;
-        if (STAGE == 3) d = fabs(r - phi0[q]);
     }
-    // the two layers boundsls rewrites are folded in by k_bc_ls; the z index is
-    // uniform over the block, so the branch is wave-uniform
-    if (STAGE == 3 && k != 0 && k != g.nz + 1) wave_atomic_max(res, d);
+    if (STAGE == 3) {
+        double b = block_reduce(dmax, OpMax(), smem);
+        if (threadIdx.x == 0 && threadIdx.y == 0) atomic_max_nonneg(res, b);
+    }
 }
 
 int launch_reinit_stage(Ctx& c, int stage, const double* phi0, const double* phis, double* out) {
-    ProfScope ps(c, "k_reinit_stage");
+    ProfScope ps(c, stage == 1 ? "k_reinit_stage1" : "k_reinit_stage23");
     dim3 blk(BX, BY, 1);
-    dim3 grd = grid3(c.g, c.g.nz + 6, blk);
+    dim3 grd = grid3(c.g, (c.g.nz + 6 + RKZ - 1) / RKZ, blk);
     double* res = c.dscal + DS_REINIT_RES;
-    if (stage == 1) hipLaunchKernelGGL(k_reinit_stage<1>, grd, blk, 0, c.stream, c.g, c.NB2, res, phi0, phis, out);
-    else if (stage == 2) hipLaunchKernelGGL(k_reinit_stage<2>, grd, blk, 0, c.stream, c.g, c.NB2, res, phi0, phis, out);
-    else hipLaunchKernelGGL(k_reinit_stage<3>, grd, blk, 0, c.stream, c.g, c.NB2, res, phi0, phis, out);
+#define RS(S, P) \
+    hipLaunchKernelGGL((k_reinit_stage<S, P>), grd, blk, 0, c.stream, c.g, c.NB2, res, phi0, phis, out, c.stash)
+    if (c.g.pow2) {
+        if (stage == 1) RS(1, true); else if (stage == 2) RS(2, true); else RS(3, true);
+    } else {
+        if (stage == 1) RS(1, false); else if (stage == 2) RS(2, false); else RS(3, false);
+    }
+#undef RS
     HIP_OK(hipGetLastError());
     return ICLS_OK;
 }
 
 // boundsls after the last re-init stage, folding the two wall layers into the residual
-int launch_reinit_residual(Ctx& c, const double* phi0, const double* phinew) {
+// against the phi0 layers stage 3 stashed
+int launch_reinit_residual(Ctx& c, const double* /*phi0*/, const double* phinew) {
     ProfScope ps(c, "k_bc_ls");
     dim3 blk(BX, BY, 1);
-    hipLaunchKernelGGL(k_bc_ls, grid3(c.g, 1, blk), blk, 0, c.stream, c.g, (double*)phinew, phi0,
+    hipLaunchKernelGGL(k_bc_ls, grid3(c.g, 1, blk), blk, 0, c.stream, c.g, (double*)phinew, (const double*)c.stash,
                        c.dscal + DS_REINIT_RES, (const double*)nullptr);
     HIP_OK(hipGetLastError());
     return ICLS_OK;

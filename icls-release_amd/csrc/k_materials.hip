@@ -238,9 +238,10 @@ int launch_inflation_velocity(Ctx& c, int l) {
     hipLaunchKernelGGL(k_infl_sums, grd, blk, 0, c.stream, c.g, flag, (const double*)c.lvset[l], (const double*)c.curv2,
                        (const double*)c.normal_m, s1, c.pi, c.dheav, c.dpart, nb);
     double* sums = c.dscal + DS_COUNT_FIXED + 2 * c.lmax;
-    hipLaunchKernelGGL(k_reduce_final<OpSum>, dim3(1), dim3(256), 0, c.stream, (const double*)c.dpart, nb, 0., sums);
-    hipLaunchKernelGGL(k_reduce_final<OpSum>, dim3(1), dim3(256), 0, c.stream, (const double*)(c.dpart + nb), nb, 0.,
-                       sums + 1);
+    hipLaunchKernelGGL(k_reduce_final_pred<OpSum>, dim3(1), dim3(256), 0, c.stream, (const double*)c.dpart, nb, 0.,
+                       sums, flag);
+    hipLaunchKernelGGL(k_reduce_final_pred<OpSum>, dim3(1), dim3(256), 0, c.stream, (const double*)(c.dpart + nb), nb,
+                       0., sums + 1, flag);
     double ncell = 1. * c.g.nx * c.g.ny * c.g.nz;
     double lxyz = c.P.lx * c.P.ly * c.P.lz;
     double inflate_dt = c.P.inflate_step * c.dt;

@@ -4,6 +4,7 @@
 // reductions (chkdiv, chkdt, max_vel_at_center).
 #include "icls_internal.hpp"
 #include "icls_reduce.hpp"
+#include "icls_stencil.hpp"
 
 namespace icls {
 
@@ -19,7 +20,7 @@ struct MomParams {
 // delta_diff (mom.f90:134-273), NS_grav (mom.f90:277-301) and momxp/yp/zp
 // (mom.f90:779-851), one interior cell per thread.  Face densities are the
 // arithmetic means of update_materials (misc.f90:151-172), formed here from rho.
-template <int RHS_ONLY>
+template <int RHS_ONLY, bool P2>
 __global__ void k_momentum(Grid g, MomParams m, const double* __restrict__ dscal, const double* __restrict__ u,
                            const double* __restrict__ v, const double* __restrict__ w, const double* __restrict__ miu,
                            const double* __restrict__ rho, double* __restrict__ duold, double* __restrict__ dvold,
@@ -33,7 +34,7 @@ __global__ void k_momentum(Grid g, MomParams m, const double* __restrict__ dscal
     const int im = wrap(i0 - 1, g.nx), ip = wrap(i0 + 1, g.nx), jm = wrap(j0 - 1, g.ny), jp = wrap(j0 + 1, g.ny);
     const int km = k - 1, kp = k + 1;
     auto I = [&](int a, int b, int c) { return gidx(g, a, b, c); };
-    const double dx = g.dx, dy = g.dy, dz = g.dz;
+    const Sp<double, P2> sx{g.dx, g.dxi}, sy{g.dy, g.dyi}, sz{g.dz, g.dzi};
     const int i = i0, j = j0;
     const long q = I(i, j, k);
 
@@ -44,21 +45,21 @@ __global__ void k_momentum(Grid g, MomParams m, const double* __restrict__ dscal
     double uvjm = 0.25 * (u[I(i, jm, k)] + u[q]) * (v[I(ip, jm, k)] + v[I(i, jm, k)]);
     double uwkp = 0.25 * (u[I(i, j, kp)] + u[q]) * (w[I(ip, j, k)] + w[q]);
     double uwkm = 0.25 * (u[I(i, j, km)] + u[q]) * (w[I(ip, j, km)] + w[I(i, j, km)]);
-    double RCu = (-uuip + uuim) / dx + (-uvjp + uvjm) / dy + (-uwkp + uwkm) / dz;
+    double RCu = sx.div(-uuip + uuim) + sy.div(-uvjp + uvjm) + sz.div(-uwkp + uwkm);
     double uvip = 0.25 * (u[q] + u[I(i, jp, k)]) * (v[q] + v[I(ip, j, k)]);
     double uvim = 0.25 * (u[I(im, j, k)] + u[I(im, jp, k)]) * (v[q] + v[I(im, j, k)]);
     double vvjp = 0.25 * (v[q] + v[I(i, jp, k)]) * (v[q] + v[I(i, jp, k)]);
     double vvjm = 0.25 * (v[q] + v[I(i, jm, k)]) * (v[q] + v[I(i, jm, k)]);
     double wvkp = 0.25 * (w[q] + w[I(i, jp, k)]) * (v[I(i, j, kp)] + v[q]);
     double wvkm = 0.25 * (w[I(i, j, km)] + w[I(i, jp, km)]) * (v[I(i, j, km)] + v[q]);
-    double RCv = (-uvip + uvim) / dx + (-vvjp + vvjm) / dy + (-wvkp + wvkm) / dz;
+    double RCv = sx.div(-uvip + uvim) + sy.div(-vvjp + vvjm) + sz.div(-wvkp + wvkm);
     double uwip = 0.25 * (w[q] + w[I(ip, j, k)]) * (u[q] + u[I(i, j, kp)]);
     double uwim = 0.25 * (w[q] + w[I(im, j, k)]) * (u[I(im, j, k)] + u[I(im, j, kp)]);
     double vwjp = 0.25 * (w[q] + w[I(i, jp, k)]) * (v[q] + v[I(i, j, kp)]);
     double vwjm = 0.25 * (w[q] + w[I(i, jm, k)]) * (v[I(i, jm, k)] + v[I(i, jm, kp)]);
     double wwkp = 0.25 * (w[q] + w[I(i, j, kp)]) * (w[q] + w[I(i, j, kp)]);
     double wwkm = 0.25 * (w[q] + w[I(i, j, km)]) * (w[q] + w[I(i, j, km)]);
-    double RCw = (-uwip + uwim) / dx + (-vwjp + vwjm) / dy + (-wwkp + wwkm) / dz;
+    double RCw = sx.div(-uwip + uwim) + sy.div(-vwjp + vwjm) + sz.div(-wwkp + wwkm);
 
     // ---- face densities
     double rq = rho[q];
@@ -74,69 +75,69 @@ __global__ void k_momentum(Grid g, MomParams m, const double* __restrict__ dscal
     const double mq = miu[q];
     double RDu, RDv, RDw;
     {
-        double dudxp = (u[I(ip, j, k)] - u[q]) / dx;
-        double dudxm = (u[q] - u[I(im, j, k)]) / dx;
-        double dudyp = (u[I(i, jp, k)] - u[q]) / dy;
-        double dudym = (u[q] - u[I(i, jm, k)]) / dy;
-        double dvdxp = (v[I(ip, j, k)] - v[q]) / dx;
-        double dvdxm = (v[I(ip, jm, k)] - v[I(i, jm, k)]) / dx;
-        double dudzp = (u[I(i, j, kp)] - u[q]) / dz;
-        double dudzm = (u[q] - u[I(i, j, km)]) / dz;
-        double dwdxp = (w[I(ip, j, k)] - w[q]) / dx;
-        double dwdxm = (w[I(ip, j, km)] - w[I(i, j, km)]) / dx;
+        double dudxp = sx.div(u[I(ip, j, k)] - u[q]);
+        double dudxm = sx.div(u[q] - u[I(im, j, k)]);
+        double dudyp = sy.div(u[I(i, jp, k)] - u[q]);
+        double dudym = sy.div(u[q] - u[I(i, jm, k)]);
+        double dvdxp = sx.div(v[I(ip, j, k)] - v[q]);
+        double dvdxm = sx.div(v[I(ip, jm, k)] - v[I(i, jm, k)]);
+        double dudzp = sz.div(u[I(i, j, kp)] - u[q]);
+        double dudzm = sz.div(u[q] - u[I(i, j, km)]);
+        double dwdxp = sx.div(w[I(ip, j, k)] - w[q]);
+        double dwdxm = sx.div(w[I(ip, j, km)] - w[I(i, j, km)]);
         double ma1 = 0.25 * (mq + miu[I(i, jm, k)] + miu[I(ip, jm, k)] + miu[I(ip, j, k)]);
         double ma2 = 0.25 * (mq + miu[I(i, jp, k)] + miu[I(ip, jp, k)] + miu[I(ip, j, k)]);
         double ma3 = 0.25 * (mq + miu[I(i, j, km)] + miu[I(ip, j, km)] + miu[I(ip, j, k)]);
         double ma4 = 0.25 * (mq + miu[I(i, j, kp)] + miu[I(ip, j, kp)] + miu[I(ip, j, k)]);
-        double RD1 = 2. * (dudxp * miu[I(ip, j, k)] - dudxm * mq) / dx;
-        double RD2 = (dudyp * ma2 - dudym * ma1) / dy;
-        double RD3 = (dvdxp * ma2 - dvdxm * ma1) / dy;
-        double RD4 = (dudzp * ma4 - dudzm * ma3) / dz;
-        double RD5 = (dwdxp * ma4 - dwdxm * ma3) / dz;
+        double RD1 = sx.div(2. * (dudxp * miu[I(ip, j, k)] - dudxm * mq));
+        double RD2 = sy.div(dudyp * ma2 - dudym * ma1);
+        double RD3 = sy.div(dvdxp * ma2 - dvdxm * ma1);
+        double RD4 = sz.div(dudzp * ma4 - dudzm * ma3);
+        double RD5 = sz.div(dwdxp * ma4 - dwdxm * ma3);
         RDu = (RD1 + RD2 + RD3 + RD4 + RD5) / rho_u;
     }
     {
-        double dvdxp = (v[I(ip, j, k)] - v[q]) / dx;
-        double dvdxm = (v[q] - v[I(im, j, k)]) / dx;
-        double dudyp = (u[I(i, jp, k)] - u[q]) / dy;
-        double dudym = (u[I(im, jp, k)] - u[I(im, j, k)]) / dy;
-        double dvdyp = (v[I(i, jp, k)] - v[q]) / dy;
-        double dvdym = (v[q] - v[I(i, jm, k)]) / dy;
-        double dvdzp = (v[I(i, j, kp)] - v[q]) / dz;
-        double dvdzm = (v[q] - v[I(i, j, km)]) / dz;
-        double dwdyp = (w[I(i, jp, k)] - w[q]) / dy;
-        double dwdym = (w[I(i, jp, km)] - w[I(i, j, km)]) / dy;
+        double dvdxp = sx.div(v[I(ip, j, k)] - v[q]);
+        double dvdxm = sx.div(v[q] - v[I(im, j, k)]);
+        double dudyp = sy.div(u[I(i, jp, k)] - u[q]);
+        double dudym = sy.div(u[I(im, jp, k)] - u[I(im, j, k)]);
+        double dvdyp = sy.div(v[I(i, jp, k)] - v[q]);
+        double dvdym = sy.div(v[q] - v[I(i, jm, k)]);
+        double dvdzp = sz.div(v[I(i, j, kp)] - v[q]);
+        double dvdzm = sz.div(v[q] - v[I(i, j, km)]);
+        double dwdyp = sy.div(w[I(i, jp, k)] - w[q]);
+        double dwdym = sy.div(w[I(i, jp, km)] - w[I(i, j, km)]);
         double ma2 = 0.25 * (mq + miu[I(i, jp, k)] + miu[I(ip, jp, k)] + miu[I(ip, j, k)]);
         double ma5 = 0.25 * (mq + miu[I(i, jp, k)] + miu[I(im, jp, k)] + miu[I(im, j, k)]);
         double ma6 = 0.25 * (mq + miu[I(i, j, km)] + miu[I(i, jp, km)] + miu[I(i, jp, k)]);
         double ma7 = 0.25 * (mq + miu[I(i, j, kp)] + miu[I(i, jp, kp)] + miu[I(i, jp, k)]);
-        double RD1 = (dvdxp * ma2 - dvdxm * ma5) / dx;
-        double RD2 = (dudyp * ma2 - dudym * ma5) / dx;
-        double RD3 = 2. * (dvdyp * miu[I(i, jp, k)] - dvdym * mq) / dy;
-        double RD4 = (dvdzp * ma7 - dvdzm * ma6) / dz;
-        double RD5 = (dwdyp * ma7 - dwdym * ma6) / dz;
+        double RD1 = sx.div(dvdxp * ma2 - dvdxm * ma5);
+        double RD2 = sx.div(dudyp * ma2 - dudym * ma5);
+        double RD3 = sy.div(2. * (dvdyp * miu[I(i, jp, k)] - dvdym * mq));
+        double RD4 = sz.div(dvdzp * ma7 - dvdzm * ma6);
+        double RD5 = sz.div(dwdyp * ma7 - dwdym * ma6);
         RDv = (RD1 + RD2 + RD3 + RD4 + RD5) / rho_v;
     }
     {
-        double dwdxp = (w[I(ip, j, k)] - w[q]) / dx;
-        double dwdxm = (w[q] - w[I(im, j, k)]) / dx;
-        double dudzp = (u[I(i, j, kp)] - u[q]) / dz;
-        double dudzm = (u[I(im, j, kp)] - u[I(im, j, k)]) / dz;
-        double dwdyp = (w[I(i, jp, k)] - w[q]) / dy;
-        double dwdym = (w[q] - w[I(i, jm, k)]) / dy;
-        double dvdzp = (v[I(i, j, kp)] - v[q]) / dz;
-        double dvdzm = (v[I(i, jm, kp)] - v[I(i, jm, k)]) / dz;
-        double dwdzp = (w[I(i, j, kp)] - w[q]) / dz;
-        double dwdzm = (w[q] - w[I(i, j, km)]) / dz;
+        double dwdxp = sx.div(w[I(ip, j, k)] - w[q]);
+        double dwdxm = sx.div(w[q] - w[I(im, j, k)]);
+        double dudzp = sz.div(u[I(i, j, kp)] - u[q]);
+        double dudzm = sz.div(u[I(im, j, kp)] - u[I(im, j, k)]);
+        double dwdyp = sy.div(w[I(i, jp, k)] - w[q]);
+        double dwdym = sy.div(w[q] - w[I(i, jm, k)]);
+        double dvdzp = sz.div(v[I(i, j, kp)] - v[q]);
+        double dvdzm = sz.div(v[I(i, jm, kp)] - v[I(i, jm, k)]);
+        double dwdzp = sz.div(w[I(i, j, kp)] - w[q]);
+        double dwdzm = sz.div(w[q] - w[I(i, j, km)]);
         double ma4 = 0.25 * (mq + miu[I(i, j, kp)] + miu[I(ip, j, kp)] + miu[I(ip, j, k)]);
         double ma8 = 0.25 * (mq + miu[I(i, j, kp)] + miu[I(im, j, kp)] + miu[I(im, j, k)]);
         double ma7 = 0.25 * (mq + miu[I(i, j, kp)] + miu[I(i, jp, kp)] + miu[I(i, jp, k)]);
         double ma9 = 0.25 * (mq + miu[I(i, j, kp)] + miu[I(i, jm, kp)] + miu[I(i, jm, k)]);
-        double RD1 = (dwdxp * ma4 - dwdxm * ma8) / dx;
-        double RD2 = (dudzp * ma4 - dudzm * ma8) / dx;
-        double RD3 = (dwdyp * ma7 - dwdym * ma9) / dy;
-        double RD4 = (dvdzp * ma7 - dvdzm * ma9) / dy;
-        double RD5 = 2. * (dwdzp * miu[I(i, j, kp)] - dwdzm * mq) / dz;
+        double RD1 = sx.div(dwdxp * ma4 - dwdxm * ma8);
+        double RD2 = sx.div(dudzp * ma4 - dudzm * ma8);
+        double RD3 = sy.div(dwdyp * ma7 - dwdym * ma9);
+        double RD4 = sy.div(dvdzp * ma7 - dvdzm * ma9);
+        double RD5 = sz.div(2. * (dwdzp * miu[I(i, j, kp)] - dwdzm * mq));
         RDw = (RD1 + RD2 + RD3 + RD4 + RD5) / rho_w;
     }
 
@@ -185,14 +186,17 @@ static int momentum_launch(Ctx& c, int rhs_only) {
     m.forceytot = c.P.constPoi ? -12. * c.visc / (c.P.lz * c.P.lz) : 0.;
     m.rho_equal = c.P.rho1 == c.P.rho2;
     dim3 blk(BX, BY, 1);
-#define MOM(R)                                                                                                     \
-    hipLaunchKernelGGL(k_momentum<R>, grid3(c.g, c.g.nz, blk), blk, 0, c.stream, c.g, m, (const double*)c.dscal,   \
+#define MOM(R, P)                                                                                                  \
+    hipLaunchKernelGGL((k_momentum<R, P>), grid3(c.g, c.g.nz, blk), blk, 0, c.stream, c.g, m, (const double*)c.dscal,   \
                        (const double*)c.unew, (const double*)c.vnew, (const double*)c.wnew, (const double*)c.miu,   \
                        (const double*)c.rho, c.duold, c.dvold, c.dwold, (const double*)c.pnew, (const double*)c.pold, \
                        (const double*)c.p_x, (const double*)c.p_y, (const double*)c.p_z, (const double*)c.p_xold,   \
                        (const double*)c.p_yold, (const double*)c.p_zold, c.dudt, c.dvdt, c.dwdt)
-    if (rhs_only) MOM(1);
-    else MOM(0);
+    if (c.g.pow2) {
+        if (rhs_only) MOM(1, true); else MOM(0, true);
+    } else {
+        if (rhs_only) MOM(1, false); else MOM(0, false);
+    }
 #undef MOM
     HIP_OK(hipGetLastError());
     return ICLS_OK;
@@ -288,14 +292,12 @@ int launch_correc(Ctx& c) {
 // ---------------------------------------------------------------- pressure rotation
 // norm = sum(sum(pnew(:,:,1),2),1)/(itot*jtot) over the full C-halo plane, halos
 // included (main.f90:172-175); partial sums over rows of the (i1+1) x (j1+1) plane
+// one block per row j = 0 .. j1 of the plane, threads over i = 0 .. i1
 __global__ void k_plane_sum(Grid g, const double* __restrict__ p, double* __restrict__ part) {
     __shared__ double smem[16];
-    int i = blockIdx.x * blockDim.x + threadIdx.x;   // Fortran i = 0 .. i1
+    const int j0 = wrap((int)blockIdx.x - 1, g.ny);
     double acc = 0.;
-    if (i <= g.nx + 1) {
-        int i0 = wrap(i - 1, g.nx);
-        for (int j = 0; j <= g.ny + 1; j++) acc += p[gidx(g, i0, wrap(j - 1, g.ny), 1)];
-    }
+    for (int i = threadIdx.x; i <= g.nx + 1; i += blockDim.x) acc += p[gidx(g, wrap(i - 1, g.nx), j0, 1)];
     double r = block_reduce(acc, OpSum(), smem);
     if (threadIdx.x == 0) part[blockIdx.x] = r;
 }
@@ -310,8 +312,7 @@ __global__ void k_norm_final(const double* __restrict__ part, int n, double deno
 
 int launch_pressure_norm(Ctx& c) {
     ProfScope ps(c, "k_plane_sum");
-    int n = c.g.nx + 2;
-    int nb = (n + 255) / 256;
+    int nb = c.g.ny + 2;
     hipLaunchKernelGGL(k_plane_sum, dim3(nb), dim3(256), 0, c.stream, c.g, (const double*)c.p, c.dpart);
     hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(256), 0, c.stream, (const double*)c.dpart, nb,
                        1. * c.g.nx * c.g.ny, c.dscal);

@@ -182,34 +182,66 @@ __global__ void k_thomas(Grid g, const double* __restrict__ xyrt, double* __rest
     const double bmid = -(a + cc);
     const double b1 = bmid + a, bN = bmid + cc;
     const double lam = xyrt[i0 + (long)g.nx * j0];
-    long q = gidx(g, i0, j0, 1);
     const long sz = g.sz;
-    double z = 1. / (b1 + lam);
-    double dprev = cc * z;
-    double pprev = p[q] * z;
-    d[q] = dprev;
-    p[q] = pprev;
-    for (int k = 2; k <= kmax - 1; k++) {
-        q += sz;
-        double bb = bmid + lam;
-        z = 1. / (bb - a * dprev);
-        dprev = cc * z;
-        pprev = (p[q] - a * pprev) * z;
-        d[q] = dprev;
-        p[q] = pprev;
+    const long q1 = gidx(g, i0, j0, 1);
+    // The recurrences are serial in k; the loads they consume are not, so each sweep
+    // fetches a chunk of CH values ahead of the arithmetic that uses them.
+    constexpr int CH = 16;
+    double buf[CH], dbuf[CH];
+    double dprev = 0., pprev = 0.;
+    for (int kb = 1; kb <= kmax; kb += CH) {
+#pragma unroll
+        for (int t = 0; t < CH; t++)
+            if (kb + t <= kmax) buf[t] = p[q1 + (long)(kb + t - 1) * sz];
+#pragma unroll
+        for (int t = 0; t < CH; t++) {
+            const int k = kb + t;
+            if (k > kmax) break;
+            double z;
+            if (k == 1) {
+                z = 1. / (b1 + lam);
+                dprev = cc * z;
+                pprev = buf[t] * z;
+            } else if (k < kmax) {
+                double bb = bmid + lam;
+                z = 1. / (bb - a * dprev);
+                dprev = cc * z;
+                pprev = (buf[t] - a * pprev) * z;
+            } else {
+                double bb = bN + lam;
+                z = bb - a * dprev;
+                if (z != 0.) pprev = (buf[t] - a * pprev) / z;
+                else pprev = 0.;
+            }
+            buf[t] = pprev;
+            dbuf[t] = dprev;
+        }
+#pragma unroll
+        for (int t = 0; t < CH; t++) {
+            const int k = kb + t;
+            if (k > kmax) break;
+            p[q1 + (long)(k - 1) * sz] = buf[t];
+            if (k < kmax) d[q1 + (long)(k - 1) * sz] = dbuf[t];
+        }
     }
-    q += sz;
-    {
-        double bb = bN + lam;
-        z = bb - a * dprev;
-        if (z != 0.) pprev = (p[q] - a * pprev) / z;
-        else pprev = 0.;
-        p[q] = pprev;
-    }
-    for (int k = kmax - 1; k >= 1; k--) {
-        q -= sz;
-        pprev = p[q] - d[q] * pprev;
-        p[q] = pprev;
+    // back substitution from k = kmax-1 down to 1; pprev holds p(kmax)
+    for (int kt = kmax - 1; kt >= 1; kt -= CH) {
+#pragma unroll
+        for (int t = 0; t < CH; t++)
+            if (kt - t >= 1) {
+                long q = q1 + (long)(kt - t - 1) * sz;
+                buf[t] = p[q];
+                dbuf[t] = d[q];
+            }
+#pragma unroll
+        for (int t = 0; t < CH; t++) {
+            if (kt - t < 1) break;
+            pprev = buf[t] - dbuf[t] * pprev;
+            buf[t] = pprev;
+        }
+#pragma unroll
+        for (int t = 0; t < CH; t++)
+            if (kt - t >= 1) p[q1 + (long)(kt - t - 1) * sz] = buf[t];
     }
 }
 

@@ -21,10 +21,11 @@ ERRORS = {0: "ICLS_OK", 1: "ICLS_ERR_PARAM", 2: "ICLS_ERR_HIP", 3: "ICLS_ERR_NAN
 EXPORTS = [
     "icls_last_error", "icls_version", "icls_init", "icls_finalize", "icls_field_shape", "icls_init_program",
     "icls_init_derived", "icls_upload", "icls_download", "icls_get_scalar", "icls_set_scalar", "icls_get_volume",
+    "icls_set_volume",
     "icls_sync", "icls_ls_main", "icls_average_stream_vel", "icls_ab2", "icls_fillps", "icls_gfm4p",
     "icls_add_pjump", "icls_solver2d", "icls_boundp", "icls_correc", "icls_bounduvw", "icls_update_pressure",
     "icls_average_vel_at_center", "icls_check_nan", "icls_chkdiv", "icls_chkdt", "icls_max_vel_at_center",
-    "icls_step", "icls_profile_enable", "icls_profile_reset", "icls_kernel_times",
+    "icls_step", "icls_profile_enable", "icls_profile_only", "icls_profile_reset", "icls_kernel_times",
 ]
 
 
@@ -55,6 +56,7 @@ def load():
     L.icls_get_scalar.argtypes = [i32, dp]
     L.icls_set_scalar.argtypes = [i32, dbl]
     L.icls_get_volume.argtypes = [i32, dp, dp]
+    L.icls_set_volume.argtypes = [i32, dbl, dbl]
     for name in ["icls_ls_main", "icls_boundp", "icls_bounduvw", "icls_update_pressure", "icls_profile_enable"]:
         getattr(L, name).argtypes = [i32]
     L.icls_average_stream_vel.argtypes = [dp]
@@ -62,6 +64,7 @@ def load():
     L.icls_chkdt.argtypes = [dp]
     L.icls_max_vel_at_center.argtypes = [dp]
     L.icls_step.argtypes = [i32, i32]
+    L.icls_profile_only.argtypes = [ctypes.c_char_p]
     L.icls_kernel_times.argtypes = [i32, ctypes.c_char_p, dp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(i32)]
     for name in EXPORTS:
         fn = getattr(L, name)
@@ -137,6 +140,9 @@ class Solver:
         self._call("icls_get_volume", l, ctypes.byref(v), ctypes.byref(v0))
         return v.value, v0.value
 
+    def set_volume(self, l, volume, init_volume):
+        self._call("icls_set_volume", l, float(volume), float(init_volume))
+
     def sync(self):
         self._call("icls_sync")
 
@@ -210,6 +216,9 @@ class Solver:
     # --- profiling
     def profile(self, on=True):
         self._call("icls_profile_enable", 1 if on else 0)
+
+    def profile_only(self, name=None):
+        self._call("icls_profile_only", name.encode() if name else None)
 
     def profile_reset(self):
         self._call("icls_profile_reset")

@@ -146,6 +146,8 @@ int32_t icls_download(int32_t field, int32_t l, double* host);
 int32_t icls_get_scalar(int32_t which, double* value);
 int32_t icls_set_scalar(int32_t which, double value);
 int32_t icls_get_volume(int32_t l, double* volume, double* init_volume);
+/* Restore volume(l) and init_volume(l) (common.f90:52), e.g. on restart. */
+int32_t icls_set_volume(int32_t l, double volume, double init_volume);
 int32_t icls_sync(void);
 
 /* ---- the per-step operators (src/main.f90:103-291) ---- */
@@ -173,6 +175,8 @@ int32_t icls_step(int32_t istep, int32_t begin);
 /* Per-kernel HIP-event timing on the library's stream.  When enabled every launch
  * is bracketed with events; icls_kernel_times copies up to n (name, total ms, count). */
 int32_t icls_profile_enable(int32_t on);
+/* Restrict event bracketing to launches named `name` (NULL or "" = all). */
+int32_t icls_profile_only(const char* name);
 int32_t icls_profile_reset(void);
 int32_t icls_kernel_times(int32_t n, char* names /* n*64 */, double* total_ms, int64_t* counts, int32_t* n_out);
 

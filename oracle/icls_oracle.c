@@ -1557,6 +1557,11 @@ void oracle_set_scalar(oracle_state* S, int which, double v) {
     }
 }
 
+void oracle_set_volume(oracle_state* S, int l, double volume, double init_volume) {
+    S->volume[l] = volume;
+    S->init_volume[l] = init_volume;
+}
+
 double oracle_volume(oracle_state* S, int l, double* init_volume) {
     if (init_volume) *init_volume = S->init_volume[l];
     return S->volume[l];

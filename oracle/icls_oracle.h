@@ -63,6 +63,7 @@ double* oracle_field(oracle_state* S, int field, int l, int* shape, long* count)
 double  oracle_scalar(oracle_state* S, int which);
 void    oracle_set_scalar(oracle_state* S, int which, double v);
 double  oracle_volume(oracle_state* S, int l, double* init_volume);
+void    oracle_set_volume(oracle_state* S, int l, double volume, double init_volume);
 const double* oracle_pseudo_inv_A(oracle_state* S);          /* 10 x 27, column-major */
 
 /* stand-alone numerics (no state) */

@@ -70,6 +70,7 @@ def lib():
         L.oracle_set_scalar.argtypes = [vp, i32, dbl]
         L.oracle_volume.argtypes = [vp, i32, dp]
         L.oracle_volume.restype = dbl
+        L.oracle_set_volume.argtypes = [vp, i32, dbl, dbl]
         L.oracle_pseudo_inv_A.argtypes = [vp]
         L.oracle_pseudo_inv_A.restype = dp
         L.oracle_rfft_forward.argtypes = [i32, dp]
@@ -139,6 +140,9 @@ class Oracle:
         iv = ctypes.c_double()
         v = self.L.oracle_volume(self.h, l, ctypes.byref(iv))
         return v, iv.value
+
+    def set_volume(self, l, volume, init_volume):
+        self.L.oracle_set_volume(self.h, l, volume, init_volume)
 
     # --- operators
     def init_program(self):
