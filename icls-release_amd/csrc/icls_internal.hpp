@@ -80,6 +80,9 @@ struct Ctx {
     double* stash = nullptr;              // two z planes (re-init residual reference)
     double* scratch = nullptr;            // one field for on-demand downloads
     int nscal = 0;                        // doubles in dscal / hscal
+    double* hvol = nullptr;               // pinned copy of volume(l), init_volume(l)
+    hipEvent_t ev_vol = nullptr;          // recorded after that copy
+    bool vol_published = false;
     double* hscal = nullptr;              // pinned host mirror of dscal
     // constant tables
     double* pinvA = nullptr;              // 10 x 27 column-major

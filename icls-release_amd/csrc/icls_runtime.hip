@@ -147,6 +147,8 @@ static void free_ctx(Ctx* c) {
     for (auto* p : ptrs) if (p) hipFree(p);
     for (auto* p : c->lvset) if (p) hipFree(p);
     if (c->hscal) hipHostFree(c->hscal);
+    if (c->hvol) hipHostFree(c->hvol);
+    if (c->ev_vol) hipEventDestroy(c->ev_vol);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c->prof;
     delete c;
@@ -220,6 +222,9 @@ static int build_ctx(const icls_params* prm) {
     HIP_OK(hipHostMalloc((void**)&C.hscal, sizeof(double) * nscal));
     memset(C.hscal, 0, sizeof(double) * nscal);
     C.nscal = nscal;
+    HIP_OK(hipHostMalloc((void**)&C.hvol, sizeof(double) * 2 * C.lmax));
+    memset(C.hvol, 0, sizeof(double) * 2 * C.lmax);
+    HIP_OK(hipEventCreateWithFlags(&C.ev_vol, hipEventDisableTiming));
     // reduction partials: enough for lmax per-block sums of any sweep
     // (the chkdt sweep covers 1..i1 x 1..j1 x 1..k1, one row and column more than the others)
     long nb = (long)((g.nx + 2 + 63) / 64 + 1) * ((g.ny + 2 + 3) / 4 + 1) * ((g.nz + 2 + 7) / 8 + 1);
@@ -368,6 +373,7 @@ int32_t icls_download(int32_t field, int32_t l, double* host) {
     if (!host) { set_error("null host buffer"); return ICLS_ERR_PARAM; }
     double* d = field_ptr(c, field, l);
     if (field == ICLS_F_JIEMIAN) { TRY(launch_jiemian(c, c.scratch)); d = c.scratch; }
+    if (field == ICLS_F_UCC || field == ICLS_F_VCC || field == ICLS_F_WCC) TRY(launch_avg_vel(c));
     if (field == ICLS_F_RHO_U || field == ICLS_F_RHO_V || field == ICLS_F_RHO_W) d = c.rho;
     if (!d) { set_error("field %d (l=%d) cannot be downloaded", field, l); return ICLS_ERR_PARAM; }
     int sh = shape_of(field), hx, hy, hz, lb;
@@ -446,6 +452,8 @@ int32_t icls_get_volume(int32_t l, double* volume, double* init_volume) {
     return ICLS_OK;
 }
 
+static int publish_volumes(Ctx& c);
+
 int32_t icls_set_volume(int32_t l, double volume, double init_volume) {
     CTX_OR_FAIL();
     if (l < 0 || l >= c.lmax) { set_error("bad level-set index %d", l); return ICLS_ERR_PARAM; }
@@ -455,6 +463,31 @@ int32_t icls_set_volume(int32_t l, double volume, double init_volume) {
     v[0] = init_volume;
     HIP_OK(hipMemcpyAsync(c.dscal + DS_COUNT_FIXED + c.lmax + l, v, sizeof(double), hipMemcpyHostToDevice, c.stream));
     HIP_OK(hipStreamSynchronize(c.stream));
+    return publish_volumes(c);
+}
+
+// The mass-correction trigger needs volume(l) from the previous update_materials.
+// The volumes are copied to pinned host memory right after they are formed and the
+// host waits for that copy only when it reaches the trigger, by which time the device
+// is still working through the queued Navier-Stokes half of the previous step.
+static int publish_volumes(Ctx& c) {
+    HIP_OK(hipMemcpyAsync(c.hvol, c.dscal + DS_COUNT_FIXED, sizeof(double) * 2 * c.lmax, hipMemcpyDeviceToHost,
+                          c.stream));
+    HIP_OK(hipEventRecord(c.ev_vol, c.stream));
+    c.vol_published = true;
+    return ICLS_OK;
+}
+
+// ls_mass_correction trigger (levelset.f90:386-395), evaluated on the host copy of
+// the same doubles the device flag kernel uses
+static int mass_correction_due(Ctx& c, int l, int istep, bool& due) {
+    due = false;
+    if (istep % c.P.inflate_step != 0) return ICLS_OK;
+    if (!c.vol_published) TRY(publish_volumes(c));
+    HIP_OK(hipEventSynchronize(c.ev_vol));
+    double V = c.hvol[l], V0 = c.hvol[c.lmax + l];
+    double massloss = (V - V0) / V0;
+    due = fabs(massloss) > 1e-5;
     return ICLS_OK;
 }
 
@@ -465,26 +498,32 @@ static int ls_main_impl(Ctx& c, int istep) {
         // tube_construction (levelset.f90:87-197)
         TRY(launch_tube(c, ph, c.ls_a));
         TRY(launch_bc_ls(c, c.ls_a));
-        // ls_advection, SSP-RK3 (levelset.f90:235-243 / 268-276)
+        // ls_advection, SSP-RK3 (levelset.f90:235-243 / 268-276), cell-centre velocity
+        // averaged from unew/vnew/wnew inside the stencil (misc.f90:595-597)
         const int sch = c.P.ls_adv_space;
-        TRY(launch_adv_stage(c, sch, 1, c.ls_a, c.ls_a, c.ls_b, c.ucc, c.vcc, c.wcc, 0, nullptr, nullptr));
+        TRY(launch_adv_stage(c, sch, 1, c.ls_a, c.ls_a, c.ls_b, c.unew, c.vnew, c.wnew, 1, nullptr, nullptr));
         TRY(launch_bc_ls(c, c.ls_b));
-        TRY(launch_adv_stage(c, sch, 2, c.ls_a, c.ls_b, c.ls_c, c.ucc, c.vcc, c.wcc, 0, nullptr, nullptr));
+        TRY(launch_adv_stage(c, sch, 2, c.ls_a, c.ls_b, c.ls_c, c.unew, c.vnew, c.wnew, 1, nullptr, nullptr));
         TRY(launch_bc_ls(c, c.ls_c));
-        TRY(launch_adv_stage(c, sch, 3, c.ls_a, c.ls_c, ph, c.ucc, c.vcc, c.wcc, 0, nullptr, nullptr));
+        TRY(launch_adv_stage(c, sch, 3, c.ls_a, c.ls_c, ph, c.unew, c.vnew, c.wnew, 1, nullptr, nullptr));
         TRY(launch_bc_ls(c, ph));
-        // ls_mass_correction, predicated on the device (levelset.f90:377-582)
+        // ls_mass_correction (levelset.f90:377-582)
         if (c.P.mass_correction) {
-            const double* flag = c.dscal + DS_INFL_FLAG;
-            TRY(launch_inflation_prepare(c, l, istep));
-            TRY(launch_curvature(c, ph, c.curv2, c.normal_m, flag));
-            TRY(launch_inflation_velocity(c, l));
-            TRY(launch_adv_stage(c, ICLS_ADV_HOUC5, 1, ph, ph, c.ls_a, c.ucc_l, c.vcc_l, c.wcc_l, 0, nullptr, flag));
-            TRY(launch_bc_ls_pred(c, c.ls_a, flag));
-            TRY(launch_adv_stage(c, ICLS_ADV_HOUC5, 2, ph, c.ls_a, c.ls_b, c.ucc_l, c.vcc_l, c.wcc_l, 0, nullptr, flag));
-            TRY(launch_bc_ls_pred(c, c.ls_b, flag));
-            TRY(launch_adv_stage(c, ICLS_ADV_HOUC5, 3, ph, c.ls_b, ph, c.ucc_l, c.vcc_l, c.wcc_l, 0, nullptr, flag));
-            TRY(launch_bc_ls_pred(c, ph, flag));
+            bool due = false;
+            TRY(mass_correction_due(c, l, istep, due));
+            TRY(launch_inflation_prepare(c, l, istep));   // maxchk_glb = 0 and the device flag
+            if (due) {
+                const double* flag = c.dscal + DS_INFL_FLAG;
+                TRY(launch_curvature(c, ph, c.curv2, c.normal_m, flag));
+                TRY(launch_inflation_velocity(c, l));
+                TRY(launch_adv_stage(c, ICLS_ADV_HOUC5, 1, ph, ph, c.ls_a, c.ucc_l, c.vcc_l, c.wcc_l, 0, nullptr, flag));
+                TRY(launch_bc_ls_pred(c, c.ls_a, flag));
+                TRY(launch_adv_stage(c, ICLS_ADV_HOUC5, 2, ph, c.ls_a, c.ls_b, c.ucc_l, c.vcc_l, c.wcc_l, 0, nullptr,
+                                     flag));
+                TRY(launch_bc_ls_pred(c, c.ls_b, flag));
+                TRY(launch_adv_stage(c, ICLS_ADV_HOUC5, 3, ph, c.ls_b, ph, c.ucc_l, c.vcc_l, c.wcc_l, 0, nullptr, flag));
+                TRY(launch_bc_ls_pred(c, ph, flag));
+            }
         }
         // ls_reinit_main, WENO5 + RK3 (levelset.f90:592-882)
         if (c.P.re_initialization && istep % c.P.reinit_step == 0) {
@@ -497,7 +536,8 @@ static int ls_main_impl(Ctx& c, int istep) {
         }
     }
     // update_materials (misc.f90:79-290)
-    return launch_materials(c);
+    TRY(launch_materials(c));
+    return publish_volumes(c);
 }
 
 int32_t icls_ls_main(int32_t istep) {
@@ -662,6 +702,7 @@ int32_t icls_chkdt(double* dtmax) {
 
 int32_t icls_max_vel_at_center(double* vmax) {
     CTX_OR_FAIL();
+    TRY(launch_avg_vel(c));
     TRY(launch_max_vel(c));
     TRY(pull_scalars(c));
     if (vmax) *vmax = c.hscal[DS_VMAX];
@@ -680,6 +721,7 @@ static int init_derived_impl(Ctx& c) {
     // init_volume = volume (init.f90:86)
     HIP_OK(hipMemcpyAsync(c.dscal + DS_COUNT_FIXED + c.lmax, c.dscal + DS_COUNT_FIXED, sizeof(double) * c.lmax,
                           hipMemcpyDeviceToDevice, c.stream));
+    TRY(publish_volumes(c));
     TRY(launch_momentum_rhs_only(c));                      // init.f90:90
     // pold = pnew, p_xold = p_x ... (init.f90:107-110)
     HIP_OK(hipMemcpyAsync(c.pold, c.pnew, sizeof(double) * c.nfield, hipMemcpyDeviceToDevice, c.stream));
@@ -723,7 +765,8 @@ int32_t icls_step(int32_t istep, int32_t begin) {
     TRY(launch_bc_uvw(c, c.unew, c.vnew, c.wnew));
     TRY(launch_pressure_norm(c));                          // main.f90:164-185
     TRY(launch_pressure_rotate(c, istep == begin + 1));
-    TRY(launch_avg_vel(c));                                // main.f90:190
+    // main.f90:190: the cell-centre velocities are averaged from unew/vnew/wnew where
+    // they are consumed (the next step's advection), and on download
     TRY(launch_nan_check(c));                              // main.f90:207 (sticky flag)
     return ICLS_OK;
 }

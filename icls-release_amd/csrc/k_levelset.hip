@@ -176,7 +176,10 @@ __device__ __forceinline__ double adv_rhs(const Grid& g, const double* __restric
 // The bracket is the reference's whole-array temporary, applied to every stored
 // cell including the z ghost layers.  flag (optional) predicates the launch on a
 // device-side switch (used by the mass correction).
-template <int SCHEME, int STAGE, bool P2>
+// VELF: vu/vv/vw are the face velocities unew/vnew/wnew and the cell-centre values
+// are averaged here as average_vel_at_center does (misc.f90:595-597); otherwise they
+// are cell-centre arrays already.
+template <int SCHEME, int STAGE, bool P2, bool VELF>
 __global__ void k_adv_stage(Grid g, double dt, const double* phi0, const double* __restrict__ phis, double* out,
                             const uint8_t* __restrict__ mask, const double* __restrict__ vu,
                             const double* __restrict__ vv, const double* __restrict__ vw, const double* __restrict__ flag) {
@@ -191,30 +194,40 @@ __global__ void k_adv_stage(Grid g, double dt, const double* phi0, const double*
     else { comb = 1. / 3. * phi0[q] + 2. / 3. * phis[q]; factor = 2. / 3.; }
     double r = comb;
     if (k >= 1 && k <= g.nz && mask[q]) {
-        double dfdt = adv_rhs<SCHEME, P2>(g, phis, i0, j0, k, vu[q], vv[q], vw[q]);
+        double uc, vc, wc;
+        if (VELF) {
+            uc = (vu[gidx(g, wrap(i0 - 1, g.nx), j0, k)] + vu[q]) / 2.;
+            vc = (vv[gidx(g, i0, wrap(j0 - 1, g.ny), k)] + vv[q]) / 2.;
+            wc = (vw[gidx(g, i0, j0, k - 1)] + vw[q]) / 2.;
+        } else {
+            uc = vu[q]; vc = vv[q]; wc = vw[q];
+        }
+        double dfdt = adv_rhs<SCHEME, P2>(g, phis, i0, j0, k, uc, vc, wc);
         r = comb + factor * dt * dfdt;
     }
     out[q] = r;
 }
 
 int launch_adv_stage(Ctx& c, int scheme, int stage, const double* phi0, const double* phis, double* out,
-                     const double* vu, const double* vv, const double* vw, int /*vel_mode*/, const double* /*kappa*/,
+                     const double* vu, const double* vv, const double* vw, int velf, const double* /*kappa*/,
                      const double* flag) {
     dim3 blk(BX, BY, 1);
     dim3 grd = grid3(c.g, c.g.nz + 6, blk);
-#define ADV(S, T)                                                                                                  \
-    if (scheme == S && stage == T) {                                                                               \
-        ProfScope ps(c, T == 1 ? "k_adv_stage1" : "k_adv_stage23");                                                \
-        if (c.g.pow2)                                                                                              \
-            hipLaunchKernelGGL((k_adv_stage<S, T, true>), grd, blk, 0, c.stream, c.g, c.dt, phi0, phis, out, c.mask, \
-                               vu, vv, vw, flag);                                                                  \
-        else                                                                                                       \
-            hipLaunchKernelGGL((k_adv_stage<S, T, false>), grd, blk, 0, c.stream, c.g, c.dt, phi0, phis, out, c.mask,\
-                               vu, vv, vw, flag);                                                                  \
+#define ADV1(S, T, P, V) \
+    hipLaunchKernelGGL((k_adv_stage<S, T, P, V>), grd, blk, 0, c.stream, c.g, c.dt, phi0, phis, out, c.mask, vu, vv, vw, flag)
+#define ADV(S, T)                                                          \
+    if (scheme == S && stage == T) {                                       \
+        ProfScope ps(c, T == 1 ? "k_adv_stage1" : "k_adv_stage23");        \
+        if (c.g.pow2) {                                                    \
+            if (velf) ADV1(S, T, true, true); else ADV1(S, T, true, false);   \
+        } else {                                                           \
+            if (velf) ADV1(S, T, false, true); else ADV1(S, T, false, false); \
+        }                                                                  \
     }
     ADV(ICLS_ADV_HOUC5, 1) ADV(ICLS_ADV_HOUC5, 2) ADV(ICLS_ADV_HOUC5, 3)
     ADV(ICLS_ADV_WENO5, 1) ADV(ICLS_ADV_WENO5, 2) ADV(ICLS_ADV_WENO5, 3)
 #undef ADV
+#undef ADV1
     HIP_OK(hipGetLastError());
     return ICLS_OK;
 }

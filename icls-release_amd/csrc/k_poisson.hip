@@ -135,9 +135,141 @@ __global__ void k_rfft(Grid g, int N, int LPB, int TPL, const double* __restrict
     }
 }
 
+// The same transform with the line length N and the lines per block LPB fixed at
+// compile time: every index split is a shift and every stage loop unrolls.
+template <int DIR, int INV, int N, int LPB>
+__global__ void __launch_bounds__(LPB*((N / 4) < 64 ? (N / 4) : 64) < 64 ? 64 : LPB*((N / 4) < 64 ? (N / 4) : 64))
+k_rfft_n(Grid g, const double* __restrict__ twg, double* __restrict__ f) {
+    constexpr int M = N / 2;
+    constexpr int TPL = (M / 2) < 64 ? (M / 2) : 64;
+    constexpr int NTHR = LPB * TPL < 64 ? 64 : LPB * TPL;
+    __shared__ cplx bufa[LPB * M];
+    __shared__ cplx bufb[LPB * M];
+    __shared__ cplx tw[M];
+    const int tid = threadIdx.x;
+    const int k = blockIdx.y + 1;
+    const int line0 = blockIdx.x * LPB;
+    for (int t = tid; t < M; t += NTHR) tw[t] = {twg[2 * t], twg[2 * t + 1]};
+    const double scale = sqrt(1. / (double)N);
+    auto gq = [&](int ln, int n) -> long { return DIR == 0 ? gidx(g, n, line0 + ln, k) : gidx(g, line0 + ln, n, k); };
+    auto split = [&](int e, int& ln, int& n) {
+        if (DIR == 0) { ln = e / N; n = e % N; } else { n = e / LPB; ln = e % LPB; }
+    };
+    if (!INV) {
+#pragma unroll 4
+        for (int e = tid; e < LPB * N; e += NTHR) {
+            int ln, n;
+            split(e, ln, n);
+            double v = f[gq(ln, n)];
+            cplx* z = bufa + ln * M + (n >> 1);
+            if (n & 1) z->im = v; else z->re = v;
+        }
+    } else {
+#pragma unroll 4
+        for (int e = tid; e < LPB * N; e += NTHR) {
+            int ln, n;
+            split(e, ln, n);
+            ((double*)(bufb + ln * M))[n] = f[gq(ln, n)];
+        }
+        __syncthreads();
+        for (int e = tid; e < LPB * M; e += NTHR) {
+            int ln = e / M, kk = e % M;
+            const double* r = (const double*)(bufb + ln * M);
+            cplx X, Y;
+            if (kk == 0) { X = {r[0], 0.}; Y = {r[N - 1], 0.}; }
+            else { X = {r[2 * kk - 1], r[2 * kk]}; Y = {r[2 * (M - kk) - 1], r[2 * (M - kk)]}; }
+            cplx cy = cconj(Y);
+            cplx s = cadd(X, cy), dlt = csub(X, cy);
+            cplx t = cmul(tw[kk], dlt);
+            cplx it = {-t.im, t.re};
+            bufa[ln * M + kk] = cadd(s, it);
+        }
+    }
+    __syncthreads();
+    const int ln = tid / TPL, lane = tid % TPL;
+    cplx* x = bufa + (ln < LPB ? ln : 0) * M;
+    cplx* y = bufb + (ln < LPB ? ln : 0) * M;
+#pragma unroll
+    for (int Ns = 1; Ns < M; Ns <<= 1) {
+        if (ln < LPB) {
+#pragma unroll
+            for (int j = lane; j < M / 2; j += TPL) {
+                int kk = j & (Ns - 1);
+                cplx a = x[j], b = x[j + M / 2];
+                cplx w = tw[kk * (M / Ns)];
+                if (!INV) w.im = -w.im;
+                b = cmul(b, w);
+                int d = ((j - kk) << 1) + kk;
+                y[d] = cadd(a, b);
+                y[d + Ns] = csub(a, b);
+            }
+        }
+        __syncthreads();
+        cplx* t = x; x = y; y = t;
+    }
+    constexpr int NST = (M >= 2) ? __builtin_ctz(M) : 0;
+    cplx* outb = (NST & 1) ? bufb : bufa;
+    if (!INV) {
+#pragma unroll 4
+        for (int e = tid; e < LPB * N; e += NTHR) {
+            int lnn, n;
+            split(e, lnn, n);
+            const cplx* Z = outb + lnn * M;
+            double v;
+            if (n == 0) v = Z[0].re + Z[0].im;
+            else if (n == N - 1) v = Z[0].re - Z[0].im;
+            else {
+                int kk = (n + 1) >> 1;
+                cplx a = Z[kk], b = cconj(Z[M - kk]);
+                cplx ev = {(a.re + b.re) * 0.5, (a.im + b.im) * 0.5};
+                cplx od = {(a.im - b.im) * 0.5, -(a.re - b.re) * 0.5};
+                cplx w = tw[kk];
+                w.im = -w.im;
+                cplx X = cadd(ev, cmul(w, od));
+                v = (n & 1) ? X.re : X.im;
+            }
+            f[gq(lnn, n)] = scale * v;
+        }
+    } else {
+#pragma unroll 4
+        for (int e = tid; e < LPB * N; e += NTHR) {
+            int lnn, n;
+            split(e, lnn, n);
+            cplx z = outb[lnn * M + (n >> 1)];
+            f[gq(lnn, n)] = scale * ((n & 1) ? z.im : z.re);
+        }
+    }
+}
+
+template <int DIR, int INV, int N, int LPB>
+static bool try_rfft_n(Ctx& c, double* f, int n, int nlines) {
+    if (n != N || nlines % LPB != 0) return false;
+    constexpr int TPL = (N / 4) < 64 ? (N / 4) : 64;
+    constexpr int NTHR = LPB * TPL < 64 ? 64 : LPB * TPL;
+    hipLaunchKernelGGL((k_rfft_n<DIR, INV, N, LPB>), dim3(nlines / LPB, c.g.nz, 1), dim3(NTHR), 0, c.stream, c.g,
+                       (const double*)(DIR == 0 ? c.tw_x : c.tw_y), f);
+    return true;
+}
+
+template <int DIR, int INV>
+static bool rfft_fixed(Ctx& c, double* f, int n, int nlines) {
+    constexpr int L = DIR == 0 ? 4 : 8;
+    return try_rfft_n<DIR, INV, 32, L>(c, f, n, nlines) || try_rfft_n<DIR, INV, 64, L>(c, f, n, nlines) ||
+           try_rfft_n<DIR, INV, 128, L>(c, f, n, nlines) || try_rfft_n<DIR, INV, 256, L>(c, f, n, nlines) ||
+           try_rfft_n<DIR, INV, 512, 4>(c, f, n, nlines) || try_rfft_n<DIR, INV, 1024, 2>(c, f, n, nlines);
+}
+
 static int rfft_launch(Ctx& c, double* f, int dir, int inverse) {
     const int N = dir == 0 ? c.g.nx : c.g.ny;
     const int nlines = dir == 0 ? c.g.ny : c.g.nx;
+    bool done;
+    if (dir == 0) done = inverse ? rfft_fixed<0, 1>(c, f, N, nlines) : rfft_fixed<0, 0>(c, f, N, nlines);
+    else done = inverse ? rfft_fixed<1, 1>(c, f, N, nlines) : rfft_fixed<1, 0>(c, f, N, nlines);
+    if (done) {
+        HIP_OK(hipGetLastError());
+        return ICLS_OK;
+    }
+    // any other power of two: the runtime-sized kernel
     const int M = N / 2;
     int LPB = dir == 0 ? 4 : 16;
     if (LPB > nlines) LPB = nlines;

@@ -163,3 +163,27 @@ def test_fortran_driver_matches_oracle(tmp_path):
     for f, a in got.items():
         b = o.get(f).ravel(order="F")
         assert rel_linf(a, b) <= TOL, P.FIELD_NAMES[f]
+
+
+@pytest.mark.parametrize("n,nz", [(256, 16), (64, 32), (16, 8)])
+def test_solver2d_parity(n, nz):
+    """solver2d alone on a random zero-mean right-hand side: FFT x, FFT y, Thomas z,
+    inverse FFTs (solver.f90:68-196), at the line lengths of the workloads."""
+    p = small_drop(n, nz=nz)
+    o, g = _pair(p)
+    try:
+        o.init_program()
+        g.init_program()
+        rng = np.random.default_rng(n + nz)
+        f = np.zeros(P.field_dims(p, P.F_P), order="F")
+        f[1:-1, 1:-1, 1:-1] = rng.standard_normal((n, n, nz))
+        f[1:-1, 1:-1, 1:-1] -= f[1:-1, 1:-1, 1:-1].mean()
+        o.set(P.F_P, f)
+        g.upload(P.F_P, f)
+        o.solver2d()
+        g.solver2d()
+        a = g.download(P.F_P)[1:-1, 1:-1, 1:-1]
+        b = o.get(P.F_P)[1:-1, 1:-1, 1:-1]
+        assert rel_linf(a, b) <= TOL
+    finally:
+        g.close()
