@@ -53,9 +53,9 @@ def traffic_for(kernel, workload):
     if not os.path.exists(path):
         return None
     try:
-        data = json.load(open(path))
-        return data.get(workload, {}).get(kernel)
-    except Exception:
+        rec = json.load(open(path)).get(workload, {}).get(kernel)
+        return None if rec is None else rec["bytes_per_launch"]
+    except (OSError, ValueError, KeyError):
         return None
 
 
