@@ -8,6 +8,12 @@
 //   Rows along x are contiguous and 256-byte aligned for nx a multiple of 32 (FP64).
 // Fortran C-halo arrays (0:i1,0:j1,0:k1) use k = 0 .. ktot+1 of this layout, LS-halo
 // arrays (-2:i1+2,...) use all of it, interior arrays (1:imax,...) use k = 1 .. ktot.
+//
+// Precision classes.  "Field" arrays (level sets, velocities, momentum history,
+// materials, curvature, GFM jumps) hold the mode's element type T: double in the FP64
+// mode, float in the mixed modes.  "Pressure" arrays (pnew, pold, p, the Thomas
+// multipliers) are double in every mode.  Context members are raw device addresses
+// typed double*; launchers reinterpret field arrays as T*.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstdint>
@@ -36,31 +42,59 @@ __host__ __device__ inline int wrap(int i, int n) { return i < 0 ? i + n : (i >=
 enum DevScalar {
     DS_RHO_AVR = 0, DS_V_BULK, DS_NORM, DS_NORM_OLD, DS_REINIT_RES, DS_MAXCHK, DS_AREA, DS_INFL_TEMP,
     DS_INFL_FLAG, DS_DIV_TOT, DS_DIV_MAX, DS_VMAX, DS_UMAX, DS_VMAXC, DS_WMAX, DS_NAN_FLAG,
-    DS_COUNT_FIXED = 32
+    DS_RES_MAX, DS_COUNT_FIXED = 32
 };
 // volume[l] at DS_COUNT_FIXED + l, init_volume[l] at DS_COUNT_FIXED + lmax + l
 
-// Deterministic two-pass reductions: blocks write partials, one block folds them in order.
-constexpr int RED_BLOCK = 256;
-constexpr int RED_MAX_BLOCKS = 2048;
-
 struct Profiler;
+
+// Learned curvature closure: a 30 -> 64 -> 64 -> 1 tanh MLP on the 3 x 3 x 3 phi
+// stencil (over dz) and the unit normal, giving kappa * dz.  Parameters are one flat
+// FP32 vector: W1 (64 x 32, input padded to 32), b1, W2 (64 x 64), b2, w3, b3.
+constexpr int MLP_NIN = 30, MLP_NPAD = 32, MLP_NH = 64;
+constexpr int MLP_OW1 = 0;
+constexpr int MLP_OB1 = MLP_OW1 + MLP_NH * MLP_NPAD;
+constexpr int MLP_OW2 = MLP_OB1 + MLP_NH;
+constexpr int MLP_OB2 = MLP_OW2 + MLP_NH * MLP_NH;
+constexpr int MLP_OW3 = MLP_OB2 + MLP_NH;
+constexpr int MLP_OB3 = MLP_OW3 + MLP_NH;
+constexpr int MLP_NPARAM = MLP_OB3 + 1;
+constexpr int MLP_GBLOCKS = 64;                 // blocks of the gradient kernel
+
+struct Mlp {
+    float* w = nullptr;                 // master weights
+    float *adam_m = nullptr, *adam_v = nullptr;
+    uint16_t* w1b = nullptr;            // bf16 W1 (64 x 32), W2 (64 x 64) and W2^T
+    uint16_t* w2b = nullptr;
+    uint16_t* w2tb = nullptr;
+    float* gpart = nullptr;             // MLP_GBLOCKS x (MLP_NPARAM + 1) gradient partials
+    float* grad = nullptr;              // reduced gradient (MLP_NPARAM) and loss
+    double* pinvA_fine = nullptr;       // least-squares matrix at the 4x finer spacing
+    int* list = nullptr;                // under-resolved interface cells of one step
+    int* count = nullptr;
+    int list_cap = 0;
+    long adam_t = 0;
+    bool ready = false;                 // weights set or trained
+    float gate = 0.4f;                  // |kappa| dz at which the closure is consulted
+    float tol = 0.5f;                   // accepted |kappa_mlp - kappa_fd| dz
+};
 
 struct Ctx {
     icls_params P{};
     Grid g{};
     int lmax = 1;
     long nfield = 0;       // elements per device 3D field
+    bool fp32 = false;     // field arrays hold float (mixed modes)
+    size_t esz = 8;        // bytes per field element
     double NB2 = 0, rho0 = 0, visc = 0, pi = 0, dt = 0, time = 0;
     hipStream_t stream = nullptr;
     int device = 0;
     bool initialised = false;
-    bool first_pressure_update_done = false;
 
-    // FP64 fields (C-halo/interior/LS all in the padded layout)
+    // field class (T)
     double *unew = nullptr, *vnew = nullptr, *wnew = nullptr;
     double *dudt = nullptr, *dvdt = nullptr, *dwdt = nullptr;
-    double *pnew = nullptr, *pold = nullptr, *p = nullptr, *p_jump = nullptr;
+    double *p_jump = nullptr;
     double *duold = nullptr, *dvold = nullptr, *dwold = nullptr;
     double *p_x = nullptr, *p_y = nullptr, *p_z = nullptr;
     double *p_xold = nullptr, *p_yold = nullptr, *p_zold = nullptr;
@@ -71,28 +105,43 @@ struct Ctx {
     double *normal_m = nullptr;
     std::vector<double*> lvset;           // lmax level sets
     double *ls_a = nullptr, *ls_b = nullptr, *ls_c = nullptr;  // RK / re-init stage buffers
+    double* frhs = nullptr;               // Poisson right-hand side (the array p itself in FP64)
+    double* pf = nullptr;                 // mixed: float work array of the transforms
+    // pressure class (double)
+    double *pnew = nullptr, *pold = nullptr, *p = nullptr;
+    double* thomas_d = nullptr;           // Thomas multipliers (interior)
+    double* thomas_y = nullptr;           // mixed: Thomas forward-sweep values
+    double* scratch = nullptr;            // one double array for on-demand downloads
     uint8_t* mask = nullptr;              // narrow band (interior)
-    double* thomas_d = nullptr;           // Thomas scratch (interior)
     double* dscal = nullptr;              // device scalars
     double* dpart = nullptr;              // reduction partials
     long npart_cap = 0;                   // capacity of dpart in doubles
     double** dls = nullptr;               // device table of the lmax level-set pointers
     double* stash = nullptr;              // two z planes (re-init residual reference)
-    double* scratch = nullptr;            // one field for on-demand downloads
     int nscal = 0;                        // doubles in dscal / hscal
     double* hvol = nullptr;               // pinned copy of volume(l), init_volume(l)
     hipEvent_t ev_vol = nullptr;          // recorded after that copy
     bool vol_published = false;
     double* hscal = nullptr;              // pinned host mirror of dscal
-    // constant tables
+    // constant tables (double, and float copies for the mixed modes)
     double* pinvA = nullptr;              // 10 x 27 column-major
     double* xyrt = nullptr;               // nx * ny eigenvalues
-    double* tw_x = nullptr;               // FFT twiddles along x (nx complex)
+    double* tw_x = nullptr;               // FFT twiddles along x (nx/2 complex)
     double* tw_y = nullptr;               // along y
+    double* pinvA_f = nullptr;            // float copies (raw float arrays)
+    double* tw_x_f = nullptr;
+    double* tw_y_f = nullptr;
     std::vector<double> host_pinvA;
 
+    Mlp mlp;
+    bool learned = false;                 // ICLS_MODE_MIXED_LEARNED
     Profiler* prof = nullptr;
 };
+
+template <class T>
+inline T* as(double* p) { return reinterpret_cast<T*>(p); }
+template <class T>
+inline const T* as(const double* p) { return reinterpret_cast<const T*>(p); }
 
 // ---- error handling
 void set_error(const char* fmt, ...);
@@ -120,36 +169,62 @@ inline dim3 grid3(const Grid& g, int kcount, dim3 blk) {
     return dim3((g.nx + blk.x - 1) / blk.x, (g.ny + blk.y - 1) / blk.y, kcount);
 }
 
-// ---- launchers (defined in the k_*.hip files); each returns ICLS_OK or an error
+// XCD-aware block order.  The hardware hands consecutive workgroup ids round-robin to
+// the 8 XCDs, each with its own L2.  Kernels that share halos between neighbouring
+// blocks launch a 1D grid and take their tile from xcd_tile(): XCD x then works
+// through one contiguous range of tile ids, so blocks that run side by side on an
+// XCD are spatial neighbours and their halos meet in the same L2.
+constexpr int NUM_XCD = 8;
+__device__ __forceinline__ int xcd_tile(int b, int n) {
+    const int per = n / NUM_XCD, rem = n % NUM_XCD;
+    const int x = b % NUM_XCD, i = b / NUM_XCD;
+    return x * per + (x < rem ? x : rem) + i;
+}
+
+// ---- launchers (defined in the k_*.hip files); each returns ICLS_OK or an error.
+// Field-class pointers are reinterpreted as the mode's element type.
 // level set
 int launch_tube(Ctx& c, const double* phi_in, double* phi_out);
 int launch_bc_ls(Ctx& c, double* phi);
+int launch_bc_ls_pred(Ctx& c, double* phi, const double* flag);
 int launch_adv_stage(Ctx& c, int scheme, int stage, const double* phi0, const double* phis, double* out,
-                     const double* vu, const double* vv, const double* vw, int vel_mode, const double* kappa,
-                     const double* flag);
+                     const double* vu, const double* vv, const double* vw, int velf, const double* flag);
 int launch_reinit_stage(Ctx& c, int stage, const double* phi0, const double* phis, double* out);
-int launch_reinit_residual(Ctx& c, const double* phi0, const double* phinew);
+int launch_reinit_residual(Ctx& c, const double* phinew);
 int launch_boundmls(Ctx& c, double* phi);
 int launch_init_levelset(Ctx& c, int l, double* phi);
 // materials / reductions
 int launch_materials(Ctx& c);
+int launch_jiemian(Ctx& c, double* out);
 int launch_sum_interior(Ctx& c, const double* f, int slot);
 int launch_inflation_prepare(Ctx& c, int l, int istep);
 int launch_inflation_velocity(Ctx& c, int l);
 // interface / gfm
-int launch_curvature(Ctx& c, const double* phi, double* kappa, double* normal_m, const double* flag);
+int launch_curvature(Ctx& c, const double* phi, double* kappa, double* normal_m, const double* flag,
+                     bool listed = false);
 int launch_recurvature(Ctx& c, const double* phi, const double* kappa, double* kappa2);
 int launch_gfm(Ctx& c, int l, const double* kappa2, bool fuse_fillps, bool store_pjump);
 int launch_fillps(Ctx& c);
 int launch_add_pjump(Ctx& c);
-// poisson
-int launch_fft_x(Ctx& c, double* f, int inverse);
-int launch_fft_y(Ctx& c, double* f, int inverse);
-int launch_thomas(Ctx& c, double* f);
+// poisson: solve L p = frhs into p (solver2d; in the mixed modes with one refinement)
+int launch_solve(Ctx& c);
+// learned curvature closure (k_mlp.hip)
+int mlp_alloc(Ctx& c);
+void mlp_free(Ctx& c);
+int mlp_set_weights(Ctx& c, const float* w);
+int mlp_forward(Ctx& c, const uint16_t* x_bf16, int n, float* y);          // device pointers
+int mlp_grad(Ctx& c, const uint16_t* x_bf16, const float* t, int n, float* grad_loss);
+int mlp_adam(Ctx& c, float lr);                                            // after mlp_grad
+int mlp_samples(Ctx& c, long n, unsigned long long seed, float r_min, float r_max, uint16_t* x_bf16, float* t);
+int launch_mlp_reset_list(Ctx& c);
+int launch_mlp_curv(Ctx& c, const double* phi, double* kappa);
 // momentum / correction
 int launch_momentum(Ctx& c);
+int launch_momentum_rhs_only(Ctx& c);
 int launch_bc_uvw(Ctx& c, double* u, double* v, double* w);
+int launch_bc_uvw_pred(Ctx& c, double* u, double* v, double* w, const double* flag);
 int launch_bc_p(Ctx& c, double* p);
+int launch_bc_p_t_pred(Ctx& c, double* p, const double* flag);
 int launch_correc(Ctx& c);
 int launch_pressure_norm(Ctx& c);
 int launch_pressure_rotate(Ctx& c, int first_step);
@@ -159,13 +234,8 @@ int launch_chkdiv(Ctx& c);
 int launch_chkdt_umax(Ctx& c);
 int launch_max_vel(Ctx& c);
 int launch_init_velocity(Ctx& c);
-int launch_fill(Ctx& c, double* f, double v);
-int launch_copy_interior(Ctx& c, const double* a, double* b);
-int launch_momentum_rhs_only(Ctx& c);
-int launch_jiemian(Ctx& c, double* out);
-int launch_bc_p_pred(Ctx& c, double* p, const double* flag);
-int launch_bc_uvw_pred(Ctx& c, double* u, double* v, double* w, const double* flag);
-int launch_bc_ls_pred(Ctx& c, double* phi, const double* flag);
-int launch_scalar_kernel(Ctx& c, int op, int a, int b);
+int launch_fill(Ctx& c, double* f, double v, int fp32);
+int launch_copy_field(Ctx& c, const double* a, double* b);
+int launch_convert(Ctx& c, const double* src, int src_fp32, double* dst, int dst_fp32);
 
 }  // namespace icls

@@ -110,4 +110,39 @@ __device__ __forceinline__ T icurvature(int nb, const T* ph, const T* cv) {
     return T(0);
 }
 
+// get_curvature's arithmetic (interface.f90:59-109) on 27 gathered values v:
+// gradient and Hessian from the least-squares rows of A (10 x 27, column-major),
+// kappa = -div(grad phi / |grad phi|), zero when |grad phi| < 0.9, clamped to
+// +-max_kappa, zero below 1e-9.  m receives |grad phi|.
+template <class T>
+__device__ __forceinline__ T curv_lsq(const T* __restrict__ A, const T* v, T max_kappa, T& m) {
+    auto row = [&](int r) {
+        T s = T(0.);
+#pragma unroll
+        for (int q = 0; q < 27; q++) s = s + A[r + 10 * q] * v[q];
+        return s;
+    };
+    T dphidx = row(1), dphidy = row(2), dphidz = row(3);
+    T d2phidx = row(4), d2phidy = row(5), d2phidz = row(6);
+    T d2phidxdy = row(7), d2phidxdz = row(8), d2phidydz = row(9);
+    T norm_val = sqrt(dphidx * dphidx + dphidy * dphidy + dphidz * dphidz);
+    m = norm_val;
+    T cv;
+    if (norm_val < T(0.9)) {
+        cv = T(0.);
+    } else {
+        T term1 = (d2phidx + d2phidy + d2phidz) / norm_val;
+        T n3 = norm_val * norm_val * norm_val;
+        T term2 = dphidx * dphidx * d2phidx + dphidy * dphidy * d2phidy + dphidz * dphidz * d2phidz;
+        term2 = term2 / n3;
+        T term3 = dphidx * dphidy * d2phidxdy + dphidx * dphidz * d2phidxdz + dphidy * dphidz * d2phidydz;
+        term3 = T(2.) * term3 / n3;
+        cv = -(term1 - term2 - term3);
+    }
+    if (cv < -max_kappa) cv = -max_kappa;
+    else if (cv > max_kappa) cv = max_kappa;
+    else if (fabs(cv) < T(1e-9)) cv = T(0.);
+    return cv;
+}
+
 }  // namespace icls

@@ -1,6 +1,10 @@
 // k_interface.hip -- least-squares curvature (get_curvature), its smoothing
 // (re_curvature), the ghost-fluid pressure jumps (gfm4p) and the Poisson right-hand
 // side (fillps), fused where the reference applies them back to back.
+//
+// T is the element type of the field arrays; the right-hand side the Poisson solve
+// reads (c.frhs) has the same type.  In the FP64 mode c.frhs is the pressure array p
+// itself, as in the reference.
 #include "icls_internal.hpp"
 #include "icls_stencil.hpp"
 
@@ -9,7 +13,8 @@ namespace icls {
 static constexpr int BX = 64, BY = 4;
 
 // 27-point gather in reshape order (i fastest, then j, then k), interface.f90:59
-__device__ __forceinline__ void gather27(const Grid& g, const double* __restrict__ f, int i0, int j0, int k, double* v) {
+template <class T>
+__device__ __forceinline__ void gather27(const Grid& g, const T* __restrict__ f, int i0, int j0, int k, T* v) {
     int im = wrap(i0 - 1, g.nx), ip = wrap(i0 + 1, g.nx), jm = wrap(j0 - 1, g.ny), jp = wrap(j0 + 1, g.ny);
     int is[3] = {im, i0, ip}, js[3] = {jm, j0, jp};
     int q = 0;
@@ -22,8 +27,9 @@ __device__ __forceinline__ void gather27(const Grid& g, const double* __restrict
 }
 
 // row r of matmul(pseudo_inv_A, v), accumulated over the 27 columns in order
-__device__ __forceinline__ double lsq_row(const double* __restrict__ A, int r, const double* v) {
-    double s = 0.;
+template <class T>
+__device__ __forceinline__ T lsq_row(const T* __restrict__ A, int r, const T* v) {
+    T s = T(0.);
 #pragma unroll
     for (int q = 0; q < 27; q++) s = s + A[r + 10 * q] * v[q];
     return s;
@@ -33,60 +39,59 @@ __device__ __forceinline__ double lsq_row(const double* __restrict__ A, int r, c
 // quadratic of init_matrix to the 27 values, kappa = -div(grad phi/|grad phi|) with
 // the 0.9 gradient filter and the +-1/dz and 1e-9 limits; zero elsewhere.  The
 // z-ghost layers of kappa are never written and stay zero (interface.f90:49).
-__global__ void k_curvature(Grid g, double NB2, const double* __restrict__ A, const double* __restrict__ phi,
-                            double* __restrict__ kappa, double* __restrict__ nm, const double* __restrict__ flag) {
+// With a cell list (the learned mode) the interface cells whose curvature is
+// under-resolved -- |phi| <= 4 dz, |grad phi| >= 0.9 and |kappa| dz >= gate -- are
+// appended to it for the learned closure (k_mlp_curv).
+template <class T>
+__global__ void k_curvature(Grid g, T NB2, const T* __restrict__ A, const T* __restrict__ phi, T* __restrict__ kappa,
+                            T* __restrict__ nm, const double* __restrict__ flag, int* __restrict__ list,
+                            int* __restrict__ count, int cap, T gate) {
     if (flag && flag[0] == 0.) return;
     int i0 = blockIdx.x * BX + threadIdx.x, j0 = blockIdx.y * BY + threadIdx.y, k = blockIdx.z + 1;
     if (i0 >= g.nx || j0 >= g.ny) return;
     long q = gidx(g, i0, j0, k);
-    const double dz = g.dz, max_kappa = 1. / dz;
-    double cv = 0., m = 1.;
-    if (fabs(phi[q]) <= NB2 - 1. * dz) {
-        double v[27];
+    const T dz = T(g.dz), max_kappa = T(1.) / dz;
+    T cv = T(0.), m = T(1.);
+    if (fabs(phi[q]) <= NB2 - T(1.) * dz) {
+        T v[27];
         gather27(g, phi, i0, j0, k, v);
-        double dphidx = lsq_row(A, 1, v), dphidy = lsq_row(A, 2, v), dphidz = lsq_row(A, 3, v);
-        double d2phidx = lsq_row(A, 4, v), d2phidy = lsq_row(A, 5, v), d2phidz = lsq_row(A, 6, v);
-        double d2phidxdy = lsq_row(A, 7, v), d2phidxdz = lsq_row(A, 8, v), d2phidydz = lsq_row(A, 9, v);
-        double norm_val = sqrt(dphidx * dphidx + dphidy * dphidy + dphidz * dphidz);
-        m = norm_val;
-        if (norm_val < 0.9) {
-            cv = 0.;
-        } else {
-            double term1 = (d2phidx + d2phidy + d2phidz) / norm_val;
-            double n3 = norm_val * norm_val * norm_val;
-            double term2 = dphidx * dphidx * d2phidx + dphidy * dphidy * d2phidy + dphidz * dphidz * d2phidz;
-            term2 = term2 / n3;
-            double term3 = dphidx * dphidy * d2phidxdy + dphidx * dphidz * d2phidxdz + dphidy * dphidz * d2phidydz;
-            term3 = 2. * term3 / n3;
-            cv = -(term1 - term2 - term3);
+        cv = curv_lsq(A, v, max_kappa, m);
+        if (list && fabs(phi[q]) <= T(4.) * dz && m >= T(0.9) && fabs(cv) * dz >= gate) {
+            const int slot = atomicAdd(count, 1);
+            if (slot < cap) list[slot] = (int)(i0 + g.sy * j0 + g.sz * (long)(k - 1));
         }
-        if (cv < -max_kappa) cv = -max_kappa;
-        else if (cv > max_kappa) cv = max_kappa;
-        else if (fabs(cv) < 1e-9) cv = 0.;
     }
     kappa[q] = cv;
     if (nm) nm[q] = m;
 }
 
-int launch_curvature(Ctx& c, const double* phi, double* kappa, double* normal_m, const double* flag) {
+int launch_curvature(Ctx& c, const double* phi, double* kappa, double* normal_m, const double* flag, bool listed) {
     ProfScope ps(c, "k_curvature");
     dim3 blk(BX, BY, 1);
-    hipLaunchKernelGGL(k_curvature, grid3(c.g, c.g.nz, blk), blk, 0, c.stream, c.g, c.NB2, (const double*)c.pinvA, phi,
-                       kappa, normal_m, flag);
+    dim3 grd = grid3(c.g, c.g.nz, blk);
+    int* list = listed ? c.mlp.list : nullptr;
+    if (c.fp32)
+        hipLaunchKernelGGL(k_curvature<float>, grd, blk, 0, c.stream, c.g, (float)c.NB2, as<float>(c.pinvA_f),
+                           as<float>(phi), as<float>(kappa), as<float>(normal_m), flag, list, c.mlp.count,
+                           c.mlp.list_cap, c.mlp.gate);
+    else
+        hipLaunchKernelGGL(k_curvature<double>, grd, blk, 0, c.stream, c.g, c.NB2, (const double*)c.pinvA, phi, kappa,
+                           normal_m, flag, list, c.mlp.count, c.mlp.list_cap, (double)c.mlp.gate);
     HIP_OK(hipGetLastError());
     return ICLS_OK;
 }
 
 // re_curvature (interface.f90:186-218): for |phi| < 2 dz the constant term of the
 // least-squares fit to the 27 curvature values; zero elsewhere.
-__global__ void k_recurvature(Grid g, const double* __restrict__ A, const double* __restrict__ phi,
-                              const double* __restrict__ kappa, double* __restrict__ kappa2) {
+template <class T>
+__global__ void k_recurvature(Grid g, const T* __restrict__ A, const T* __restrict__ phi, const T* __restrict__ kappa,
+                              T* __restrict__ kappa2) {
     int i0 = blockIdx.x * BX + threadIdx.x, j0 = blockIdx.y * BY + threadIdx.y, k = blockIdx.z + 1;
     if (i0 >= g.nx || j0 >= g.ny) return;
     long q = gidx(g, i0, j0, k);
-    double r = 0.;
-    if (fabs(phi[q]) < 2. * g.dz) {
-        double v[27];
+    T r = T(0.);
+    if (fabs(phi[q]) < T(2.) * T(g.dz)) {
+        T v[27];
         gather27(g, kappa, i0, j0, k, v);
         r = lsq_row(A, 0, v);
     }
@@ -96,35 +101,39 @@ __global__ void k_recurvature(Grid g, const double* __restrict__ A, const double
 int launch_recurvature(Ctx& c, const double* phi, const double* kappa, double* kappa2) {
     ProfScope ps(c, "k_recurvature");
     dim3 blk(BX, BY, 1);
-    hipLaunchKernelGGL(k_recurvature, grid3(c.g, c.g.nz, blk), blk, 0, c.stream, c.g, (const double*)c.pinvA, phi, kappa,
-                       kappa2);
+    dim3 grd = grid3(c.g, c.g.nz, blk);
+    if (c.fp32)
+        hipLaunchKernelGGL(k_recurvature<float>, grd, blk, 0, c.stream, c.g, as<float>(c.pinvA_f), as<float>(phi),
+                           as<float>(kappa), as<float>(kappa2));
+    else
+        hipLaunchKernelGGL(k_recurvature<double>, grd, blk, 0, c.stream, c.g, (const double*)c.pinvA, phi, kappa, kappa2);
     HIP_OK(hipGetLastError());
     return ICLS_OK;
 }
 
 // fillps (fillps.f90:36-53): rho0/dt * div(u*)
-__device__ __forceinline__ double fillps_rhs(const Grid& g, double coef, const double* __restrict__ du,
-                                             const double* __restrict__ dv, const double* __restrict__ dw, int i0,
-                                             int j0, int k) {
+template <class T>
+__device__ __forceinline__ T fillps_rhs(const Grid& g, T coef, const T* __restrict__ du, const T* __restrict__ dv,
+                                        const T* __restrict__ dw, int i0, int j0, int k) {
     long q = gidx(g, i0, j0, k);
-    double vel_x = (du[q] - du[gidx(g, wrap(i0 - 1, g.nx), j0, k)]) * g.dxi;
-    double vel_y = (dv[q] - dv[gidx(g, i0, wrap(j0 - 1, g.ny), k)]) * g.dyi;
-    double vel_z = (dw[q] - dw[gidx(g, i0, j0, k - 1)]) * g.dzi;
+    T vel_x = (du[q] - du[gidx(g, wrap(i0 - 1, g.nx), j0, k)]) * T(g.dxi);
+    T vel_y = (dv[q] - dv[gidx(g, i0, wrap(j0 - 1, g.ny), k)]) * T(g.dyi);
+    T vel_z = (dw[q] - dw[gidx(g, i0, j0, k - 1)]) * T(g.dzi);
     return coef * (vel_x + vel_y + vel_z);
 }
 
 // gfm4p jumps of one level set at one cell (gfm.f90:99-215).  Returns the separate
 // jump Laplacian pj_sep and the + face jumps px/py/pz (already sign-flipped inside
 // the drop); false when the cell is outside the |phi| <= 2 dz shell.
-__device__ __forceinline__ bool gfm_cell(const Grid& g, double nondi, const double* __restrict__ ls,
-                                         const double* __restrict__ cvf, int i0, int j0, int k, double& pjs, double& pxs,
-                                         double& pys, double& pzs) {
+template <class T>
+__device__ __forceinline__ bool gfm_cell(const Grid& g, T nondi, const T* __restrict__ ls, const T* __restrict__ cvf,
+                                         int i0, int j0, int k, T& pjs, T& pxs, T& pys, T& pzs) {
     long q = gidx(g, i0, j0, k);
-    double phc = ls[q];
-    if (!(fabs(phc) <= 2. * g.dz)) return false;
-    const double dx2 = g.dx * g.dx, dy2 = g.dy * g.dy, dz2 = g.dz * g.dz;
-    double ph[3], cv[3], jump[3];
-    double face[3] = {0., 0., 0.};
+    T phc = ls[q];
+    if (!(fabs(phc) <= T(2.) * T(g.dz))) return false;
+    const T dx2 = T(g.dx) * T(g.dx), dy2 = T(g.dy) * T(g.dy), dz2 = T(g.dz) * T(g.dz);
+    T ph[3], cv[3], jump[3];
+    T face[3] = {T(0.), T(0.), T(0.)};
 #pragma unroll
     for (int ax = 0; ax < 3; ax++) {
         long qm, qp;
@@ -133,19 +142,19 @@ __device__ __forceinline__ bool gfm_cell(const Grid& g, double nondi, const doub
         else { qm = gidx(g, i0, j0, k - 1); qp = gidx(g, i0, j0, k + 1); }
         ph[0] = ls[qm]; ph[1] = phc; ph[2] = ls[qp];
         cv[0] = cvf[qm]; cv[1] = cvf[q]; cv[2] = cvf[qp];
-        double jm = 0., jp = 0.;
-        if (ph[1] * ph[0] < 0.) jm = nondi * (-icurvature<double>(-1, ph + 1, cv + 1));
-        if (ph[1] * ph[2] < 0.) { jp = nondi * (-icurvature<double>(1, ph + 1, cv + 1)); face[ax] = jp; }
-        jump[ax] = jm + 0. + jp;   // sum(jp_x) over (-1, 0, 1)
+        T jm = T(0.), jp = T(0.);
+        if (ph[1] * ph[0] < T(0.)) jm = nondi * (-icurvature<T>(-1, ph + 1, cv + 1));
+        if (ph[1] * ph[2] < T(0.)) { jp = nondi * (-icurvature<T>(1, ph + 1, cv + 1)); face[ax] = jp; }
+        jump[ax] = jm + T(0.) + jp;   // sum(jp_x) over (-1, 0, 1)
     }
-    if (phc > 0.) {
+    if (phc > T(0.)) {
         pjs = (jump[0] / dx2 + jump[1] / dy2 + jump[2] / dz2);
         pxs = face[0]; pys = face[1]; pzs = face[2];
-    } else if (phc < 0.) {
+    } else if (phc < T(0.)) {
         pjs = -(jump[0] / dx2 + jump[1] / dy2 + jump[2] / dz2);
         pxs = -face[0]; pys = -face[1]; pzs = -face[2];
     } else {
-        pjs = 0.; pxs = 0.; pys = 0.; pzs = 0.;
+        pjs = T(0.); pxs = T(0.); pys = T(0.); pzs = T(0.);
     }
     return true;
 }
@@ -153,17 +162,17 @@ __device__ __forceinline__ bool gfm_cell(const Grid& g, double nondi, const doub
 // MODE 0: first level set -- write p_jump, p_x, p_y, p_z over the interior (zero off
 //         the shell), gfm.f90:71-79 and 219-224.
 // MODE 1: further level sets -- add inside their shell (gfm.f90:226-232).
-// MODE 2: single level set fused with fillps and main.f90:142 -- p = rhs + p_jump.
-template <int MODE>
-__global__ void k_gfm(Grid g, double nondi, double coef, const double* __restrict__ ls, const double* __restrict__ cvf,
-                      double* __restrict__ pjump, double* __restrict__ px, double* __restrict__ py,
-                      double* __restrict__ pz, const double* __restrict__ du, const double* __restrict__ dv,
-                      const double* __restrict__ dw, double* __restrict__ p) {
+// MODE 2: single level set fused with fillps and main.f90:142 -- rhs = fillps + p_jump.
+template <class T, int MODE>
+__global__ void k_gfm(Grid g, T nondi, T coef, const T* __restrict__ ls, const T* __restrict__ cvf,
+                      T* __restrict__ pjump, T* __restrict__ px, T* __restrict__ py, T* __restrict__ pz,
+                      const T* __restrict__ du, const T* __restrict__ dv, const T* __restrict__ dw,
+                      T* __restrict__ rhs) {
     int i0 = blockIdx.x * BX + threadIdx.x, j0 = blockIdx.y * BY + threadIdx.y, k = blockIdx.z + 1;
     if (i0 >= g.nx || j0 >= g.ny) return;
     long q = gidx(g, i0, j0, k);
-    double pjs = 0., pxs = 0., pys = 0., pzs = 0.;
-    bool shell = gfm_cell(g, nondi, ls, cvf, i0, j0, k, pjs, pxs, pys, pzs);
+    T pjs = T(0.), pxs = T(0.), pys = T(0.), pzs = T(0.);
+    bool shell = gfm_cell<T>(g, nondi, ls, cvf, i0, j0, k, pjs, pxs, pys, pzs);
     if (MODE == 1) {
         if (shell) {
             pjump[q] = pjump[q] + pjs;
@@ -173,48 +182,65 @@ __global__ void k_gfm(Grid g, double nondi, double coef, const double* __restric
     }
     px[q] = pxs; py[q] = pys; pz[q] = pzs;
     if (pjump) pjump[q] = pjs;
-    if (MODE == 2) p[q] = fillps_rhs(g, coef, du, dv, dw, i0, j0, k) + pjs;
+    if (MODE == 2) rhs[q] = fillps_rhs<T>(g, coef, du, dv, dw, i0, j0, k) + pjs;
 }
 
-int launch_gfm(Ctx& c, int l, const double* kappa2, bool fuse_fillps, bool store_pjump) {
-    ProfScope ps(c, "k_gfm");
+template <class T>
+static void gfm_dispatch(Ctx& c, int l, const double* kappa2, int mode, bool store_pjump) {
     dim3 blk(BX, BY, 1);
     dim3 grd = grid3(c.g, c.g.nz, blk);
-    double nondi = 1. / c.P.We, coef = c.rho0 / c.dt;
-    double* pj = store_pjump ? c.p_jump : nullptr;
-    if (fuse_fillps)
-        hipLaunchKernelGGL(k_gfm<2>, grd, blk, 0, c.stream, c.g, nondi, coef, (const double*)c.lvset[l], kappa2, pj, c.p_x,
-                           c.p_y, c.p_z, (const double*)c.dudt, (const double*)c.dvdt, (const double*)c.dwdt, c.p);
-    else if (l == 0)
-        hipLaunchKernelGGL(k_gfm<0>, grd, blk, 0, c.stream, c.g, nondi, coef, (const double*)c.lvset[l], kappa2, c.p_jump,
-                           c.p_x, c.p_y, c.p_z, (const double*)nullptr, (const double*)nullptr, (const double*)nullptr,
-                           (double*)nullptr);
+    const T nondi = (T)(1. / c.P.We), coef = (T)(c.rho0 / c.dt);
+    const T* ls = as<T>(c.lvset[l]);
+    const T* k2 = as<T>(kappa2);
+    T *px = as<T>(c.p_x), *py = as<T>(c.p_y), *pz = as<T>(c.p_z);
+    if (mode == 2)
+        hipLaunchKernelGGL((k_gfm<T, 2>), grd, blk, 0, c.stream, c.g, nondi, coef, ls, k2,
+                           store_pjump ? as<T>(c.p_jump) : (T*)nullptr, px, py, pz, as<T>(c.dudt), as<T>(c.dvdt),
+                           as<T>(c.dwdt), as<T>(c.frhs));
+    else if (mode == 0)
+        hipLaunchKernelGGL((k_gfm<T, 0>), grd, blk, 0, c.stream, c.g, nondi, coef, ls, k2, as<T>(c.p_jump), px, py, pz,
+                           (const T*)nullptr, (const T*)nullptr, (const T*)nullptr, (T*)nullptr);
     else
-        hipLaunchKernelGGL(k_gfm<1>, grd, blk, 0, c.stream, c.g, nondi, coef, (const double*)c.lvset[l], kappa2, c.p_jump,
-                           c.p_x, c.p_y, c.p_z, (const double*)nullptr, (const double*)nullptr, (const double*)nullptr,
-                           (double*)nullptr);
+        hipLaunchKernelGGL((k_gfm<T, 1>), grd, blk, 0, c.stream, c.g, nondi, coef, ls, k2, as<T>(c.p_jump), px, py, pz,
+                           (const T*)nullptr, (const T*)nullptr, (const T*)nullptr, (T*)nullptr);
+}
+
+// fuse_fillps: the single-level-set step (MODE 2); otherwise MODE 0 for l = 0 and
+// MODE 1 for the further level sets
+int launch_gfm(Ctx& c, int l, const double* kappa2, bool fuse_fillps, bool store_pjump) {
+    ProfScope ps(c, "k_gfm");
+    const int mode = fuse_fillps ? 2 : (l == 0 ? 0 : 1);
+    if (c.fp32) gfm_dispatch<float>(c, l, kappa2, mode, store_pjump);
+    else gfm_dispatch<double>(c, l, kappa2, mode, store_pjump);
     HIP_OK(hipGetLastError());
     return ICLS_OK;
 }
 
-__global__ void k_fillps(Grid g, double coef, const double* __restrict__ du, const double* __restrict__ dv,
-                         const double* __restrict__ dw, double* __restrict__ p) {
+template <class T>
+__global__ void k_fillps(Grid g, T coef, const T* __restrict__ du, const T* __restrict__ dv, const T* __restrict__ dw,
+                         T* __restrict__ rhs) {
     int i0 = blockIdx.x * BX + threadIdx.x, j0 = blockIdx.y * BY + threadIdx.y, k = blockIdx.z + 1;
     if (i0 >= g.nx || j0 >= g.ny) return;
-    p[gidx(g, i0, j0, k)] = fillps_rhs(g, coef, du, dv, dw, i0, j0, k);
+    rhs[gidx(g, i0, j0, k)] = fillps_rhs<T>(g, coef, du, dv, dw, i0, j0, k);
 }
 
 int launch_fillps(Ctx& c) {
     ProfScope ps(c, "k_fillps");
     dim3 blk(BX, BY, 1);
-    hipLaunchKernelGGL(k_fillps, grid3(c.g, c.g.nz, blk), blk, 0, c.stream, c.g, c.rho0 / c.dt, (const double*)c.dudt,
-                       (const double*)c.dvdt, (const double*)c.dwdt, c.p);
+    dim3 grd = grid3(c.g, c.g.nz, blk);
+    if (c.fp32)
+        hipLaunchKernelGGL(k_fillps<float>, grd, blk, 0, c.stream, c.g, (float)(c.rho0 / c.dt), as<float>(c.dudt),
+                           as<float>(c.dvdt), as<float>(c.dwdt), as<float>(c.frhs));
+    else
+        hipLaunchKernelGGL(k_fillps<double>, grd, blk, 0, c.stream, c.g, c.rho0 / c.dt, (const double*)c.dudt,
+                           (const double*)c.dvdt, (const double*)c.dwdt, c.frhs);
     HIP_OK(hipGetLastError());
     return ICLS_OK;
 }
 
-// p = p + p_jump (main.f90:142) over the interior
-__global__ void k_add(Grid g, double* __restrict__ p, const double* __restrict__ a) {
+// rhs = rhs + p_jump (main.f90:142) over the interior
+template <class T>
+__global__ void k_add(Grid g, T* __restrict__ p, const T* __restrict__ a) {
     int i0 = blockIdx.x * BX + threadIdx.x, j0 = blockIdx.y * BY + threadIdx.y, k = blockIdx.z + 1;
     if (i0 >= g.nx || j0 >= g.ny) return;
     long q = gidx(g, i0, j0, k);
@@ -224,7 +250,9 @@ __global__ void k_add(Grid g, double* __restrict__ p, const double* __restrict__
 int launch_add_pjump(Ctx& c) {
     ProfScope ps(c, "k_add");
     dim3 blk(BX, BY, 1);
-    hipLaunchKernelGGL(k_add, grid3(c.g, c.g.nz, blk), blk, 0, c.stream, c.g, c.p, (const double*)c.p_jump);
+    dim3 grd = grid3(c.g, c.g.nz, blk);
+    if (c.fp32) hipLaunchKernelGGL(k_add<float>, grd, blk, 0, c.stream, c.g, as<float>(c.frhs), as<float>(c.p_jump));
+    else hipLaunchKernelGGL(k_add<double>, grd, blk, 0, c.stream, c.g, c.frhs, (const double*)c.p_jump);
     HIP_OK(hipGetLastError());
     return ICLS_OK;
 }

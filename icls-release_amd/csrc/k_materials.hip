@@ -1,6 +1,9 @@
 // k_materials.hip -- update_materials (Heaviside, viscosity, density, volume), the
 // mass-correction "inflation" sums and velocities, scalar bookkeeping kernels and the
 // generic interior sums.
+//
+// T is the element type of the field arrays.  Every sum (volumes, interface area) is
+// accumulated in double whatever T is.
 #include "icls_internal.hpp"
 #include "icls_reduce.hpp"
 #include "icls_stencil.hpp"
@@ -23,9 +26,10 @@ __device__ __forceinline__ int block_linear() { return blockIdx.x + gridDim.x * 
 //   H_l  = Heaviside('0','trig',l,1.5 dz),  H = min_l H_l   (jiemian and h_rho)
 //   miu  = miu1 H + miu2 (1 - H),  rho = rho1 H + rho2 (1 - H)
 // and the partial interior sums of every H_l for volume(l) (misc.f90:128-134).
-__global__ void k_materials(Grid g, int lmax, const double* const* __restrict__ ls, double eps, double pi, double miu1,
-                            double miu2, double rho1, double rho2, double* __restrict__ miu, double* __restrict__ rho,
-                            double* __restrict__ jiemian, double* __restrict__ part, int nblocks) {
+template <class T>
+__global__ void k_materials(Grid g, int lmax, const T* const* __restrict__ ls, T eps, T pi, T miu1, T miu2, T rho1,
+                            T rho2, T* __restrict__ miu, T* __restrict__ rho, T* __restrict__ jiemian,
+                            double* __restrict__ part, int nblocks) {
     __shared__ double smem[16];
     int i0 = blockIdx.x * BX + threadIdx.x, j0 = blockIdx.y * BY + threadIdx.y;
     bool valid = i0 < g.nx && j0 < g.ny;
@@ -36,17 +40,17 @@ __global__ void k_materials(Grid g, int lmax, const double* const* __restrict__ 
             int k = blockIdx.z * KCH + kk;   // k = 0 .. nz+1
             if (!valid || k > g.nz + 1) continue;
             long q = gidx(g, i0, j0, k);
-            double H = 0.;
+            T H = T(0.);
             if (l0 == 0) {
-                H = heaviside_trig<double>(ls[0][q], eps, pi);
-                for (int l = 1; l < lmax; l++) H = fmin(H, heaviside_trig<double>(ls[l][q], eps, pi));
-                miu[q] = miu1 * H + miu2 * (1. - H);
-                rho[q] = rho1 * H + rho2 * (1. - H);
+                H = heaviside_trig<T>(ls[0][q], eps, pi);
+                for (int l = 1; l < lmax; l++) H = fmin(H, heaviside_trig<T>(ls[l][q], eps, pi));
+                miu[q] = miu1 * H + miu2 * (T(1.) - H);
+                rho[q] = rho1 * H + rho2 * (T(1.) - H);
                 if (jiemian) jiemian[q] = H;
             }
             if (k >= 1 && k <= g.nz) {
                 for (int t = 0; t < 4 && l0 + t < lmax; t++)
-                    acc[t] += (l0 == 0 && t == 0 && lmax == 1) ? H : heaviside_trig<double>(ls[l0 + t][q], eps, pi);
+                    acc[t] += (double)((l0 == 0 && t == 0 && lmax == 1) ? H : heaviside_trig<T>(ls[l0 + t][q], eps, pi));
             }
         }
         for (int t = 0; t < 4 && l0 + t < lmax; t++) {
@@ -75,60 +79,75 @@ int launch_materials(Ctx& c) {
     dim3 grd = red_grid(c.g, 0, c.g.nz + 1);
     int nb = red_nblocks(grd);
     if ((long)nb * c.lmax > c.npart_cap) { set_error("reduction scratch too small"); return ICLS_ERR_STATE; }
-    hipLaunchKernelGGL(k_materials, grd, blk, 0, c.stream, c.g, c.lmax, (const double* const*)c.dls, 1.5 * c.g.dz, c.pi,
-                       c.P.miu1, c.P.miu2, c.P.rho1, c.P.rho2, c.miu, c.rho, (double*)nullptr, c.dpart, nb);
+    const icls_params& P = c.P;
+    if (c.fp32)
+        hipLaunchKernelGGL(k_materials<float>, grd, blk, 0, c.stream, c.g, c.lmax, (const float* const*)c.dls,
+                           (float)(1.5 * c.g.dz), (float)c.pi, (float)P.miu1, (float)P.miu2, (float)P.rho1,
+                           (float)P.rho2, as<float>(c.miu), as<float>(c.rho), (float*)nullptr, c.dpart, nb);
+    else
+        hipLaunchKernelGGL(k_materials<double>, grd, blk, 0, c.stream, c.g, c.lmax, (const double* const*)c.dls,
+                           1.5 * c.g.dz, c.pi, P.miu1, P.miu2, P.rho1, P.rho2, c.miu, c.rho, (double*)nullptr, c.dpart,
+                           nb);
     HIP_OK(hipGetLastError());
     double* sums = c.dscal + DS_COUNT_FIXED + 2 * c.lmax;   // lmax scratch sums after volume/init_volume
     for (int l = 0; l < c.lmax; l++)
         hipLaunchKernelGGL(k_reduce_final<OpSum>, dim3(1), dim3(256), 0, c.stream, c.dpart + (long)l * nb, nb, 0.,
                            sums + l);
     double ncell = 1. * c.g.nx * c.g.ny * c.g.nz;
-    hipLaunchKernelGGL(k_volume_final, dim3(1), dim3(64), 0, c.stream, c.lmax, ncell, c.P.rho1, c.P.rho2,
+    hipLaunchKernelGGL(k_volume_final, dim3(1), dim3(64), 0, c.stream, c.lmax, ncell, P.rho1, P.rho2,
                        (const double*)sums, c.dscal);
     HIP_OK(hipGetLastError());
     return ICLS_OK;
 }
 
-// jiemian = min_l H_l over the C-halo range (misc.f90:115-120), on request only
-__global__ void k_jiemian(Grid g, int lmax, const double* const* __restrict__ ls, double eps, double pi,
-                          double* __restrict__ out) {
+// jiemian = min_l H_l over the C-halo range (misc.f90:115-120), on request only,
+// written as double
+template <class T>
+__global__ void k_jiemian(Grid g, int lmax, const T* const* __restrict__ ls, T eps, T pi, double* __restrict__ out) {
     int i0 = blockIdx.x * BX + threadIdx.x, j0 = blockIdx.y * BY + threadIdx.y, k = blockIdx.z;
     if (i0 >= g.nx || j0 >= g.ny) return;
     long q = gidx(g, i0, j0, k);
-    double H = heaviside_trig<double>(ls[0][q], eps, pi);
-    for (int l = 1; l < lmax; l++) H = fmin(H, heaviside_trig<double>(ls[l][q], eps, pi));
-    out[q] = H;
+    T H = heaviside_trig<T>(ls[0][q], eps, pi);
+    for (int l = 1; l < lmax; l++) H = fmin(H, heaviside_trig<T>(ls[l][q], eps, pi));
+    out[q] = (double)H;
 }
 
 int launch_jiemian(Ctx& c, double* out) {
     dim3 blk(BX, BY, 1);
-    hipLaunchKernelGGL(k_jiemian, grid3(c.g, c.g.nz + 2, blk), blk, 0, c.stream, c.g, c.lmax, (const double* const*)c.dls,
-                       1.5 * c.g.dz, c.pi, out);
+    dim3 grd = grid3(c.g, c.g.nz + 2, blk);
+    if (c.fp32)
+        hipLaunchKernelGGL(k_jiemian<float>, grd, blk, 0, c.stream, c.g, c.lmax, (const float* const*)c.dls,
+                           (float)(1.5 * c.g.dz), (float)c.pi, out);
+    else
+        hipLaunchKernelGGL(k_jiemian<double>, grd, blk, 0, c.stream, c.g, c.lmax, (const double* const*)c.dls,
+                           1.5 * c.g.dz, c.pi, out);
     HIP_OK(hipGetLastError());
     return ICLS_OK;
 }
 
 // ---------------------------------------------------------------- interior sums
-template <int KIND>
-__global__ void k_sum_interior(Grid g, const double* __restrict__ f, double* __restrict__ part) {
+template <class T>
+__global__ void k_sum_interior(Grid g, const T* __restrict__ f, double* __restrict__ part) {
     __shared__ double smem[16];
     int i0 = blockIdx.x * BX + threadIdx.x, j0 = blockIdx.y * BY + threadIdx.y;
     double acc = 0.;
     if (i0 < g.nx && j0 < g.ny)
         for (int kk = 0; kk < KCH; kk++) {
             int k = 1 + blockIdx.z * KCH + kk;
-            if (k <= g.nz) acc += f[gidx(g, i0, j0, k)];
+            if (k <= g.nz) acc += (double)f[gidx(g, i0, j0, k)];
         }
     double r = block_reduce(acc, OpSum(), smem);
     if (threadIdx.x == 0 && threadIdx.y == 0) part[block_linear()] = r;
 }
 
-// dscal[slot] = sum over the interior of f
+// dscal[slot] = sum over the interior of the field-class array f
 int launch_sum_interior(Ctx& c, const double* f, int slot) {
+    ProfScope ps(c, "k_sum_interior");
     dim3 blk(BX, BY, 1);
     dim3 grd = red_grid(c.g, 1, c.g.nz);
     int nb = red_nblocks(grd);
-    hipLaunchKernelGGL(k_sum_interior<0>, grd, blk, 0, c.stream, c.g, f, c.dpart);
+    if (c.fp32) hipLaunchKernelGGL(k_sum_interior<float>, grd, blk, 0, c.stream, c.g, as<float>(f), c.dpart);
+    else hipLaunchKernelGGL(k_sum_interior<double>, grd, blk, 0, c.stream, c.g, f, c.dpart);
     hipLaunchKernelGGL(k_reduce_final<OpSum>, dim3(1), dim3(256), 0, c.stream, (const double*)c.dpart, nb, 0.,
                        c.dscal + slot);
     HIP_OK(hipGetLastError());
@@ -148,9 +167,10 @@ __global__ void k_infl_flag(int l, int lmax, int due, double* dscal) {
 
 // partial sums of delta*kappa*|grad phi| (levelset.f90:499) and of the Heaviside
 // (levelset.f90:505) over the interior; dheav over the C-halo range
-__global__ void k_infl_sums(Grid g, const double* __restrict__ flag, const double* __restrict__ phi,
-                            const double* __restrict__ kappa, const double* __restrict__ nm, double s1, double pi,
-                            double* __restrict__ dheav, double* __restrict__ part, int nblocks) {
+template <class T>
+__global__ void k_infl_sums(Grid g, const double* __restrict__ flag, const T* __restrict__ phi,
+                            const T* __restrict__ kappa, const T* __restrict__ nm, T s1, T pi, T* __restrict__ dheav,
+                            double* __restrict__ part, int nblocks) {
     if (flag[0] == 0.) return;
     __shared__ double smem[16];
     int i0 = blockIdx.x * BX + threadIdx.x, j0 = blockIdx.y * BY + threadIdx.y;
@@ -160,12 +180,12 @@ __global__ void k_infl_sums(Grid g, const double* __restrict__ flag, const doubl
         int k = blockIdx.z * KCH + kk;
         if (!valid || k > g.nz + 1) continue;
         long q = gidx(g, i0, j0, k);
-        double h = heaviside_trig<double>(phi[q], s1, pi);
+        T h = heaviside_trig<T>(phi[q], s1, pi);
         dheav[q] = h;
         if (k >= 1 && k <= g.nz) {
-            double del = dirac_trig<double>(phi[q], s1, pi);
-            sa += del * kappa[q] * nm[q];
-            sv += h;
+            T del = dirac_trig<T>(phi[q], s1, pi);
+            sa += (double)(del * kappa[q] * nm[q]);
+            sv += (double)h;
         }
     }
     double ra = block_reduce(sa, OpSum(), smem);
@@ -188,32 +208,33 @@ __global__ void k_infl_scalars(int l, int lmax, double ncell, double dx, double 
 }
 
 // u_l = -temp*(dheav(i+1)-dheav(i))/dx etc. (levelset.f90:527-535) after boundp(dheav)
-__global__ void k_infl_face_vel(Grid g, const double* __restrict__ dscal, const double* __restrict__ dheav,
-                                double* __restrict__ ul, double* __restrict__ vl, double* __restrict__ wl) {
+template <class T>
+__global__ void k_infl_face_vel(Grid g, const double* __restrict__ dscal, const T* __restrict__ dheav,
+                                T* __restrict__ ul, T* __restrict__ vl, T* __restrict__ wl) {
     if (dscal[DS_INFL_FLAG] == 0.) return;
     int i0 = blockIdx.x * BX + threadIdx.x, j0 = blockIdx.y * BY + threadIdx.y, k = blockIdx.z + 1;
     if (i0 >= g.nx || j0 >= g.ny) return;
-    const double temp = dscal[DS_INFL_TEMP];
+    const T temp = (T)dscal[DS_INFL_TEMP];
     long q = gidx(g, i0, j0, k);
-    double h = dheav[q];
-    ul[q] = -temp * (dheav[gidx(g, wrap(i0 + 1, g.nx), j0, k)] - h) / g.dx;
-    vl[q] = -temp * (dheav[gidx(g, i0, wrap(j0 + 1, g.ny), k)] - h) / g.dy;
-    wl[q] = -temp * (dheav[gidx(g, i0, j0, k + 1)] - h) / g.dz;
+    T h = dheav[q];
+    ul[q] = -temp * (dheav[gidx(g, wrap(i0 + 1, g.nx), j0, k)] - h) / T(g.dx);
+    vl[q] = -temp * (dheav[gidx(g, i0, wrap(j0 + 1, g.ny), k)] - h) / T(g.dy);
+    wl[q] = -temp * (dheav[gidx(g, i0, j0, k + 1)] - h) / T(g.dz);
 }
 
 // u_lc = (u_l(i-1)+u_l(i))/2*kappa (levelset.f90:541-549)
-__global__ void k_infl_center_vel(Grid g, const double* __restrict__ dscal, const double* __restrict__ kappa,
-                                  const double* __restrict__ ul, const double* __restrict__ vl,
-                                  const double* __restrict__ wl, double* __restrict__ uc, double* __restrict__ vc,
-                                  double* __restrict__ wc) {
+template <class T>
+__global__ void k_infl_center_vel(Grid g, const double* __restrict__ dscal, const T* __restrict__ kappa,
+                                  const T* __restrict__ ul, const T* __restrict__ vl, const T* __restrict__ wl,
+                                  T* __restrict__ uc, T* __restrict__ vc, T* __restrict__ wc) {
     if (dscal[DS_INFL_FLAG] == 0.) return;
     int i0 = blockIdx.x * BX + threadIdx.x, j0 = blockIdx.y * BY + threadIdx.y, k = blockIdx.z + 1;
     if (i0 >= g.nx || j0 >= g.ny) return;
     long q = gidx(g, i0, j0, k);
-    double kap = kappa[q];
-    uc[q] = (ul[gidx(g, wrap(i0 - 1, g.nx), j0, k)] + ul[q]) / 2. * kap;
-    vc[q] = (vl[gidx(g, i0, wrap(j0 - 1, g.ny), k)] + vl[q]) / 2. * kap;
-    wc[q] = (wl[gidx(g, i0, j0, k - 1)] + wl[q]) / 2. * kap;
+    T kap = kappa[q];
+    uc[q] = (ul[gidx(g, wrap(i0 - 1, g.nx), j0, k)] + ul[q]) / T(2.) * kap;
+    vc[q] = (vl[gidx(g, i0, wrap(j0 - 1, g.ny), k)] + vl[q]) / T(2.) * kap;
+    wc[q] = (wl[gidx(g, i0, j0, k - 1)] + wl[q]) / T(2.) * kap;
 }
 
 int launch_inflation_prepare(Ctx& c, int l, int istep) {
@@ -227,16 +248,15 @@ int launch_inflation_prepare(Ctx& c, int l, int istep) {
 // Everything of inflation (levelset.f90:407-549) up to the cell-centre correction
 // velocities; the caller has filled c.curv2 (kappa) and c.normal_m from the current
 // level set with get_curvature predicated on the same flag.
-
-int launch_inflation_velocity(Ctx& c, int l) {
-    ProfScope ps(c, "k_infl");
+template <class T>
+static int inflation_velocity(Ctx& c, int l) {
     const double* flag = c.dscal + DS_INFL_FLAG;
     const double s1 = 1.5 * c.g.dz;
     dim3 blk(BX, BY, 1);
     dim3 grd = red_grid(c.g, 0, c.g.nz + 1);
     int nb = red_nblocks(grd);
-    hipLaunchKernelGGL(k_infl_sums, grd, blk, 0, c.stream, c.g, flag, (const double*)c.lvset[l], (const double*)c.curv2,
-                       (const double*)c.normal_m, s1, c.pi, c.dheav, c.dpart, nb);
+    hipLaunchKernelGGL(k_infl_sums<T>, grd, blk, 0, c.stream, c.g, flag, as<T>(c.lvset[l]), as<T>(c.curv2),
+                       as<T>(c.normal_m), (T)s1, (T)c.pi, as<T>(c.dheav), c.dpart, nb);
     double* sums = c.dscal + DS_COUNT_FIXED + 2 * c.lmax;
     hipLaunchKernelGGL(k_reduce_final_pred<OpSum>, dim3(1), dim3(256), 0, c.stream, (const double*)c.dpart, nb, 0.,
                        sums, flag);
@@ -248,18 +268,23 @@ int launch_inflation_velocity(Ctx& c, int l) {
     hipLaunchKernelGGL(k_infl_scalars, dim3(1), dim3(64), 0, c.stream, l, c.lmax, ncell, c.g.dx, c.g.dy, c.g.dz, lxyz,
                        inflate_dt, s1, (const double*)sums, c.dscal);
     HIP_OK(hipGetLastError());
-    int rc = launch_bc_p_pred(c, c.dheav, flag);
+    int rc = launch_bc_p_t_pred(c, c.dheav, flag);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_infl_face_vel, grid3(c.g, c.g.nz, blk), blk, 0, c.stream, c.g, (const double*)c.dscal,
-                       (const double*)c.dheav, c.u_l, c.v_l, c.w_l);
+    hipLaunchKernelGGL(k_infl_face_vel<T>, grid3(c.g, c.g.nz, blk), blk, 0, c.stream, c.g, (const double*)c.dscal,
+                       as<T>(c.dheav), as<T>(c.u_l), as<T>(c.v_l), as<T>(c.w_l));
     HIP_OK(hipGetLastError());
     rc = launch_bc_uvw_pred(c, c.u_l, c.v_l, c.w_l, flag);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_infl_center_vel, grid3(c.g, c.g.nz, blk), blk, 0, c.stream, c.g, (const double*)c.dscal,
-                       (const double*)c.curv2, (const double*)c.u_l, (const double*)c.v_l, (const double*)c.w_l,
-                       c.ucc_l, c.vcc_l, c.wcc_l);
+    hipLaunchKernelGGL(k_infl_center_vel<T>, grid3(c.g, c.g.nz, blk), blk, 0, c.stream, c.g, (const double*)c.dscal,
+                       as<T>(c.curv2), as<T>(c.u_l), as<T>(c.v_l), as<T>(c.w_l), as<T>(c.ucc_l), as<T>(c.vcc_l),
+                       as<T>(c.wcc_l));
     HIP_OK(hipGetLastError());
     return ICLS_OK;
+}
+
+int launch_inflation_velocity(Ctx& c, int l) {
+    ProfScope ps(c, "k_infl");
+    return c.fp32 ? inflation_velocity<float>(c, l) : inflation_velocity<double>(c, l);
 }
 
 }  // namespace icls

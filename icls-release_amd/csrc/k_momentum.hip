@@ -2,6 +2,10 @@
 // split-pressure term), the velocity/pressure boundary conditions, correc, the
 // pressure rotation and normalisation, cell-centre velocities and the diagnostic
 // reductions (chkdiv, chkdt, max_vel_at_center).
+//
+// T is the element type of the field arrays; the pressures pnew, pold and p are
+// double in every mode, and pressure differences are formed in double before they
+// meet a field value.
 #include "icls_internal.hpp"
 #include "icls_reduce.hpp"
 #include "icls_stencil.hpp"
@@ -18,163 +22,249 @@ struct MomParams {
 
 // ab2 (ab2.f90:49-129) with mom_main (mom.f90:13-62), CNTR2_conv (mom.f90:67-130),
 // delta_diff (mom.f90:134-273), NS_grav (mom.f90:277-301) and momxp/yp/zp
-// (mom.f90:779-851), one interior cell per thread.  Face densities are the
-// arithmetic means of update_materials (misc.f90:151-172), formed here from rho.
-template <int RHS_ONLY, bool P2>
-__global__ void k_momentum(Grid g, MomParams m, const double* __restrict__ dscal, const double* __restrict__ u,
-                           const double* __restrict__ v, const double* __restrict__ w, const double* __restrict__ miu,
-                           const double* __restrict__ rho, double* __restrict__ duold, double* __restrict__ dvold,
-                           double* __restrict__ dwold, const double* __restrict__ pnew, const double* __restrict__ pold,
-                           const double* __restrict__ p_x, const double* __restrict__ p_y,
-                           const double* __restrict__ p_z, const double* __restrict__ p_xold,
-                           const double* __restrict__ p_yold, const double* __restrict__ p_zold,
-                           double* __restrict__ dudt, double* __restrict__ dvdt, double* __restrict__ dwdt) {
-    int i0 = blockIdx.x * BX + threadIdx.x, j0 = blockIdx.y * BY + threadIdx.y, k = blockIdx.z + 1;
-    if (i0 >= g.nx || j0 >= g.ny) return;
-    const int im = wrap(i0 - 1, g.nx), ip = wrap(i0 + 1, g.nx), jm = wrap(j0 - 1, g.ny), jp = wrap(j0 + 1, g.ny);
-    const int km = k - 1, kp = k + 1;
-    auto I = [&](int a, int b, int c) { return gidx(g, a, b, c); };
-    const Sp<double, P2> sx{g.dx, g.dxi}, sy{g.dy, g.dyi}, sz{g.dz, g.dzi};
-    const int i = i0, j = j0;
-    const long q = I(i, j, k);
+// (mom.f90:779-851).  Face densities are the arithmetic means of update_materials
+// (misc.f90:151-172), formed here from rho.
+//
+// The kernel marches in z.  A block owns a 64 x 4 tile of (i,j) columns over MZ
+// consecutive layers.  u, v, w, miu and rho -- the arrays read
+// at neighbouring cells -- pass through a ring of three LDS planes of the tile with a
+// one-cell x/y halo, so every element of them is fetched once per block; the next
+// plane is loaded into registers while the current one is computed.  Tiles are taken
+// in XCD order (xcd_tile).  x and y wrap with a mask: itot and jtot are powers of two.
+constexpr int MZ = 32;
+constexpr int TW = BX + 2, TH = BY + 2, TP = TW * TH;   // tile plane with halo
+constexpr int NLD = (TP + BX * BY - 1) / (BX * BY);      // plane elements per thread
 
-    // ---- convection (CNTR2)
-    double uuip = 0.25 * (u[I(ip, j, k)] + u[q]) * (u[I(ip, j, k)] + u[q]);
-    double uuim = 0.25 * (u[I(im, j, k)] + u[q]) * (u[I(im, j, k)] + u[q]);
-    double uvjp = 0.25 * (u[I(i, jp, k)] + u[q]) * (v[I(ip, j, k)] + v[q]);
-    double uvjm = 0.25 * (u[I(i, jm, k)] + u[q]) * (v[I(ip, jm, k)] + v[I(i, jm, k)]);
-    double uwkp = 0.25 * (u[I(i, j, kp)] + u[q]) * (w[I(ip, j, k)] + w[q]);
-    double uwkm = 0.25 * (u[I(i, j, km)] + u[q]) * (w[I(ip, j, km)] + w[I(i, j, km)]);
-    double RCu = sx.div(-uuip + uuim) + sy.div(-uvjp + uvjm) + sz.div(-uwkp + uwkm);
-    double uvip = 0.25 * (u[q] + u[I(i, jp, k)]) * (v[q] + v[I(ip, j, k)]);
-    double uvim = 0.25 * (u[I(im, j, k)] + u[I(im, jp, k)]) * (v[q] + v[I(im, j, k)]);
-    double vvjp = 0.25 * (v[q] + v[I(i, jp, k)]) * (v[q] + v[I(i, jp, k)]);
-    double vvjm = 0.25 * (v[q] + v[I(i, jm, k)]) * (v[q] + v[I(i, jm, k)]);
-    double wvkp = 0.25 * (w[q] + w[I(i, jp, k)]) * (v[I(i, j, kp)] + v[q]);
-    double wvkm = 0.25 * (w[I(i, j, km)] + w[I(i, jp, km)]) * (v[I(i, j, km)] + v[q]);
-    double RCv = sx.div(-uvip + uvim) + sy.div(-vvjp + vvjm) + sz.div(-wvkp + wvkm);
-    double uwip = 0.25 * (w[q] + w[I(ip, j, k)]) * (u[q] + u[I(i, j, kp)]);
-    double uwim = 0.25 * (w[q] + w[I(im, j, k)]) * (u[I(im, j, k)] + u[I(im, j, kp)]);
-    double vwjp = 0.25 * (w[q] + w[I(i, jp, k)]) * (v[q] + v[I(i, j, kp)]);
-    double vwjm = 0.25 * (w[q] + w[I(i, jm, k)]) * (v[I(i, jm, k)] + v[I(i, jm, kp)]);
-    double wwkp = 0.25 * (w[q] + w[I(i, j, kp)]) * (w[q] + w[I(i, j, kp)]);
-    double wwkm = 0.25 * (w[q] + w[I(i, j, km)]) * (w[q] + w[I(i, j, km)]);
-    double RCw = sx.div(-uwip + uwim) + sy.div(-vwjp + vwjm) + sz.div(-wwkp + wwkm);
-
-    // ---- face densities
-    double rq = rho[q];
-    double rho_u, rho_v, rho_w;
-    if (m.rho_equal) { rho_u = m.rho1; rho_v = m.rho1; rho_w = m.rho1; }
-    else {
-        rho_u = (rq + rho[I(ip, j, k)]) / 2.;
-        rho_v = (rq + rho[I(i, jp, k)]) / 2.;
-        rho_w = (rq + rho[I(i, j, kp)]) / 2.;
+template <class T, int RHS_ONLY, bool P2>
+__global__ void __launch_bounds__(BX * BY)
+k_momentum_zm(Grid g, MomParams m, const double* __restrict__ dscal, const T* __restrict__ u, const T* __restrict__ v,
+              const T* __restrict__ w, const T* __restrict__ miu, const T* __restrict__ rho, T* __restrict__ duold,
+              T* __restrict__ dvold, T* __restrict__ dwold, const double* __restrict__ pnew,
+              const double* __restrict__ pold, const T* __restrict__ p_x, const T* __restrict__ p_y,
+              const T* __restrict__ p_z, const T* __restrict__ p_xold, const T* __restrict__ p_yold,
+              const T* __restrict__ p_zold, T* __restrict__ dudt, T* __restrict__ dvdt, T* __restrict__ dwdt) {
+    __shared__ T ring[5][3][TP];   // u, v, w, miu, rho x three planes
+    const int ntx = (g.nx + BX - 1) / BX, nty = (g.ny + BY - 1) / BY, ntz = (g.nz + MZ - 1) / MZ;
+    const int tile = xcd_tile(blockIdx.x, ntx * nty * ntz);
+    const int bx = tile % ntx, by = (tile / ntx) % nty, bz = tile / (ntx * nty);
+    const int x0 = bx * BX, y0 = by * BY, kb = 1 + bz * MZ;
+    const int ke = kb + MZ - 1 < g.nz ? kb + MZ - 1 : g.nz;
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = tx + BX * ty;
+    const int xm = g.nx - 1, ym = g.ny - 1;
+    const T* src[5] = {u, v, w, miu, rho};
+    // plane element e of the tile -> global offset within a z layer
+    long poff[NLD];
+#pragma unroll
+    for (int r = 0; r < NLD; r++) {
+        int e = tid + r * BX * BY;
+        if (e >= TP) e = TP - 1;
+        const int lx = e % TW - 1, ly = e / TW - 1;
+        poff[r] = (long)((x0 + lx) & xm) + g.sy * (long)((y0 + ly) & ym);
     }
+    T nxt[5][NLD];
+    auto fetch = [&](int k) {
+        const long base = g.sz * (long)(k + 2);
+#pragma unroll
+        for (int f = 0; f < 5; f++)
+#pragma unroll
+            for (int r = 0; r < NLD; r++) nxt[f][r] = src[f][base + poff[r]];
+    };
+    auto stash = [&](int slot) {
+#pragma unroll
+        for (int f = 0; f < 5; f++)
+#pragma unroll
+            for (int r = 0; r < NLD; r++) {
+                const int e = tid + r * BX * BY;
+                if (e < TP) ring[f][slot][e] = nxt[f][r];
+            }
+    };
+    auto slot_of = [&](int k) { return (k - kb + 3) % 3; };   // k - 1 .. k + 1 -> three slots
+    fetch(kb - 1);
+    stash(slot_of(kb - 1));
+    fetch(kb);
+    stash(slot_of(kb));
+    fetch(kb + 1);
 
-    // ---- diffusion (delta formulation)
-    const double mq = miu[q];
-    double RDu, RDv, RDw;
-    {
-        double dudxp = sx.div(u[I(ip, j, k)] - u[q]);
-        double dudxm = sx.div(u[q] - u[I(im, j, k)]);
-        double dudyp = sy.div(u[I(i, jp, k)] - u[q]);
-        double dudym = sy.div(u[q] - u[I(i, jm, k)]);
-        double dvdxp = sx.div(v[I(ip, j, k)] - v[q]);
-        double dvdxm = sx.div(v[I(ip, jm, k)] - v[I(i, jm, k)]);
-        double dudzp = sz.div(u[I(i, j, kp)] - u[q]);
-        double dudzm = sz.div(u[q] - u[I(i, j, km)]);
-        double dwdxp = sx.div(w[I(ip, j, k)] - w[q]);
-        double dwdxm = sx.div(w[I(ip, j, km)] - w[I(i, j, km)]);
-        double ma1 = 0.25 * (mq + miu[I(i, jm, k)] + miu[I(ip, jm, k)] + miu[I(ip, j, k)]);
-        double ma2 = 0.25 * (mq + miu[I(i, jp, k)] + miu[I(ip, jp, k)] + miu[I(ip, j, k)]);
-        double ma3 = 0.25 * (mq + miu[I(i, j, km)] + miu[I(ip, j, km)] + miu[I(ip, j, k)]);
-        double ma4 = 0.25 * (mq + miu[I(i, j, kp)] + miu[I(ip, j, kp)] + miu[I(ip, j, k)]);
-        double RD1 = sx.div(2. * (dudxp * miu[I(ip, j, k)] - dudxm * mq));
-        double RD2 = sy.div(dudyp * ma2 - dudym * ma1);
-        double RD3 = sy.div(dvdxp * ma2 - dvdxm * ma1);
-        double RD4 = sz.div(dudzp * ma4 - dudzm * ma3);
-        double RD5 = sz.div(dwdxp * ma4 - dwdxm * ma3);
-        RDu = (RD1 + RD2 + RD3 + RD4 + RD5) / rho_u;
-    }
-    {
-        double dvdxp = sx.div(v[I(ip, j, k)] - v[q]);
-        double dvdxm = sx.div(v[q] - v[I(im, j, k)]);
-        double dudyp = sy.div(u[I(i, jp, k)] - u[q]);
-        double dudym = sy.div(u[I(im, jp, k)] - u[I(im, j, k)]);
-        double dvdyp = sy.div(v[I(i, jp, k)] - v[q]);
-        double dvdym = sy.div(v[q] - v[I(i, jm, k)]);
-        double dvdzp = sz.div(v[I(i, j, kp)] - v[q]);
-        double dvdzm = sz.div(v[q] - v[I(i, j, km)]);
-        double dwdyp = sy.div(w[I(i, jp, k)] - w[q]);
-        double dwdym = sy.div(w[I(i, jp, km)] - w[I(i, j, km)]);
-        double ma2 = 0.25 * (mq + miu[I(i, jp, k)] + miu[I(ip, jp, k)] + miu[I(ip, j, k)]);
-        double ma5 = 0.25 * (mq + miu[I(i, jp, k)] + miu[I(im, jp, k)] + miu[I(im, j, k)]);
-        double ma6 = 0.25 * (mq + miu[I(i, j, km)] + miu[I(i, jp, km)] + miu[I(i, jp, k)]);
-        double ma7 = 0.25 * (mq + miu[I(i, j, kp)] + miu[I(i, jp, kp)] + miu[I(i, jp, k)]);
-        double RD1 = sx.div(dvdxp * ma2 - dvdxm * ma5);
-        double RD2 = sx.div(dudyp * ma2 - dudym * ma5);
-        double RD3 = sy.div(2. * (dvdyp * miu[I(i, jp, k)] - dvdym * mq));
-        double RD4 = sz.div(dvdzp * ma7 - dvdzm * ma6);
-        double RD5 = sz.div(dwdyp * ma7 - dwdym * ma6);
-        RDv = (RD1 + RD2 + RD3 + RD4 + RD5) / rho_v;
-    }
-    {
-        double dwdxp = sx.div(w[I(ip, j, k)] - w[q]);
-        double dwdxm = sx.div(w[q] - w[I(im, j, k)]);
-        double dudzp = sz.div(u[I(i, j, kp)] - u[q]);
-        double dudzm = sz.div(u[I(im, j, kp)] - u[I(im, j, k)]);
-        double dwdyp = sy.div(w[I(i, jp, k)] - w[q]);
-        double dwdym = sy.div(w[q] - w[I(i, jm, k)]);
-        double dvdzp = sz.div(v[I(i, j, kp)] - v[q]);
-        double dvdzm = sz.div(v[I(i, jm, kp)] - v[I(i, jm, k)]);
-        double dwdzp = sz.div(w[I(i, j, kp)] - w[q]);
-        double dwdzm = sz.div(w[q] - w[I(i, j, km)]);
-        double ma4 = 0.25 * (mq + miu[I(i, j, kp)] + miu[I(ip, j, kp)] + miu[I(ip, j, k)]);
-        double ma8 = 0.25 * (mq + miu[I(i, j, kp)] + miu[I(im, j, kp)] + miu[I(im, j, k)]);
-        double ma7 = 0.25 * (mq + miu[I(i, j, kp)] + miu[I(i, jp, kp)] + miu[I(i, jp, k)]);
-        double ma9 = 0.25 * (mq + miu[I(i, j, kp)] + miu[I(i, jm, kp)] + miu[I(i, jm, k)]);
-        double RD1 = sx.div(dwdxp * ma4 - dwdxm * ma8);
-        double RD2 = sx.div(dudzp * ma4 - dudzm * ma8);
-        double RD3 = sy.div(dwdyp * ma7 - dwdym * ma9);
-        double RD4 = sy.div(dvdzp * ma7 - dvdzm * ma9);
-        double RD5 = sz.div(2. * (dwdzp * miu[I(i, j, kp)] - dwdzm * mq));
-        RDw = (RD1 + RD2 + RD3 + RD4 + RD5) / rho_w;
-    }
+    const int i = (x0 + tx) & xm, j = (y0 + ty) & ym;
+    const bool valid = x0 + tx < g.nx && y0 + ty < g.ny;
+    const int ip = (i + 1) & xm, jp = (j + 1) & ym;
+    const Sp<T, P2> sx{T(g.dx), T(g.dxi)}, sy{T(g.dy), T(g.dyi)}, sz{T(g.dz), T(g.dzi)};
+    const T qt = T(0.25), two = T(2.), one = T(1.);
+    const T rho_avr = T(dscal[DS_RHO_AVR]);
+    const T iFr = T(m.iFr), visc = T(m.visc), dt = T(m.dt), c15 = T(1.5), c05 = T(0.5), irho0 = T(m.inv_rho0);
+    const int c0 = (ty + 1) * TW + tx + 1;   // own cell in the tile plane
 
-    // ---- buoyancy and RHS
-    const double rho_avr = dscal[DS_RHO_AVR];
-    double RGu = m.iFr * (1. - rho_avr / rho_u) * m.g0;
-    double RGv = m.iFr * (1. - rho_avr / rho_v) * m.g1;
-    double RGw = m.iFr * (1. - rho_avr / rho_w) * m.g2;
-    double dun = RCu + RDu * m.visc + RGu;
-    double dvn = RCv + RDv * m.visc + RGv;
-    double dwn = RCw + RDw * m.visc + RGw;
+    for (int k = kb; k <= ke; k++) {
+        stash(slot_of(k + 1));
+        __syncthreads();
+        if (k + 1 <= ke) fetch(k + 2);
+        const T* U[3] = {ring[0][slot_of(k - 1)], ring[0][slot_of(k)], ring[0][slot_of(k + 1)]};
+        const T* V[3] = {ring[1][slot_of(k - 1)], ring[1][slot_of(k)], ring[1][slot_of(k + 1)]};
+        const T* W[3] = {ring[2][slot_of(k - 1)], ring[2][slot_of(k)], ring[2][slot_of(k + 1)]};
+        const T* MU[3] = {ring[3][slot_of(k - 1)], ring[3][slot_of(k)], ring[3][slot_of(k + 1)]};
+        const T* RH = ring[4][slot_of(k)];
+        const T* RHP = ring[4][slot_of(k + 1)];
+        // a(dx, dy, dz) at offset (dx, dy) in plane k + dz
+#define AT(A, dx_, dy_, dz_) A[(dz_) + 1][c0 + (dy_) * TW + (dx_)]
+        if (valid) {
+            const long q = gidx(g, i, j, k);
+            const T uq = AT(U, 0, 0, 0), vq = AT(V, 0, 0, 0), wq = AT(W, 0, 0, 0);
+            // ---- convection (CNTR2)
+            T uuip = qt * (AT(U, 1, 0, 0) + uq) * (AT(U, 1, 0, 0) + uq);
+            T uuim = qt * (AT(U, -1, 0, 0) + uq) * (AT(U, -1, 0, 0) + uq);
+            T uvjp = qt * (AT(U, 0, 1, 0) + uq) * (AT(V, 1, 0, 0) + vq);
+            T uvjm = qt * (AT(U, 0, -1, 0) + uq) * (AT(V, 1, -1, 0) + AT(V, 0, -1, 0));
+            T uwkp = qt * (AT(U, 0, 0, 1) + uq) * (AT(W, 1, 0, 0) + wq);
+            T uwkm = qt * (AT(U, 0, 0, -1) + uq) * (AT(W, 1, 0, -1) + AT(W, 0, 0, -1));
+            T RCu = sx.div(-uuip + uuim) + sy.div(-uvjp + uvjm) + sz.div(-uwkp + uwkm);
+            T uvip = qt * (uq + AT(U, 0, 1, 0)) * (vq + AT(V, 1, 0, 0));
+            T uvim = qt * (AT(U, -1, 0, 0) + AT(U, -1, 1, 0)) * (vq + AT(V, -1, 0, 0));
+            T vvjp = qt * (vq + AT(V, 0, 1, 0)) * (vq + AT(V, 0, 1, 0));
+            T vvjm = qt * (vq + AT(V, 0, -1, 0)) * (vq + AT(V, 0, -1, 0));
+            T wvkp = qt * (wq + AT(W, 0, 1, 0)) * (AT(V, 0, 0, 1) + vq);
+            T wvkm = qt * (AT(W, 0, 0, -1) + AT(W, 0, 1, -1)) * (AT(V, 0, 0, -1) + vq);
+            T RCv = sx.div(-uvip + uvim) + sy.div(-vvjp + vvjm) + sz.div(-wvkp + wvkm);
+            T uwip = qt * (wq + AT(W, 1, 0, 0)) * (uq + AT(U, 0, 0, 1));
+            T uwim = qt * (wq + AT(W, -1, 0, 0)) * (AT(U, -1, 0, 0) + AT(U, -1, 0, 1));
+            T vwjp = qt * (wq + AT(W, 0, 1, 0)) * (vq + AT(V, 0, 0, 1));
+            T vwjm = qt * (wq + AT(W, 0, -1, 0)) * (AT(V, 0, -1, 0) + AT(V, 0, -1, 1));
+            T wwkp = qt * (wq + AT(W, 0, 0, 1)) * (wq + AT(W, 0, 0, 1));
+            T wwkm = qt * (wq + AT(W, 0, 0, -1)) * (wq + AT(W, 0, 0, -1));
+            T RCw = sx.div(-uwip + uwim) + sy.div(-vwjp + vwjm) + sz.div(-wwkp + wwkm);
 
-    if (RHS_ONLY) {   // init.f90:90, mom_main into duold
-        duold[q] = dun; dvold[q] = dvn; dwold[q] = dwn;
-        return;
+            // ---- face densities
+            const T rq = RH[c0];
+            T rho_u, rho_v, rho_w;
+            if (m.rho_equal) { rho_u = T(m.rho1); rho_v = T(m.rho1); rho_w = T(m.rho1); }
+            else {
+                rho_u = (rq + RH[c0 + 1]) / two;
+                rho_v = (rq + RH[c0 + TW]) / two;
+                rho_w = (rq + RHP[c0]) / two;
+            }
+
+            // ---- diffusion (delta formulation)
+            const T mq = AT(MU, 0, 0, 0);
+            T RDu, RDv, RDw;
+            {
+                T dudxp = sx.div(AT(U, 1, 0, 0) - uq);
+                T dudxm = sx.div(uq - AT(U, -1, 0, 0));
+                T dudyp = sy.div(AT(U, 0, 1, 0) - uq);
+                T dudym = sy.div(uq - AT(U, 0, -1, 0));
+                T dvdxp = sx.div(AT(V, 1, 0, 0) - vq);
+                T dvdxm = sx.div(AT(V, 1, -1, 0) - AT(V, 0, -1, 0));
+                T dudzp = sz.div(AT(U, 0, 0, 1) - uq);
+                T dudzm = sz.div(uq - AT(U, 0, 0, -1));
+                T dwdxp = sx.div(AT(W, 1, 0, 0) - wq);
+                T dwdxm = sx.div(AT(W, 1, 0, -1) - AT(W, 0, 0, -1));
+                T ma1 = qt * (mq + AT(MU, 0, -1, 0) + AT(MU, 1, -1, 0) + AT(MU, 1, 0, 0));
+                T ma2 = qt * (mq + AT(MU, 0, 1, 0) + AT(MU, 1, 1, 0) + AT(MU, 1, 0, 0));
+                T ma3 = qt * (mq + AT(MU, 0, 0, -1) + AT(MU, 1, 0, -1) + AT(MU, 1, 0, 0));
+                T ma4 = qt * (mq + AT(MU, 0, 0, 1) + AT(MU, 1, 0, 1) + AT(MU, 1, 0, 0));
+                T RD1 = sx.div(two * (dudxp * AT(MU, 1, 0, 0) - dudxm * mq));
+                T RD2 = sy.div(dudyp * ma2 - dudym * ma1);
+                T RD3 = sy.div(dvdxp * ma2 - dvdxm * ma1);
+                T RD4 = sz.div(dudzp * ma4 - dudzm * ma3);
+                T RD5 = sz.div(dwdxp * ma4 - dwdxm * ma3);
+                RDu = (RD1 + RD2 + RD3 + RD4 + RD5) / rho_u;
+            }
+            {
+                T dvdxp = sx.div(AT(V, 1, 0, 0) - vq);
+                T dvdxm = sx.div(vq - AT(V, -1, 0, 0));
+                T dudyp = sy.div(AT(U, 0, 1, 0) - uq);
+                T dudym = sy.div(AT(U, -1, 1, 0) - AT(U, -1, 0, 0));
+                T dvdyp = sy.div(AT(V, 0, 1, 0) - vq);
+                T dvdym = sy.div(vq - AT(V, 0, -1, 0));
+                T dvdzp = sz.div(AT(V, 0, 0, 1) - vq);
+                T dvdzm = sz.div(vq - AT(V, 0, 0, -1));
+                T dwdyp = sy.div(AT(W, 0, 1, 0) - wq);
+                T dwdym = sy.div(AT(W, 0, 1, -1) - AT(W, 0, 0, -1));
+                T ma2 = qt * (mq + AT(MU, 0, 1, 0) + AT(MU, 1, 1, 0) + AT(MU, 1, 0, 0));
+                T ma5 = qt * (mq + AT(MU, 0, 1, 0) + AT(MU, -1, 1, 0) + AT(MU, -1, 0, 0));
+                T ma6 = qt * (mq + AT(MU, 0, 0, -1) + AT(MU, 0, 1, -1) + AT(MU, 0, 1, 0));
+                T ma7 = qt * (mq + AT(MU, 0, 0, 1) + AT(MU, 0, 1, 1) + AT(MU, 0, 1, 0));
+                T RD1 = sx.div(dvdxp * ma2 - dvdxm * ma5);
+                T RD2 = sx.div(dudyp * ma2 - dudym * ma5);
+                T RD3 = sy.div(two * (dvdyp * AT(MU, 0, 1, 0) - dvdym * mq));
+                T RD4 = sz.div(dvdzp * ma7 - dvdzm * ma6);
+                T RD5 = sz.div(dwdyp * ma7 - dwdym * ma6);
+                RDv = (RD1 + RD2 + RD3 + RD4 + RD5) / rho_v;
+            }
+            {
+                T dwdxp = sx.div(AT(W, 1, 0, 0) - wq);
+                T dwdxm = sx.div(wq - AT(W, -1, 0, 0));
+                T dudzp = sz.div(AT(U, 0, 0, 1) - uq);
+                T dudzm = sz.div(AT(U, -1, 0, 1) - AT(U, -1, 0, 0));
+                T dwdyp = sy.div(AT(W, 0, 1, 0) - wq);
+                T dwdym = sy.div(wq - AT(W, 0, -1, 0));
+                T dvdzp = sz.div(AT(V, 0, 0, 1) - vq);
+                T dvdzm = sz.div(AT(V, 0, -1, 1) - AT(V, 0, -1, 0));
+                T dwdzp = sz.div(AT(W, 0, 0, 1) - wq);
+                T dwdzm = sz.div(wq - AT(W, 0, 0, -1));
+                T ma4 = qt * (mq + AT(MU, 0, 0, 1) + AT(MU, 1, 0, 1) + AT(MU, 1, 0, 0));
+                T ma8 = qt * (mq + AT(MU, 0, 0, 1) + AT(MU, -1, 0, 1) + AT(MU, -1, 0, 0));
+                T ma7 = qt * (mq + AT(MU, 0, 0, 1) + AT(MU, 0, 1, 1) + AT(MU, 0, 1, 0));
+                T ma9 = qt * (mq + AT(MU, 0, 0, 1) + AT(MU, 0, -1, 1) + AT(MU, 0, -1, 0));
+                T RD1 = sx.div(dwdxp * ma4 - dwdxm * ma8);
+                T RD2 = sx.div(dudzp * ma4 - dudzm * ma8);
+                T RD3 = sy.div(dwdyp * ma7 - dwdym * ma9);
+                T RD4 = sy.div(dvdzp * ma7 - dvdzm * ma9);
+                T RD5 = sz.div(two * (dwdzp * AT(MU, 0, 0, 1) - dwdzm * mq));
+                RDw = (RD1 + RD2 + RD3 + RD4 + RD5) / rho_w;
+            }
+
+            // ---- buoyancy and RHS
+            T RGu = iFr * (one - rho_avr / rho_u) * T(m.g0);
+            T RGv = iFr * (one - rho_avr / rho_v) * T(m.g1);
+            T RGw = iFr * (one - rho_avr / rho_w) * T(m.g2);
+            T dun = RCu + RDu * visc + RGu;
+            T dvn = RCv + RDv * visc + RGv;
+            T dwn = RCw + RDw * visc + RGw;
+            if (RHS_ONLY) {
+                duold[q] = dun; dvold[q] = dvn; dwold[q] = dwn;
+            } else {
+                // ---- AB2 (ab2.f90:55-65)
+                T us = uq + dt * (c15 * dun - c05 * duold[q]);
+                T vs = vq + dt * (c15 * dvn - c05 * dvold[q]);
+                T ws = wq + dt * (c15 * dwn - c05 * dwold[q]);
+                duold[q] = dun; dvold[q] = dvn; dwold[q] = dwn;
+                // ---- split pressure (ab2.f90:79-101, mom.f90:793,818,843)
+                const long qx = gidx(g, ip, j, k), qy = gidx(g, i, jp, k), qz = q + g.sz;
+                double phq = 2. * pnew[q] - pold[q];
+                double phx = 2. * pnew[qx] - pold[qx];
+                double phy = 2. * pnew[qy] - pold[qy];
+                double phz = 2. * pnew[qz] - pold[qz];
+                T pjx = two * p_x[q] - p_xold[q];
+                T pjy = two * p_y[q] - p_yold[q];
+                T pjz = two * p_z[q] - p_zold[q];
+                T mpx = -(one / rho_u - irho0) * T(g.dxi) * T(phx - (double)pjx - phq);
+                T mpy = -(one / rho_v - irho0) * T(g.dyi) * T(phy - (double)pjy - phq);
+                T mpz = -(one / rho_w - irho0) * T(g.dzi) * T(phz - (double)pjz - phq);
+                dudt[q] = us + dt * mpx;
+                dvdt[q] = vs + dt * mpy - T(m.forceytot) * dt / rho_v;
+                dwdt[q] = ws + dt * mpz;
+            }
+        }
+#undef AT
+        __syncthreads();
     }
-    // ---- AB2 (ab2.f90:55-65)
-    const double dt = m.dt;
-    double us = u[q] + dt * (1.5 * dun - 0.5 * duold[q]);
-    double vs = v[q] + dt * (1.5 * dvn - 0.5 * dvold[q]);
-    double ws = w[q] + dt * (1.5 * dwn - 0.5 * dwold[q]);
-    duold[q] = dun; dvold[q] = dvn; dwold[q] = dwn;
+}
 
-    // ---- split pressure (ab2.f90:79-101, mom.f90:793,818,843)
-    double phq = 2. * pnew[q] - pold[q];
-    double phx = 2. * pnew[I(ip, j, k)] - pold[I(ip, j, k)];
-    double phy = 2. * pnew[I(i, jp, k)] - pold[I(i, jp, k)];
-    double phz = 2. * pnew[I(i, j, kp)] - pold[I(i, j, kp)];
-    double pjx = 2. * p_x[q] - p_xold[q];
-    double pjy = 2. * p_y[q] - p_yold[q];
-    double pjz = 2. * p_z[q] - p_zold[q];
-    double mpx = -(1. / rho_u - m.inv_rho0) * g.dxi * (phx - pjx - phq);
-    double mpy = -(1. / rho_v - m.inv_rho0) * g.dyi * (phy - pjy - phq);
-    double mpz = -(1. / rho_w - m.inv_rho0) * g.dzi * (phz - pjz - phq);
-    dudt[q] = us + dt * mpx;
-    dvdt[q] = vs + dt * mpy - m.forceytot * dt / rho_v;
-    dwdt[q] = ws + dt * mpz;
+template <class T>
+static void momentum_dispatch(Ctx& c, const MomParams& m, int rhs_only) {
+    dim3 blk(BX, BY, 1);
+    const int ntiles = ((c.g.nx + BX - 1) / BX) * ((c.g.ny + BY - 1) / BY) * ((c.g.nz + MZ - 1) / MZ);
+#define MOM(R, P)                                                                                                  \
+    hipLaunchKernelGGL((k_momentum_zm<T, R, P>), dim3(ntiles), blk, 0, c.stream, c.g, m,                            \
+                       (const double*)c.dscal, as<T>(c.unew), as<T>(c.vnew), as<T>(c.wnew), as<T>(c.miu),          \
+                       as<T>(c.rho), as<T>(c.duold), as<T>(c.dvold), as<T>(c.dwold), (const double*)c.pnew,        \
+                       (const double*)c.pold, as<T>(c.p_x), as<T>(c.p_y), as<T>(c.p_z), as<T>(c.p_xold),          \
+                       as<T>(c.p_yold), as<T>(c.p_zold), as<T>(c.dudt), as<T>(c.dvdt), as<T>(c.dwdt))
+    if (c.g.pow2) {
+        if (rhs_only) MOM(1, true); else MOM(0, true);
+    } else {
+        if (rhs_only) MOM(1, false); else MOM(0, false);
+    }
+#undef MOM
 }
 
 static int momentum_launch(Ctx& c, int rhs_only) {
@@ -185,37 +275,19 @@ static int momentum_launch(Ctx& c, int rhs_only) {
     m.rho0 = c.rho0; m.inv_rho0 = 1. / c.rho0; m.rho1 = c.P.rho1;
     m.forceytot = c.P.constPoi ? -12. * c.visc / (c.P.lz * c.P.lz) : 0.;
     m.rho_equal = c.P.rho1 == c.P.rho2;
-    dim3 blk(BX, BY, 1);
-#define MOM(R, P)                                                                                                  \
-    hipLaunchKernelGGL((k_momentum<R, P>), grid3(c.g, c.g.nz, blk), blk, 0, c.stream, c.g, m, (const double*)c.dscal,   \
-                       (const double*)c.unew, (const double*)c.vnew, (const double*)c.wnew, (const double*)c.miu,   \
-                       (const double*)c.rho, c.duold, c.dvold, c.dwold, (const double*)c.pnew, (const double*)c.pold, \
-                       (const double*)c.p_x, (const double*)c.p_y, (const double*)c.p_z, (const double*)c.p_xold,   \
-                       (const double*)c.p_yold, (const double*)c.p_zold, c.dudt, c.dvdt, c.dwdt)
-    if (c.g.pow2) {
-        if (rhs_only) MOM(1, true); else MOM(0, true);
-    } else {
-        if (rhs_only) MOM(1, false); else MOM(0, false);
-    }
-#undef MOM
+    if (c.fp32) momentum_dispatch<float>(c, m, rhs_only);
+    else momentum_dispatch<double>(c, m, rhs_only);
     HIP_OK(hipGetLastError());
     return ICLS_OK;
 }
 int launch_momentum(Ctx& c) { return momentum_launch(c, 0); }
 int launch_momentum_rhs_only(Ctx& c) { return momentum_launch(c, 1); }
 
-// mom_main alone into caller arrays (init.f90:90 starts AB2 with it)
-__global__ void k_copy_interior(Grid g, const double* __restrict__ a, double* __restrict__ b) {
-    int i0 = blockIdx.x * BX + threadIdx.x, j0 = blockIdx.y * BY + threadIdx.y, k = blockIdx.z + 1;
-    if (i0 >= g.nx || j0 >= g.ny) return;
-    long q = gidx(g, i0, j0, k);
-    b[q] = a[q];
-}
-
 // ---------------------------------------------------------------- boundary conditions
 // bounduvw, BC_in_z = 'Dirichl' (bound.f90:26-47); x/y by wrap
-__global__ void k_bc_uvw(Grid g, double vlo, double vup, double* __restrict__ u, double* __restrict__ v,
-                         double* __restrict__ w, const double* __restrict__ flag) {
+template <class T>
+__global__ void k_bc_uvw(Grid g, T vlo, T vup, T* __restrict__ u, T* __restrict__ v, T* __restrict__ w,
+                         const double* __restrict__ flag) {
     if (flag && flag[0] == 0.) return;
     int i0 = blockIdx.x * BX + threadIdx.x, j0 = blockIdx.y * BY + threadIdx.y;
     if (i0 >= g.nx || j0 >= g.ny) return;
@@ -223,10 +295,10 @@ __global__ void k_bc_uvw(Grid g, double vlo, double vup, double* __restrict__ u,
     auto I = [&](int k) { return gidx(g, i0, j0, k); };
     u[I(0)] = -u[I(1)];
     u[I(k1)] = -u[I(kmax)];
-    v[I(0)] = 2. * vlo - v[I(1)];
-    v[I(k1)] = 2. * vup - v[I(kmax)];
-    w[I(0)] = 0.;
-    w[I(kmax)] = 0.;
+    v[I(0)] = T(2.) * vlo - v[I(1)];
+    v[I(k1)] = T(2.) * vup - v[I(kmax)];
+    w[I(0)] = T(0.);
+    w[I(kmax)] = T(0.);
     w[I(k1)] = w[I(kmax - 1)];
 }
 
@@ -240,14 +312,19 @@ int launch_bc_uvw_pred(Ctx& c, double* u, double* v, double* w, const double* fl
     double vlo, vup;
     wall_speeds(c, vlo, vup);
     dim3 blk(BX, BY, 1);
-    hipLaunchKernelGGL(k_bc_uvw, grid3(c.g, 1, blk), blk, 0, c.stream, c.g, vlo, vup, u, v, w, flag);
+    if (c.fp32)
+        hipLaunchKernelGGL(k_bc_uvw<float>, grid3(c.g, 1, blk), blk, 0, c.stream, c.g, (float)vlo, (float)vup,
+                           as<float>(u), as<float>(v), as<float>(w), flag);
+    else
+        hipLaunchKernelGGL(k_bc_uvw<double>, grid3(c.g, 1, blk), blk, 0, c.stream, c.g, vlo, vup, u, v, w, flag);
     HIP_OK(hipGetLastError());
     return ICLS_OK;
 }
 int launch_bc_uvw(Ctx& c, double* u, double* v, double* w) { return launch_bc_uvw_pred(c, u, v, w, nullptr); }
 
 // boundp z part: Neumann (bound.f90:193-198)
-__global__ void k_bc_p(Grid g, double* __restrict__ p, const double* __restrict__ flag) {
+template <class T>
+__global__ void k_bc_p(Grid g, T* __restrict__ p, const double* __restrict__ flag) {
     if (flag && flag[0] == 0.) return;
     int i0 = blockIdx.x * BX + threadIdx.x, j0 = blockIdx.y * BY + threadIdx.y;
     if (i0 >= g.nx || j0 >= g.ny) return;
@@ -255,36 +332,53 @@ __global__ void k_bc_p(Grid g, double* __restrict__ p, const double* __restrict_
     p[gidx(g, i0, j0, g.nz + 1)] = p[gidx(g, i0, j0, g.nz)];
 }
 
-int launch_bc_p_pred(Ctx& c, double* p, const double* flag) {
+// on a pressure-class (double) array
+int launch_bc_p(Ctx& c, double* p) {
     ProfScope ps(c, "k_bc_p");
     dim3 blk(BX, BY, 1);
-    hipLaunchKernelGGL(k_bc_p, grid3(c.g, 1, blk), blk, 0, c.stream, c.g, p, flag);
+    hipLaunchKernelGGL(k_bc_p<double>, grid3(c.g, 1, blk), blk, 0, c.stream, c.g, p, (const double*)nullptr);
     HIP_OK(hipGetLastError());
     return ICLS_OK;
 }
-int launch_bc_p(Ctx& c, double* p) { return launch_bc_p_pred(c, p, nullptr); }
+
+// on a field-class array (the Heaviside of the mass correction), predicated
+int launch_bc_p_t_pred(Ctx& c, double* p, const double* flag) {
+    ProfScope ps(c, "k_bc_p");
+    dim3 blk(BX, BY, 1);
+    if (c.fp32) hipLaunchKernelGGL(k_bc_p<float>, grid3(c.g, 1, blk), blk, 0, c.stream, c.g, as<float>(p), flag);
+    else hipLaunchKernelGGL(k_bc_p<double>, grid3(c.g, 1, blk), blk, 0, c.stream, c.g, p, flag);
+    HIP_OK(hipGetLastError());
+    return ICLS_OK;
+}
 
 // ---------------------------------------------------------------- correc
-// correc.f90:33-35
-__global__ void k_correc(Grid g, double dt, double rho0, const double* __restrict__ p, const double* __restrict__ px,
-                         const double* __restrict__ py, const double* __restrict__ pz, const double* __restrict__ du,
-                         const double* __restrict__ dv, const double* __restrict__ dw, double* __restrict__ u,
-                         double* __restrict__ v, double* __restrict__ w) {
+// correc.f90:33-35; the pressure difference p(+1) - p - p_x is formed in double
+template <class T>
+__global__ void k_correc(Grid g, T dt, T rho0, const double* __restrict__ p, const T* __restrict__ px,
+                         const T* __restrict__ py, const T* __restrict__ pz, const T* __restrict__ du,
+                         const T* __restrict__ dv, const T* __restrict__ dw, T* __restrict__ u, T* __restrict__ v,
+                         T* __restrict__ w) {
     int i0 = blockIdx.x * BX + threadIdx.x, j0 = blockIdx.y * BY + threadIdx.y, k = blockIdx.z + 1;
     if (i0 >= g.nx || j0 >= g.ny) return;
     long q = gidx(g, i0, j0, k);
     double pq = p[q];
-    u[q] = du[q] - dt * g.dxi * (p[gidx(g, wrap(i0 + 1, g.nx), j0, k)] - pq - px[q]) / rho0;
-    v[q] = dv[q] - dt * g.dyi * (p[gidx(g, i0, wrap(j0 + 1, g.ny), k)] - pq - py[q]) / rho0;
-    w[q] = dw[q] - dt * g.dzi * (p[gidx(g, i0, j0, k + 1)] - pq - pz[q]) / rho0;
+    u[q] = du[q] - dt * T(g.dxi) * T(p[gidx(g, wrap(i0 + 1, g.nx), j0, k)] - pq - (double)px[q]) / rho0;
+    v[q] = dv[q] - dt * T(g.dyi) * T(p[gidx(g, i0, wrap(j0 + 1, g.ny), k)] - pq - (double)py[q]) / rho0;
+    w[q] = dw[q] - dt * T(g.dzi) * T(p[gidx(g, i0, j0, k + 1)] - pq - (double)pz[q]) / rho0;
 }
 
 int launch_correc(Ctx& c) {
     ProfScope ps(c, "k_correc");
     dim3 blk(BX, BY, 1);
-    hipLaunchKernelGGL(k_correc, grid3(c.g, c.g.nz, blk), blk, 0, c.stream, c.g, c.dt, c.rho0, (const double*)c.p,
-                       (const double*)c.p_x, (const double*)c.p_y, (const double*)c.p_z, (const double*)c.dudt,
-                       (const double*)c.dvdt, (const double*)c.dwdt, c.unew, c.vnew, c.wnew);
+    dim3 grd = grid3(c.g, c.g.nz, blk);
+    if (c.fp32)
+        hipLaunchKernelGGL(k_correc<float>, grd, blk, 0, c.stream, c.g, (float)c.dt, (float)c.rho0, (const double*)c.p,
+                           as<float>(c.p_x), as<float>(c.p_y), as<float>(c.p_z), as<float>(c.dudt), as<float>(c.dvdt),
+                           as<float>(c.dwdt), as<float>(c.unew), as<float>(c.vnew), as<float>(c.wnew));
+    else
+        hipLaunchKernelGGL(k_correc<double>, grd, blk, 0, c.stream, c.g, c.dt, c.rho0, (const double*)c.p,
+                           (const double*)c.p_x, (const double*)c.p_y, (const double*)c.p_z, (const double*)c.dudt,
+                           (const double*)c.dvdt, (const double*)c.dwdt, c.unew, c.vnew, c.wnew);
     HIP_OK(hipGetLastError());
     return ICLS_OK;
 }
@@ -349,35 +443,42 @@ int launch_pressure_rotate(Ctx& c, int first_step) {
 
 // ---------------------------------------------------------------- cell-centre velocity
 // average_vel_at_center (misc.f90:587-604)
-__global__ void k_avg_vel(Grid g, const double* __restrict__ u, const double* __restrict__ v,
-                          const double* __restrict__ w, double* __restrict__ uc, double* __restrict__ vc,
-                          double* __restrict__ wc) {
+template <class T>
+__global__ void k_avg_vel(Grid g, const T* __restrict__ u, const T* __restrict__ v, const T* __restrict__ w,
+                          T* __restrict__ uc, T* __restrict__ vc, T* __restrict__ wc) {
     int i0 = blockIdx.x * BX + threadIdx.x, j0 = blockIdx.y * BY + threadIdx.y, k = blockIdx.z + 1;
     if (i0 >= g.nx || j0 >= g.ny) return;
     long q = gidx(g, i0, j0, k);
-    uc[q] = (u[gidx(g, wrap(i0 - 1, g.nx), j0, k)] + u[q]) / 2.;
-    vc[q] = (v[gidx(g, i0, wrap(j0 - 1, g.ny), k)] + v[q]) / 2.;
-    wc[q] = (w[gidx(g, i0, j0, k - 1)] + w[q]) / 2.;
+    uc[q] = (u[gidx(g, wrap(i0 - 1, g.nx), j0, k)] + u[q]) / T(2.);
+    vc[q] = (v[gidx(g, i0, wrap(j0 - 1, g.ny), k)] + v[q]) / T(2.);
+    wc[q] = (w[gidx(g, i0, j0, k - 1)] + w[q]) / T(2.);
 }
 
 int launch_avg_vel(Ctx& c) {
     ProfScope ps(c, "k_avg_vel");
     dim3 blk(BX, BY, 1);
-    hipLaunchKernelGGL(k_avg_vel, grid3(c.g, c.g.nz, blk), blk, 0, c.stream, c.g, (const double*)c.unew,
-                       (const double*)c.vnew, (const double*)c.wnew, c.ucc, c.vcc, c.wcc);
+    dim3 grd = grid3(c.g, c.g.nz, blk);
+    if (c.fp32)
+        hipLaunchKernelGGL(k_avg_vel<float>, grd, blk, 0, c.stream, c.g, as<float>(c.unew), as<float>(c.vnew),
+                           as<float>(c.wnew), as<float>(c.ucc), as<float>(c.vcc), as<float>(c.wcc));
+    else
+        hipLaunchKernelGGL(k_avg_vel<double>, grd, blk, 0, c.stream, c.g, (const double*)c.unew, (const double*)c.vnew,
+                           (const double*)c.wnew, c.ucc, c.vcc, c.wcc);
     HIP_OK(hipGetLastError());
     return ICLS_OK;
 }
 
 // ---------------------------------------------------------------- NaN and diagnostics
 // isnan(vnew(1,1,1)) (main.f90:207-210) into a sticky flag
-__global__ void k_nan_check(Grid g, const double* __restrict__ v, double* dscal) {
+template <class T>
+__global__ void k_nan_check(Grid g, const T* __restrict__ v, double* dscal) {
     if (threadIdx.x != 0) return;
     if (isnan(v[gidx(g, 0, 0, 1)])) dscal[DS_NAN_FLAG] = 1.;
 }
 
 int launch_nan_check(Ctx& c) {
-    hipLaunchKernelGGL(k_nan_check, dim3(1), dim3(64), 0, c.stream, c.g, (const double*)c.vnew, c.dscal);
+    if (c.fp32) hipLaunchKernelGGL(k_nan_check<float>, dim3(1), dim3(64), 0, c.stream, c.g, as<float>(c.vnew), c.dscal);
+    else hipLaunchKernelGGL(k_nan_check<double>, dim3(1), dim3(64), 0, c.stream, c.g, (const double*)c.vnew, c.dscal);
     HIP_OK(hipGetLastError());
     return ICLS_OK;
 }
@@ -388,9 +489,11 @@ static inline dim3 red_grid(const Grid& g, int nk) {
 __device__ __forceinline__ int block_linear() { return blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z); }
 
 // chkdiv (chkdiv.f90:28-48): sum of div and max |div| with its first location in
-// (k, j, i) loop order; partials [sum | max | location]
-__global__ void k_chkdiv(Grid g, const double* __restrict__ u, const double* __restrict__ v,
-                         const double* __restrict__ w, double* __restrict__ part, int nb) {
+// (k, j, i) loop order; partials [sum | max | location].  The divergence is formed in
+// double from the stored velocities.
+template <class T>
+__global__ void k_chkdiv(Grid g, const T* __restrict__ u, const T* __restrict__ v, const T* __restrict__ w,
+                         double* __restrict__ part, int nb) {
     __shared__ double smem[16];
     __shared__ double smax[256];
     __shared__ long sloc[256];
@@ -403,8 +506,9 @@ __global__ void k_chkdiv(Grid g, const double* __restrict__ u, const double* __r
             int k = 1 + blockIdx.z * KCH + kk;
             if (k > g.nz) break;
             long q = gidx(g, i0, j0, k);
-            double div = (w[q] - w[gidx(g, i0, j0, k - 1)]) * g.dzi + (v[q] - v[gidx(g, i0, wrap(j0 - 1, g.ny), k)]) * g.dyi +
-                         (u[q] - u[gidx(g, wrap(i0 - 1, g.nx), j0, k)]) * g.dxi;
+            double div = ((double)w[q] - (double)w[gidx(g, i0, j0, k - 1)]) * g.dzi +
+                         ((double)v[q] - (double)v[gidx(g, i0, wrap(j0 - 1, g.ny), k)]) * g.dyi +
+                         ((double)u[q] - (double)u[gidx(g, wrap(i0 - 1, g.nx), j0, k)]) * g.dxi;
             s += div;
             double a = fabs(div);
             long lin = (long)i0 + (long)g.nx * ((long)j0 + (long)g.ny * (k - 1));
@@ -442,33 +546,38 @@ int launch_chkdiv(Ctx& c) {
     dim3 blk(BX, BY, 1);
     dim3 grd = red_grid(c.g, c.g.nz);
     int nb = grd.x * grd.y * grd.z;
-    hipLaunchKernelGGL(k_chkdiv, grd, blk, 0, c.stream, c.g, (const double*)c.unew, (const double*)c.vnew,
-                       (const double*)c.wnew, c.dpart, nb);
+    if (c.fp32)
+        hipLaunchKernelGGL(k_chkdiv<float>, grd, blk, 0, c.stream, c.g, as<float>(c.unew), as<float>(c.vnew),
+                           as<float>(c.wnew), c.dpart, nb);
+    else
+        hipLaunchKernelGGL(k_chkdiv<double>, grd, blk, 0, c.stream, c.g, (const double*)c.unew, (const double*)c.vnew,
+                           (const double*)c.wnew, c.dpart, nb);
     hipLaunchKernelGGL(k_chkdiv_final, dim3(1), dim3(256), 0, c.stream, (const double*)c.dpart, nb, c.dscal);
     HIP_OK(hipGetLastError());
     return ICLS_OK;
 }
 
 // chkdt (chkdt.f90:38-52): max |face-to-centre average| over 1..i1, 1..j1, 1..k1
-__global__ void k_umax(Grid g, const double* __restrict__ u, const double* __restrict__ v,
-                       const double* __restrict__ w, double* __restrict__ part, int nb) {
+template <class T>
+__global__ void k_umax(Grid g, const T* __restrict__ u, const T* __restrict__ v, const T* __restrict__ w,
+                       double* __restrict__ part, int nb) {
     __shared__ double smem[16];
     int i = blockIdx.x * BX + threadIdx.x + 1, j = blockIdx.y * BY + threadIdx.y + 1;   // Fortran 1 .. i1
-    double mu = 0., mv = 0., mw = 0.;
+    T mu = T(0.), mv = T(0.), mw = T(0.);
     if (i <= g.nx + 1 && j <= g.ny + 1) {
         int i0 = wrap(i - 1, g.nx), j0 = wrap(j - 1, g.ny), im = wrap(i - 2, g.nx), jm = wrap(j - 2, g.ny);
         for (int kk = 0; kk < KCH; kk++) {
             int k = 1 + blockIdx.z * KCH + kk;
             if (k > g.nz + 1) break;
             long q = gidx(g, i0, j0, k);
-            mu = fmax(mu, fabs((u[q] + u[gidx(g, im, j0, k)]) / 2.));
-            mv = fmax(mv, fabs((v[q] + v[gidx(g, i0, jm, k)]) / 2.));
-            mw = fmax(mw, fabs((w[q] + w[gidx(g, i0, j0, k - 1)]) / 2.));
+            mu = fmax(mu, fabs((u[q] + u[gidx(g, im, j0, k)]) / T(2.)));
+            mv = fmax(mv, fabs((v[q] + v[gidx(g, i0, jm, k)]) / T(2.)));
+            mw = fmax(mw, fabs((w[q] + w[gidx(g, i0, j0, k - 1)]) / T(2.)));
         }
     }
-    double a = block_reduce(mu, OpMax(), smem);
-    double b = block_reduce(mv, OpMax(), smem);
-    double c2 = block_reduce(mw, OpMax(), smem);
+    double a = block_reduce((double)mu, OpMax(), smem);
+    double b = block_reduce((double)mv, OpMax(), smem);
+    double c2 = block_reduce((double)mw, OpMax(), smem);
     if (threadIdx.x == 0 && threadIdx.y == 0) {
         int bl = block_linear();
         part[bl] = a; part[nb + bl] = b; part[2 * nb + bl] = c2;
@@ -480,8 +589,12 @@ int launch_chkdt_umax(Ctx& c) {
     dim3 blk(BX, BY, 1);
     dim3 grd((c.g.nx + 1 + BX - 1) / BX, (c.g.ny + 1 + BY - 1) / BY, (c.g.nz + 1 + KCH - 1) / KCH);
     int nb = grd.x * grd.y * grd.z;
-    hipLaunchKernelGGL(k_umax, grd, blk, 0, c.stream, c.g, (const double*)c.unew, (const double*)c.vnew,
-                       (const double*)c.wnew, c.dpart, nb);
+    if (c.fp32)
+        hipLaunchKernelGGL(k_umax<float>, grd, blk, 0, c.stream, c.g, as<float>(c.unew), as<float>(c.vnew),
+                           as<float>(c.wnew), c.dpart, nb);
+    else
+        hipLaunchKernelGGL(k_umax<double>, grd, blk, 0, c.stream, c.g, (const double*)c.unew, (const double*)c.vnew,
+                           (const double*)c.wnew, c.dpart, nb);
     for (int t = 0; t < 3; t++)
         hipLaunchKernelGGL(k_reduce_final<OpMax>, dim3(1), dim3(256), 0, c.stream, (const double*)(c.dpart + t * nb), nb,
                            0., c.dscal + DS_UMAX + t);
@@ -490,11 +603,12 @@ int launch_chkdt_umax(Ctx& c) {
 }
 
 // max_vel_at_center (misc.f90:621-632)
-__global__ void k_vmag_max(Grid g, const double* __restrict__ uc, const double* __restrict__ vc,
-                           const double* __restrict__ wc, double* __restrict__ part) {
+template <class T>
+__global__ void k_vmag_max(Grid g, const T* __restrict__ uc, const T* __restrict__ vc, const T* __restrict__ wc,
+                           double* __restrict__ part) {
     __shared__ double smem[16];
     int i0 = blockIdx.x * BX + threadIdx.x, j0 = blockIdx.y * BY + threadIdx.y;
-    double m = 0.;
+    T m = T(0.);
     if (i0 < g.nx && j0 < g.ny)
         for (int kk = 0; kk < KCH; kk++) {
             int k = 1 + blockIdx.z * KCH + kk;
@@ -502,7 +616,7 @@ __global__ void k_vmag_max(Grid g, const double* __restrict__ uc, const double* 
             long q = gidx(g, i0, j0, k);
             m = fmax(m, sqrt(uc[q] * uc[q] + vc[q] * vc[q] + wc[q] * wc[q]));
         }
-    double r = block_reduce(m, OpMax(), smem);
+    double r = block_reduce((double)m, OpMax(), smem);
     if (threadIdx.x == 0 && threadIdx.y == 0) part[block_linear()] = r;
 }
 
@@ -511,8 +625,12 @@ int launch_max_vel(Ctx& c) {
     dim3 blk(BX, BY, 1);
     dim3 grd = red_grid(c.g, c.g.nz);
     int nb = grd.x * grd.y * grd.z;
-    hipLaunchKernelGGL(k_vmag_max, grd, blk, 0, c.stream, c.g, (const double*)c.ucc, (const double*)c.vcc,
-                       (const double*)c.wcc, c.dpart);
+    if (c.fp32)
+        hipLaunchKernelGGL(k_vmag_max<float>, grd, blk, 0, c.stream, c.g, as<float>(c.ucc), as<float>(c.vcc),
+                           as<float>(c.wcc), c.dpart);
+    else
+        hipLaunchKernelGGL(k_vmag_max<double>, grd, blk, 0, c.stream, c.g, (const double*)c.ucc, (const double*)c.vcc,
+                           (const double*)c.wcc, c.dpart);
     hipLaunchKernelGGL(k_reduce_final<OpMax>, dim3(1), dim3(256), 0, c.stream, (const double*)c.dpart, nb, 0.,
                        c.dscal + DS_VMAX);
     HIP_OK(hipGetLastError());
@@ -521,11 +639,12 @@ int launch_max_vel(Ctx& c) {
 
 // ---------------------------------------------------------------- initial velocity
 // init_vel_p 'cou' (init.f90:191-210): v = v_lower + shear_rate*(k-0.5)*dz, k = 0..k1
-__global__ void k_init_couette(Grid g, double v_lower, double shear_rate, double* __restrict__ v) {
+template <class T>
+__global__ void k_init_couette(Grid g, double v_lower, double shear_rate, T* __restrict__ v) {
     int i0 = blockIdx.x * BX + threadIdx.x, j0 = blockIdx.y * BY + threadIdx.y, k = blockIdx.z;
     if (i0 >= g.nx || j0 >= g.ny) return;
     double coorz = (k - 0.5) * g.dz;
-    v[gidx(g, i0, j0, k)] = v_lower + shear_rate * coorz;
+    v[gidx(g, i0, j0, k)] = (T)(v_lower + shear_rate * coorz);
 }
 
 int launch_init_velocity(Ctx& c) {
@@ -533,32 +652,58 @@ int launch_init_velocity(Ctx& c) {
     double v_upper = c.P.wall_speed, v_lower = -c.P.wall_speed;
     double shear_rate = (v_upper - v_lower) / c.P.lz;
     dim3 blk(BX, BY, 1);
-    hipLaunchKernelGGL(k_init_couette, grid3(c.g, c.g.nz + 2, blk), blk, 0, c.stream, c.g, v_lower, shear_rate, c.vnew);
+    dim3 grd = grid3(c.g, c.g.nz + 2, blk);
+    if (c.fp32)
+        hipLaunchKernelGGL(k_init_couette<float>, grd, blk, 0, c.stream, c.g, v_lower, shear_rate, as<float>(c.vnew));
+    else
+        hipLaunchKernelGGL(k_init_couette<double>, grd, blk, 0, c.stream, c.g, v_lower, shear_rate, c.vnew);
     HIP_OK(hipGetLastError());
     return ICLS_OK;
 }
 
 // fill a whole padded field (used instead of memset so graph capture stays safe)
-__global__ void k_fill(long n, double* __restrict__ f, double v) {
+template <class T>
+__global__ void k_fill(long n, T* __restrict__ f, T v) {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     long stride = (long)gridDim.x * blockDim.x;
     for (; t < n; t += stride) f[t] = v;
 }
 
-int launch_fill(Ctx& c, double* f, double v) {
+int launch_fill(Ctx& c, double* f, double v, int fp32) {
     long n = c.nfield;
     int nb = (int)((n + 255) / 256);
     if (nb > 65536) nb = 65536;
-    hipLaunchKernelGGL(k_fill, dim3(nb), dim3(256), 0, c.stream, n, f, v);
+    if (fp32) hipLaunchKernelGGL(k_fill<float>, dim3(nb), dim3(256), 0, c.stream, n, as<float>(f), (float)v);
+    else hipLaunchKernelGGL(k_fill<double>, dim3(nb), dim3(256), 0, c.stream, n, f, v);
     HIP_OK(hipGetLastError());
     return ICLS_OK;
 }
 
-int launch_copy_interior(Ctx& c, const double* a, double* b) {
-    dim3 blk(BX, BY, 1);
-    hipLaunchKernelGGL(k_copy_interior, grid3(c.g, c.g.nz, blk), blk, 0, c.stream, c.g, a, b);
+// element-type conversion of a whole padded field
+template <class S, class D>
+__global__ void k_convert(long n, const S* __restrict__ a, D* __restrict__ b) {
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long stride = (long)gridDim.x * blockDim.x;
+    for (; t < n; t += stride) b[t] = (D)a[t];
+}
+
+int launch_convert(Ctx& c, const double* src, int src_fp32, double* dst, int dst_fp32) {
+    long n = c.nfield;
+    int nb = (int)((n + 255) / 256);
+    if (nb > 65536) nb = 65536;
+    if (src_fp32 && dst_fp32)
+        hipLaunchKernelGGL((k_convert<float, float>), dim3(nb), dim3(256), 0, c.stream, n, as<float>(src), as<float>(dst));
+    else if (src_fp32)
+        hipLaunchKernelGGL((k_convert<float, double>), dim3(nb), dim3(256), 0, c.stream, n, as<float>(src), dst);
+    else if (dst_fp32)
+        hipLaunchKernelGGL((k_convert<double, float>), dim3(nb), dim3(256), 0, c.stream, n, src, as<float>(dst));
+    else
+        hipLaunchKernelGGL((k_convert<double, double>), dim3(nb), dim3(256), 0, c.stream, n, src, dst);
     HIP_OK(hipGetLastError());
     return ICLS_OK;
 }
+
+// copy of a whole padded field-class array
+int launch_copy_field(Ctx& c, const double* a, double* b) { return launch_convert(c, a, c.fp32, b, c.fp32); }
 
 }  // namespace icls

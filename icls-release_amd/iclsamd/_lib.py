@@ -26,7 +26,21 @@ EXPORTS = [
     "icls_add_pjump", "icls_solver2d", "icls_boundp", "icls_correc", "icls_bounduvw", "icls_update_pressure",
     "icls_average_vel_at_center", "icls_check_nan", "icls_chkdiv", "icls_chkdt", "icls_max_vel_at_center",
     "icls_step", "icls_profile_enable", "icls_profile_only", "icls_profile_reset", "icls_kernel_times",
+    "icls_mlp_defaults", "icls_mlp_train", "icls_mlp_set_weights", "icls_mlp_get_weights", "icls_mlp_set_gate",
+    "icls_mlp_forward", "icls_mlp_gradient", "icls_mlp_samples", "icls_mlp_listed",
 ]
+
+MLP_NPARAM = 6337
+MLP_NPAD = 32
+
+
+class MlpTrainOpts(ctypes.Structure):
+    """icls_mlp_train_opts (include/icls.h)."""
+    _fields_ = [
+        ("n_samples", ctypes.c_int64), ("epochs", ctypes.c_int32), ("batch", ctypes.c_int32),
+        ("lr", ctypes.c_float), ("r_min", ctypes.c_float), ("r_max", ctypes.c_float),
+        ("seed", ctypes.c_uint64), ("loss_first", ctypes.c_float), ("loss_last", ctypes.c_float),
+    ]
 
 
 class IclsError(RuntimeError):
@@ -66,6 +80,16 @@ def load():
     L.icls_step.argtypes = [i32, i32]
     L.icls_profile_only.argtypes = [ctypes.c_char_p]
     L.icls_kernel_times.argtypes = [i32, ctypes.c_char_p, dp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(i32)]
+    fp, u16p = ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_uint16)
+    L.icls_mlp_defaults.argtypes = [ctypes.POINTER(MlpTrainOpts)]
+    L.icls_mlp_train.argtypes = [ctypes.POINTER(MlpTrainOpts)]
+    L.icls_mlp_set_weights.argtypes = [fp, i32]
+    L.icls_mlp_get_weights.argtypes = [fp, i32]
+    L.icls_mlp_set_gate.argtypes = [ctypes.c_float, ctypes.c_float]
+    L.icls_mlp_forward.argtypes = [u16p, i32, fp]
+    L.icls_mlp_gradient.argtypes = [u16p, fp, i32, fp]
+    L.icls_mlp_samples.argtypes = [ctypes.c_int64, ctypes.c_uint64, ctypes.c_float, ctypes.c_float, u16p, fp]
+    L.icls_mlp_listed.argtypes = [ctypes.POINTER(i32)]
     for name in EXPORTS:
         fn = getattr(L, name)
         if fn.restype is ctypes.c_int:
@@ -211,6 +235,58 @@ class Solver:
     def max_vel_at_center(self):
         v = ctypes.c_double()
         self._call("icls_max_vel_at_center", ctypes.byref(v))
+        return v.value
+
+    # --- learned curvature closure
+    def mlp_train(self, **kw):
+        """Train on the device; keyword overrides of icls_mlp_train_opts.  Returns the
+        options with loss_first / loss_last filled in, as a dict."""
+        o = MlpTrainOpts()
+        self.L.icls_mlp_defaults(ctypes.byref(o))
+        for k, v in kw.items():
+            setattr(o, k, v)
+        self._call("icls_mlp_train", ctypes.byref(o))
+        return {k: getattr(o, k) for k, _ in MlpTrainOpts._fields_}
+
+    def mlp_set_weights(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        self._call("icls_mlp_set_weights", w.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), w.size)
+
+    def mlp_get_weights(self):
+        w = np.zeros(MLP_NPARAM, dtype=np.float32)
+        self._call("icls_mlp_get_weights", w.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), w.size)
+        return w
+
+    def mlp_set_gate(self, gate, tol):
+        self._call("icls_mlp_set_gate", gate, tol)
+
+    def mlp_forward(self, x_bf16):
+        """x_bf16: (n, 32) uint16 bfloat16 bit patterns."""
+        x = np.ascontiguousarray(x_bf16, dtype=np.uint16)
+        y = np.zeros(x.shape[0], dtype=np.float32)
+        self._call("icls_mlp_forward", x.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16)), x.shape[0],
+                   y.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
+        return y
+
+    def mlp_gradient(self, x_bf16, t):
+        x = np.ascontiguousarray(x_bf16, dtype=np.uint16)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        g = np.zeros(MLP_NPARAM + 1, dtype=np.float32)
+        self._call("icls_mlp_gradient", x.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16)),
+                   t.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), x.shape[0],
+                   g.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
+        return g[:-1], float(g[-1])
+
+    def mlp_samples(self, n, seed=20261202, r_min=1.5, r_max=12.0):
+        x = np.zeros((n, MLP_NPAD), dtype=np.uint16)
+        t = np.zeros(n, dtype=np.float32)
+        self._call("icls_mlp_samples", n, seed, r_min, r_max, x.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16)),
+                   t.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
+        return x, t
+
+    def mlp_listed(self):
+        v = ctypes.c_int32()
+        self._call("icls_mlp_listed", ctypes.byref(v))
         return v.value
 
     # --- profiling

@@ -1,39 +1,59 @@
-"""Algorithmic bytes per cell of every kernel of the step (FP64 storage).
+"""Algorithmic bytes per cell of every kernel of the step.
 
 Definition (the measurement protocol of this project): the sum over a kernel's
-distinct input and output fields of element size, per interior cell
+distinct input and output arrays of their element size, per interior cell
 N = itot*jtot*ktot.  Stencil re-reads, halos, the z ghost layers, constant tables
 and scalars are not counted; masked or band-limited sweeps still count every cell;
 a byte mask counts 1 byte.  Scratch that a kernel keeps for itself (the Thomas
-multipliers) is not an input or output and is not counted.
+multipliers and forward-sweep values) is not an input or output and is not counted.
+
+Arrays are of three classes: "T" field arrays (8 bytes in the FP64 mode, 4 in the
+mixed modes), "P" pressure arrays (8 bytes in every mode) and "B" byte masks.
 """
 
 HBM_PEAK_GBS = 8000.0       # MI355X HBM3E, vendor figure
 FP64_PEAK_TFLOPS = 78.6
 
-# bytes per interior cell, FP64 fields (8 B) unless noted
-KERNEL_BYTES_FP64 = {
-    "k_tube": 8 + 8 + 1,               # R phi; W phi_c, mask
-    "k_adv_stage1": 8 * 5 + 1,         # R phi0 (= phi_s), u, v, w, mask; W phi_1
-    "k_adv_stage23": 8 * 6 + 1,        # R phi0, phi_s, u, v, w, mask; W phi_out
-    "k_reinit_stage1": 8 * 2,          # R phi0 (= phi_s); W phi_n1
-    "k_reinit_stage23": 8 * 3,         # R phi0, phi_s; W phi_out
-    "k_materials": 8 * 3,              # R phi; W miu, rho
-    "k_curvature": 8 * 2,              # R phi; W kappa
-    "k_recurvature": 8 * 3,            # R phi, kappa; W kappa'
-    "k_gfm": 8 * 9,                    # R phi, kappa', u*, v*, w*; W p, p_x, p_y, p_z
-    "k_rfft_x_fwd": 8 * 2,             # R/W p
-    "k_rfft_y_fwd": 8 * 2,
-    "k_rfft_x_inv": 8 * 2,
-    "k_rfft_y_inv": 8 * 2,
-    "k_thomas": 8 * 2,                 # R/W p
-    "k_momentum": 8 * 22,              # R u,v,w,miu,rho,duold x3,pnew,pold,p_xyz,p_xyz_old; W dudt x3, duold x3
-    "k_correc": 8 * 10,                # R u* x3, p, p_x, p_y, p_z; W u, v, w
-    "k_pressure_rotate": 8 * 4,        # R p, pnew; W pnew, pold
-    "k_avg_vel": 8 * 6,                # R u, v, w; W ucc, vcc, wcc
+# kernel -> (number of T arrays, number of P arrays, number of byte masks) read or written
+KERNEL_ARRAYS = {
+    "k_tube": (2, 0, 1),               # R phi; W phi_c, mask
+    "k_adv_stage1": (5, 0, 1),         # R phi0 (= phi_s), u, v, w, mask; W phi_1
+    "k_adv_stage23": (6, 0, 1),        # R phi0, phi_s, u, v, w, mask; W phi_out
+    "k_reinit_stage1": (2, 0, 0),      # R phi0 (= phi_s); W phi_n1
+    "k_reinit_stage23": (3, 0, 0),     # R phi0, phi_s; W phi_out
+    "k_materials": (3, 0, 0),          # R phi; W miu, rho
+    "k_curvature": (2, 0, 0),          # R phi; W kappa
+    "k_recurvature": (3, 0, 0),        # R phi, kappa; W kappa'
+    "k_gfm": (9, 0, 0),                # R phi, kappa', u*, v*, w*; W rhs, p_x, p_y, p_z
+    "k_momentum": (20, 2, 0),          # R u,v,w,miu,rho,duold x3,p_xyz,p_xyz_old; R pnew,pold; W dudt x3, duold x3
+    "k_correc": (9, 1, 0),             # R u* x3, p_x, p_y, p_z; R p; W u, v, w
+    "k_pressure_rotate": (0, 4, 0),    # R p, pnew; W pnew, pold
+    "k_avg_vel": (6, 0, 0),            # R u, v, w; W ucc, vcc, wcc
+    "k_residual": (2, 1, 0),           # R rhs, p; W r (mixed modes only)
 }
 
+# the Poisson passes differ by mode: FP64 works in place on p (P class); the mixed
+# passes read and write the float work array, and the last inverse transform writes p
+_POISSON_FP64 = {"k_rfft_x_fwd": 16, "k_rfft_y_fwd": 16, "k_rfft_y_inv": 16, "k_rfft_x_inv": 16, "k_thomas": 16}
+_POISSON_MIXED = {"k_rfft_x_fwd": 8, "k_rfft_y_fwd": 8, "k_rfft_y_inv": 8, "k_rfft_x_inv": 12, "k_thomas": 8}
 
-def step_bytes_per_cell(calls):
+
+def kernel_bytes(mode_fp32):
+    """{kernel: algorithmic bytes per interior cell} for the FP64 (False) or a mixed
+    (True) mode."""
+    t = 4 if mode_fp32 else 8
+    out = {k: nt * t + npr * 8 + nb for k, (nt, npr, nb) in KERNEL_ARRAYS.items()}
+    out.update(_POISSON_MIXED if mode_fp32 else _POISSON_FP64)
+    if not mode_fp32:
+        del out["k_residual"]
+    return out
+
+
+KERNEL_BYTES_FP64 = kernel_bytes(False)
+KERNEL_BYTES_MIXED = kernel_bytes(True)
+
+
+def step_bytes_per_cell(calls, mode_fp32=False):
     """Algorithmic bytes of one step given {kernel: launches per step}."""
-    return sum(KERNEL_BYTES_FP64.get(k, 0) * n for k, n in calls.items())
+    kb = kernel_bytes(mode_fp32)
+    return sum(kb.get(k, 0) * n for k, n in calls.items())

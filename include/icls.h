@@ -180,6 +180,46 @@ int32_t icls_profile_only(const char* name);
 int32_t icls_profile_reset(void);
 int32_t icls_kernel_times(int32_t n, char* names /* n*64 */, double* total_ms, int64_t* counts, int32_t* n_out);
 
+/* ---- learned curvature closure ----
+ * No counterpart in the reference: an MLP (30 -> 64 -> 64 -> 1, tanh, BF16 matrix
+ * cores with FP32 accumulation) that, in ICLS_MODE_MIXED_LEARNED, replaces the
+ * least-squares curvature of get_curvature (src/interface.f90:20-119) on
+ * under-resolved interface cells: |phi| <= 4 dz and |kappa| dz >= gate.  The
+ * finite-difference value is kept where the network departs from it by more than
+ * tol / dz.
+ *   input  (32): phi on the 3x3x3 stencil / dz (i fastest, then j, then k), the
+ *                unit normal of the least-squares gradient, two zeros
+ *   output     : kappa * dz
+ *   weights    : ICLS_MLP_NPARAM floats, W1 (64 x 32, row-major), b1 (64),
+ *                W2 (64 x 64), b2 (64), w3 (64), b3 (1)
+ * Training runs on the device: samples are spheres and their complements of radius
+ * r_min .. r_max dz whose target curvature is the solver's own operator on a 4x finer
+ * grid; the loss is the mean squared error, minimised with Adam. */
+#define ICLS_MLP_NPARAM 6337
+typedef struct {
+    int64_t n_samples;    /* training samples (default 131072)                 */
+    int32_t epochs;       /* passes over them (default 30)                     */
+    int32_t batch;        /* rows per Adam step, a multiple of 16 (1024)       */
+    float lr;             /* Adam learning rate (default 2e-3)                 */
+    float r_min, r_max;   /* sample radii in dz (default 1.5, 12)              */
+    uint64_t seed;        /* sample generator seed                             */
+    float loss_first;     /* out: mean loss of the first epoch                 */
+    float loss_last;      /* out: mean loss of the last epoch                  */
+} icls_mlp_train_opts;
+int32_t icls_mlp_defaults(icls_mlp_train_opts* o);
+int32_t icls_mlp_train(icls_mlp_train_opts* o);
+int32_t icls_mlp_set_weights(const float* w, int32_t n);
+int32_t icls_mlp_get_weights(float* w, int32_t n);
+int32_t icls_mlp_set_gate(float gate, float tol);
+/* Host-buffer entries for checks: n rows of 32 bf16 features (n a multiple of 16). */
+int32_t icls_mlp_forward(const uint16_t* x_bf16, int32_t n, float* y);
+/* Gradient of the mean squared error: ICLS_MLP_NPARAM values, then the loss. */
+int32_t icls_mlp_gradient(const uint16_t* x_bf16, const float* t, int32_t n, float* grad_loss);
+/* The training generator's samples (features and kappa*dz targets). */
+int32_t icls_mlp_samples(int64_t n, uint64_t seed, float r_min, float r_max, uint16_t* x_bf16, float* t);
+/* Cells the closure was consulted on in the last step (capped at the list size). */
+int32_t icls_mlp_listed(int32_t* n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,153 @@
+"""The learned curvature closure on the GPU: forward and gradients against the CPU
+oracle (oracle/mlp_oracle.py), the training-sample targets against the analytic
+curvature of spheres, training itself, and an ablation against the plain
+finite-difference curvature on under-resolved drops.
+
+Tolerances.  The GPU rounds the matrix operands to bfloat16 where the oracle's
+bf16 emulation does, but accumulates in float32 and uses tanhf; an accumulation
+difference can move a bf16 rounding of an activation by one unit (2^-8 relative).
+Forward outputs are compared to 1e-2 of their range, gradients to 2e-2 of each
+tensor's largest entry, the loss to 1e-3 relative.
+"""
+import numpy as np
+import pytest
+
+from helpers import small_drop
+from iclsamd import params as P
+
+pytestmark = pytest.mark.gpu
+
+
+def _solver(p=None):
+    from iclsamd import _lib
+    return _lib.Solver(p if p is not None else small_drop(16, nz=8))
+
+
+def _params(seed):
+    import mlp_oracle as M
+    rng = np.random.default_rng(seed)
+    p = M.init_params(seed)
+    for k in ("b1", "b2", "b3"):
+        p[k] = rng.standard_normal(p[k].shape) * 0.2
+    # the float32 weights the GPU holds
+    return M.unflatten(M.flatten(p).astype(np.float32).astype(np.float64))
+
+
+def _batch(n, seed):
+    import mlp_oracle as M
+    rng = np.random.default_rng(seed)
+    X = np.zeros((n, M.NPAD))
+    X[:, :M.NIN] = rng.standard_normal((n, M.NIN)) * 2.0
+    xb = M.to_bf16_bits(X)
+    return xb, M.from_bf16_bits(xb), rng.standard_normal(n) * 0.3
+
+
+def test_mlp_forward_matches_oracle():
+    import mlp_oracle as M
+    p = _params(3)
+    xb, X, _ = _batch(512, 8)
+    g = _solver()
+    try:
+        g.mlp_set_weights(M.flatten(p))
+        y = g.mlp_forward(xb)
+    finally:
+        g.close()
+    yo, _ = M.forward(p, X, bf=True)
+    scale = np.max(np.abs(yo))
+    err = np.max(np.abs(y - yo))
+    print("mlp forward: max |gpu - oracle| = %.3e of range %.3e, median %.3e" %
+          (err, scale, np.median(np.abs(y - yo))))
+    assert err <= 1e-2 * scale
+
+
+def test_mlp_gradients_match_oracle():
+    import mlp_oracle as M
+    p = _params(5)
+    xb, X, t = _batch(256, 9)
+    t = t.astype(np.float32).astype(np.float64)
+    g = _solver()
+    try:
+        g.mlp_set_weights(M.flatten(p))
+        grad, loss = g.mlp_gradient(xb, t)
+    finally:
+        g.close()
+    lo, go = M.loss_and_grads(p, X, t, bf=True)
+    gp = M.unflatten(grad)
+    assert abs(loss - lo) <= 1e-3 * lo, (loss, lo)
+    for k in ("W1", "b1", "W2", "b2", "w3", "b3"):
+        e = np.max(np.abs(gp[k] - go[k]))
+        s = np.max(np.abs(go[k]))
+        print("mlp gradient %s: max |gpu - oracle| = %.3e of %.3e" % (k, e, s))
+        assert e <= 2e-2 * s, k
+
+
+def test_mlp_sample_targets_are_sphere_curvature():
+    """Radius 8 dz: the target (the solver's curvature operator on a 4x finer grid,
+    averaged over the 8 fine cells around the coarse centre) must match the
+    curvature of the level surface through the centre, -2/d for a drop and +2/d for
+    a bubble, d = R + phi or R - phi."""
+    import mlp_oracle as M
+    g = _solver()
+    try:
+        xb, t = g.mlp_samples(4096, seed=77, r_min=8.0, r_max=8.0001)
+    finally:
+        g.close()
+    dz = 1.0 / 8                                      # small_drop(16, nz=8): lz = 1
+    phi = M.from_bf16_bits(xb[:, 13]) * dz            # centre value, bf16-rounded
+    R = 8.0 * dz
+    drop = t < 0
+    d = np.where(drop, R + phi, R - phi)
+    exact = np.where(drop, -2.0 / d, 2.0 / d) * dz
+    rel = np.abs(t - exact) / np.abs(exact)
+    nrm = np.sqrt(np.sum(M.from_bf16_bits(xb[:, 27:30]) ** 2, axis=1))
+    print("sample targets: median rel error %.2e, 99%% %.2e; drops %d of %d" %
+          (np.median(rel), np.quantile(rel, 0.99), drop.sum(), len(t)))
+    assert 0.3 < drop.mean() < 0.7
+    assert np.all(np.abs(phi) <= 4.0 * dz * 1.01)
+    assert np.all(np.abs(nrm - 1.0) < 1e-2)
+    # phi is bf16 in the features (2^-9 relative), the fine fit is second order
+    assert np.quantile(rel, 0.99) < 2e-2
+
+
+def test_mlp_training_reduces_loss():
+    g = _solver()
+    try:
+        o = g.mlp_train(n_samples=32768, epochs=8, batch=1024, lr=3e-3)
+    finally:
+        g.close()
+    print("training: loss %.3e -> %.3e" % (o["loss_first"], o["loss_last"]))
+    assert o["loss_last"] < 0.2 * o["loss_first"]
+
+
+def _curvature_errors(mode, R_cells, n=32):
+    p = small_drop(n, r=R_cells / n, mass_correction=0)
+    p.rho2 = p.rho1
+    p.mode = mode
+    g = _solver(p)
+    try:
+        g.init_program()
+        g.gfm4p()
+        kap = g.download(P.F_CURV)[1:-1, 1:-1, 1:-1]
+        phi = g.download(P.F_LVSET)[3:-3, 3:-3, 3:-3]
+        listed = g.mlp_listed() if mode == P.MODE_MIXED_LEARNED else 0
+    finally:
+        g.close()
+    dz = 1.0 / n
+    R = R_cells * dz
+    sel = np.abs(phi) <= 2 * dz
+    exact = -2.0 / (R + phi[sel])
+    rel = np.abs(kap[sel] - exact) / np.abs(exact)
+    return float(np.mean(rel)), float(np.max(rel)), listed
+
+
+@pytest.mark.parametrize("R_cells", [2.5, 3.5])
+def test_learned_curvature_beats_finite_differences_on_small_drops(R_cells):
+    """Ablation: the same under-resolved drop, curvature near the interface (the
+    values the GFM jumps use) against the exact -2/(R + phi), without (MIXED) and
+    with (MIXED_LEARNED) the closure."""
+    fd_mean, fd_max, _ = _curvature_errors(P.MODE_MIXED, R_cells)
+    ml_mean, ml_max, listed = _curvature_errors(P.MODE_MIXED_LEARNED, R_cells)
+    print("R = %.1f dz: mean rel curvature error FD %.3e (max %.3e), learned %.3e (max %.3e), %d cells listed" %
+          (R_cells, fd_mean, fd_max, ml_mean, ml_max, listed))
+    assert listed > 0
+    assert ml_mean < fd_mean
