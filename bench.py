@@ -3,18 +3,28 @@
 
 Workload: the 256^3 rising bubble (1 x 1 x 2 box, density ratio 100:1, viscosity
 ratio 50:1, gravity -z, Eotvos ~ 10, WENO5 level-set advection, re-initialisation and
-mass-correction check every step), plain FP64, synthetic initial fields (no data
-files).  A step is one pass of the reference's time loop body (src/main.f90:105-210):
-level set, AB2 momentum, GFM + Poisson projection, correction, pressure update.
+mass-correction check every step), synthetic initial fields (no data files).  A step
+is one pass of the reference's time loop body (src/main.f90:105-210): level set, AB2
+momentum, GFM + Poisson projection, correction, pressure update.
+
+Modes (--mode):
+  both          (default) mixed+learned, then plain FP64, on fresh contexts
+  fp64          plain FP64, every field and kernel in double
+  mixed         FP32 fields; FP64 pressure, Thomas, Poisson residual + one
+                refinement solve, volume sums
+  learned       mixed plus the learned curvature closure (trained on the device
+                before the timed steps)
 
 One JSON line on stdout:
-  value       = itot*jtot*ktot*K / (seconds of K timed steps) / 1e6
-  roofline    = the dominant kernel's algorithmic bytes per launch / its mean launch
-                time (HIP events on the library stream over the timed region) vs the
-                8 TB/s HBM peak; traffic from rocprofv3 FETCH_SIZE + WRITE_SIZE if a
-                committed measurement for that kernel exists under profiles/
-  cpu_baseline= the CPU oracle (C restatement of the reference, 1 core) on one step
-                of the same workload
+  value       = itot*jtot*ktot*K / (seconds of K timed steps) / 1e6 of the first mode
+                (mixed+learned by default); "fp64" holds the plain FP64 run and
+                "speedup_vs_fp64" the ratio
+  roofline    = that run's dominant kernel: algorithmic bytes per launch / its mean
+                launch time (HIP events on the library stream over the timed region)
+                vs the 8 TB/s HBM peak; traffic from rocprofv3 FETCH_SIZE + WRITE_SIZE
+                where a committed measurement for that kernel exists under profiles/
+  cpu_baseline= the CPU oracle (C restatement of the reference, FP64, 1 core) on one
+                step of the same workload
 """
 import argparse
 import json
@@ -29,6 +39,10 @@ from iclsamd import params as P  # noqa: E402
 from iclsamd import roofline as R  # noqa: E402
 
 METRIC = "Mcell-steps/s at 256^3: mixed+learned vs plain FP64 GPU; %HBM roofline"
+MODES = {"fp64": P.MODE_FP64, "mixed": P.MODE_MIXED, "learned": P.MODE_MIXED_LEARNED}
+DESCR = {"fp64": "plain FP64",
+         "mixed": "FP32 fields with FP64 pressure/Thomas/residual/volumes",
+         "learned": "FP32 fields with FP64 pressure/Thomas/residual/volumes + learned curvature closure"}
 
 
 def cpu_baseline(p, nsteps=1):
@@ -44,7 +58,7 @@ def cpu_baseline(p, nsteps=1):
     o.close()
     ncell = p.itot * p.jtot * p.ktot
     return {"value": ncell * nsteps / el / 1e6, "unit": "Mcell-steps/s", "cores": 1, "kind": "port",
-            "sample": "%d step(s) of the %dx%dx%d workload after an untimed init, %.1f s" %
+            "sample": "%d step(s) of the %dx%dx%d workload (FP64) after an untimed init, %.1f s" %
                       (nsteps, p.itot, p.jtot, p.ktot, el)}
 
 
@@ -59,14 +73,72 @@ def traffic_for(kernel, workload):
         return None
 
 
+def run_mode(mode, n, steps, warmup, show_kernels):
+    """Time `steps` steps of the workload in one precision mode; returns a dict."""
+    from iclsamd import _lib
+    fp32 = mode != "fp64"
+    p = P.bubble256(n, mode=MODES[mode])
+    workload = "bubble%d" % n if mode == "fp64" else "bubble%d_%s" % (n, mode)
+    kbytes = R.kernel_bytes(fp32)
+    ncell = p.itot * p.jtot * p.ktot
+    g = _lib.Solver(p)
+    try:
+        t0 = time.perf_counter()
+        g.init_program()                       # the learned mode trains its closure here
+        g.sync()
+        t_init = time.perf_counter() - t0
+        istep = 0
+        # warm-up: the first step loads every kernel's code object; the others bracket
+        # every launch with events to find the dominant kernel
+        nprof = max(warmup - 1, 1)
+        istep += 1
+        g.step(istep, 0)
+        g.profile(True)
+        for _ in range(nprof):
+            istep += 1
+            g.step(istep, 0)
+        kt = g.kernel_times()
+        known = {k: v for k, v in kt.items() if k in kbytes}
+        dominant = max(known, key=lambda k: known[k][0])
+        if show_kernels:
+            tot = sum(v[0] for v in kt.values())
+            print("-- %s" % mode, file=sys.stderr)
+            for k, v in sorted(kt.items(), key=lambda kv: -kv[1][0]):
+                print("%-22s %9.3f ms/step %6.1f%%" % (k, v[0] / nprof, 100 * v[0] / tot), file=sys.stderr)
+        # timed region: only the dominant kernel is bracketed by events
+        g.profile_reset()
+        g.profile_only(dominant)
+        g.sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            istep += 1
+            g.step(istep, 0)
+        g.sync()
+        el = time.perf_counter() - t0
+        kt = g.kernel_times()
+        g.check_nan()
+        listed = g.mlp_listed() if mode == "learned" else None
+    finally:
+        g.close()
+    ms, launches = kt[dominant]
+    avg_s = ms / 1e3 / launches
+    bytes_per_launch = kbytes[dominant] * ncell
+    achieved = bytes_per_launch / avg_s / 1e9
+    roof = {"kernel": dominant, "bound": "hbm", "achieved": round(achieved, 1), "peak": R.HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / R.HBM_PEAK_GBS, 4),
+            "traffic": traffic_for(dominant, workload), "algorithmic_bytes_per_launch": bytes_per_launch,
+            "avg_launch_ms": round(avg_s * 1e3, 4)}
+    return {"p": p, "workload": workload, "value": ncell * steps / el / 1e6, "ms_per_step": 1e3 * el / steps,
+            "roofline": roof, "init_s": t_init, "listed_cells": listed}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=256, help="cells per direction")
-    ap.add_argument("--mode", choices=["fp64", "mixed"], default="fp64",
-                    help="fp64: plain FP64; mixed: FP32 fields, FP64 pressure, Thomas, residual and volumes")
+    ap.add_argument("--mode", choices=["both", "fp64", "mixed", "learned"], default="both")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--kernels", action="store_true", help="also print the per-kernel table to stderr")
     a = ap.parse_args()
@@ -76,67 +148,30 @@ def main():
               "(multi-GPU sharding is design-only)" % a.gpus, file=sys.stderr)
         sys.exit(2)
 
-    from iclsamd import _lib
-    fp32 = a.mode == "mixed"
-    p = P.bubble256(a.n, mode=P.MODE_MIXED if fp32 else P.MODE_FP64)
-    workload = "bubble%d" % a.n if not fp32 else "bubble%d_mixed" % a.n
-    kbytes = R.kernel_bytes(fp32)
-    ncell = p.itot * p.jtot * p.ktot
-    g = _lib.Solver(p)
-    g.init_program()
-    istep = 0
-    # warm-up: the first step loads every kernel's code object; the others bracket
-    # every launch with events to find the dominant kernel
-    nprof = max(a.warmup - 1, 1)
-    istep += 1
-    g.step(istep, 0)
-    g.profile(True)
-    for _ in range(nprof):
-        istep += 1
-        g.step(istep, 0)
-    kt = g.kernel_times()
-    known = {k: v for k, v in kt.items() if k in kbytes}
-    dominant = max(known, key=lambda k: known[k][0])
-    if a.kernels:
-        tot = sum(v[0] for v in kt.values())
-        for k, v in sorted(kt.items(), key=lambda kv: -kv[1][0]):
-            print("%-22s %9.3f ms/step %6.1f%%" % (k, v[0] / nprof, 100 * v[0] / tot), file=sys.stderr)
-    # timed region: only the dominant kernel is bracketed by events
-    g.profile_reset()
-    g.profile_only(dominant)
-    g.sync()
-    t0 = time.perf_counter()
-    for _ in range(a.steps):
-        istep += 1
-        g.step(istep, 0)
-    g.sync()
-    el = time.perf_counter() - t0
-    kt = g.kernel_times()
-    g.check_nan()
-    g.close()
-
-    ms, launches = kt[dominant]
-    avg_s = ms / 1e3 / launches
-    bytes_per_launch = kbytes[dominant] * ncell
-    achieved = bytes_per_launch / avg_s / 1e9
-    tr = traffic_for(dominant, workload)
-    roof = {"kernel": dominant, "bound": "hbm", "achieved": round(achieved, 1), "peak": R.HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(achieved / R.HBM_PEAK_GBS, 4),
-            "traffic": tr, "algorithmic_bytes_per_launch": bytes_per_launch,
-            "avg_launch_ms": round(avg_s * 1e3, 4)}
-    value = ncell * a.steps / el / 1e6
+    first = "learned" if a.mode == "both" else a.mode
+    r = run_mode(first, a.n, a.steps, a.warmup, a.kernels)
+    p = r["p"]
     line = {
-        "metric": METRIC, "value": round(value, 2), "unit": "Mcell-steps/s", "n_gpus": 1, "steps": a.steps,
-        "warmup": a.warmup, "ms_per_step": round(1e3 * el / a.steps, 4), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64" if fp32 else "f64", "data": "synthetic",
+        "metric": METRIC, "value": round(r["value"], 2), "unit": "Mcell-steps/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": round(r["ms_per_step"], 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64" if first == "fp64" else "f32+f64",
+        "data": "synthetic",
         "config": {"workload": "%s: %dx%dx%d rising bubble, 1x1x2 box, %s, WENO5" %
-                               (workload, p.itot, p.jtot, p.ktot,
-                                "FP32 fields with FP64 pressure/Thomas/residual/volumes" if fp32 else "plain FP64"),
-                   "mode": a.mode, "grid": [p.itot, p.jtot, p.ktot]},
-        "roofline": roof,
+                               (r["workload"], p.itot, p.jtot, p.ktot, DESCR[first]),
+                   "mode": first, "grid": [p.itot, p.jtot, p.ktot]},
+        "roofline": r["roofline"],
     }
+    if first == "learned":
+        line["config"]["closure_cells_last_step"] = r["listed_cells"]
+        line["config"]["init_s_with_training"] = round(r["init_s"], 2)
+    if a.mode == "both":
+        f = run_mode("fp64", a.n, a.steps, a.warmup, a.kernels)
+        line["fp64"] = {"value": round(f["value"], 2), "ms_per_step": round(f["ms_per_step"], 4),
+                        "roofline": f["roofline"]}
+        line["speedup_vs_fp64"] = round(r["value"] / f["value"], 3)
     if not a.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(p)
+        q = P.bubble256(a.n, mode=P.MODE_FP64)
+        line["cpu_baseline"] = cpu_baseline(q)
     print(json.dumps(line))
 
 

@@ -199,6 +199,7 @@ int launch_jiemian(Ctx& c, double* out);
 int launch_sum_interior(Ctx& c, const double* f, int slot);
 int launch_inflation_prepare(Ctx& c, int l, int istep);
 int launch_inflation_velocity(Ctx& c, int l);
+int launch_ipot(Ctx& c, double* dtemp);
 // interface / gfm
 int launch_curvature(Ctx& c, const double* phi, double* kappa, double* normal_m, const double* flag,
                      bool listed = false);
@@ -215,7 +216,8 @@ int mlp_set_weights(Ctx& c, const float* w);
 int mlp_forward(Ctx& c, const uint16_t* x_bf16, int n, float* y);          // device pointers
 int mlp_grad(Ctx& c, const uint16_t* x_bf16, const float* t, int n, float* grad_loss);
 int mlp_adam(Ctx& c, float lr);                                            // after mlp_grad
-int mlp_samples(Ctx& c, long n, unsigned long long seed, float r_min, float r_max, uint16_t* x_bf16, float* t);
+int mlp_samples(Ctx& c, long n, unsigned long long seed, float r_min, float r_max, uint16_t* x_bf16, float* t,
+                float* fd);
 int launch_mlp_reset_list(Ctx& c);
 int launch_mlp_curv(Ctx& c, const double* phi, double* kappa);
 // momentum / correction

@@ -715,6 +715,30 @@ int32_t icls_check_nan(void) {
     return ICLS_OK;
 }
 
+// ipot (interface.f90:225-277) on request: dmin(m,n) = (dtemp(m,n) + dtemp(n,m))/2
+// for m < n, zero elsewhere; lmax x lmax, column-major as the reference's dmin
+int32_t icls_ipot(double* dmin) {
+    CTX_OR_FAIL();
+    if (!dmin) { set_error("null dmin"); return ICLS_ERR_PARAM; }
+    const int L = c.lmax;
+    for (int q = 0; q < L * L; q++) dmin[q] = 0.;
+    if (L < 2) return ICLS_OK;
+    double* dt = nullptr;
+    HIP_OK(hipMalloc((void**)&dt, sizeof(double) * L * L));
+    int rc = launch_ipot(c, dt);
+    std::vector<double> h((size_t)L * L);
+    if (!rc) {
+        hipError_t e = hipMemcpyAsync(h.data(), dt, sizeof(double) * L * L, hipMemcpyDeviceToHost, c.stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
+        if (e != hipSuccess) { set_error("ipot copy: %s", hipGetErrorString(e)); rc = ICLS_ERR_HIP; }
+    }
+    hipFree(dt);
+    if (rc) return rc;
+    for (int m = 0; m < L - 1; m++)
+        for (int n = m + 1; n < L; n++) dmin[m + L * n] = 0.5 * (h[m + (size_t)L * n] + h[n + (size_t)L * m]);
+    return ICLS_OK;
+}
+
 int32_t icls_chkdiv(int32_t istep, double* divtot, double* divmax, int32_t* ijk) {
     CTX_OR_FAIL();
     TRY(launch_chkdiv(c));
@@ -824,6 +848,93 @@ int32_t icls_init_program(void) {
     return init_derived_impl(c);
 }
 
+// ---------------------------------------------------------------- checkpoint / restart
+// the fields of an fld file, in file order (src/loadd.f90:76-100)
+static std::vector<std::pair<int, int>> fld_fields(const Ctx& c) {
+    std::vector<std::pair<int, int>> f = {{ICLS_F_UNEW, 0}, {ICLS_F_VNEW, 0}, {ICLS_F_WNEW, 0}, {ICLS_F_PNEW, 0}};
+    for (int l = 0; l < c.lmax; l++) f.push_back({ICLS_F_LVSET, l});
+    f.push_back({ICLS_F_JIEMIAN, 0});
+    return f;
+}
+
+// between a host array of the field's Fortran shape and its interior
+// (1:imax,1:jmax,1:kmax), x fastest: full -> in when to_interior, else in -> full
+static void copy_interior(const Ctx& c, int field, double* full, double* in, bool to_interior) {
+    int hx, hy, hz, lb;
+    host_dims(c, shape_of(field), hx, hy, hz, lb);
+    const int nx = c.g.nx, ny = c.g.ny, nz = c.g.nz;
+    for (int k = 1; k <= nz; k++)
+        for (int j = 1; j <= ny; j++)
+            for (int i = 1; i <= nx; i++) {
+                const size_t h = (size_t)(i - lb) + (size_t)hx * ((size_t)(j - lb) + (size_t)hy * (size_t)(k - lb));
+                const size_t q = (size_t)(i - 1) + (size_t)nx * ((size_t)(j - 1) + (size_t)ny * (size_t)(k - 1));
+                if (to_interior) in[q] = full[h];
+                else full[h] = in[q];
+            }
+}
+
+static long host_count(const Ctx& c, int field) {
+    int hx, hy, hz, lb;
+    host_dims(c, shape_of(field), hx, hy, hz, lb);
+    return (long)hx * hy * hz;
+}
+
+int32_t icls_write_fld(const char* path, int32_t nr) {
+    CTX_OR_FAIL();
+    if (!path) { set_error("null path"); return ICLS_ERR_PARAM; }
+    const size_t n = (size_t)c.g.nx * c.g.ny * c.g.nz;
+    std::vector<double> full, in(n);
+    FILE* fp = fopen(path, "wb");
+    if (!fp) { set_error("cannot open %s for writing", path); return ICLS_ERR_PARAM; }
+    int rc = ICLS_OK;
+    for (auto& fl : fld_fields(c)) {
+        full.assign((size_t)host_count(c, fl.first), 0.);
+        rc = icls_download(fl.first, fl.second, full.data());
+        if (rc) break;
+        copy_interior(c, fl.first, full.data(), in.data(), true);
+        if (fwrite(in.data(), sizeof(double), n, fp) != n) { set_error("short write to %s", path); rc = ICLS_ERR_PARAM; break; }
+    }
+    if (!rc) {
+        const double info[3] = {c.time, 1. * nr, c.dt};
+        if (fwrite(info, sizeof(double), 3, fp) != 3) { set_error("short write to %s", path); rc = ICLS_ERR_PARAM; }
+    }
+    if (fclose(fp) != 0 && !rc) { set_error("cannot close %s", path); rc = ICLS_ERR_PARAM; }
+    return rc;
+}
+
+int32_t icls_init_program_restart(const char* path, int32_t* begin) {
+    CTX_OR_FAIL();
+    if (!path || !begin) { set_error("null argument"); return ICLS_ERR_PARAM; }
+    FILE* fp = fopen(path, "rb");
+    if (!fp) { set_error("cannot open %s", path); return ICLS_ERR_PARAM; }
+    const size_t n = (size_t)c.g.nx * c.g.ny * c.g.nz;
+    std::vector<double> full, in(n);
+    // init_vel_p (init.f90:148-210), then loadflds(0, begin)
+    double* zf[] = {c.unew, c.vnew, c.wnew, c.dudt, c.dvdt, c.dwdt};
+    int rc = ICLS_OK;
+    for (auto* f : zf) if (!rc) rc = launch_fill(c, f, 0., c.fp32);
+    if (!rc) rc = launch_fill(c, c.pnew, 0., 0);
+    if (!rc) rc = launch_init_velocity(c);
+    for (auto& fl : fld_fields(c)) {
+        if (rc) break;
+        if (fread(in.data(), sizeof(double), n, fp) != n) { set_error("%s: file too short", path); rc = ICLS_ERR_PARAM; break; }
+        if (fl.first == ICLS_F_JIEMIAN) continue;   // update_materials recomputes it
+        full.assign((size_t)host_count(c, fl.first), 0.);
+        if (fl.first != ICLS_F_LVSET) rc = icls_download(fl.first, fl.second, full.data());   // keep halos
+        if (rc) break;
+        copy_interior(c, fl.first, full.data(), in.data(), false);
+        rc = icls_upload(fl.first, fl.second, full.data());
+    }
+    double info[3] = {0., 0., 0.};
+    if (!rc && fread(info, sizeof(double), 3, fp) != 3) { set_error("%s: file too short", path); rc = ICLS_ERR_PARAM; }
+    fclose(fp);
+    if (rc) return rc;
+    c.time = info[0];
+    *begin = (int32_t)info[1];
+    c.dt = info[2];
+    return init_derived_impl(c);   // sets dt = timestep (init.f90:68)
+}
+
 // ---------------------------------------------------------------- one time step
 int32_t icls_step(int32_t istep, int32_t begin) {
     CTX_OR_FAIL();
@@ -872,11 +983,11 @@ int32_t icls_profile_reset(void) {
 int32_t icls_mlp_defaults(icls_mlp_train_opts* o) {
     if (!o) return ICLS_ERR_PARAM;
     o->n_samples = 131072;
-    o->epochs = 30;
+    o->epochs = 60;
     o->batch = 1024;
-    o->lr = 2e-3f;
+    o->lr = 3e-3f;
     o->r_min = 1.5f;
-    o->r_max = 12.f;
+    o->r_max = 8.f;
     o->seed = 20261202ull;
     o->loss_first = o->loss_last = 0.f;
     return ICLS_OK;
@@ -899,15 +1010,17 @@ static int train_impl(Ctx& c, icls_mlp_train_opts* o) {
     HIP_OK(hipMalloc(&x.p, sizeof(uint16_t) * MLP_NPAD * n));
     HIP_OK(hipMalloc(&t.p, sizeof(float) * n));
     HIP_OK(hipMalloc(&lb.p, sizeof(float) * nb));   // the loss of each batch, read once per epoch
-    TRY(mlp_samples(c, n, o->seed, o->r_min, o->r_max, (uint16_t*)x.p, (float*)t.p));
+    TRY(mlp_samples(c, n, o->seed, o->r_min, o->r_max, (uint16_t*)x.p, (float*)t.p, nullptr));
     std::vector<float> losses((size_t)nb);
     for (int e = 0; e < o->epochs; e++) {
+        // cosine decay of the learning rate to 2% over the epochs
+        const float lr = o->lr * (0.02f + 0.98f * 0.5f * (1.f + cosf(3.14159265f * (float)e / (float)o->epochs)));
         for (int b = 0; b < nb; b++) {
             const long r0 = (long)b * o->batch;
             TRY(mlp_grad(c, (const uint16_t*)x.p + r0 * MLP_NPAD, (const float*)t.p + r0, o->batch, nullptr));
             HIP_OK(hipMemcpyAsync((float*)lb.p + b, c.mlp.grad + MLP_NPARAM, sizeof(float), hipMemcpyDeviceToDevice,
                                   c.stream));
-            TRY(mlp_adam(c, o->lr));
+            TRY(mlp_adam(c, lr));
         }
         HIP_OK(hipMemcpyAsync(losses.data(), lb.p, sizeof(float) * nb, hipMemcpyDeviceToHost, c.stream));
         HIP_OK(hipStreamSynchronize(c.stream));
@@ -978,15 +1091,17 @@ int32_t icls_mlp_gradient(const uint16_t* x, const float* t, int32_t n, float* g
     return ICLS_OK;
 }
 
-int32_t icls_mlp_samples(int64_t n, uint64_t seed, float r_min, float r_max, uint16_t* x, float* t) {
+int32_t icls_mlp_samples(int64_t n, uint64_t seed, float r_min, float r_max, uint16_t* x, float* t, float* fd) {
     CTX_OR_FAIL();
     if (!x || !t || n <= 0 || !(r_max > r_min) || !(r_min > 0.f)) { set_error("bad sample request"); return ICLS_ERR_PARAM; }
-    DevBuf dx, dt;
+    DevBuf dx, dt, df;
     HIP_OK(hipMalloc(&dx.p, sizeof(uint16_t) * MLP_NPAD * n));
     HIP_OK(hipMalloc(&dt.p, sizeof(float) * n));
-    TRY(mlp_samples(c, n, seed, r_min, r_max, (uint16_t*)dx.p, (float*)dt.p));
+    if (fd) HIP_OK(hipMalloc(&df.p, sizeof(float) * n));
+    TRY(mlp_samples(c, n, seed, r_min, r_max, (uint16_t*)dx.p, (float*)dt.p, (float*)df.p));
     HIP_OK(hipMemcpyAsync(x, dx.p, sizeof(uint16_t) * MLP_NPAD * n, hipMemcpyDeviceToHost, c.stream));
     HIP_OK(hipMemcpyAsync(t, dt.p, sizeof(float) * n, hipMemcpyDeviceToHost, c.stream));
+    if (fd) HIP_OK(hipMemcpyAsync(fd, df.p, sizeof(float) * n, hipMemcpyDeviceToHost, c.stream));
     HIP_OK(hipStreamSynchronize(c.stream));
     return ICLS_OK;
 }

@@ -279,12 +279,18 @@ int launch_adv_stage(Ctx& c, int scheme, int stage, const double* phi0, const do
 // block; the next plane is loaded into registers while the current one is computed.
 // Interior layers are split into chunks of RZ; the first and the last chunk also
 // write the three ghost layers below and above.  Tiles are taken in XCD order.
-constexpr int RZ = 32;
+#ifndef ICLS_REINIT_RZ
+#define ICLS_REINIT_RZ 32
+#endif
+#ifndef ICLS_REINIT_PD
+#define ICLS_REINIT_PD 4
+#endif
+constexpr int RZ = ICLS_REINIT_RZ;                        // layers per block
 constexpr int RH = 3;                                    // stencil radius
 constexpr int RTW = BX + 2 * RH, RTH = BY + 2 * RH, RTP = RTW * RTH;
 constexpr int RNLD = (RTP + BX * BY - 1) / (BX * BY);
 constexpr int RNS = 2 * RH + 1;                           // planes in the ring
-constexpr int RPD = 4;                                    // planes in flight ahead of the ring
+constexpr int RPD = ICLS_REINIT_PD;                       // planes in flight ahead of the ring
 
 template <class T, int STAGE, bool P2>
 __device__ __forceinline__ T reinit_combine(T p0, T ps, T rhs, T dtau) {

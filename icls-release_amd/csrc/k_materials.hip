@@ -154,6 +154,57 @@ int launch_sum_interior(Ctx& c, const double* f, int slot) {
     return ICLS_OK;
 }
 
+// ---------------------------------------------------------------- ipot
+// interface.f90:225-277: for the ordered pair (m, n), n != m, the minimum over every
+// stored cell of phi_m + phi_n where -dz < phi_m < dz and phi_n < NarrowBand_2, 1e3
+// elsewhere.  The x/y halos of the reference's arrays are periodic copies of stored
+// cells, so the minimum over the stored cells (all z ghost layers included) is the
+// reference's.  One block partial per partner n.
+template <class T>
+__global__ void k_ipot(Grid g, int lmax, int m, const T* const* __restrict__ ls, T dz, T cutoff,
+                       double* __restrict__ part, int nblocks) {
+    __shared__ double smem[16];
+    int i0 = blockIdx.x * BX + threadIdx.x, j0 = blockIdx.y * BY + threadIdx.y;
+    bool valid = i0 < g.nx && j0 < g.ny;
+    for (int n = 0; n < lmax; n++) {
+        if (n == m) continue;
+        double d = 1e3;
+        for (int kk = 0; kk < KCH; kk++) {
+            int k = blockIdx.z * KCH + kk - 2;   // k = -2 .. nz+3
+            if (!valid || k > g.nz + 3) continue;
+            long q = gidx(g, i0, j0, k);
+            T pm = ls[m][q], pn = ls[n][q];
+            if (pm > -T(1.) * dz && pm < T(1.) * dz && pn < cutoff) d = fmin(d, (double)(pm + pn));
+        }
+        double r = block_reduce(d, OpMin(), smem);
+        if (threadIdx.x == 0 && threadIdx.y == 0) part[(long)n * nblocks + block_linear()] = r;
+    }
+}
+
+// dtemp[m + lmax n] for every ordered pair (device array of lmax * lmax doubles)
+int launch_ipot(Ctx& c, double* dtemp) {
+    ProfScope ps(c, "k_ipot");
+    dim3 blk(BX, BY, 1);
+    dim3 grd = red_grid(c.g, -2, c.g.nz + 3);
+    int nb = red_nblocks(grd);
+    if ((long)nb * c.lmax > c.npart_cap) { set_error("reduction scratch too small"); return ICLS_ERR_STATE; }
+    for (int m = 0; m < c.lmax; m++) {
+        if (c.fp32)
+            hipLaunchKernelGGL(k_ipot<float>, grd, blk, 0, c.stream, c.g, c.lmax, m, (const float* const*)c.dls,
+                               (float)c.g.dz, (float)c.NB2, c.dpart, nb);
+        else
+            hipLaunchKernelGGL(k_ipot<double>, grd, blk, 0, c.stream, c.g, c.lmax, m, (const double* const*)c.dls,
+                               c.g.dz, c.NB2, c.dpart, nb);
+        for (int n = 0; n < c.lmax; n++) {
+            if (n == m) continue;
+            hipLaunchKernelGGL(k_reduce_final<OpMin>, dim3(1), dim3(256), 0, c.stream, (const double*)(c.dpart + (long)n * nb),
+                               nb, 1e3, dtemp + m + (long)c.lmax * n);
+        }
+    }
+    HIP_OK(hipGetLastError());
+    return ICLS_OK;
+}
+
 // ---------------------------------------------------------------- mass correction
 // ls_mass_correction trigger (levelset.f90:384-395): maxchk_glb = 0, then inflate
 // when mod(istep,inflate_step) == 0 and |(V - V0)/V0| > 1e-5.

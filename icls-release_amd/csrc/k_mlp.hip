@@ -308,11 +308,13 @@ __device__ __forceinline__ double urand(unsigned long long& s) { return (double)
 // phi = R - |x - c|) of radius R in [r_min, r_max] dz, placed so that the coarse cell
 // centred at the origin lies within 4 dz of the interface.  Features: the exact
 // signed distance on the coarse 3 x 3 x 3 stencil over dz and the unit normal of the
-// least-squares gradient.  Target: kappa dz, where kappa is the mean of the solver's
-// curvature operator (curv_lsq) on the 8 cells of a 4x finer grid around the origin.
+// least-squares gradient.  Reference curvature: the mean of the solver's curvature
+// operator (curv_lsq) on the 8 cells of a 4x finer grid around the origin.  The
+// network learns the correction to the coarse operator's value, so the target is
+// (kappa_fine - kappa_coarse) dz; fd (optional) receives kappa_coarse dz.
 __global__ void k_mlp_samples(long n, unsigned long long seed, double r_min, double r_max, double dx, double dy,
                               double dz, const double* __restrict__ A, const double* __restrict__ Af,
-                              uint16_t* __restrict__ x, float* __restrict__ t) {
+                              uint16_t* __restrict__ x, float* __restrict__ t, float* __restrict__ fd) {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     unsigned long long s = seed ^ (0xD1B54A32D192ED03ull * (unsigned long long)(i + 1));
@@ -361,12 +363,16 @@ __global__ void k_mlp_samples(long n, unsigned long long seed, double r_min, dou
         double m;
         ksum += curv_lsq<double>(Af, vf, 1. / fz, m);
     }
-    t[i] = (float)(ksum / 8. * dz);
+    double mc;
+    const double kc = curv_lsq<double>(A, v, 1. / dz, mc);
+    t[i] = (float)((ksum / 8. - kc) * dz);
+    if (fd) fd[i] = (float)(kc * dz);
 }
 
 // ---------------------------------------------------------------- inference
-// The listed cells (k_curvature): features from phi, the closure's kappa, accepted
-// when within tol / dz of the finite-difference value (which stays otherwise).
+// The listed cells (k_curvature): features from phi; the closure's kappa is the
+// finite-difference value plus the network's correction, accepted when the
+// correction is within tol / dz (the finite-difference value stays otherwise).
 template <class T>
 __global__ void __launch_bounds__(64)
 k_mlp_curv(Grid g, const int* __restrict__ list, const int* __restrict__ count, int cap, const T* __restrict__ A,
@@ -421,7 +427,7 @@ k_mlp_curv(Grid g, const int* __restrict__ list, const int* __restrict__ count, 
                 const long q = qs[4 * lg + r];
                 if (q < 0) continue;
                 const double kfd = (double)kappa[q];
-                double km = (double)o.y[r] / dz;
+                double km = kfd + (double)o.y[r] / dz;
                 const double kmax = 1. / dz;
                 km = km < -kmax ? -kmax : (km > kmax ? kmax : km);
                 if (isfinite(km) && fabs(km - kfd) * dz <= (double)tol) kappa[q] = (T)km;
@@ -535,11 +541,12 @@ int mlp_adam(Ctx& c, float lr) {
     return ICLS_OK;
 }
 
-int mlp_samples(Ctx& c, long n, unsigned long long seed, float r_min, float r_max, uint16_t* x, float* t) {
+int mlp_samples(Ctx& c, long n, unsigned long long seed, float r_min, float r_max, uint16_t* x, float* t,
+                float* fd) {
     ProfScope ps(c, "k_mlp_samples");
     hipLaunchKernelGGL(k_mlp_samples, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, c.stream, n, seed,
                        (double)r_min, (double)r_max, c.g.dx, c.g.dy, c.g.dz, (const double*)c.pinvA,
-                       (const double*)c.mlp.pinvA_fine, x, t);
+                       (const double*)c.mlp.pinvA_fine, x, t, fd);
     HIP_OK(hipGetLastError());
     return ICLS_OK;
 }

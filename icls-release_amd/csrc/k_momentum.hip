@@ -31,7 +31,14 @@ struct MomParams {
 // one-cell x/y halo, so every element of them is fetched once per block; the next
 // plane is loaded into registers while the current one is computed.  Tiles are taken
 // in XCD order (xcd_tile).  x and y wrap with a mask: itot and jtot are powers of two.
-constexpr int MZ = 32;
+#ifndef ICLS_MOM_MZ
+#define ICLS_MOM_MZ 32
+#endif
+#ifndef ICLS_MOM_PD
+#define ICLS_MOM_PD 2
+#endif
+constexpr int MZ = ICLS_MOM_MZ;                          // layers per block
+constexpr int MPD = ICLS_MOM_PD;                         // planes in flight ahead of the ring
 constexpr int TW = BX + 2, TH = BY + 2, TP = TW * TH;   // tile plane with halo
 constexpr int NLD = (TP + BX * BY - 1) / (BX * BY);      // plane elements per thread
 
@@ -61,29 +68,32 @@ k_momentum_zm(Grid g, MomParams m, const double* __restrict__ dscal, const T* __
         const int lx = e % TW - 1, ly = e / TW - 1;
         poff[r] = (long)((x0 + lx) & xm) + g.sy * (long)((y0 + ly) & ym);
     }
-    T nxt[5][NLD];
-    auto fetch = [&](int k) {
+    // planes travel global -> registers -> LDS; the register queue holds MPD planes
+    T nxt[MPD][5][NLD];
+    auto fetch = [&](int k, T (*dst)[NLD]) {
         const long base = g.sz * (long)(k + 2);
 #pragma unroll
         for (int f = 0; f < 5; f++)
 #pragma unroll
-            for (int r = 0; r < NLD; r++) nxt[f][r] = src[f][base + poff[r]];
+            for (int r = 0; r < NLD; r++) dst[f][r] = src[f][base + poff[r]];
     };
-    auto stash = [&](int slot) {
+    auto stash = [&](int slot, T (*from)[NLD]) {
 #pragma unroll
         for (int f = 0; f < 5; f++)
 #pragma unroll
             for (int r = 0; r < NLD; r++) {
                 const int e = tid + r * BX * BY;
-                if (e < TP) ring[f][slot][e] = nxt[f][r];
+                if (e < TP) ring[f][slot][e] = from[f][r];
             }
     };
     auto slot_of = [&](int k) { return (k - kb + 3) % 3; };   // k - 1 .. k + 1 -> three slots
-    fetch(kb - 1);
-    stash(slot_of(kb - 1));
-    fetch(kb);
-    stash(slot_of(kb));
-    fetch(kb + 1);
+    fetch(kb - 1, nxt[0]);
+    stash(slot_of(kb - 1), nxt[0]);
+    fetch(kb, nxt[0]);
+    stash(slot_of(kb), nxt[0]);
+#pragma unroll
+    for (int u = 0; u < MPD; u++)
+        if (kb + u <= ke) fetch(kb + 1 + u, nxt[u]);
 
     const int i = (x0 + tx) & xm, j = (y0 + ty) & ym;
     const bool valid = x0 + tx < g.nx && y0 + ty < g.ny;
@@ -94,10 +104,30 @@ k_momentum_zm(Grid g, MomParams m, const double* __restrict__ dscal, const T* __
     const T iFr = T(m.iFr), visc = T(m.visc), dt = T(m.dt), c15 = T(1.5), c05 = T(0.5), irho0 = T(m.inv_rho0);
     const int c0 = (ty + 1) * TW + tx + 1;   // own cell in the tile plane
 
-    for (int k = kb; k <= ke; k++) {
-        stash(slot_of(k + 1));
+    for (int k0 = kb; k0 <= ke; k0 += MPD)
+#pragma unroll
+    for (int u = 0; u < MPD; u++) {
+        const int k = k0 + u;
+        if (k > ke) continue;   // uniform over the block
+        // the cell's point data, issued before the barrier and the stencil arithmetic
+        const long q = gidx(g, i, j, k);
+        T du0 = T(0.), dv0 = T(0.), dw0 = T(0.), pjx = T(0.), pjy = T(0.), pjz = T(0.);
+        double phq = 0., phx = 0., phy = 0., phz = 0.;
+        if (!RHS_ONLY && valid) {
+            const long qx = gidx(g, ip, j, k), qy = gidx(g, i, jp, k), qz = q + g.sz;
+            du0 = duold[q]; dv0 = dvold[q]; dw0 = dwold[q];
+            // split pressure (ab2.f90:79-101): p_hat = 2 p^n - p^(n-1), p_hat_j = 2 p_j - p_j,old
+            phq = 2. * pnew[q] - pold[q];
+            phx = 2. * pnew[qx] - pold[qx];
+            phy = 2. * pnew[qy] - pold[qy];
+            phz = 2. * pnew[qz] - pold[qz];
+            pjx = two * p_x[q] - p_xold[q];
+            pjy = two * p_y[q] - p_yold[q];
+            pjz = two * p_z[q] - p_zold[q];
+        }
+        stash(slot_of(k + 1), nxt[u]);
         __syncthreads();
-        if (k + 1 <= ke) fetch(k + 2);
+        if (k + MPD <= ke) fetch(k + 1 + MPD, nxt[u]);
         const T* U[3] = {ring[0][slot_of(k - 1)], ring[0][slot_of(k)], ring[0][slot_of(k + 1)]};
         const T* V[3] = {ring[1][slot_of(k - 1)], ring[1][slot_of(k)], ring[1][slot_of(k + 1)]};
         const T* W[3] = {ring[2][slot_of(k - 1)], ring[2][slot_of(k)], ring[2][slot_of(k + 1)]};
@@ -107,7 +137,6 @@ k_momentum_zm(Grid g, MomParams m, const double* __restrict__ dscal, const T* __
         // a(dx, dy, dz) at offset (dx, dy) in plane k + dz
 #define AT(A, dx_, dy_, dz_) A[(dz_) + 1][c0 + (dy_) * TW + (dx_)]
         if (valid) {
-            const long q = gidx(g, i, j, k);
             const T uq = AT(U, 0, 0, 0), vq = AT(V, 0, 0, 0), wq = AT(W, 0, 0, 0);
             // ---- convection (CNTR2)
             T uuip = qt * (AT(U, 1, 0, 0) + uq) * (AT(U, 1, 0, 0) + uq);
@@ -223,19 +252,11 @@ k_momentum_zm(Grid g, MomParams m, const double* __restrict__ dscal, const T* __
                 duold[q] = dun; dvold[q] = dvn; dwold[q] = dwn;
             } else {
                 // ---- AB2 (ab2.f90:55-65)
-                T us = uq + dt * (c15 * dun - c05 * duold[q]);
-                T vs = vq + dt * (c15 * dvn - c05 * dvold[q]);
-                T ws = wq + dt * (c15 * dwn - c05 * dwold[q]);
+                T us = uq + dt * (c15 * dun - c05 * du0);
+                T vs = vq + dt * (c15 * dvn - c05 * dv0);
+                T ws = wq + dt * (c15 * dwn - c05 * dw0);
                 duold[q] = dun; dvold[q] = dvn; dwold[q] = dwn;
-                // ---- split pressure (ab2.f90:79-101, mom.f90:793,818,843)
-                const long qx = gidx(g, ip, j, k), qy = gidx(g, i, jp, k), qz = q + g.sz;
-                double phq = 2. * pnew[q] - pold[q];
-                double phx = 2. * pnew[qx] - pold[qx];
-                double phy = 2. * pnew[qy] - pold[qy];
-                double phz = 2. * pnew[qz] - pold[qz];
-                T pjx = two * p_x[q] - p_xold[q];
-                T pjy = two * p_y[q] - p_yold[q];
-                T pjz = two * p_z[q] - p_zold[q];
+                // ---- split-pressure terms (mom.f90:793,818,843)
                 T mpx = -(one / rho_u - irho0) * T(g.dxi) * T(phx - (double)pjx - phq);
                 T mpy = -(one / rho_v - irho0) * T(g.dyi) * T(phy - (double)pjy - phq);
                 T mpz = -(one / rho_w - irho0) * T(g.dzi) * T(phz - (double)pjz - phq);

@@ -25,6 +25,7 @@ EXPORTS = [
     "icls_sync", "icls_ls_main", "icls_average_stream_vel", "icls_ab2", "icls_fillps", "icls_gfm4p",
     "icls_add_pjump", "icls_solver2d", "icls_boundp", "icls_correc", "icls_bounduvw", "icls_update_pressure",
     "icls_average_vel_at_center", "icls_check_nan", "icls_chkdiv", "icls_chkdt", "icls_max_vel_at_center",
+    "icls_ipot", "icls_write_fld", "icls_init_program_restart",
     "icls_step", "icls_profile_enable", "icls_profile_only", "icls_profile_reset", "icls_kernel_times",
     "icls_mlp_defaults", "icls_mlp_train", "icls_mlp_set_weights", "icls_mlp_get_weights", "icls_mlp_set_gate",
     "icls_mlp_forward", "icls_mlp_gradient", "icls_mlp_samples", "icls_mlp_listed",
@@ -77,6 +78,9 @@ def load():
     L.icls_chkdiv.argtypes = [i32, dp, dp, ctypes.POINTER(i32)]
     L.icls_chkdt.argtypes = [dp]
     L.icls_max_vel_at_center.argtypes = [dp]
+    L.icls_ipot.argtypes = [dp]
+    L.icls_write_fld.argtypes = [ctypes.c_char_p, i32]
+    L.icls_init_program_restart.argtypes = [ctypes.c_char_p, ctypes.POINTER(i32)]
     L.icls_step.argtypes = [i32, i32]
     L.icls_profile_only.argtypes = [ctypes.c_char_p]
     L.icls_kernel_times.argtypes = [i32, ctypes.c_char_p, dp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(i32)]
@@ -88,7 +92,7 @@ def load():
     L.icls_mlp_set_gate.argtypes = [ctypes.c_float, ctypes.c_float]
     L.icls_mlp_forward.argtypes = [u16p, i32, fp]
     L.icls_mlp_gradient.argtypes = [u16p, fp, i32, fp]
-    L.icls_mlp_samples.argtypes = [ctypes.c_int64, ctypes.c_uint64, ctypes.c_float, ctypes.c_float, u16p, fp]
+    L.icls_mlp_samples.argtypes = [ctypes.c_int64, ctypes.c_uint64, ctypes.c_float, ctypes.c_float, u16p, fp, fp]
     L.icls_mlp_listed.argtypes = [ctypes.POINTER(i32)]
     for name in EXPORTS:
         fn = getattr(L, name)
@@ -237,6 +241,23 @@ class Solver:
         self._call("icls_max_vel_at_center", ctypes.byref(v))
         return v.value
 
+    def ipot(self):
+        """dmin (lmax x lmax) of interface.f90:225-277."""
+        L = self.p.lmax
+        d = np.zeros(L * L)
+        self._call("icls_ipot", _dp(d))
+        return d.reshape((L, L), order="F")
+
+    # --- checkpoint / restart (src/loadd.f90)
+    def write_fld(self, path, nr):
+        self._call("icls_write_fld", os.fsencode(path), nr)
+
+    def init_program_restart(self, path):
+        """Restart from an fld file; returns the step number stored in it."""
+        b = ctypes.c_int32()
+        self._call("icls_init_program_restart", os.fsencode(path), ctypes.byref(b))
+        return b.value
+
     # --- learned curvature closure
     def mlp_train(self, **kw):
         """Train on the device; keyword overrides of icls_mlp_train_opts.  Returns the
@@ -277,12 +298,15 @@ class Solver:
                    g.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
         return g[:-1], float(g[-1])
 
-    def mlp_samples(self, n, seed=20261202, r_min=1.5, r_max=12.0):
+    def mlp_samples(self, n, seed=20261202, r_min=1.5, r_max=8.0):
+        """(features (n, 32) bf16 bits, targets = correction * dz, coarse kappa * dz)"""
         x = np.zeros((n, MLP_NPAD), dtype=np.uint16)
         t = np.zeros(n, dtype=np.float32)
+        fd = np.zeros(n, dtype=np.float32)
+        fp = ctypes.POINTER(ctypes.c_float)
         self._call("icls_mlp_samples", n, seed, r_min, r_max, x.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16)),
-                   t.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
-        return x, t
+                   t.ctypes.data_as(fp), fd.ctypes.data_as(fp))
+        return x, t, fd
 
     def mlp_listed(self):
         v = ctypes.c_int32()

@@ -167,6 +167,20 @@ int32_t icls_check_nan(void);                            /* main.f90:207-210    
 int32_t icls_chkdiv(int32_t istep, double* divtot, double* divmax, int32_t* ijk); /* chkdiv.f90:10-90 */
 int32_t icls_chkdt(double* dtmax);                       /* chkdt.f90:11-105            */
 int32_t icls_max_vel_at_center(double* vmax);            /* misc.f90:610-636            */
+/* ipot (interface.f90:225-277): minimal distances between interfaces, dmin(lmax,lmax)
+ * column-major, on request.  In the reference only exjump (the 'attractive' option,
+ * not built) and the per-output diagnostic read dmin, so it is not part of icls_step. */
+int32_t icls_ipot(double* dmin);
+
+/* ---- checkpoint / restart ----
+ * loadflds (src/loadd.f90:16-108) writes the file fld####### the reference reads back:
+ * unew, vnew, wnew, pnew, lvset(l) for l = 1..lmax and jiemian, each the interior
+ * (1:imax,1:jmax,1:kmax) in FP64, x fastest, native byte order; then time, nr, dt. */
+int32_t icls_write_fld(const char* path, int32_t nr);
+/* init_program's restart branch (src/init.f90:44-51, 63-110): init_vel_p, the fields of
+ * the file, then the derived state as icls_init_derived (which resets dt to timestep
+ * and init_volume to the restarted volume, as the reference does).  *begin receives nr. */
+int32_t icls_init_program_restart(const char* path, int32_t* begin);
 /* One whole time step: main.f90:105-210 in the reference order, with the kernels
  * fused where the arithmetic per element is unchanged. */
 int32_t icls_step(int32_t istep, int32_t begin);
@@ -189,19 +203,20 @@ int32_t icls_kernel_times(int32_t n, char* names /* n*64 */, double* total_ms, i
  * tol / dz.
  *   input  (32): phi on the 3x3x3 stencil / dz (i fastest, then j, then k), the
  *                unit normal of the least-squares gradient, two zeros
- *   output     : kappa * dz
+ *   output     : the correction to the least-squares kappa * dz
  *   weights    : ICLS_MLP_NPARAM floats, W1 (64 x 32, row-major), b1 (64),
  *                W2 (64 x 64), b2 (64), w3 (64), b3 (1)
  * Training runs on the device: samples are spheres and their complements of radius
- * r_min .. r_max dz whose target curvature is the solver's own operator on a 4x finer
- * grid; the loss is the mean squared error, minimised with Adam. */
+ * r_min .. r_max dz whose reference curvature is the solver's own operator on a 4x
+ * finer grid; the target is its difference from the coarse operator (times dz), the
+ * loss the mean squared error, minimised with Adam (cosine learning-rate decay). */
 #define ICLS_MLP_NPARAM 6337
 typedef struct {
     int64_t n_samples;    /* training samples (default 131072)                 */
-    int32_t epochs;       /* passes over them (default 30)                     */
+    int32_t epochs;       /* passes over them (default 60)                     */
     int32_t batch;        /* rows per Adam step, a multiple of 16 (1024)       */
-    float lr;             /* Adam learning rate (default 2e-3)                 */
-    float r_min, r_max;   /* sample radii in dz (default 1.5, 12)              */
+    float lr;             /* initial Adam learning rate (default 3e-3)         */
+    float r_min, r_max;   /* sample radii in dz (default 1.5, 8)               */
     uint64_t seed;        /* sample generator seed                             */
     float loss_first;     /* out: mean loss of the first epoch                 */
     float loss_last;      /* out: mean loss of the last epoch                  */
@@ -215,8 +230,9 @@ int32_t icls_mlp_set_gate(float gate, float tol);
 int32_t icls_mlp_forward(const uint16_t* x_bf16, int32_t n, float* y);
 /* Gradient of the mean squared error: ICLS_MLP_NPARAM values, then the loss. */
 int32_t icls_mlp_gradient(const uint16_t* x_bf16, const float* t, int32_t n, float* grad_loss);
-/* The training generator's samples (features and kappa*dz targets). */
-int32_t icls_mlp_samples(int64_t n, uint64_t seed, float r_min, float r_max, uint16_t* x_bf16, float* t);
+/* The training generator's samples: features, targets (the correction times dz) and,
+ * if fd is not NULL, the coarse least-squares kappa * dz. */
+int32_t icls_mlp_samples(int64_t n, uint64_t seed, float r_min, float r_max, uint16_t* x_bf16, float* t, float* fd);
 /* Cells the closure was consulted on in the last step (capped at the list size). */
 int32_t icls_mlp_listed(int32_t* n);
 

@@ -82,16 +82,18 @@ def test_mlp_gradients_match_oracle():
 
 
 def test_mlp_sample_targets_are_sphere_curvature():
-    """Radius 8 dz: the target (the solver's curvature operator on a 4x finer grid,
-    averaged over the 8 fine cells around the coarse centre) must match the
-    curvature of the level surface through the centre, -2/d for a drop and +2/d for
-    a bubble, d = R + phi or R - phi."""
+    """Radius 8 dz: the reference curvature (the solver's curvature operator on a 4x
+    finer grid, averaged over the 8 fine cells around the coarse centre; target +
+    coarse value) must match the curvature of the level surface through the centre,
+    -2/d for a drop and +2/d for a bubble, d = R + phi or R - phi; the coarse operator
+    alone is further off."""
     import mlp_oracle as M
     g = _solver()
     try:
-        xb, t = g.mlp_samples(4096, seed=77, r_min=8.0, r_max=8.0001)
+        xb, corr, fd = g.mlp_samples(4096, seed=77, r_min=8.0, r_max=8.0001)
     finally:
         g.close()
+    t = corr + fd                                     # fine-grid kappa * dz
     dz = 1.0 / 8                                      # small_drop(16, nz=8): lz = 1
     phi = M.from_bf16_bits(xb[:, 13]) * dz            # centre value, bf16-rounded
     R = 8.0 * dz
@@ -107,6 +109,9 @@ def test_mlp_sample_targets_are_sphere_curvature():
     assert np.all(np.abs(nrm - 1.0) < 1e-2)
     # phi is bf16 in the features (2^-9 relative), the fine fit is second order
     assert np.quantile(rel, 0.99) < 2e-2
+    rel_fd = np.abs(fd - exact) / np.abs(exact)
+    print("coarse operator: median rel error %.2e" % np.median(rel_fd))
+    assert np.median(rel_fd) > 4 * np.median(rel)
 
 
 def test_mlp_training_reduces_loss():

@@ -99,6 +99,85 @@ def test_steps_parity_anisotropic_bubble():
         g.close()
 
 
+def test_steps_parity_three_drops():
+    """lmax = 3 ('mdp', init.f90:901-986): one level set per drop, materials from the
+    minimum Heaviside (misc.f90:102-120), GFM jumps accumulated over the drops
+    (gfm.f90:219-233) with fillps and p_jump added separately (main.f90:139-142), and
+    ipot's interface distances (interface.f90:225-277)."""
+    p = P.base_params(32)
+    p.set_drops([(0.3, 0.3, 0.5, 0.12), (0.62, 0.45, 0.5, 0.10), (0.45, 0.7, 0.35, 0.09)])
+    p.mass_correction = 0
+    o, g = _pair(p)
+    try:
+        o.init_program()
+        g.init_program()
+        for s in range(1, 6):
+            o.step(s, 0)
+            g.step(s, 0)
+        _compare(o, g, [f for f in FIELDS if f != P.F_LVSET], TOL, "three drops, 5 steps")
+        for l in range(3):
+            e = rel_linf(g.download(P.F_LVSET, l), o.get(P.F_LVSET, l))
+            assert e <= TOL, (l, e)
+        dmin = g.ipot()
+        for m in range(3):
+            for n in range(3):
+                want = o.ipot_dmin(m, n)
+                assert abs(dmin[m, n] - want) <= TOL * max(abs(want), 1e-300), (m, n, dmin[m, n], want)
+        assert dmin[0, 1] != 0.0 and dmin[0, 1] < 1.0      # drops 1 and 2 lie within the band
+    finally:
+        g.close()
+
+
+def test_restart_from_fld_matches_oracle(tmp_path):
+    """Checkpoint in the reference's fld format (loadd.f90:16-108) and restart
+    (init.f90:44-51): the file holds u, v, w, p, each level set and jiemian (interiors,
+    FP64, x fastest), then time, nr, dt; the restarted GPU run and the oracle loaded
+    from the same file and re-derived (init.f90:63-110) agree step by step."""
+    import pyoracle
+    from iclsamd import _lib
+    p = small_drop(32, mass_correction=0)
+    n = p.itot * p.jtot * p.ktot
+    path = str(tmp_path / "fld0000004")
+    g = _lib.Solver(p)
+    try:
+        g.init_program()
+        for s in range(1, 5):
+            g.step(s, 0)
+        u_before = g.download(P.F_UNEW)
+        g.write_fld(path, 4)
+    finally:
+        g.close()
+    raw = np.fromfile(path, dtype=np.float64)
+    assert raw.size == (4 + p.lmax + 1) * n + 3
+    time_, nr, dt = raw[-3:]
+    assert nr == 4.0 and dt == p.timestep and abs(time_ - 4 * p.timestep) <= 1e-15
+    blocks = [raw[b * n:(b + 1) * n].reshape((p.itot, p.jtot, p.ktot), order="F") for b in range(4 + p.lmax + 1)]
+    assert np.array_equal(blocks[0], u_before[1:-1, 1:-1, 1:-1])
+
+    o = pyoracle.Oracle(p)
+    o.init_program()
+    for b, f in enumerate([P.F_UNEW, P.F_VNEW, P.F_WNEW, P.F_PNEW]):
+        a = np.zeros(P.field_dims(p, f), order="F")
+        a[1:-1, 1:-1, 1:-1] = blocks[b]
+        o.set(f, a)
+    a = np.zeros(P.field_dims(p, P.F_LVSET), order="F")
+    a[3:-3, 3:-3, 3:-3] = blocks[4]
+    o.set(P.F_LVSET, a)
+    o.set_scalar(P.S_TIME, time_)
+    o.init_derived()
+    g = _lib.Solver(p)
+    try:
+        begin = g.init_program_restart(path)
+        assert begin == 4
+        _compare(o, g, FIELDS, 1e-13, "restart state")
+        for s in range(begin + 1, begin + 5):
+            o.step(s, begin)
+            g.step(s, begin)
+        _compare(o, g, FIELDS, TOL, "4 steps after restart")
+    finally:
+        g.close()
+
+
 def test_inflation_parity():
     """One step with the mass correction firing (|V - V0|/V0 = 1e-3)."""
     p = small_drop(32)
