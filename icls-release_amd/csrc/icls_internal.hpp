@@ -109,8 +109,10 @@ struct Ctx {
     double* pf = nullptr;                 // mixed: float work array of the transforms
     // pressure class (double)
     double *pnew = nullptr, *pold = nullptr, *p = nullptr;
-    double* thomas_d = nullptr;           // Thomas multipliers (interior)
-    double* thomas_y = nullptr;           // mixed: Thomas forward-sweep values
+    double* thomas_d = nullptr;           // serial Thomas: multipliers (interior)
+    double* thomas_y = nullptr;           // serial Thomas, mixed: forward-sweep values
+    int thomas_seg = 0;                   // partitioned Thomas: segment length (0: serial)
+    double* thomas_seed = nullptr;        // partitioned Thomas: d below each segment
     double* scratch = nullptr;            // one double array for on-demand downloads
     uint8_t* mask = nullptr;              // narrow band (interior)
     double* dscal = nullptr;              // device scalars
@@ -209,6 +211,8 @@ int launch_fillps(Ctx& c);
 int launch_add_pjump(Ctx& c);
 // poisson: solve L p = frhs into p (solver2d; in the mixed modes with one refinement)
 int launch_solve(Ctx& c);
+int thomas_seg_len(int nz);
+int launch_thomas_seed(Ctx& c);
 // learned curvature closure (k_mlp.hip)
 int mlp_alloc(Ctx& c);
 void mlp_free(Ctx& c);
@@ -216,8 +220,8 @@ int mlp_set_weights(Ctx& c, const float* w);
 int mlp_forward(Ctx& c, const uint16_t* x_bf16, int n, float* y);          // device pointers
 int mlp_grad(Ctx& c, const uint16_t* x_bf16, const float* t, int n, float* grad_loss);
 int mlp_adam(Ctx& c, float lr);                                            // after mlp_grad
-int mlp_samples(Ctx& c, long n, unsigned long long seed, float r_min, float r_max, uint16_t* x_bf16, float* t,
-                float* fd);
+int mlp_samples(Ctx& c, long n, unsigned long long seed, float r_min, float r_max, float gate, uint16_t* x_bf16,
+                float* t, float* fd);
 int launch_mlp_reset_list(Ctx& c);
 int launch_mlp_curv(Ctx& c, const double* phi, double* kappa);
 // momentum / correction

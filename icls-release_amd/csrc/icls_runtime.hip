@@ -144,7 +144,7 @@ static void free_ctx(Ctx* c) {
                                c->u_l, c->v_l, c->w_l, c->dheav, c->ucc_l, c->vcc_l, c->wcc_l, c->normal_m,
                                c->ls_a, c->ls_b, c->ls_c, c->mask, c->thomas_d, c->dscal, c->dpart, c->pinvA,
                                c->xyrt, c->tw_x, c->tw_y, c->dls, c->stash, c->scratch, c->pf, c->thomas_y,
-                               c->pinvA_f, c->tw_x_f, c->tw_y_f};
+                               c->thomas_seed, c->pinvA_f, c->tw_x_f, c->tw_y_f};
     if (c->frhs != c->p) ptrs.push_back(c->frhs);   // in FP64 the right-hand side is p itself
     for (auto* p : ptrs) if (p) hipFree(p);
     for (auto* p : c->lvset) if (p) hipFree(p);
@@ -232,12 +232,19 @@ static int build_ctx(const icls_params* prm) {
     C.lvset.assign(C.lmax, nullptr);
     for (int l = 0; l < C.lmax; l++) TRY(alloc_field(C, &C.lvset[l], C.esz));
     // pressure class: double in every mode
-    double** pfields[] = {&C.pnew, &C.pold, &C.p, &C.thomas_d, &C.scratch};
+    double** pfields[] = {&C.pnew, &C.pold, &C.p, &C.scratch};
     for (auto** f : pfields) TRY(alloc_field(C, f, sizeof(double)));
+    C.thomas_seg = thomas_seg_len(g.nz);
+    if (C.thomas_seg) {
+        const long nseed = (long)g.nx * g.ny * (g.nz / C.thomas_seg - 1);
+        HIP_OK(hipMalloc((void**)&C.thomas_seed, sizeof(double) * (nseed > 0 ? nseed : 1)));
+    } else {
+        TRY(alloc_field(C, &C.thomas_d, sizeof(double)));
+        if (C.fp32) TRY(alloc_field(C, &C.thomas_y, sizeof(double)));
+    }
     if (C.fp32) {
         TRY(alloc_field(C, &C.frhs, sizeof(float)));
         TRY(alloc_field(C, &C.pf, sizeof(float)));
-        TRY(alloc_field(C, &C.thomas_y, sizeof(double)));
     } else {
         C.frhs = C.p;
     }
@@ -273,6 +280,7 @@ static int build_ctx(const icls_params* prm) {
         for (int i = 1; i <= g.nx; i++) xy[(size_t)(i - 1) + (size_t)g.nx * (j - 1)] = xrt[i] + yrt[j];
     HIP_OK(hipMalloc((void**)&C.xyrt, sizeof(double) * xy.size()));
     HIP_OK(hipMemcpy(C.xyrt, xy.data(), sizeof(double) * xy.size(), hipMemcpyHostToDevice));
+    TRY(launch_thomas_seed(C));
     for (int dir = 0; dir < 2; dir++) {
         int n = dir == 0 ? g.nx : g.ny;
         std::vector<double> tw((size_t)n);   // n/2 complex exp(+2 pi i t/n)
@@ -1010,7 +1018,8 @@ static int train_impl(Ctx& c, icls_mlp_train_opts* o) {
     HIP_OK(hipMalloc(&x.p, sizeof(uint16_t) * MLP_NPAD * n));
     HIP_OK(hipMalloc(&t.p, sizeof(float) * n));
     HIP_OK(hipMalloc(&lb.p, sizeof(float) * nb));   // the loss of each batch, read once per epoch
-    TRY(mlp_samples(c, n, o->seed, o->r_min, o->r_max, (uint16_t*)x.p, (float*)t.p, nullptr));
+    // the cells the closure is consulted on, with a margin below the gate
+    TRY(mlp_samples(c, n, o->seed, o->r_min, o->r_max, 0.75f * c.mlp.gate, (uint16_t*)x.p, (float*)t.p, nullptr));
     std::vector<float> losses((size_t)nb);
     for (int e = 0; e < o->epochs; e++) {
         // cosine decay of the learning rate to 2% over the epochs
@@ -1098,7 +1107,7 @@ int32_t icls_mlp_samples(int64_t n, uint64_t seed, float r_min, float r_max, uin
     HIP_OK(hipMalloc(&dx.p, sizeof(uint16_t) * MLP_NPAD * n));
     HIP_OK(hipMalloc(&dt.p, sizeof(float) * n));
     if (fd) HIP_OK(hipMalloc(&df.p, sizeof(float) * n));
-    TRY(mlp_samples(c, n, seed, r_min, r_max, (uint16_t*)dx.p, (float*)dt.p, (float*)df.p));
+    TRY(mlp_samples(c, n, seed, r_min, r_max, 0.f, (uint16_t*)dx.p, (float*)dt.p, (float*)df.p));
     HIP_OK(hipMemcpyAsync(x, dx.p, sizeof(uint16_t) * MLP_NPAD * n, hipMemcpyDeviceToHost, c.stream));
     HIP_OK(hipMemcpyAsync(t, dt.p, sizeof(float) * n, hipMemcpyDeviceToHost, c.stream));
     if (fd) HIP_OK(hipMemcpyAsync(fd, df.p, sizeof(float) * n, hipMemcpyDeviceToHost, c.stream));

@@ -290,7 +290,7 @@ constexpr int RH = 3;                                    // stencil radius
 constexpr int RTW = BX + 2 * RH, RTH = BY + 2 * RH, RTP = RTW * RTH;
 constexpr int RNLD = (RTP + BX * BY - 1) / (BX * BY);
 constexpr int RNS = 2 * RH + 1;                           // planes in the ring
-constexpr int RPD = ICLS_REINIT_PD;                       // planes in flight ahead of the ring
+constexpr int RPD_MAX = ICLS_REINIT_PD;                   // planes in flight ahead of the ring
 
 template <class T, int STAGE, bool P2>
 __device__ __forceinline__ T reinit_combine(T p0, T ps, T rhs, T dtau) {
@@ -299,10 +299,15 @@ __device__ __forceinline__ T reinit_combine(T p0, T ps, T rhs, T dtau) {
     return T(1.) / T(3.) * p0 + T(2.) / T(3.) * (ps + dtau * rhs);
 }
 
-template <class T, int STAGE, bool P2>
+// RPD planes (and phi0 values) are in flight ahead of the step that uses them.  The
+// steps of a chunk run in groups of RPD, unrolled, so the chunk length must be a
+// multiple of RPD (the launcher falls back to RPD = 1 otherwise): a step skipped
+// inside a group would make the memory-counter waits of every step conservative.
+template <class T, int STAGE, bool P2, int RPD>
 __global__ void __launch_bounds__(BX * BY)
 k_reinit_zm(Grid g, T NB2, double* res, const T* phi0, const T* __restrict__ phis, T* out, T* __restrict__ stash) {
     __shared__ T ring[RNS][RTP];
+    __shared__ T p0s[BX * BY];
     __shared__ double smem[16];
     const int ntx = (g.nx + BX - 1) / BX, nty = (g.ny + BY - 1) / BY, ntz = (g.nz + RZ - 1) / RZ;
     const int tile = xcd_tile(blockIdx.x, ntx * nty * ntz);
@@ -341,11 +346,11 @@ k_reinit_zm(Grid g, T NB2, double* res, const T* phi0, const T* __restrict__ phi
         fetch(k, nxt[0]);
         stash_plane(k, nxt[0]);
     }
-#pragma unroll
-    for (int u = 0; u < RPD; u++)
-        if (kb + u <= ke) fetch(kb + u + RH, nxt[u]);
 
-    const bool valid = x0 + tx < g.nx && y0 + ty < g.ny;
+    // A thread past the end of x or y (nx < 64 or ny < 4) takes the wrapped cell and
+    // computes and stores the very values of the thread that owns it (its ring
+    // positions and loads wrap the same way); no branch guards the arithmetic and the
+    // stores, which keeps the memory-counter waits of the loop exact.
     const int i = (x0 + tx) & xm, j = (y0 + ty) & ym;
     const T dz = T(g.dz), dtau = T(0.5) * dz;
     const Sp<T, P2> sx{T(g.dx), T(g.dxi)}, sz{T(g.dz), T(g.dzi)};
@@ -364,24 +369,33 @@ k_reinit_zm(Grid g, T NB2, double* res, const T* phi0, const T* __restrict__ phi
         }
         out[q] = r;
     };
-    if (valid && bz == 0)
+    if (bz == 0)
         for (int k = -2; k <= 0; k++) ghost(k);
 
-    // phi0 of the cell, RPD z steps ahead
+    // The queues: planes of phi_s and the cell's phi0, RPD z steps ahead.  Every queue
+    // load is unconditional, its plane clamped to the last one the chunk reads (a
+    // conditional load would be merged into its register by a copy, and the copy
+    // waits for every load in flight), and the prologue issues them in the order of
+    // the loop, so that the waits on the loop's entry path are no tighter than on its
+    // back edge.  Each step moves its phi0 to the thread's own LDS slot before the
+    // queue register is loaded again, for the same reason.
     T p0q[RPD];
 #pragma unroll
-    for (int u = 0; u < RPD; u++) p0q[u] = (valid && kb + u <= ke) ? phi0[gidx(g, i, j, kb + u)] : T(0.);
+    for (int u = 0; u < RPD; u++) {
+        fetch(min(kb + u, ke) + RH, nxt[u]);
+        p0q[u] = phi0[gidx(g, i, j, min(kb + u, ke))];
+    }
     for (int k0 = kb; k0 <= ke; k0 += RPD) {
 #pragma unroll
         for (int u = 0; u < RPD; u++) {
             const int k = k0 + u;
-            if (k > ke) continue;   // uniform over the block
             stash_plane(k + RH, nxt[u]);
+            p0s[tid] = p0q[u];
             __syncthreads();
-            if (k + RPD <= ke) fetch(k + RPD + RH, nxt[u]);
-            const T p0 = p0q[u];
-            if (valid && k + RPD <= ke) p0q[u] = phi0[gidx(g, i, j, k + RPD)];
-            if (valid) {
+            fetch(min(k + RPD, ke) + RH, nxt[u]);
+            p0q[u] = phi0[gidx(g, i, j, min(k + RPD, ke))];
+            const T p0 = p0s[tid];
+            {
                 const long q = gidx(g, i, j, k);
                 const T* P0 = ring[slot_of(k)];
                 const T ps = P0[c0];
@@ -430,7 +444,7 @@ k_reinit_zm(Grid g, T NB2, double* res, const T* phi0, const T* __restrict__ phi
             __syncthreads();
         }
     }
-    if (valid && bz == ntz - 1)
+    if (bz == ntz - 1)
         for (int k = g.nz + 1; k <= g.nz + 3; k++) ghost(k);
     if (STAGE == 3) {
         double b = block_reduce((double)dmax, OpMax(), smem);
@@ -438,20 +452,28 @@ k_reinit_zm(Grid g, T NB2, double* res, const T* phi0, const T* __restrict__ phi
     }
 }
 
-template <class T>
-static void reinit_dispatch(Ctx& c, int stage, const double* phi0, const double* phis, double* out) {
+template <class T, int RPD>
+static void reinit_dispatch_pd(Ctx& c, int stage, const double* phi0, const double* phis, double* out) {
     dim3 blk(BX, BY, 1);
     const int ntiles = ((c.g.nx + BX - 1) / BX) * ((c.g.ny + BY - 1) / BY) * ((c.g.nz + RZ - 1) / RZ);
     double* res = c.dscal + DS_REINIT_RES;
 #define RS(S, P)                                                                                                  \
-    hipLaunchKernelGGL((k_reinit_zm<T, S, P>), dim3(ntiles), blk, 0, c.stream, c.g, (T)c.NB2, res, as<T>(phi0),    \
-                       as<T>(phis), as<T>(out), as<T>(c.stash))
+    hipLaunchKernelGGL((k_reinit_zm<T, S, P, RPD>), dim3(ntiles), blk, 0, c.stream, c.g, (T)c.NB2, res,           \
+                       as<T>(phi0), as<T>(phis), as<T>(out), as<T>(c.stash))
     if (c.g.pow2) {
         if (stage == 1) RS(1, true); else if (stage == 2) RS(2, true); else RS(3, true);
     } else {
         if (stage == 1) RS(1, false); else if (stage == 2) RS(2, false); else RS(3, false);
     }
 #undef RS
+}
+
+template <class T>
+static void reinit_dispatch(Ctx& c, int stage, const double* phi0, const double* phis, double* out) {
+    // every chunk (RZ layers, the last one nz % RZ) a multiple of RPD_MAX
+    const bool deep = RZ % RPD_MAX == 0 && (c.g.nz % RZ) % RPD_MAX == 0;
+    if (deep) reinit_dispatch_pd<T, RPD_MAX>(c, stage, phi0, phis, out);
+    else reinit_dispatch_pd<T, 1>(c, stage, phi0, phis, out);
 }
 
 int launch_reinit_stage(Ctx& c, int stage, const double* phi0, const double* phis, double* out) {

@@ -311,30 +311,45 @@ __device__ __forceinline__ double urand(unsigned long long& s) { return (double)
 // least-squares gradient.  Reference curvature: the mean of the solver's curvature
 // operator (curv_lsq) on the 8 cells of a 4x finer grid around the origin.  The
 // network learns the correction to the coarse operator's value, so the target is
-// (kappa_fine - kappa_coarse) dz; fd (optional) receives kappa_coarse dz.
-__global__ void k_mlp_samples(long n, unsigned long long seed, double r_min, double r_max, double dx, double dy,
-                              double dz, const double* __restrict__ A, const double* __restrict__ Af,
+// (kappa_fine - kappa_coarse) dz, with kappa_fine limited to the +-1/dz the solver's
+// curvature is limited to; fd (optional) receives kappa_coarse dz.
+//
+// With gate > 0 a sample is drawn again (up to 64 times) until the coarse cell is one
+// the closure is consulted on: |grad phi| >= 0.9 in the least-squares fit and
+// |kappa_coarse| dz >= gate.
+__global__ void k_mlp_samples(long n, unsigned long long seed, double r_min, double r_max, double gate, double dx,
+                              double dy, double dz, const double* __restrict__ A, const double* __restrict__ Af,
                               uint16_t* __restrict__ x, float* __restrict__ t, float* __restrict__ fd) {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     unsigned long long s = seed ^ (0xD1B54A32D192ED03ull * (unsigned long long)(i + 1));
-    const double R = (r_min + (r_max - r_min) * urand(s)) * dz;
-    const double zc = 2. * urand(s) - 1., ang = 2. * 3.14159265358979323846 * urand(s);
-    const double rs = sqrt(fmax(0., 1. - zc * zc));
-    const double nx = rs * cos(ang), ny = rs * sin(ang), nz = zc;
-    const double sg = urand(s) < 0.5 ? 1. : -1.;
-    const double dlo = fmax(0.1 * R, R - 4. * dz), dhi = R + 4. * dz;
-    const double dist = dlo + (dhi - dlo) * urand(s);
-    const double cx = -dist * nx, cy = -dist * ny, cz = -dist * nz;
+    double R, sg, cx, cy, cz, kc, v[27];
+    for (int attempt = 0;; attempt++) {
+        R = (r_min + (r_max - r_min) * urand(s)) * dz;
+        const double zc = 2. * urand(s) - 1., ang = 2. * 3.14159265358979323846 * urand(s);
+        const double rs = sqrt(fmax(0., 1. - zc * zc));
+        const double nx = rs * cos(ang), ny = rs * sin(ang), nz = zc;
+        sg = urand(s) < 0.5 ? 1. : -1.;
+        const double dlo = fmax(0.1 * R, R - 4. * dz), dhi = R + 4. * dz;
+        const double dist = dlo + (dhi - dlo) * urand(s);
+        cx = -dist * nx;
+        cy = -dist * ny;
+        cz = -dist * nz;
+        int q = 0;
+        for (int c = -1; c <= 1; c++)
+            for (int b = -1; b <= 1; b++)
+                for (int a = -1; a <= 1; a++) {
+                    const double ex = a * dx - cx, ey = b * dy - cy, ez = c * dz - cz;
+                    v[q++] = sg * (sqrt(ex * ex + ey * ey + ez * ez) - R);
+                }
+        double mc;
+        kc = curv_lsq<double>(A, v, 1. / dz, mc);
+        if (gate <= 0. || attempt >= 63 || (mc >= 0.9 && fabs(kc) * dz >= gate)) break;
+    }
     auto phi = [&](double px, double py, double pz) {
         const double ex = px - cx, ey = py - cy, ez = pz - cz;
         return sg * (sqrt(ex * ex + ey * ey + ez * ez) - R);
     };
-    double v[27];
-    int q = 0;
-    for (int c = -1; c <= 1; c++)
-        for (int b = -1; b <= 1; b++)
-            for (int a = -1; a <= 1; a++) v[q++] = phi(a * dx, b * dy, c * dz);
     double gx = 0., gy = 0., gz = 0.;
     for (int k = 0; k < 27; k++) {
         gx += A[1 + 10 * k] * v[k];
@@ -363,9 +378,8 @@ __global__ void k_mlp_samples(long n, unsigned long long seed, double r_min, dou
         double m;
         ksum += curv_lsq<double>(Af, vf, 1. / fz, m);
     }
-    double mc;
-    const double kc = curv_lsq<double>(A, v, 1. / dz, mc);
-    t[i] = (float)((ksum / 8. - kc) * dz);
+    const double kf = fmin(fmax(ksum / 8., -1. / dz), 1. / dz);
+    t[i] = (float)((kf - kc) * dz);
     if (fd) fd[i] = (float)(kc * dz);
 }
 
@@ -541,11 +555,11 @@ int mlp_adam(Ctx& c, float lr) {
     return ICLS_OK;
 }
 
-int mlp_samples(Ctx& c, long n, unsigned long long seed, float r_min, float r_max, uint16_t* x, float* t,
-                float* fd) {
+int mlp_samples(Ctx& c, long n, unsigned long long seed, float r_min, float r_max, float gate, uint16_t* x,
+                float* t, float* fd) {
     ProfScope ps(c, "k_mlp_samples");
     hipLaunchKernelGGL(k_mlp_samples, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, c.stream, n, seed,
-                       (double)r_min, (double)r_max, c.g.dx, c.g.dy, c.g.dz, (const double*)c.pinvA,
+                       (double)r_min, (double)r_max, (double)gate, c.g.dx, c.g.dy, c.g.dz, (const double*)c.pinvA,
                        (const double*)c.mlp.pinvA_fine, x, t, fd);
     HIP_OK(hipGetLastError());
     return ICLS_OK;

@@ -26,19 +26,18 @@ struct MomParams {
 // (misc.f90:151-172), formed here from rho.
 //
 // The kernel marches in z.  A block owns a 64 x 4 tile of (i,j) columns over MZ
-// consecutive layers.  u, v, w, miu and rho -- the arrays read
-// at neighbouring cells -- pass through a ring of three LDS planes of the tile with a
-// one-cell x/y halo, so every element of them is fetched once per block; the next
-// plane is loaded into registers while the current one is computed.  Tiles are taken
-// in XCD order (xcd_tile).  x and y wrap with a mask: itot and jtot are powers of two.
+// consecutive layers.  u, v, w, miu and rho -- the arrays read at neighbouring cells
+// -- pass through a ring of four LDS planes of the tile with a one-cell x/y halo, so
+// every element of them is fetched once per block.  Step k reads planes k-1 .. k+1;
+// right after the step's barrier the block issues the loads of plane k+2 and of the
+// cell's point data for step k+1, computes step k while they are in flight, and
+// stores plane k+2 into the fourth slot (that of plane k-2, which no thread reads any
+// more) before the one barrier of the step.  Tiles are taken in XCD order
+// (xcd_tile).  x and y wrap with a mask: itot and jtot are powers of two.
 #ifndef ICLS_MOM_MZ
 #define ICLS_MOM_MZ 32
 #endif
-#ifndef ICLS_MOM_PD
-#define ICLS_MOM_PD 2
-#endif
 constexpr int MZ = ICLS_MOM_MZ;                          // layers per block
-constexpr int MPD = ICLS_MOM_PD;                         // planes in flight ahead of the ring
 constexpr int TW = BX + 2, TH = BY + 2, TP = TW * TH;   // tile plane with halo
 constexpr int NLD = (TP + BX * BY - 1) / (BX * BY);      // plane elements per thread
 
@@ -50,7 +49,7 @@ k_momentum_zm(Grid g, MomParams m, const double* __restrict__ dscal, const T* __
               const double* __restrict__ pold, const T* __restrict__ p_x, const T* __restrict__ p_y,
               const T* __restrict__ p_z, const T* __restrict__ p_xold, const T* __restrict__ p_yold,
               const T* __restrict__ p_zold, T* __restrict__ dudt, T* __restrict__ dvdt, T* __restrict__ dwdt) {
-    __shared__ T ring[5][3][TP];   // u, v, w, miu, rho x three planes
+    __shared__ T ring[5][4][TP];   // u, v, w, miu, rho x four planes
     const int ntx = (g.nx + BX - 1) / BX, nty = (g.ny + BY - 1) / BY, ntz = (g.nz + MZ - 1) / MZ;
     const int tile = xcd_tile(blockIdx.x, ntx * nty * ntz);
     const int bx = tile % ntx, by = (tile / ntx) % nty, bz = tile / (ntx * nty);
@@ -68,8 +67,8 @@ k_momentum_zm(Grid g, MomParams m, const double* __restrict__ dscal, const T* __
         const int lx = e % TW - 1, ly = e / TW - 1;
         poff[r] = (long)((x0 + lx) & xm) + g.sy * (long)((y0 + ly) & ym);
     }
-    // planes travel global -> registers -> LDS; the register queue holds MPD planes
-    T nxt[MPD][5][NLD];
+    // planes travel global -> registers -> LDS
+    T nxt[5][NLD];
     auto fetch = [&](int k, T (*dst)[NLD]) {
         const long base = g.sz * (long)(k + 2);
 #pragma unroll
@@ -86,48 +85,68 @@ k_momentum_zm(Grid g, MomParams m, const double* __restrict__ dscal, const T* __
                 if (e < TP) ring[f][slot][e] = from[f][r];
             }
     };
-    auto slot_of = [&](int k) { return (k - kb + 3) % 3; };   // k - 1 .. k + 1 -> three slots
-    fetch(kb - 1, nxt[0]);
-    stash(slot_of(kb - 1), nxt[0]);
-    fetch(kb, nxt[0]);
-    stash(slot_of(kb), nxt[0]);
-#pragma unroll
-    for (int u = 0; u < MPD; u++)
-        if (kb + u <= ke) fetch(kb + 1 + u, nxt[u]);
+    auto slot_of = [&](int k) { return (k - kb + 4) & 3; };   // k >= kb - 1
+    for (int k = kb - 1; k <= kb + 1; k++) {
+        fetch(k, nxt);
+        stash(slot_of(k), nxt);
+    }
 
+    // A thread past the end of x or y (nx < 64 or ny < 4) takes the wrapped cell: its
+    // ring positions and point loads wrap the same way, so it computes and stores the
+    // very values of the thread that owns the cell.  No branch guards the arithmetic
+    // and the stores, which keeps the memory-counter waits of the loop exact.
     const int i = (x0 + tx) & xm, j = (y0 + ty) & ym;
-    const bool valid = x0 + tx < g.nx && y0 + ty < g.ny;
     const int ip = (i + 1) & xm, jp = (j + 1) & ym;
+    // the cell's point data (AB2 history, pressures, face jumps), loaded one z step
+    // ahead and turned into the values the step uses (du0 .. gpz) before it is loaded
+    // again.  Every load of the loop is unconditional, its layer clamped to the last
+    // one the chunk reads: a conditional load would be merged into its register by a
+    // copy, and the copy waits for every load in flight.
+    T pd[9];
+    double pp[8];
+    auto fetch_point = [&](int k) {
+        if (RHS_ONLY) return;
+        const long q = gidx(g, i, j, k), qx = gidx(g, ip, j, k), qy = gidx(g, i, jp, k), qz = q + g.sz;
+        pd[0] = duold[q]; pd[1] = dvold[q]; pd[2] = dwold[q];
+        pd[3] = p_x[q]; pd[4] = p_xold[q];
+        pd[5] = p_y[q]; pd[6] = p_yold[q];
+        pd[7] = p_z[q]; pd[8] = p_zold[q];
+        pp[0] = pnew[q]; pp[1] = pold[q];
+        pp[2] = pnew[qx]; pp[3] = pold[qx];
+        pp[4] = pnew[qy]; pp[5] = pold[qy];
+        pp[6] = pnew[qz]; pp[7] = pold[qz];
+    };
+    T du0 = T(0.), dv0 = T(0.), dw0 = T(0.);
+    double gpx = 0., gpy = 0., gpz = 0.;
+    const T two_ = T(2.);
+    auto derive = [&]() {
+        if (RHS_ONLY) return;
+        du0 = pd[0]; dv0 = pd[1]; dw0 = pd[2];
+        // split pressure (ab2.f90:79-101): p_hat = 2 p^n - p^(n-1), p_hat_j = 2 p_j - p_j,old
+        const double phq = 2. * pp[0] - pp[1];
+        const double phx = 2. * pp[2] - pp[3];
+        const double phy = 2. * pp[4] - pp[5];
+        const double phz = 2. * pp[6] - pp[7];
+        const T pjx = two_ * pd[3] - pd[4];
+        const T pjy = two_ * pd[5] - pd[6];
+        const T pjz = two_ * pd[7] - pd[8];
+        gpx = phx - (double)pjx - phq;
+        gpy = phy - (double)pjy - phq;
+        gpz = phz - (double)pjz - phq;
+    };
+    fetch_point(kb);
+    derive();
+    __syncthreads();
     const Sp<T, P2> sx{T(g.dx), T(g.dxi)}, sy{T(g.dy), T(g.dyi)}, sz{T(g.dz), T(g.dzi)};
     const T qt = T(0.25), two = T(2.), one = T(1.);
     const T rho_avr = T(dscal[DS_RHO_AVR]);
     const T iFr = T(m.iFr), visc = T(m.visc), dt = T(m.dt), c15 = T(1.5), c05 = T(0.5), irho0 = T(m.inv_rho0);
     const int c0 = (ty + 1) * TW + tx + 1;   // own cell in the tile plane
 
-    for (int k0 = kb; k0 <= ke; k0 += MPD)
-#pragma unroll
-    for (int u = 0; u < MPD; u++) {
-        const int k = k0 + u;
-        if (k > ke) continue;   // uniform over the block
-        // the cell's point data, issued before the barrier and the stencil arithmetic
+    for (int k = kb; k <= ke; k++) {
         const long q = gidx(g, i, j, k);
-        T du0 = T(0.), dv0 = T(0.), dw0 = T(0.), pjx = T(0.), pjy = T(0.), pjz = T(0.);
-        double phq = 0., phx = 0., phy = 0., phz = 0.;
-        if (!RHS_ONLY && valid) {
-            const long qx = gidx(g, ip, j, k), qy = gidx(g, i, jp, k), qz = q + g.sz;
-            du0 = duold[q]; dv0 = dvold[q]; dw0 = dwold[q];
-            // split pressure (ab2.f90:79-101): p_hat = 2 p^n - p^(n-1), p_hat_j = 2 p_j - p_j,old
-            phq = 2. * pnew[q] - pold[q];
-            phx = 2. * pnew[qx] - pold[qx];
-            phy = 2. * pnew[qy] - pold[qy];
-            phz = 2. * pnew[qz] - pold[qz];
-            pjx = two * p_x[q] - p_xold[q];
-            pjy = two * p_y[q] - p_yold[q];
-            pjz = two * p_z[q] - p_zold[q];
-        }
-        stash(slot_of(k + 1), nxt[u]);
-        __syncthreads();
-        if (k + MPD <= ke) fetch(k + 1 + MPD, nxt[u]);
+        fetch(min(k + 2, ke + 1), nxt);
+        fetch_point(min(k + 1, ke));
         const T* U[3] = {ring[0][slot_of(k - 1)], ring[0][slot_of(k)], ring[0][slot_of(k + 1)]};
         const T* V[3] = {ring[1][slot_of(k - 1)], ring[1][slot_of(k)], ring[1][slot_of(k + 1)]};
         const T* W[3] = {ring[2][slot_of(k - 1)], ring[2][slot_of(k)], ring[2][slot_of(k + 1)]};
@@ -136,7 +155,7 @@ k_momentum_zm(Grid g, MomParams m, const double* __restrict__ dscal, const T* __
         const T* RHP = ring[4][slot_of(k + 1)];
         // a(dx, dy, dz) at offset (dx, dy) in plane k + dz
 #define AT(A, dx_, dy_, dz_) A[(dz_) + 1][c0 + (dy_) * TW + (dx_)]
-        if (valid) {
+        {
             const T uq = AT(U, 0, 0, 0), vq = AT(V, 0, 0, 0), wq = AT(W, 0, 0, 0);
             // ---- convection (CNTR2)
             T uuip = qt * (AT(U, 1, 0, 0) + uq) * (AT(U, 1, 0, 0) + uq);
@@ -257,15 +276,17 @@ k_momentum_zm(Grid g, MomParams m, const double* __restrict__ dscal, const T* __
                 T ws = wq + dt * (c15 * dwn - c05 * dw0);
                 duold[q] = dun; dvold[q] = dvn; dwold[q] = dwn;
                 // ---- split-pressure terms (mom.f90:793,818,843)
-                T mpx = -(one / rho_u - irho0) * T(g.dxi) * T(phx - (double)pjx - phq);
-                T mpy = -(one / rho_v - irho0) * T(g.dyi) * T(phy - (double)pjy - phq);
-                T mpz = -(one / rho_w - irho0) * T(g.dzi) * T(phz - (double)pjz - phq);
+                T mpx = -(one / rho_u - irho0) * T(g.dxi) * T(gpx);
+                T mpy = -(one / rho_v - irho0) * T(g.dyi) * T(gpy);
+                T mpz = -(one / rho_w - irho0) * T(g.dzi) * T(gpz);
                 dudt[q] = us + dt * mpx;
                 dvdt[q] = vs + dt * mpy - T(m.forceytot) * dt / rho_v;
                 dwdt[q] = ws + dt * mpz;
             }
         }
 #undef AT
+        derive();
+        stash(slot_of(k + 2), nxt);
         __syncthreads();
     }
 }

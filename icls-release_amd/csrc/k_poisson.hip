@@ -13,6 +13,8 @@
 // on float storage, the last inverse transform writes double p, and one step of
 // iterative refinement follows: r = rhs - L p in double (L is the discrete operator
 // the transforms diagonalise), the same solve on r, p += the correction.
+#include <type_traits>
+
 #include "icls_internal.hpp"
 
 namespace icls {
@@ -29,6 +31,24 @@ template <class TO, class C, int ACC>
 __device__ __forceinline__ void put(TO* out, long q, C v) {
     if (ACC) out[q] = out[q] + (TO)v;
     else out[q] = (TO)v;
+}
+
+// Residual of the discrete Poisson problem the transforms and the Thomas solve
+// invert exactly (periodic 3-point second differences in x and y, the Neumann
+// tridiagonal rows of initsolver.f90:36-49 in z): rhs - L p at one cell, in double.
+template <class T>
+__device__ __forceinline__ double residual_at(const Grid& g, const T* __restrict__ rhs, const double* __restrict__ p,
+                                              int i0, int j0, int k) {
+    const long q = gidx(g, i0, j0, k);
+    const double pc = p[q];
+    const double ax = g.dxi * g.dxi, ay = g.dyi * g.dyi, az = g.dzi * g.dzi;
+    double lx = (p[gidx(g, wrap(i0 + 1, g.nx), j0, k)] - 2. * pc + p[gidx(g, wrap(i0 - 1, g.nx), j0, k)]) * ax;
+    double ly = (p[gidx(g, i0, wrap(j0 + 1, g.ny), k)] - 2. * pc + p[gidx(g, i0, wrap(j0 - 1, g.ny), k)]) * ay;
+    double lz;
+    if (k == 1) lz = (p[gidx(g, i0, j0, 2)] - pc) * az;
+    else if (k == g.nz) lz = (p[gidx(g, i0, j0, k - 1)] - pc) * az;
+    else lz = (p[gidx(g, i0, j0, k + 1)] - 2. * pc + p[gidx(g, i0, j0, k - 1)]) * az;
+    return (double)rhs[q] - (lx + ly + lz);
 }
 
 // In-LDS Stockham radix-2 FFT of M points for one line; TPL threads per line.
@@ -148,7 +168,10 @@ __global__ void k_rfft(Grid g, int N, int LPB, int TPL, const C* __restrict__ tw
 }
 
 // The same transform with the line length N and the lines per block LPB fixed at
-// compile time: every index split is a shift and every stage loop unrolls.
+// compile time: every index split is a shift and every stage loop unrolls.  With
+// RES (forward x only) the input is not read from in but formed as the residual
+// rhs - L p of the double solution p, so the refinement pass starts without a
+// residual array.
 template <int N, int LPB>
 struct RfftShape {
     static constexpr int M = N / 2;
@@ -156,9 +179,10 @@ struct RfftShape {
     static constexpr int NTHR = LPB * TPL < 64 ? 64 : LPB * TPL;
 };
 
-template <int DIR, int INV, int N, int LPB, class C, class TO, int ACC>
+template <int DIR, int INV, int N, int LPB, class C, class TO, int ACC, int RES>
 __global__ void __launch_bounds__((RfftShape<N, LPB>::NTHR))
-k_rfft_n(Grid g, const C* __restrict__ twg, const C* in, TO* out) {
+k_rfft_n(Grid g, const C* __restrict__ twg, const C* in, TO* out, const double* __restrict__ pres) {
+    static_assert(!RES || (DIR == 0 && !INV), "the residual feeds the forward x transform");
     constexpr int M = N / 2;
     constexpr int TPL = RfftShape<N, LPB>::TPL;
     constexpr int NTHR = RfftShape<N, LPB>::NTHR;
@@ -181,7 +205,9 @@ k_rfft_n(Grid g, const C* __restrict__ twg, const C* in, TO* out) {
         for (int e = tid; e < LPB * N; e += NTHR) {
             int ln, n;
             split(e, ln, n);
-            C v = in[gq(ln, n)];
+            C v;
+            if (RES) v = (C)residual_at(g, in, pres, n, line0 + ln, k);
+            else v = in[gq(ln, n)];
             cx<C>* z = bufa + ln * M + (n >> 1);
             if (n & 1) z->im = v; else z->re = v;
         }
@@ -262,35 +288,361 @@ k_rfft_n(Grid g, const C* __restrict__ twg, const C* in, TO* out) {
     }
 }
 
-template <int DIR, int INV, int N, int LPB, class C, class TO, int ACC>
-static bool try_rfft_n(Ctx& c, const C* tw, const C* in, TO* out, int n, int nlines) {
+// ---------------------------------------------------------------- register transform
+// Float lines of length N = 256: the 128-point complex FFT of z_m = x_2m + i x_2m+1
+// with its 128 points spread over 8 threads ("slots"), point m = s + 8 r in register
+// r of slot s.  Each thread does a 16-point DFT over r in registers and applies the
+// twiddles W128^(s k1); one LDS exchange gives every thread two k1 values from all 8
+// slots for the 8-point DFTs (X_k1+16k2); a second exchange pairs X_k with X_128-k for
+// the even/odd split (or, inverse, the split comes first, from global memory).
+// Along y a block is 8 waves over 64 consecutive i: the lane is the line and the wave
+// the slot, so every global access is a 256-byte row segment.  Along x a block is 4
+// waves over 32 lines (consecutive j): 8 lanes per line, slot = lane % 8.
+namespace r16 {
+constexpr int N = 256, M = 128;
+
+__device__ __forceinline__ cx<float> w16(int j, int sg) {
+    constexpr float C1 = 0.92387953251128674f, S1 = 0.38268343236508978f, C2 = 0.70710678118654752f;
+    const float co[16] = {1.f, C1, C2, S1, 0.f, -S1, -C2, -C1, -1.f, -C1, -C2, -S1, 0.f, S1, C2, C1};
+    const float si[16] = {0.f, S1, C2, C1, 1.f, C1, C2, S1, 0.f, -S1, -C2, -C1, -1.f, -C1, -C2, -S1};
+    j &= 15;
+    return {co[j], sg > 0 ? si[j] : -si[j]};
+}
+
+// 4-point DFT in place, exp(SG 2 pi i / 4) = SG i
+template <int SG>
+__device__ __forceinline__ void dft4(cx<float>& x0, cx<float>& x1, cx<float>& x2, cx<float>& x3) {
+    const cx<float> t0 = cadd(x0, x2), t1 = csub(x0, x2), t2 = cadd(x1, x3), t3 = csub(x1, x3);
+    const cx<float> jt3 = SG > 0 ? cx<float>{-t3.im, t3.re} : cx<float>{t3.im, -t3.re};
+    x0 = cadd(t0, t2);
+    x1 = cadd(t1, jt3);
+    x2 = csub(t0, t2);
+    x3 = csub(t1, jt3);
+}
+
+// 16-point DFT of v[0..15] in place (4 x 4); X_k ends at v[4 (k % 4) + k / 4]
+template <int SG>
+__device__ __forceinline__ void dft16(cx<float>* v) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) dft4<SG>(v[q], v[q + 4], v[q + 8], v[q + 12]);
+#pragma unroll
+    for (int q = 1; q < 4; q++)
+#pragma unroll
+        for (int ka = 1; ka < 4; ka++) v[q + 4 * ka] = cmul(v[q + 4 * ka], w16(q * ka, SG));
+#pragma unroll
+    for (int ka = 0; ka < 4; ka++) dft4<SG>(v[4 * ka], v[4 * ka + 1], v[4 * ka + 2], v[4 * ka + 3]);
+}
+
+// 8-point DFT of u[0..7] in place (4 x 2); X_k ends at u[2 (k % 4) + k / 4]
+template <int SG>
+__device__ __forceinline__ void dft8(cx<float>* u) {
+    dft4<SG>(u[0], u[2], u[4], u[6]);
+    dft4<SG>(u[1], u[3], u[5], u[7]);
+    u[3] = cmul(u[3], w16(2, SG));
+    u[5] = cmul(u[5], w16(4, SG));
+    u[7] = cmul(u[7], w16(6, SG));
+#pragma unroll
+    for (int ka = 0; ka < 4; ka++) {
+        const cx<float> a = u[2 * ka], b = u[2 * ka + 1];
+        u[2 * ka] = cadd(a, b);
+        u[2 * ka + 1] = csub(a, b);
+    }
+}
+
+// exp(+2 pi i t / N) for 0 <= t < N from the half table
+__device__ __forceinline__ cx<float> twf(const cx<float>* tw, int t) {
+    return t < M ? tw[t] : cx<float>{-tw[t - M].re, -tw[t - M].im};
+}
+
+// Z_k of the inverse split from X_k and X_(M-k) of the half-complex line
+__device__ __forceinline__ cx<float> inv_split(const cx<float>* tw, int kk, cx<float> X, cx<float> Y) {
+    const cx<float> cy = cconj(Y);
+    const cx<float> t = cmul(tw[kk], csub(X, cy));
+    return cadd(cadd(X, cy), cx<float>{-t.im, t.re});
+}
+
+// F_k of the forward split from X_k and X_(M-k) (k >= 1)
+__device__ __forceinline__ cx<float> fwd_split(const cx<float>* tw, int kk, cx<float> a, cx<float> y) {
+    const cx<float> b = cconj(y);
+    const cx<float> ev = {(a.re + b.re) * 0.5f, (a.im + b.im) * 0.5f};
+    const cx<float> od = {(a.im - b.im) * 0.5f, -(a.re - b.re) * 0.5f};
+    return cadd(ev, cmul(cconj(tw[kk]), od));
+}
+
+// the complex FFT from "v[r] = z_(s + 8 r)" to "u[h][2 (k2 % 4) + k2 / 4] = X_(2s+h+16k2)";
+// xch(k1, t) addresses the exchange slot of (k1, slot t) of this thread's line
+template <int SG, class XCH>
+__device__ __forceinline__ void cfft128(cx<float>* v, cx<float> (*u)[8], int s, const cx<float>* tw, XCH xch) {
+    dft16<SG>(v);
+#pragma unroll
+    for (int k1 = 0; k1 < 16; k1++) {
+        cx<float> val = v[4 * (k1 % 4) + k1 / 4];
+        if (k1) {
+            cx<float> w = twf(tw, 2 * s * k1);
+            if (SG < 0) w.im = -w.im;
+            val = cmul(val, w);
+        }
+        *xch(k1, s) = val;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+#pragma unroll
+        for (int t = 0; t < 8; t++) u[h][t] = *xch(2 * s + h, t);
+        dft8<SG>(u[h]);
+    }
+    __syncthreads();   // the exchange buffer is free again
+}
+}  // namespace r16
+
+template <class TO, int ACC>
+__device__ __forceinline__ void put2(TO* out, long q, float a, float b) {
+    put<TO, float, ACC>(out, q, a);
+    put<TO, float, ACC>(out, q + 1, b);
+}
+
+// along y: 512 threads, 64 lines (i) x 8 slots; LDS 64 KB exchange + twiddles
+template <int INV, class TO, int ACC>
+__global__ void __launch_bounds__(512) k_rfft256_y(Grid g, const float* __restrict__ twg, const float* in, TO* out) {
+    using namespace r16;
+    __shared__ cx<float> xb[M * 64];
+    __shared__ cx<float> tw[M];
+    const int lane = threadIdx.x & 63, s = threadIdx.x >> 6;
+    const int nbl = g.nx / 64;
+    const int tile = xcd_tile(blockIdx.x, nbl * g.nz);
+    const int k = tile / nbl + 1, i = (tile % nbl) * 64 + lane;
+    for (int t = threadIdx.x; t < M; t += 512) tw[t] = {twg[2 * t], twg[2 * t + 1]};
+    const float scale = 0.0625f;   // sqrt(1/256)
+    auto row = [&](int n) -> long { return gidx(g, i, n, k); };
+    cx<float> v[16];
+    if (!INV) {
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            const int m = s + 8 * r;
+            v[r] = {in[row(2 * m)], in[row(2 * m + 1)]};
+        }
+        __syncthreads();
+    } else {
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            const int kk = s + 8 * r;
+            cx<float> X, Y;
+            if (kk == 0) { X = {in[row(0)], 0.f}; Y = {in[row(N - 1)], 0.f}; }
+            else {
+                X = {in[row(2 * kk - 1)], in[row(2 * kk)]};
+                Y = {in[row(2 * (M - kk) - 1)], in[row(2 * (M - kk))]};
+            }
+            v[r] = inv_split(tw, kk, X, Y);
+        }
+    }
+    cx<float> u[2][8];
+    auto xch = [&](int k1, int t) { return &xb[(k1 * 8 + t) * 64 + lane]; };
+    if (!INV) cfft128<-1>(v, u, s, tw, xch);
+    else cfft128<1>(v, u, s, tw, xch);
+    if (INV) {
+#pragma unroll
+        for (int h = 0; h < 2; h++)
+#pragma unroll
+            for (int k2 = 0; k2 < 8; k2++) {
+                const int m = 2 * s + h + 16 * k2;
+                const cx<float> z = u[h][2 * (k2 % 4) + k2 / 4];
+                put<TO, float, ACC>(out, row(2 * m), z.re * scale);
+                put<TO, float, ACC>(out, row(2 * m + 1), z.im * scale);
+            }
+        return;
+    }
+#pragma unroll
+    for (int h = 0; h < 2; h++)
+#pragma unroll
+        for (int k2 = 0; k2 < 8; k2++) xb[((2 * s + h) + 16 * k2) * 64 + lane] = u[h][2 * (k2 % 4) + k2 / 4];
+    __syncthreads();
+#pragma unroll
+    for (int h = 0; h < 2; h++)
+#pragma unroll
+        for (int k2 = 0; k2 < 8; k2++) {
+            const int kk = 2 * s + h + 16 * k2;
+            const cx<float> a = u[h][2 * (k2 % 4) + k2 / 4];
+            if (kk == 0) {
+                put<TO, float, ACC>(out, row(0), (a.re + a.im) * scale);
+                put<TO, float, ACC>(out, row(N - 1), (a.re - a.im) * scale);
+            } else {
+                const cx<float> F = fwd_split(tw, kk, a, xb[(M - kk) * 64 + lane]);
+                put<TO, float, ACC>(out, row(2 * kk - 1), F.re * scale);
+                put<TO, float, ACC>(out, row(2 * kk), F.im * scale);
+            }
+        }
+}
+
+// along x: 256 threads, 32 lines (j) x 8 slots; with RES (forward) the input is
+// rhs - L pres formed on the fly
+template <int INV, class TO, int ACC, int RES>
+__global__ void __launch_bounds__(256) k_rfft256_x(Grid g, const float* __restrict__ twg, const float* in, TO* out,
+                                                  const double* __restrict__ pres) {
+    using namespace r16;
+    static_assert(!RES || !INV, "the residual feeds the forward transform");
+    static_assert(INV || (!ACC && sizeof(TO) == 4), "the forward transform writes the float work array");
+    constexpr int LS = 16 * 8 + 8;    // exchange stride of a line (complex)
+    __shared__ cx<float> xb[32 * LS];
+    __shared__ cx<float> tw[M];
+    const int lane = threadIdx.x & 63, s = lane & 7, line = (threadIdx.x >> 6) * 8 + (lane >> 3);
+    const int nbl = g.ny / 32;
+    const int tile = xcd_tile(blockIdx.x, nbl * g.nz);
+    const int k = tile / nbl + 1, j = (tile % nbl) * 32 + line;
+    for (int t = threadIdx.x; t < M; t += 256) tw[t] = {twg[2 * t], twg[2 * t + 1]};
+    const float scale = 0.0625f;
+    const long q0 = gidx(g, 0, j, k);
+    cx<float>* xl = xb + line * LS;
+    cx<float> v[16];
+    if (RES) {
+        // the residual line goes through LDS so that its stencils are not all in flight
+        float* fl = reinterpret_cast<float*>(xl);
+#pragma unroll 2
+        for (int x = s; x < N; x += 8) fl[x] = (float)residual_at(g, in, pres, x, j, k);
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < 16; r++) v[r] = xl[s + 8 * r];
+        __syncthreads();
+    } else if (!INV) {
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            const float2 a = *reinterpret_cast<const float2*>(in + q0 + 2 * (s + 8 * r));
+            v[r] = {a.x, a.y};
+        }
+        __syncthreads();
+    } else {
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            const int kk = s + 8 * r;
+            cx<float> X, Y;
+            if (kk == 0) { X = {in[q0], 0.f}; Y = {in[q0 + N - 1], 0.f}; }
+            else {
+                X = {in[q0 + 2 * kk - 1], in[q0 + 2 * kk]};
+                Y = {in[q0 + 2 * (M - kk) - 1], in[q0 + 2 * (M - kk)]};
+            }
+            v[r] = inv_split(tw, kk, X, Y);
+        }
+    }
+    cx<float> u[2][8];
+    auto xch = [&](int k1, int t) { return &xl[k1 * 8 + t]; };
+    if (!INV) cfft128<-1>(v, u, s, tw, xch);
+    else cfft128<1>(v, u, s, tw, xch);
+    if (INV) {
+        // x = 4 s + 32 k2 .. +3 holds z_(2s+16k2) and z_(2s+1+16k2)
+#pragma unroll
+        for (int k2 = 0; k2 < 8; k2++) {
+            const cx<float> z0 = u[0][2 * (k2 % 4) + k2 / 4], z1 = u[1][2 * (k2 % 4) + k2 / 4];
+            const long q = q0 + 4 * s + 32 * k2;
+            if (sizeof(TO) == 4 && !ACC) {
+                *reinterpret_cast<float4*>(out + q) = make_float4(z0.re * scale, z0.im * scale, z1.re * scale, z1.im * scale);
+            } else {
+                put2<TO, ACC>(out, q, z0.re * scale, z0.im * scale);
+                put2<TO, ACC>(out, q + 2, z1.re * scale, z1.im * scale);
+            }
+        }
+        return;
+    }
+#pragma unroll
+    for (int h = 0; h < 2; h++)
+#pragma unroll
+        for (int k2 = 0; k2 < 8; k2++) xl[(2 * s + h) + 16 * k2] = u[h][2 * (k2 % 4) + k2 / 4];
+    __syncthreads();
+    float o[2][8][2];
+#pragma unroll
+    for (int h = 0; h < 2; h++)
+#pragma unroll
+        for (int k2 = 0; k2 < 8; k2++) {
+            const int kk = 2 * s + h + 16 * k2;
+            const cx<float> a = u[h][2 * (k2 % 4) + k2 / 4];
+            if (kk == 0) {
+                o[h][k2][0] = (a.re + a.im) * scale;   // x = 0
+                o[h][k2][1] = (a.re - a.im) * scale;   // x = N - 1
+            } else {
+                const cx<float> F = fwd_split(tw, kk, a, xl[M - kk]);
+                o[h][k2][0] = F.re * scale;            // x = 2 kk - 1
+                o[h][k2][1] = F.im * scale;            // x = 2 kk
+            }
+        }
+    __syncthreads();
+    // stage the line in LDS (floats) for aligned, coalesced stores
+    float* fl = reinterpret_cast<float*>(xl);
+#pragma unroll
+    for (int h = 0; h < 2; h++)
+#pragma unroll
+        for (int k2 = 0; k2 < 8; k2++) {
+            const int kk = 2 * s + h + 16 * k2;
+            if (kk == 0) { fl[0] = o[h][k2][0]; fl[N - 1] = o[h][k2][1]; }
+            else { fl[2 * kk - 1] = o[h][k2][0]; fl[2 * kk] = o[h][k2][1]; }
+        }
+    __syncthreads();
+    // 32 lines x 64 float4: thread t stores float4 t % 64 of lines t / 64 + 4 it
+#pragma unroll
+    for (int it = 0; it < 8; it++) {
+        const int ln = (threadIdx.x >> 6) + 4 * it, c4 = threadIdx.x & 63;
+        const float* src = reinterpret_cast<const float*>(xb + ln * LS) + 4 * c4;
+        const long q = gidx(g, 4 * c4, (tile % nbl) * 32 + ln, k);
+        *reinterpret_cast<float4*>(out + q) = make_float4(src[0], src[1], src[2], src[3]);
+    }
+}
+
+// the register kernels cover float lines of 256 with 64 | nx (y) or 32 | ny (x)
+template <int DIR, int INV, class C, class TO, int ACC, int RES>
+static bool try_rfft256(Ctx& c, const C* tw, const C* in, TO* out, const double* pres) {
+    if constexpr (!std::is_same<C, float>::value) return false;
+    else {
+        const int N = DIR == 0 ? c.g.nx : c.g.ny;
+        if (N != 256) return false;
+        if (DIR == 1) {
+            if constexpr (RES) return false;
+            else {
+                if (c.g.nx % 64) return false;
+                hipLaunchKernelGGL((k_rfft256_y<INV, TO, ACC>), dim3((c.g.nx / 64) * c.g.nz), dim3(512), 0, c.stream,
+                                   c.g, tw, in, out);
+                return true;
+            }
+        }
+        if (c.g.ny % 32) return false;
+        hipLaunchKernelGGL((k_rfft256_x<INV, TO, ACC, RES>), dim3((c.g.ny / 32) * c.g.nz), dim3(256), 0, c.stream,
+                           c.g, tw, in, out, pres);
+        return true;
+    }
+}
+
+template <int DIR, int INV, int N, int LPB, class C, class TO, int ACC, int RES>
+static bool try_rfft_n(Ctx& c, const C* tw, const C* in, TO* out, int n, int nlines, const double* pres) {
     if (n != N || nlines % LPB != 0) return false;
-    hipLaunchKernelGGL((k_rfft_n<DIR, INV, N, LPB, C, TO, ACC>), dim3((nlines / LPB) * c.g.nz),
-                       dim3(RfftShape<N, LPB>::NTHR), 0, c.stream, c.g, tw, in, out);
+    hipLaunchKernelGGL((k_rfft_n<DIR, INV, N, LPB, C, TO, ACC, RES>), dim3((nlines / LPB) * c.g.nz),
+                       dim3(RfftShape<N, LPB>::NTHR), 0, c.stream, c.g, tw, in, out, pres);
     return true;
 }
 
 // lines per block: along x 4; along y enough lines for 64-byte row segments
-template <int DIR, int INV, class C, class TO, int ACC>
-static bool rfft_fixed(Ctx& c, const C* tw, const C* in, TO* out, int n, int nlines) {
+template <int DIR, int INV, class C, class TO, int ACC, int RES>
+static bool rfft_fixed(Ctx& c, const C* tw, const C* in, TO* out, int n, int nlines, const double* pres) {
     constexpr int L = DIR == 0 ? 4 : (sizeof(C) == 4 ? 16 : 8);
     constexpr int L512 = L > 8 ? 8 : 4, L1024 = L > 8 ? 4 : 2;
-    return try_rfft_n<DIR, INV, 32, L, C, TO, ACC>(c, tw, in, out, n, nlines) ||
-           try_rfft_n<DIR, INV, 64, L, C, TO, ACC>(c, tw, in, out, n, nlines) ||
-           try_rfft_n<DIR, INV, 128, L, C, TO, ACC>(c, tw, in, out, n, nlines) ||
-           try_rfft_n<DIR, INV, 256, L, C, TO, ACC>(c, tw, in, out, n, nlines) ||
-           try_rfft_n<DIR, INV, 512, L512, C, TO, ACC>(c, tw, in, out, n, nlines) ||
-           try_rfft_n<DIR, INV, 1024, L1024, C, TO, ACC>(c, tw, in, out, n, nlines);
+    return try_rfft_n<DIR, INV, 32, L, C, TO, ACC, RES>(c, tw, in, out, n, nlines, pres) ||
+           try_rfft_n<DIR, INV, 64, L, C, TO, ACC, RES>(c, tw, in, out, n, nlines, pres) ||
+           try_rfft_n<DIR, INV, 128, L, C, TO, ACC, RES>(c, tw, in, out, n, nlines, pres) ||
+           try_rfft_n<DIR, INV, 256, L, C, TO, ACC, RES>(c, tw, in, out, n, nlines, pres) ||
+           try_rfft_n<DIR, INV, 512, L512, C, TO, ACC, RES>(c, tw, in, out, n, nlines, pres) ||
+           try_rfft_n<DIR, INV, 1024, L1024, C, TO, ACC, RES>(c, tw, in, out, n, nlines, pres);
 }
 
-template <int DIR, int INV, class C, class TO, int ACC>
-static int rfft_launch(Ctx& c, const C* tw, const C* in, TO* out) {
+// Transform of every line along DIR.  With RES the forward x transform takes
+// rhs - L pres as its input (in = rhs); false means no fixed-size kernel covers the
+// shape and the caller forms the residual itself.
+template <int DIR, int INV, class C, class TO, int ACC, int RES = 0>
+static int rfft_launch(Ctx& c, const C* tw, const C* in, TO* out, const double* pres = nullptr) {
     const int N = DIR == 0 ? c.g.nx : c.g.ny;
     const int nlines = DIR == 0 ? c.g.ny : c.g.nx;
-    if (rfft_fixed<DIR, INV, C, TO, ACC>(c, tw, in, out, N, nlines)) {
+    if (try_rfft256<DIR, INV, C, TO, ACC, RES>(c, tw, in, out, pres) ||
+        rfft_fixed<DIR, INV, C, TO, ACC, RES>(c, tw, in, out, N, nlines, pres)) {
         HIP_OK(hipGetLastError());
         return ICLS_OK;
     }
+    if (RES) return -1;
     // any other power of two: the runtime-sized kernel
     const int M = N / 2;
     int LPB = DIR == 0 ? 4 : 16;
@@ -399,9 +751,188 @@ __global__ void __launch_bounds__(64) k_thomas(Grid g, const double* __restrict_
     }
 }
 
+// The same elimination with every column cut into P = nz / S segments of S layers.
+// Wave s of a block holds segment s of 64 consecutive columns in registers, so the
+// multipliers and the forward-sweep values never leave the chip.
+//
+// The multipliers d(k) do not depend on the right-hand side: each segment restarts
+// their recurrence from d at the layer below it (dseed, computed once per grid by
+// k_thomas_seed with the same operations), so every d(k) and 1/pivot is bit for bit
+// the serial one.  The forward sweep y(k) = (r(k) - a y(k-1)) / pivot(k) is affine in
+// the value entering a segment: a first pass over the segment with y = 0 entering
+// gives its exit value and the coefficient of the entering value; the carries are
+// folded across segments through LDS, and a second pass repeats the serial
+// recurrence from the true entering value.  The back substitution
+// p(k) = y(k) - d(k) p(k+1) is treated the same way from the top of the segment.
+// Only the entering values differ from the serial sweep, by rounding.
+template <int S>
+struct ThomasRow {
+    double a, cc, bmid, b1, bN, lam;
+    int n;
+    // 1 / pivot at layer k (k < n), or the pivot itself at k = n; advances d
+    __device__ __forceinline__ double piv(int k, double& dprev) const {
+        double z;
+        if (k == 1) {
+            z = 1. / (b1 + lam);
+            dprev = cc * z;
+        } else if (k < n) {
+            double bb = bmid + lam;
+            z = 1. / (bb - a * dprev);
+            dprev = cc * z;
+        } else {
+            double bb = bN + lam;
+            z = bb - a * dprev;
+        }
+        return z;
+    }
+    __device__ __forceinline__ double fwd(int k, double rk, double z, double yprev) const {
+        if (k == 1) return rk * z;
+        if (k < n) return (rk - a * yprev) * z;
+        return z != 0. ? (rk - a * yprev) / z : 0.;
+    }
+};
+
+__global__ void k_thomas_seed(Grid g, const double* __restrict__ xyrt, double* __restrict__ dseed, int S) {
+    const long ncol = (long)g.nx * g.ny;
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncol) return;
+    ThomasRow<1> R;
+    R.a = g.dzi * g.dzi;
+    R.cc = g.dzi * g.dzi;
+    R.bmid = -(R.a + R.cc);
+    R.b1 = R.bmid + R.a;
+    R.bN = R.bmid + R.cc;
+    R.lam = xyrt[c];
+    R.n = g.nz;
+    double dprev = 0.;
+    for (int k = 1; k < g.nz; k++) {
+        R.piv(k, dprev);
+        if (k % S == 0) dseed[(long)(k / S - 1) * ncol + c] = dprev;
+    }
+}
+
+// CPW columns per wave: with 64 a wave is one segment of 64 columns; with 32 a wave
+// holds two segments of 32 columns (fewer registers per lane at S = 8, so that two
+// blocks fit a CU).  A block is P segments x CPW columns.
+#ifndef ICLS_THOMAS_CPW
+#define ICLS_THOMAS_CPW 64
+#endif
+constexpr int TCPW = ICLS_THOMAS_CPW;
+constexpr int TMAXSEG = 16 * (64 / TCPW);   // segments per block at 1024 threads
+
+template <class TS, int S>
+__global__ void __launch_bounds__(1024, (TCPW == 32 && S <= 8 ? 8 : 4))
+k_thomas_seg(Grid g, const double* __restrict__ xyrt, const double* __restrict__ dseed, TS* p) {
+    __shared__ double cv[TMAXSEG][TCPW], cf[TMAXSEG][TCPW];   // segment carries: value, coefficient
+    const int lane = threadIdx.x % TCPW, s = threadIdx.x / TCPW;
+    const int P = blockDim.x / TCPW;
+    const long ncol = (long)g.nx * g.ny;
+    const long c = (long)blockIdx.x * TCPW + lane;
+    const bool act = c < ncol;
+    ThomasRow<S> R;
+    R.a = g.dzi * g.dzi;
+    R.cc = g.dzi * g.dzi;
+    R.bmid = -(R.a + R.cc);
+    R.b1 = R.bmid + R.a;
+    R.bN = R.bmid + R.cc;
+    R.lam = act ? xyrt[c] : 0.;
+    R.n = g.nz;
+    const int k0 = s * S + 1;
+    const long q0 = gidx(g, 0, 0, k0) + c;
+    double y[S], z[S];
+#pragma unroll
+    for (int t = 0; t < S; t++) y[t] = act ? (double)p[q0 + (long)t * g.sz] : 0.;
+    double dprev = (s > 0 && act) ? dseed[(long)(s - 1) * ncol + c] : 0.;
+#pragma unroll
+    for (int t = 0; t < S; t++) z[t] = R.piv(k0 + t, dprev);
+    // forward: exit value with 0 entering, and the coefficient of the entering value
+    {
+        double v = 0., f = 1.;
+#pragma unroll
+        for (int t = 0; t < S; t++) {
+            const int k = k0 + t;
+            v = R.fwd(k, y[t], z[t], v);
+            if (k == 1) f = 0.;
+            else if (k < R.n) f = -R.a * z[t] * f;
+            else f = z[t] != 0. ? -R.a * f / z[t] : 0.;
+        }
+        cv[s][lane] = v;
+        cf[s][lane] = f;
+    }
+    __syncthreads();
+    double yin = 0.;
+    for (int t = 0; t < s; t++) yin = cv[t][lane] + cf[t][lane] * yin;
+#pragma unroll
+    for (int t = 0; t < S; t++) {
+        yin = R.fwd(k0 + t, y[t], z[t], yin);
+        y[t] = yin;
+    }
+    __syncthreads();   // the carries are read before they are replaced
+    // back substitution: value at the segment's first layer with 0 above it, and the
+    // coefficient of the value above
+    {
+        double v = 0., f = 1.;
+#pragma unroll
+        for (int t = S - 1; t >= 0; t--) {
+            const int k = k0 + t;
+            if (k == R.n) { v = y[t]; f = 0.; }
+            else {
+                const double d = R.cc * z[t];
+                v = y[t] - d * v;
+                f = -d * f;
+            }
+        }
+        cv[s][lane] = v;
+        cf[s][lane] = f;
+    }
+    __syncthreads();
+    double pa = 0.;
+    for (int t = P - 1; t > s; t--) pa = cv[t][lane] + cf[t][lane] * pa;
+#pragma unroll
+    for (int t = S - 1; t >= 0; t--) {
+        const int k = k0 + t;
+        if (k == R.n) pa = y[t];
+        else pa = y[t] - (R.cc * z[t]) * pa;
+        y[t] = pa;
+    }
+    if (act)
+#pragma unroll
+        for (int t = 0; t < S; t++) p[q0 + (long)t * g.sz] = (TS)y[t];
+}
+
+// Segment length of the partitioned solve for nz layers, 0 where the serial kernel
+// runs (a block holds at most TMAXSEG segments).
+int thomas_seg_len(int nz) {
+    if (TCPW == 64 && nz % 16 == 0 && nz / 16 <= TMAXSEG) return 16;
+    if (nz % 8 == 0 && nz / 8 <= TMAXSEG) return 8;
+    return 0;
+}
+
+int launch_thomas_seed(Ctx& c) {
+    if (!c.thomas_seg) return ICLS_OK;
+    const long ncol = (long)c.g.nx * c.g.ny;
+    hipLaunchKernelGGL(k_thomas_seed, dim3((unsigned)((ncol + 63) / 64)), dim3(64), 0, c.stream, c.g,
+                       (const double*)c.xyrt, c.thomas_seed, c.thomas_seg);
+    HIP_OK(hipGetLastError());
+    return ICLS_OK;
+}
+
 template <class TS>
 static int thomas_launch(Ctx& c, TS* f, double* y) {
     ProfScope ps(c, "k_thomas");
+    if (c.thomas_seg) {
+        const long ncol = (long)c.g.nx * c.g.ny;
+        const int P = c.g.nz / c.thomas_seg;
+        dim3 grd((unsigned)((ncol + TCPW - 1) / TCPW)), blk(TCPW * P);
+        if (c.thomas_seg == 16)
+            hipLaunchKernelGGL((k_thomas_seg<TS, 16>), grd, blk, 0, c.stream, c.g, (const double*)c.xyrt,
+                               (const double*)c.thomas_seed, f);
+        else
+            hipLaunchKernelGGL((k_thomas_seg<TS, 8>), grd, blk, 0, c.stream, c.g, (const double*)c.xyrt,
+                               (const double*)c.thomas_seed, f);
+        HIP_OK(hipGetLastError());
+        return ICLS_OK;
+    }
     int bx = c.g.nx < 64 ? c.g.nx : 64;
     hipLaunchKernelGGL(k_thomas<TS>, dim3((c.g.nx + bx - 1) / bx, c.g.ny, 1), dim3(bx), 0, c.stream, c.g,
                        (const double*)c.xyrt, f, c.thomas_d, y);
@@ -409,23 +940,12 @@ static int thomas_launch(Ctx& c, TS* f, double* y) {
     return ICLS_OK;
 }
 
-// Residual of the discrete Poisson problem the transforms and the Thomas solve
-// invert exactly (periodic 3-point second differences in x and y, the Neumann
-// tridiagonal rows of initsolver.f90:36-49 in z): r = rhs - L p, in double.
+// the residual as an array, for shapes the fused forward transform does not cover
 template <class T>
 __global__ void k_residual(Grid g, const T* __restrict__ rhs, const double* __restrict__ p, T* __restrict__ r) {
     int i0 = blockIdx.x * 64 + threadIdx.x, j0 = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z + 1;
     if (i0 >= g.nx || j0 >= g.ny) return;
-    const long q = gidx(g, i0, j0, k);
-    const double pc = p[q];
-    const double ax = g.dxi * g.dxi, ay = g.dyi * g.dyi, az = g.dzi * g.dzi;
-    double lx = (p[gidx(g, wrap(i0 + 1, g.nx), j0, k)] - 2. * pc + p[gidx(g, wrap(i0 - 1, g.nx), j0, k)]) * ax;
-    double ly = (p[gidx(g, i0, wrap(j0 + 1, g.ny), k)] - 2. * pc + p[gidx(g, i0, wrap(j0 - 1, g.ny), k)]) * ay;
-    double lz;
-    if (k == 1) lz = (p[gidx(g, i0, j0, 2)] - pc) * az;
-    else if (k == g.nz) lz = (p[gidx(g, i0, j0, k - 1)] - pc) * az;
-    else lz = (p[gidx(g, i0, j0, k + 1)] - 2. * pc + p[gidx(g, i0, j0, k - 1)]) * az;
-    r[q] = (T)((double)rhs[q] - (lx + ly + lz));
+    r[gidx(g, i0, j0, k)] = (T)residual_at(g, rhs, p, i0, j0, k);
 }
 
 // ---------------------------------------------------------------- the solve
@@ -454,17 +974,31 @@ static int solve_fp64(Ctx& c) {
     return rfft_launch<0, 1, double, double, 0>(c, twx, p, p);
 }
 
-// one float-transform solve of L x = src (src float, overwritten as work array),
-// result written to (ACC = 0) or added into (ACC = 1) the double pressure
+// one float-transform solve, result written to (ACC = 0) or added into (ACC = 1) the
+// double pressure: of L x = rhs (ACC = 0), or of L x = rhs - L p (ACC = 1), the
+// residual formed inside the first transform where a fixed-size kernel covers the
+// line length and by k_residual otherwise
 template <int ACC>
-static int solve_fp32_pass(Ctx& c, const float* src) {
+static int solve_fp32_pass(Ctx& c) {
     const float* twx = as<float>(c.tw_x_f);
     const float* twy = as<float>(c.tw_y_f);
+    const float* rhs = as<float>(c.frhs);
     float* w = as<float>(c.pf);
     int rc;
-    {
+    if (!ACC) {
         ProfScope ps(c, "k_rfft_x_fwd");
-        rc = rfft_launch<0, 0, float, float, 0>(c, twx, src, w);
+        rc = rfft_launch<0, 0, float, float, 0>(c, twx, rhs, w);
+        if (rc) return rc;
+    } else {
+        ProfScope ps(c, "k_rfft_x_fwd_res");
+        rc = rfft_launch<0, 0, float, float, 0, 1>(c, twx, rhs, w, (const double*)c.p);
+        if (rc < 0) {
+            dim3 blk(64, 4, 1);
+            hipLaunchKernelGGL(k_residual<float>, grid3(c.g, c.g.nz, blk), blk, 0, c.stream, c.g, rhs,
+                               (const double*)c.p, w);
+            HIP_OK(hipGetLastError());
+            rc = rfft_launch<0, 0, float, float, 0>(c, twx, w, w);
+        }
         if (rc) return rc;
     }
     {
@@ -484,16 +1018,9 @@ static int solve_fp32_pass(Ctx& c, const float* src) {
 }
 
 static int solve_mixed(Ctx& c) {
-    int rc = solve_fp32_pass<0>(c, as<float>(c.frhs));
+    int rc = solve_fp32_pass<0>(c);
     if (rc) return rc;
-    {
-        ProfScope ps(c, "k_residual");
-        dim3 blk(64, 4, 1);
-        hipLaunchKernelGGL(k_residual<float>, grid3(c.g, c.g.nz, blk), blk, 0, c.stream, c.g, as<float>(c.frhs),
-                           (const double*)c.p, as<float>(c.pf));
-        HIP_OK(hipGetLastError());
-    }
-    return solve_fp32_pass<1>(c, as<float>(c.pf));
+    return solve_fp32_pass<1>(c);
 }
 
 int launch_solve(Ctx& c) { return c.fp32 ? solve_mixed(c) : solve_fp64(c); }

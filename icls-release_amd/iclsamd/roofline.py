@@ -5,7 +5,8 @@ distinct input and output arrays of their element size, per interior cell
 N = itot*jtot*ktot.  Stencil re-reads, halos, the z ghost layers, constant tables
 and scalars are not counted; masked or band-limited sweeps still count every cell;
 a byte mask counts 1 byte.  Scratch that a kernel keeps for itself (the Thomas
-multipliers and forward-sweep values) is not an input or output and is not counted.
+multipliers and forward-sweep values of the serial Thomas kernel, used where the
+partitioned one does not fit) is not an input or output and is not counted.
 
 Arrays are of three classes: "T" field arrays (8 bytes in the FP64 mode, 4 in the
 mixed modes), "P" pressure arrays (8 bytes in every mode) and "B" byte masks.
@@ -33,9 +34,14 @@ KERNEL_ARRAYS = {
 }
 
 # the Poisson passes differ by mode: FP64 works in place on p (P class); the mixed
-# passes read and write the float work array, and the last inverse transform writes p
+# passes read and write the float work array.  Mixed: the first pass's inverse x
+# transform writes p (R w, W p: 12), the refinement pass's adds into it (R w, p; W p:
+# 20), 16 on average over the two launches; the refinement pass's forward x transform
+# forms rhs - L p itself (R rhs, p; W w).  k_residual runs only for line lengths
+# without a fixed-size transform.
 _POISSON_FP64 = {"k_rfft_x_fwd": 16, "k_rfft_y_fwd": 16, "k_rfft_y_inv": 16, "k_rfft_x_inv": 16, "k_thomas": 16}
-_POISSON_MIXED = {"k_rfft_x_fwd": 8, "k_rfft_y_fwd": 8, "k_rfft_y_inv": 8, "k_rfft_x_inv": 12, "k_thomas": 8}
+_POISSON_MIXED = {"k_rfft_x_fwd": 8, "k_rfft_x_fwd_res": 16, "k_rfft_y_fwd": 8, "k_rfft_y_inv": 8,
+                  "k_rfft_x_inv": 16, "k_thomas": 8}
 
 
 def kernel_bytes(mode_fp32):
