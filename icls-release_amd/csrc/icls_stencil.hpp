@@ -29,6 +29,16 @@ __device__ __forceinline__ T upwind1(T vel, const T* f) {
     return f[1] - f[0];
 }
 
+// The quotient a / b.  The FP64 instantiation divides, with the reference's rounding;
+// the float one multiplies by the hardware reciprocal of b (v_rcp_f32, 1 ulp), about a
+// tenth of the instructions of an IEEE division.  The mixed modes bound their error
+// against FP64 instead of matching it bit for bit, and a repeated divisor's
+// reciprocal is formed once.
+template <typename T>
+__device__ __forceinline__ T qdiv(T a, T b) { return a / b; }
+template <>
+__device__ __forceinline__ float qdiv<float>(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+
 // Jiang-Shu WENO5 reconstruction from five differences (hyperPDE.f90:551-576)
 template <typename T>
 __device__ __forceinline__ T weno5(T v1, T v2, T v3, T v4, T v5) {
@@ -40,12 +50,12 @@ __device__ __forceinline__ T weno5(T v1, T v2, T v3, T v4, T v5) {
     T t5 = v3 - T(2.) * v4 + v5, t6 = T(3.) * v3 - T(4.) * v4 + v5;
     T S3 = (T(13.) / T(12.)) * (t5 * t5) + (T(1.) / T(4.)) * (t6 * t6);
     T e1 = eps_weno + S1, e2 = eps_weno + S2, e3 = eps_weno + S3;
-    T a1 = sigma_1 / (e1 * e1);
-    T a2 = sigma_2 / (e2 * e2);
-    T a3 = sigma_3 / (e3 * e3);
-    T w1 = a1 / (a1 + a2 + a3);
-    T w2 = a2 / (a1 + a2 + a3);
-    T w3 = a3 / (a1 + a2 + a3);
+    T a1 = qdiv(sigma_1, e1 * e1);
+    T a2 = qdiv(sigma_2, e2 * e2);
+    T a3 = qdiv(sigma_3, e3 * e3);
+    T w1 = qdiv(a1, a1 + a2 + a3);
+    T w2 = qdiv(a2, a1 + a2 + a3);
+    T w3 = qdiv(a3, a1 + a2 + a3);
     return w1 * ((T(1.) / T(3.)) * v1 - (T(7.) / T(6.)) * v2 + (T(11.) / T(6.)) * v3) +
            w2 * ((-T(1.) / T(6.)) * v2 + (T(5.) / T(6.)) * v3 + (T(1.) / T(3.)) * v4) +
            w3 * ((T(1.) / T(3.)) * v3 + (T(5.) / T(6.)) * v4 - (T(1.) / T(6.)) * v5);

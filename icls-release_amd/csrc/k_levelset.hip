@@ -404,7 +404,7 @@ k_reinit_zm(Grid g, T NB2, double* res, const T* phi0, const T* __restrict__ phi
                     T s;
                     if (p0 < -dz) s = T(-1.);
                     else if (p0 > dz) s = T(1.);
-                    else s = p0 / sqrt(p0 * p0 + dz * dz);
+                    else s = qdiv(p0, sqrt(p0 * p0 + dz * dz));
                     // reinit_rhs on the tile (hyperPDE.f90:2662-2736, 2932-3197)
                     T f[7];
 #pragma unroll

@@ -213,7 +213,7 @@ k_momentum_zm(Grid g, MomParams m, const double* __restrict__ dscal, const T* __
                 T RD3 = sy.div(dvdxp * ma2 - dvdxm * ma1);
                 T RD4 = sz.div(dudzp * ma4 - dudzm * ma3);
                 T RD5 = sz.div(dwdxp * ma4 - dwdxm * ma3);
-                RDu = (RD1 + RD2 + RD3 + RD4 + RD5) / rho_u;
+                RDu = qdiv(RD1 + RD2 + RD3 + RD4 + RD5, rho_u);
             }
             {
                 T dvdxp = sx.div(AT(V, 1, 0, 0) - vq);
@@ -235,7 +235,7 @@ k_momentum_zm(Grid g, MomParams m, const double* __restrict__ dscal, const T* __
                 T RD3 = sy.div(two * (dvdyp * AT(MU, 0, 1, 0) - dvdym * mq));
                 T RD4 = sz.div(dvdzp * ma7 - dvdzm * ma6);
                 T RD5 = sz.div(dwdyp * ma7 - dwdym * ma6);
-                RDv = (RD1 + RD2 + RD3 + RD4 + RD5) / rho_v;
+                RDv = qdiv(RD1 + RD2 + RD3 + RD4 + RD5, rho_v);
             }
             {
                 T dwdxp = sx.div(AT(W, 1, 0, 0) - wq);
@@ -257,13 +257,13 @@ k_momentum_zm(Grid g, MomParams m, const double* __restrict__ dscal, const T* __
                 T RD3 = sy.div(dwdyp * ma7 - dwdym * ma9);
                 T RD4 = sy.div(dvdzp * ma7 - dvdzm * ma9);
                 T RD5 = sz.div(two * (dwdzp * AT(MU, 0, 0, 1) - dwdzm * mq));
-                RDw = (RD1 + RD2 + RD3 + RD4 + RD5) / rho_w;
+                RDw = qdiv(RD1 + RD2 + RD3 + RD4 + RD5, rho_w);
             }
 
             // ---- buoyancy and RHS
-            T RGu = iFr * (one - rho_avr / rho_u) * T(m.g0);
-            T RGv = iFr * (one - rho_avr / rho_v) * T(m.g1);
-            T RGw = iFr * (one - rho_avr / rho_w) * T(m.g2);
+            T RGu = iFr * (one - qdiv(rho_avr, rho_u)) * T(m.g0);
+            T RGv = iFr * (one - qdiv(rho_avr, rho_v)) * T(m.g1);
+            T RGw = iFr * (one - qdiv(rho_avr, rho_w)) * T(m.g2);
             T dun = RCu + RDu * visc + RGu;
             T dvn = RCv + RDv * visc + RGv;
             T dwn = RCw + RDw * visc + RGw;
@@ -276,11 +276,11 @@ k_momentum_zm(Grid g, MomParams m, const double* __restrict__ dscal, const T* __
                 T ws = wq + dt * (c15 * dwn - c05 * dw0);
                 duold[q] = dun; dvold[q] = dvn; dwold[q] = dwn;
                 // ---- split-pressure terms (mom.f90:793,818,843)
-                T mpx = -(one / rho_u - irho0) * T(g.dxi) * T(gpx);
-                T mpy = -(one / rho_v - irho0) * T(g.dyi) * T(gpy);
-                T mpz = -(one / rho_w - irho0) * T(g.dzi) * T(gpz);
+                T mpx = -(qdiv(one, rho_u) - irho0) * T(g.dxi) * T(gpx);
+                T mpy = -(qdiv(one, rho_v) - irho0) * T(g.dyi) * T(gpy);
+                T mpz = -(qdiv(one, rho_w) - irho0) * T(g.dzi) * T(gpz);
                 dudt[q] = us + dt * mpx;
-                dvdt[q] = vs + dt * mpy - T(m.forceytot) * dt / rho_v;
+                dvdt[q] = vs + dt * mpy - qdiv(T(m.forceytot) * dt, rho_v);
                 dwdt[q] = ws + dt * mpz;
             }
         }
