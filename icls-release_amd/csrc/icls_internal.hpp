@@ -187,12 +187,9 @@ __device__ __forceinline__ int xcd_tile(int b, int n) {
 // Field-class pointers are reinterpreted as the mode's element type.
 // level set
 int launch_tube(Ctx& c, const double* phi_in, double* phi_out);
-int launch_bc_ls(Ctx& c, double* phi);
-int launch_bc_ls_pred(Ctx& c, double* phi, const double* flag);
 int launch_adv_stage(Ctx& c, int scheme, int stage, const double* phi0, const double* phis, double* out,
                      const double* vu, const double* vv, const double* vw, int velf, const double* flag);
 int launch_reinit_stage(Ctx& c, int stage, const double* phi0, const double* phis, double* out);
-int launch_reinit_residual(Ctx& c, const double* phinew);
 int launch_boundmls(Ctx& c, double* phi);
 int launch_init_levelset(Ctx& c, int l, double* phi);
 // materials / reductions

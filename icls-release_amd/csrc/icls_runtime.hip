@@ -566,18 +566,15 @@ static int mass_correction_due(Ctx& c, int l, int istep, bool& due) {
 static int ls_main_impl(Ctx& c, int istep) {
     for (int l = 0; l < c.lmax; l++) {
         double* ph = c.lvset[l];
-        // tube_construction (levelset.f90:87-197)
+        // tube_construction (levelset.f90:87-197); every level-set kernel below folds
+        // the boundsls that follows it (bound.f90:404-411) into its own launch
         TRY(launch_tube(c, ph, c.ls_a));
-        TRY(launch_bc_ls(c, c.ls_a));
         // ls_advection, SSP-RK3 (levelset.f90:235-243 / 268-276), cell-centre velocity
         // averaged from unew/vnew/wnew inside the stencil (misc.f90:595-597)
         const int sch = c.P.ls_adv_space;
         TRY(launch_adv_stage(c, sch, 1, c.ls_a, c.ls_a, c.ls_b, c.unew, c.vnew, c.wnew, 1, nullptr));
-        TRY(launch_bc_ls(c, c.ls_b));
         TRY(launch_adv_stage(c, sch, 2, c.ls_a, c.ls_b, c.ls_c, c.unew, c.vnew, c.wnew, 1, nullptr));
-        TRY(launch_bc_ls(c, c.ls_c));
         TRY(launch_adv_stage(c, sch, 3, c.ls_a, c.ls_c, ph, c.unew, c.vnew, c.wnew, 1, nullptr));
-        TRY(launch_bc_ls(c, ph));
         // ls_mass_correction (levelset.f90:377-582)
         if (c.P.mass_correction) {
             bool due = false;
@@ -588,21 +585,15 @@ static int ls_main_impl(Ctx& c, int istep) {
                 TRY(launch_curvature(c, ph, c.curv2, c.normal_m, flag));
                 TRY(launch_inflation_velocity(c, l));
                 TRY(launch_adv_stage(c, ICLS_ADV_HOUC5, 1, ph, ph, c.ls_a, c.ucc_l, c.vcc_l, c.wcc_l, 0, flag));
-                TRY(launch_bc_ls_pred(c, c.ls_a, flag));
                 TRY(launch_adv_stage(c, ICLS_ADV_HOUC5, 2, ph, c.ls_a, c.ls_b, c.ucc_l, c.vcc_l, c.wcc_l, 0, flag));
-                TRY(launch_bc_ls_pred(c, c.ls_b, flag));
                 TRY(launch_adv_stage(c, ICLS_ADV_HOUC5, 3, ph, c.ls_b, ph, c.ucc_l, c.vcc_l, c.wcc_l, 0, flag));
-                TRY(launch_bc_ls_pred(c, ph, flag));
             }
         }
         // ls_reinit_main, WENO5 + RK3 (levelset.f90:592-882)
         if (c.P.re_initialization && istep % c.P.reinit_step == 0) {
             TRY(launch_reinit_stage(c, 1, ph, ph, c.ls_a));
-            TRY(launch_bc_ls(c, c.ls_a));
             TRY(launch_reinit_stage(c, 2, ph, c.ls_a, c.ls_b));
-            TRY(launch_bc_ls(c, c.ls_b));
             TRY(launch_reinit_stage(c, 3, ph, c.ls_b, ph));    // in place: lvset = phi_new
-            TRY(launch_reinit_residual(c, ph));
         }
     }
     // update_materials (misc.f90:79-290)

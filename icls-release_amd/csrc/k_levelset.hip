@@ -26,14 +26,20 @@ static constexpr int BX = 64, BY = 4;
 // only when every term is).  A clamped value has |phi| < NarrowBand_2 exactly when
 // the unclamped one has, so the scan tests the input directly; the twelve
 // neighbours are loaded together rather than behind an early exit.
+// The boundsls that follows (bound.f90:404-411) is folded in: layers 0 and k1 take
+// the linear extrapolation of the clamped layers next to them.
 template <class T>
 __global__ void k_tube(Grid g, T NB2, const T* __restrict__ in, T* __restrict__ out, uint8_t* __restrict__ mask) {
     int i0 = blockIdx.x * BX + threadIdx.x, j0 = blockIdx.y * BY + threadIdx.y, ks = blockIdx.z;
     if (i0 >= g.nx || j0 >= g.ny) return;
     int k = ks - 2;
     long q = gidx(g, i0, j0, k);
+    auto clampv = [&](T v) { return v > NB2 ? NB2 : (v < -NB2 ? -NB2 : v); };
     T v = in[q];
-    T c = v > NB2 ? NB2 : (v < -NB2 ? -NB2 : v);
+    T c;
+    if (k == 0) c = T(2.) * clampv(in[gidx(g, i0, j0, 1)]) - clampv(in[gidx(g, i0, j0, 2)]);
+    else if (k == g.nz + 1) c = T(2.) * clampv(in[gidx(g, i0, j0, g.nz)]) - clampv(in[gidx(g, i0, j0, g.nz - 1)]);
+    else c = clampv(v);
     out[q] = c;
     if (k < 1 || k > g.nz) return;
     T nb[12];
@@ -107,15 +113,6 @@ static int bc_ls(Ctx& c, double* phi, const double* res_ref, double* res, const 
         hipLaunchKernelGGL(k_bc_ls<double>, grd, blk, 0, c.stream, c.g, phi, res_ref, res, flag);
     HIP_OK(hipGetLastError());
     return ICLS_OK;
-}
-
-int launch_bc_ls_pred(Ctx& c, double* phi, const double* flag) { return bc_ls(c, phi, nullptr, nullptr, flag); }
-int launch_bc_ls(Ctx& c, double* phi) { return bc_ls(c, phi, nullptr, nullptr, nullptr); }
-
-// boundsls after the last re-init stage, folding the two wall layers into the residual
-// against the phi0 layers stage 3 stashed
-int launch_reinit_residual(Ctx& c, const double* phinew) {
-    return bc_ls(c, (double*)phinew, c.stash, c.dscal + DS_REINIT_RES, nullptr);
 }
 
 // boundmls: three extrapolated layers per wall (bound.f90:539-549), init only
@@ -197,6 +194,10 @@ __device__ __forceinline__ T adv_rhs(const Grid& g, const T* __restrict__ phi, i
 // VELF: vu/vv/vw are the face velocities unew/vnew/wnew and the cell-centre values
 // are averaged here as average_vel_at_center does (misc.f90:595-597); otherwise they
 // are cell-centre arrays already.
+// The boundsls that follows every stage (bound.f90:404-411) is folded in: the
+// threads of layers 0 and k1 evaluate the stage at the two layers next to the wall
+// (the same operations the threads of those layers perform) and store the linear
+// extrapolation.
 template <class T, int SCHEME, int STAGE, bool P2, bool VELF>
 __global__ void k_adv_stage(Grid g, T dt, const T* phi0, const T* __restrict__ phis, T* out,
                             const uint8_t* __restrict__ mask, const T* __restrict__ vu, const T* __restrict__ vv,
@@ -205,25 +206,32 @@ __global__ void k_adv_stage(Grid g, T dt, const T* phi0, const T* __restrict__ p
     int i0 = blockIdx.x * BX + threadIdx.x, j0 = blockIdx.y * BY + threadIdx.y, ks = blockIdx.z;
     if (i0 >= g.nx || j0 >= g.ny) return;
     int k = ks - 2;
-    long q = gidx(g, i0, j0, k);
-    T comb, factor;
-    if (STAGE == 1) { comb = phi0[q]; factor = T(1.); }
-    else if (STAGE == 2) { comb = T(3.) / T(4.) * phi0[q] + T(1.) / T(4.) * phis[q]; factor = T(1.) / T(4.); }
-    else { comb = T(1.) / T(3.) * phi0[q] + T(2.) / T(3.) * phis[q]; factor = T(2.) / T(3.); }
-    T r = comb;
-    if (k >= 1 && k <= g.nz && mask[q]) {
-        T uc, vc, wc;
-        if (VELF) {
-            uc = (vu[gidx(g, wrap(i0 - 1, g.nx), j0, k)] + vu[q]) / T(2.);
-            vc = (vv[gidx(g, i0, wrap(j0 - 1, g.ny), k)] + vv[q]) / T(2.);
-            wc = (vw[gidx(g, i0, j0, k - 1)] + vw[q]) / T(2.);
-        } else {
-            uc = vu[q]; vc = vv[q]; wc = vw[q];
+    auto value = [&](int kk) -> T {
+        const long qq = gidx(g, i0, j0, kk);
+        T comb, factor;
+        if (STAGE == 1) { comb = phi0[qq]; factor = T(1.); }
+        else if (STAGE == 2) { comb = T(3.) / T(4.) * phi0[qq] + T(1.) / T(4.) * phis[qq]; factor = T(1.) / T(4.); }
+        else { comb = T(1.) / T(3.) * phi0[qq] + T(2.) / T(3.) * phis[qq]; factor = T(2.) / T(3.); }
+        T r = comb;
+        if (kk >= 1 && kk <= g.nz && mask[qq]) {
+            T uc, vc, wc;
+            if (VELF) {
+                uc = (vu[gidx(g, wrap(i0 - 1, g.nx), j0, kk)] + vu[qq]) / T(2.);
+                vc = (vv[gidx(g, i0, wrap(j0 - 1, g.ny), kk)] + vv[qq]) / T(2.);
+                wc = (vw[gidx(g, i0, j0, kk - 1)] + vw[qq]) / T(2.);
+            } else {
+                uc = vu[qq]; vc = vv[qq]; wc = vw[qq];
+            }
+            T dfdt = adv_rhs<T, SCHEME, P2>(g, phis, i0, j0, kk, uc, vc, wc);
+            r = comb + factor * dt * dfdt;
         }
-        T dfdt = adv_rhs<T, SCHEME, P2>(g, phis, i0, j0, k, uc, vc, wc);
-        r = comb + factor * dt * dfdt;
-    }
-    out[q] = r;
+        return r;
+    };
+    T r;
+    if (k == 0) r = T(2.) * value(1) - value(2);
+    else if (k == g.nz + 1) r = T(2.) * value(g.nz) - value(g.nz - 1);
+    else r = value(k);
+    out[gidx(g, i0, j0, k)] = r;
 }
 
 template <class T>
@@ -269,9 +277,12 @@ int launch_adv_stage(Ctx& c, int scheme, int stage, const double* phi0, const do
 // equal (the clamped far field) both WENO5 derivatives are exactly +0 (every
 // difference is +0, every smoothness indicator 0, the weighted sum of zeros +0), so
 // they are set without evaluating the weights.
-// Stage 3 also folds |out - phi0| of every layer that boundsls does not rewrite into
-// the residual (levelset.f90:831), stashing phi0 of the two rewritten wall layers for
-// k_bc_ls; stage 1 resets the residual.  Stage 3 may run in place (out == phi0).
+// The boundsls after each stage (bound.f90:404-411) is folded in when the last chunk
+// holds two layers or more (WALL): the block that computes layers 1 and 2 (kmax-1 and
+// kmax) stores the linear extrapolation into layer 0 (k1).  Stage 3 also folds |out -
+// phi0| of every layer into the residual (levelset.f90:831), the two wall layers with
+// their extrapolated values; without WALL it stashes phi0 of those layers for
+// k_bc_ls.  Stage 1 resets the residual.  Stage 3 may run in place (out == phi0).
 //
 // The kernel marches in z over a 64 x 4 tile of columns.  phi_s passes through
 // a ring of seven LDS planes (k-3 .. k+3) of the tile with a three-cell x/y halo, so
@@ -303,7 +314,7 @@ __device__ __forceinline__ T reinit_combine(T p0, T ps, T rhs, T dtau) {
 // steps of a chunk run in groups of RPD, unrolled, so the chunk length must be a
 // multiple of RPD (the launcher falls back to RPD = 1 otherwise): a step skipped
 // inside a group would make the memory-counter waits of every step conservative.
-template <class T, int STAGE, bool P2, int RPD>
+template <class T, int STAGE, bool P2, int RPD, bool WALL>
 __global__ void __launch_bounds__(BX * BY)
 k_reinit_zm(Grid g, T NB2, double* res, const T* phi0, const T* __restrict__ phis, T* out, T* __restrict__ stash) {
     __shared__ T ring[RNS][RTP];
@@ -357,7 +368,9 @@ k_reinit_zm(Grid g, T NB2, double* res, const T* phi0, const T* __restrict__ phi
     const int c0 = (ty + RH) * RTW + tx + RH;
     T dmax = T(0.);
 
-    // ghost layers (no right-hand side): below the first chunk, above the last
+    // ghost layers (no right-hand side): below the first chunk, above the last; with
+    // WALL, layers 0 and k1 get their final values from the wall extrapolation
+    T r_lo = T(0.), r_hi = T(0.);
     auto ghost = [&](int k) {
         const long q = gidx(g, i, j, k);
         const T p0 = phi0[q];
@@ -370,7 +383,7 @@ k_reinit_zm(Grid g, T NB2, double* res, const T* phi0, const T* __restrict__ phi
         out[q] = r;
     };
     if (bz == 0)
-        for (int k = -2; k <= 0; k++) ghost(k);
+        for (int k = -2; k <= (WALL ? -1 : 0); k++) ghost(k);
 
     // The queues: planes of phi_s and the cell's phi0, RPD z steps ahead.  Every queue
     // load is unconditional, its plane clamped to the last one the chunk reads (a
@@ -440,25 +453,44 @@ k_reinit_zm(Grid g, T NB2, double* res, const T* phi0, const T* __restrict__ phi
                 const T r = reinit_combine<T, STAGE, P2>(p0, ps, rhs, dtau);
                 if (STAGE == 3) dmax = fmax(dmax, fabs(r - p0));
                 out[q] = r;
+                if (WALL) {
+                    // layer 0 from layers 1 and 2, layer k1 from kmax - 1 and kmax (k is
+                    // uniform over the block)
+                    if (k == 1) r_lo = r;
+                    else if (k == 2) {
+                        // phi0 of layer 0 is read before the store (stage 3 may run in place)
+                        const long qw = gidx(g, i, j, 0);
+                        const T lo = T(2.) * r_lo - r;
+                        if (STAGE == 3) dmax = fmax(dmax, fabs(lo - phi0[qw]));
+                        out[qw] = lo;
+                    }
+                    if (k == g.nz - 1) r_hi = r;
+                    else if (k == g.nz) {
+                        const T hi = T(2.) * r - r_hi;
+                        const long qw = gidx(g, i, j, g.nz + 1);
+                        if (STAGE == 3) dmax = fmax(dmax, fabs(hi - phi0[qw]));
+                        out[qw] = hi;
+                    }
+                }
             }
             __syncthreads();
         }
     }
     if (bz == ntz - 1)
-        for (int k = g.nz + 1; k <= g.nz + 3; k++) ghost(k);
+        for (int k = g.nz + (WALL ? 2 : 1); k <= g.nz + 3; k++) ghost(k);
     if (STAGE == 3) {
         double b = block_reduce((double)dmax, OpMax(), smem);
         if (tid == 0) atomic_max_nonneg(res, b);
     }
 }
 
-template <class T, int RPD>
+template <class T, int RPD, bool WALL>
 static void reinit_dispatch_pd(Ctx& c, int stage, const double* phi0, const double* phis, double* out) {
     dim3 blk(BX, BY, 1);
     const int ntiles = ((c.g.nx + BX - 1) / BX) * ((c.g.ny + BY - 1) / BY) * ((c.g.nz + RZ - 1) / RZ);
     double* res = c.dscal + DS_REINIT_RES;
 #define RS(S, P)                                                                                                  \
-    hipLaunchKernelGGL((k_reinit_zm<T, S, P, RPD>), dim3(ntiles), blk, 0, c.stream, c.g, (T)c.NB2, res,           \
+    hipLaunchKernelGGL((k_reinit_zm<T, S, P, RPD, WALL>), dim3(ntiles), blk, 0, c.stream, c.g, (T)c.NB2, res,     \
                        as<T>(phi0), as<T>(phis), as<T>(out), as<T>(c.stash))
     if (c.g.pow2) {
         if (stage == 1) RS(1, true); else if (stage == 2) RS(2, true); else RS(3, true);
@@ -468,20 +500,35 @@ static void reinit_dispatch_pd(Ctx& c, int stage, const double* phi0, const doub
 #undef RS
 }
 
+// the wall extrapolation folds into the stage when the last chunk has two layers or
+// more (a chunk of one layer would need layer kmax - 1 from another block)
+bool reinit_fuses_wall(const Grid& g) { return g.nz >= 2 && (g.nz % RZ == 0 || g.nz % RZ >= 2); }
+
 template <class T>
 static void reinit_dispatch(Ctx& c, int stage, const double* phi0, const double* phis, double* out) {
     // every chunk (RZ layers, the last one nz % RZ) a multiple of RPD_MAX
     const bool deep = RZ % RPD_MAX == 0 && (c.g.nz % RZ) % RPD_MAX == 0;
-    if (deep) reinit_dispatch_pd<T, RPD_MAX>(c, stage, phi0, phis, out);
-    else reinit_dispatch_pd<T, 1>(c, stage, phi0, phis, out);
+    if (reinit_fuses_wall(c.g)) {
+        if (deep) reinit_dispatch_pd<T, RPD_MAX, true>(c, stage, phi0, phis, out);
+        else reinit_dispatch_pd<T, 1, true>(c, stage, phi0, phis, out);
+    } else {
+        if (deep) reinit_dispatch_pd<T, RPD_MAX, false>(c, stage, phi0, phis, out);
+        else reinit_dispatch_pd<T, 1, false>(c, stage, phi0, phis, out);
+    }
 }
 
+// One re-initialisation stage with its boundsls (bound.f90:404-411); after stage 3
+// also the wall layers' share of the residual (levelset.f90:831).
 int launch_reinit_stage(Ctx& c, int stage, const double* phi0, const double* phis, double* out) {
-    ProfScope ps(c, stage == 1 ? "k_reinit_stage1" : "k_reinit_stage23");
-    if (c.fp32) reinit_dispatch<float>(c, stage, phi0, phis, out);
-    else reinit_dispatch<double>(c, stage, phi0, phis, out);
-    HIP_OK(hipGetLastError());
-    return ICLS_OK;
+    {
+        ProfScope ps(c, stage == 1 ? "k_reinit_stage1" : "k_reinit_stage23");
+        if (c.fp32) reinit_dispatch<float>(c, stage, phi0, phis, out);
+        else reinit_dispatch<double>(c, stage, phi0, phis, out);
+        HIP_OK(hipGetLastError());
+    }
+    if (reinit_fuses_wall(c.g)) return ICLS_OK;
+    if (stage == 3) return bc_ls(c, out, c.stash, c.dscal + DS_REINIT_RES, nullptr);
+    return bc_ls(c, out, nullptr, nullptr, nullptr);
 }
 
 // ---------------------------------------------------------------- initial level sets
