@@ -288,6 +288,11 @@ int launch_adv_stage(Ctx& c, int scheme, int stage, const double* phi0, const do
 // a ring of seven LDS planes (k-3 .. k+3) of the tile with a three-cell x/y halo, so
 // the 19-point WENO stencils read LDS and each phi_s element is fetched once per
 // block; the next plane is loaded into registers while the current one is computed.
+// Uniform windows: when a plane is stored, the block also records whether all its
+// values are one value.  Where the seven planes of a step are one value v (the
+// clamped far field, most of the domain) every stencil is flat, all six WENO5
+// derivatives are +0, and RHS = s exactly wherever |v| < NarrowBand_2 + dz; the step
+// then skips the stencils.
 // Interior layers are split into chunks of RZ; the first and the last chunk also
 // write the three ghost layers below and above.  Tiles are taken in XCD order.
 #ifndef ICLS_REINIT_RZ
@@ -319,6 +324,8 @@ __global__ void __launch_bounds__(BX * BY)
 k_reinit_zm(Grid g, T NB2, double* res, const T* phi0, const T* __restrict__ phis, T* out, T* __restrict__ stash) {
     __shared__ T ring[RNS][RTP];
     __shared__ T p0s[BX * BY];
+    __shared__ int wok[RNS][BX * BY / 64];   // per plane and wavefront: all values equal wv
+    __shared__ T wv[RNS][BX * BY / 64];
     __shared__ double smem[16];
     const int ntx = (g.nx + BX - 1) / BX, nty = (g.ny + BY - 1) / BY, ntz = (g.nz + RZ - 1) / RZ;
     const int tile = xcd_tile(blockIdx.x, ntx * nty * ntz);
@@ -346,17 +353,45 @@ k_reinit_zm(Grid g, T NB2, double* res, const T* phi0, const T* __restrict__ phi
     };
     auto slot_of = [&](int k) { return (k - kb + 2 * RNS) % RNS; };
     auto stash_plane = [&](int k, const T* src) {
-        T* dst = ring[slot_of(k)];
+        const int sl = slot_of(k);
+        T* dst = ring[sl];
+        const T v0 = src[0];
+        bool ok = true;
 #pragma unroll
         for (int r = 0; r < RNLD; r++) {
             const int e = tid + r * BX * BY;
-            if (e < RTP) dst[e] = src[r];
+            if (e < RTP) {
+                dst[e] = src[r];
+                ok = ok && src[r] == v0;
+            }
         }
+        const T vf = __shfl(v0, 0);
+        const int wall = __all(ok && v0 == vf);
+        if ((tid & 63) == 0) {
+            wok[sl][tid >> 6] = wall;
+            wv[sl][tid >> 6] = vf;
+        }
+    };
+    // run: how many of the planes stored so far, the newest last, are one value rval
+    // (block-uniform: every wavefront reads the same LDS words after a barrier)
+    int run = 0;
+    T rval = T(0.);
+    auto push = [&](int k) {
+        const int sl = slot_of(k);
+        const T v = wv[sl][0];
+        bool ok = true;
+#pragma unroll
+        for (int w = 0; w < BX * BY / 64; w++) ok = ok && wok[sl][w] && wv[sl][w] == v;
+        if (ok && run > 0 && v == rval) run++;
+        else if (ok) { run = 1; rval = v; }
+        else run = 0;
     };
     for (int k = kb - RH; k < kb + RH; k++) {
         fetch(k, nxt[0]);
         stash_plane(k, nxt[0]);
     }
+    __syncthreads();
+    for (int k = kb - RH; k < kb + RH; k++) push(k);
 
     // A thread past the end of x or y (nx < 64 or ny < 4) takes the wrapped cell and
     // computes and stores the very values of the thread that owns it (its ring
@@ -405,15 +440,24 @@ k_reinit_zm(Grid g, T NB2, double* res, const T* phi0, const T* __restrict__ phi
             stash_plane(k + RH, nxt[u]);
             p0s[tid] = p0q[u];
             __syncthreads();
+            push(k + RH);
             fetch(min(k + RPD, ke) + RH, nxt[u]);
             p0q[u] = phi0[gidx(g, i, j, min(k + RPD, ke))];
             const T p0 = p0s[tid];
             {
                 const long q = gidx(g, i, j, k);
                 const T* P0 = ring[slot_of(k)];
-                const T ps = P0[c0];
+                const bool flat = run >= RNS;   // planes k - 3 .. k + 3 are all rval
+                const T ps = flat ? rval : P0[c0];
                 T rhs = T(0.);
-                if (fabs(ps) < NB2 + dz) {
+                if (flat) {
+                    if (fabs(ps) < NB2 + dz) {
+                        // six derivatives +0: s * (1 - sqrt(0)) = s
+                        if (p0 < -dz) rhs = T(-1.);
+                        else if (p0 > dz) rhs = T(1.);
+                        else rhs = qdiv(p0, sqrt(p0 * p0 + dz * dz));
+                    }
+                } else if (fabs(ps) < NB2 + dz) {
                     T s;
                     if (p0 < -dz) s = T(-1.);
                     else if (p0 > dz) s = T(1.);
