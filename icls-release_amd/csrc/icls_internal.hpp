@@ -42,7 +42,7 @@ __host__ __device__ inline int wrap(int i, int n) { return i < 0 ? i + n : (i >=
 enum DevScalar {
     DS_RHO_AVR = 0, DS_V_BULK, DS_NORM, DS_NORM_OLD, DS_REINIT_RES, DS_MAXCHK, DS_AREA, DS_INFL_TEMP,
     DS_INFL_FLAG, DS_DIV_TOT, DS_DIV_MAX, DS_VMAX, DS_UMAX, DS_VMAXC, DS_WMAX, DS_NAN_FLAG,
-    DS_RES_MAX, DS_COUNT_FIXED = 32
+    DS_RES_MAX, DS_PN_A, DS_PN_B, DS_COUNT_FIXED = 32
 };
 // volume[l] at DS_COUNT_FIXED + l, init_volume[l] at DS_COUNT_FIXED + lmax + l
 
@@ -109,6 +109,16 @@ struct Ctx {
     double* pf = nullptr;                 // mixed: float work array of the transforms
     // pressure class (double)
     double *pnew = nullptr, *pold = nullptr, *p = nullptr;
+    // Lazy pressure rotation (main.f90:164-183).  After a step the pnew buffer holds
+    // the raw solution p^n and the pold buffer the raw buffer of the step before; the
+    // values the reference stores are formed from them where they are read (the
+    // momentum kernel, downloads) with the reference's own operations:
+    //   pnew = A - S_A,  po = (p_prev_lazy ? B - S_B : B),
+    //   pold = (p_first ? po - S_A : po + S_B - S_A),  S_A = DS_PN_A, S_B = DS_PN_B.
+    bool p_lazy = false;
+    bool p_prev_lazy = false;
+    bool p_first = false;
+    bool p_in_pnew = false;               // the reference's p is the pnew buffer
     double* thomas_d = nullptr;           // serial Thomas: multipliers (interior)
     double* thomas_y = nullptr;           // serial Thomas, mixed: forward-sweep values
     int thomas_seg = 0;                   // partitioned Thomas: segment length (0: serial)
@@ -208,6 +218,8 @@ int launch_fillps(Ctx& c);
 int launch_add_pjump(Ctx& c);
 // poisson: solve L p = frhs into p (solver2d; in the mixed modes with one refinement)
 int launch_solve(Ctx& c);
+int launch_pressure_materialize(Ctx& c);
+int launch_pressure_release_p(Ctx& c);   // the work array p takes the raw solution back
 int thomas_seg_len(int nz);
 int launch_thomas_seed(Ctx& c);
 // learned curvature closure (k_mlp.hip)

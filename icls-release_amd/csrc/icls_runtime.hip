@@ -415,8 +415,16 @@ int32_t icls_sync(void) {
     return ICLS_OK;
 }
 
+// the pressures of a lazy rotation are made stored values before they are accessed
+static int prepare_field(Ctx& c, int field) {
+    if (field == ICLS_F_PNEW || field == ICLS_F_POLD) return launch_pressure_materialize(c);
+    if (field == ICLS_F_P) return launch_pressure_release_p(c);
+    return ICLS_OK;
+}
+
 int32_t icls_upload(int32_t field, int32_t l, const double* host) {
     CTX_OR_FAIL();
+    TRY(prepare_field(c, field));
     double* d = field_ptr(c, field, l);
     if (!d || !host) { set_error("field %d (l=%d) cannot be uploaded", field, l); return ICLS_ERR_PARAM; }
     int sh = shape_of(field), hx, hy, hz, lb;
@@ -442,6 +450,7 @@ int32_t icls_upload(int32_t field, int32_t l, const double* host) {
 int32_t icls_download(int32_t field, int32_t l, double* host) {
     CTX_OR_FAIL();
     if (!host) { set_error("null host buffer"); return ICLS_ERR_PARAM; }
+    TRY(prepare_field(c, field));
     double* d = field_ptr(c, field, l);
     if (field == ICLS_F_JIEMIAN) { TRY(launch_jiemian(c, c.scratch)); d = c.scratch; }
     if (field == ICLS_F_UCC || field == ICLS_F_VCC || field == ICLS_F_WCC) TRY(launch_avg_vel(c));
@@ -638,6 +647,7 @@ static void rotate_jumps(Ctx& c) {
 
 static int gfm_impl(Ctx& c, bool fuse_fillps, bool store_pjump) {
     rotate_jumps(c);
+    c.p_in_pnew = false;   // fillps writes p
     const bool learned = c.learned && c.mlp.ready;
     for (int l = 0; l < c.lmax; l++) {
         if (learned) {
@@ -669,7 +679,10 @@ int32_t icls_add_pjump(void) {
 }
 
 // solver2d on the right-hand side frhs (p itself in FP64) into p
-static int solver_impl(Ctx& c) { return launch_solve(c); }
+static int solver_impl(Ctx& c) {
+    c.p_in_pnew = false;
+    return launch_solve(c);
+}
 
 int32_t icls_solver2d(void) {
     CTX_OR_FAIL();
@@ -678,6 +691,7 @@ int32_t icls_solver2d(void) {
 
 int32_t icls_boundp(int32_t field) {
     CTX_OR_FAIL();
+    TRY(prepare_field(c, field));
     double* d = field_ptr(c, field, 0);
     if (!d || field == ICLS_F_LVSET) { set_error("bad field for boundp"); return ICLS_ERR_PARAM; }
     if (field_is_fp32(c, field)) return launch_bc_p_t_pred(c, d, nullptr);
@@ -802,6 +816,7 @@ int32_t icls_max_vel_at_center(double* vmax) {
 
 // ---------------------------------------------------------------- init_program
 static int init_derived_impl(Ctx& c) {
+    TRY(launch_pressure_materialize(c));
     c.dt = c.P.timestep;                                   // init.f90:68
     TRY(launch_bc_uvw(c, c.unew, c.vnew, c.wnew));         // init.f90:72-76
     TRY(launch_bc_uvw(c, c.dudt, c.dvdt, c.dwdt));
@@ -835,6 +850,7 @@ int32_t icls_init_program(void) {
     // init_vel_p (init.f90:148-210): fields start at zero; 'cou' sets v
     double* zf[] = {c.unew, c.vnew, c.wnew, c.dudt, c.dvdt, c.dwdt};
     for (auto* f : zf) TRY(launch_fill(c, f, 0., c.fp32));
+    c.p_lazy = c.p_in_pnew = false;   // pnew and pold are stored values from here
     TRY(launch_fill(c, c.pnew, 0., 0));
     TRY(launch_init_velocity(c));
     for (int l = 0; l < c.lmax; l++) TRY(launch_init_levelset(c, l, c.lvset[l]));
@@ -912,6 +928,7 @@ int32_t icls_init_program_restart(const char* path, int32_t* begin) {
     double* zf[] = {c.unew, c.vnew, c.wnew, c.dudt, c.dvdt, c.dwdt};
     int rc = ICLS_OK;
     for (auto* f : zf) if (!rc) rc = launch_fill(c, f, 0., c.fp32);
+    c.p_lazy = c.p_in_pnew = false;
     if (!rc) rc = launch_fill(c, c.pnew, 0., 0);
     if (!rc) rc = launch_init_velocity(c);
     for (auto& fl : fld_fields(c)) {

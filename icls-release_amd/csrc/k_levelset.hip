@@ -319,8 +319,11 @@ __device__ __forceinline__ T reinit_combine(T p0, T ps, T rhs, T dtau) {
 // steps of a chunk run in groups of RPD, unrolled, so the chunk length must be a
 // multiple of RPD (the launcher falls back to RPD = 1 otherwise): a step skipped
 // inside a group would make the memory-counter waits of every step conservative.
+#ifndef ICLS_REINIT_WPE
+#define ICLS_REINIT_WPE 1
+#endif
 template <class T, int STAGE, bool P2, int RPD, bool WALL>
-__global__ void __launch_bounds__(BX * BY)
+__global__ void __launch_bounds__(BX * BY, ICLS_REINIT_WPE)
 k_reinit_zm(Grid g, T NB2, double* res, const T* phi0, const T* __restrict__ phis, T* out, T* __restrict__ stash) {
     __shared__ T ring[RNS][RTP];
     __shared__ T p0s[BX * BY];

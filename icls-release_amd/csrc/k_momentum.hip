@@ -18,6 +18,7 @@ static constexpr int KCH = 8;
 struct MomParams {
     double dt, visc, iFr, g0, g1, g2, rho0, inv_rho0, rho1, forceytot;
     int rho_equal;
+    int plazy, pprev, pfirst;   // the lazy pressure rotation (Ctx::p_lazy ..)
 };
 
 // ab2 (ab2.f90:49-129) with mom_main (mom.f90:13-62), CNTR2_conv (mom.f90:67-130),
@@ -119,14 +120,23 @@ k_momentum_zm(Grid g, MomParams m, const double* __restrict__ dscal, const T* __
     T du0 = T(0.), dv0 = T(0.), dw0 = T(0.);
     double gpx = 0., gpy = 0., gpz = 0.;
     const T two_ = T(2.);
+    // pnew and pold from the raw buffers of a lazy rotation, with the operations of
+    // the rotation itself (main.f90:164-183)
+    const double SA = dscal[DS_PN_A], SB = dscal[DS_PN_B];
+    auto pn_of = [&](double a) -> double { return m.plazy ? a - SA : a; };
+    auto po_of = [&](double b) -> double {
+        if (!m.plazy) return b;
+        const double po = m.pprev ? b - SB : b;
+        return m.pfirst ? po - SA : po + SB - SA;
+    };
     auto derive = [&]() {
         if (RHS_ONLY) return;
         du0 = pd[0]; dv0 = pd[1]; dw0 = pd[2];
         // split pressure (ab2.f90:79-101): p_hat = 2 p^n - p^(n-1), p_hat_j = 2 p_j - p_j,old
-        const double phq = 2. * pp[0] - pp[1];
-        const double phx = 2. * pp[2] - pp[3];
-        const double phy = 2. * pp[4] - pp[5];
-        const double phz = 2. * pp[6] - pp[7];
+        const double phq = 2. * pn_of(pp[0]) - po_of(pp[1]);
+        const double phx = 2. * pn_of(pp[2]) - po_of(pp[3]);
+        const double phy = 2. * pn_of(pp[4]) - po_of(pp[5]);
+        const double phz = 2. * pn_of(pp[6]) - po_of(pp[7]);
         const T pjx = two_ * pd[3] - pd[4];
         const T pjy = two_ * pd[5] - pd[6];
         const T pjz = two_ * pd[7] - pd[8];
@@ -317,6 +327,9 @@ static int momentum_launch(Ctx& c, int rhs_only) {
     m.rho0 = c.rho0; m.inv_rho0 = 1. / c.rho0; m.rho1 = c.P.rho1;
     m.forceytot = c.P.constPoi ? -12. * c.visc / (c.P.lz * c.P.lz) : 0.;
     m.rho_equal = c.P.rho1 == c.P.rho2;
+    m.plazy = c.p_lazy;
+    m.pprev = c.p_prev_lazy;
+    m.pfirst = c.p_first;
     if (c.fp32) momentum_dispatch<float>(c, m, rhs_only);
     else momentum_dispatch<double>(c, m, rhs_only);
     HIP_OK(hipGetLastError());
@@ -456,30 +469,67 @@ int launch_pressure_norm(Ctx& c) {
     return ICLS_OK;
 }
 
-// pold <- pnew, pnew <- p, both shifted by the norm (main.f90:164-183), over
-// k = 0 .. k1.  Runs in place: pold[q] from pnew[q], then pnew[q] from p[q].
-__global__ void k_pressure_rotate(Grid g, int first, const double* __restrict__ dscal, const double* __restrict__ p,
-                                  double* __restrict__ pnew, double* __restrict__ pold) {
-    int i0 = blockIdx.x * BX + threadIdx.x, j0 = blockIdx.y * BY + threadIdx.y, k = blockIdx.z;
-    if (i0 >= g.nx || j0 >= g.ny) return;
-    long q = gidx(g, i0, j0, k);
-    const double norm = dscal[DS_NORM], norm_old = dscal[DS_NORM_OLD];
-    double po = pnew[q];
-    pold[q] = first ? po - norm : po + norm_old - norm;
-    pnew[q] = p[q] - norm;
-}
-
-__global__ void k_norm_shift(double* dscal) {
-    if (threadIdx.x == 0) dscal[DS_NORM_OLD] = dscal[DS_NORM];
+// The rotation pold <- pnew, pnew <- p, both shifted by the norm (main.f90:164-183),
+// is a relabelling of the three buffers: the pnew buffer takes the raw solution, the
+// pold buffer the previous pnew buffer, and the old pold buffer becomes the next
+// solve's work array.  The shifts are recorded (S_A = norm, S_B = norm_old) and
+// applied where the values are read (Ctx::p_lazy).
+__global__ void k_pressure_lazy_scalars(double* dscal) {
+    if (threadIdx.x == 0) {
+        dscal[DS_PN_B] = dscal[DS_NORM_OLD];
+        dscal[DS_PN_A] = dscal[DS_NORM];
+        dscal[DS_NORM_OLD] = dscal[DS_NORM];
+    }
 }
 
 int launch_pressure_rotate(Ctx& c, int first_step) {
-    ProfScope ps(c, "k_pressure_rotate");
-    dim3 blk(BX, BY, 1);
-    hipLaunchKernelGGL(k_pressure_rotate, grid3(c.g, c.g.nz + 2, blk), blk, 0, c.stream, c.g, first_step,
-                       (const double*)c.dscal, (const double*)c.p, c.pnew, c.pold);
-    hipLaunchKernelGGL(k_norm_shift, dim3(1), dim3(64), 0, c.stream, c.dscal);
+    hipLaunchKernelGGL(k_pressure_lazy_scalars, dim3(1), dim3(64), 0, c.stream, c.dscal);
     HIP_OK(hipGetLastError());
+    double* a = c.p;
+    c.p = c.pold;
+    c.pold = c.pnew;
+    c.pnew = a;
+    if (!c.fp32) c.frhs = c.p;   // the FP64 solve works in place on p
+    c.p_prev_lazy = c.p_lazy;
+    c.p_lazy = true;
+    c.p_first = first_step != 0;
+    c.p_in_pnew = true;
+    return ICLS_OK;
+}
+
+// the values a lazy rotation stands for, written into the pnew and pold buffers over
+// k = 0 .. k1 (the layers the reference rotates); pold from its own buffer first
+__global__ void k_pressure_materialize(Grid g, int pprev, int pfirst, const double* __restrict__ dscal,
+                                       double* __restrict__ A, double* __restrict__ B) {
+    int i0 = blockIdx.x * BX + threadIdx.x, j0 = blockIdx.y * BY + threadIdx.y, k = blockIdx.z;
+    if (i0 >= g.nx || j0 >= g.ny) return;
+    const long q = gidx(g, i0, j0, k);
+    const double SA = dscal[DS_PN_A], SB = dscal[DS_PN_B];
+    const double b = B[q];
+    const double po = pprev ? b - SB : b;
+    B[q] = pfirst ? po - SA : po + SB - SA;
+    A[q] = A[q] - SA;
+}
+
+// Turns a lazy rotation into stored values (for downloads, uploads and boundary
+// calls on the pressures).  The raw solution moves to the work array first while
+// it is still the reference's p.
+int launch_pressure_release_p(Ctx& c) {
+    if (c.p_lazy && c.p_in_pnew) {
+        HIP_OK(hipMemcpyAsync(c.p, c.pnew, sizeof(double) * c.nfield, hipMemcpyDeviceToDevice, c.stream));
+        c.p_in_pnew = false;
+    }
+    return ICLS_OK;
+}
+
+int launch_pressure_materialize(Ctx& c) {
+    if (!c.p_lazy) return ICLS_OK;
+    if (int rc = launch_pressure_release_p(c)) return rc;
+    dim3 blk(BX, BY, 1);
+    hipLaunchKernelGGL(k_pressure_materialize, grid3(c.g, c.g.nz + 2, blk), blk, 0, c.stream, c.g,
+                       (int)c.p_prev_lazy, (int)c.p_first, (const double*)c.dscal, c.pnew, c.pold);
+    HIP_OK(hipGetLastError());
+    c.p_lazy = false;
     return ICLS_OK;
 }
 
