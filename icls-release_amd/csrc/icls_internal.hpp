@@ -119,6 +119,7 @@ struct Ctx {
     bool p_prev_lazy = false;
     bool p_first = false;
     bool p_in_pnew = false;               // the reference's p is the pnew buffer
+    bool vbulk_fresh = false;             // DS_V_BULK is the interior sum of the current v
     double* thomas_d = nullptr;           // serial Thomas: multipliers (interior)
     double* thomas_y = nullptr;           // serial Thomas, mixed: forward-sweep values
     int thomas_seg = 0;                   // partitioned Thomas: segment length (0: serial)

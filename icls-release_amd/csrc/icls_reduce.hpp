@@ -31,6 +31,9 @@ __device__ __forceinline__ double block_reduce(double v, Op op, double* smem /* 
     return r;  // valid in thread 0
 }
 
+// linear id of the block in its 3D grid (x fastest)
+__device__ __forceinline__ int block_linear() { return blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z); }
+
 // Fold n partials (in order of index) into out[0] with one 256-thread block.
 template <class Op>
 __global__ void k_reduce_final(const double* __restrict__ part, int n, double init, double* out) {

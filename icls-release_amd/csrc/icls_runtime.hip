@@ -424,6 +424,7 @@ static int prepare_field(Ctx& c, int field) {
 
 int32_t icls_upload(int32_t field, int32_t l, const double* host) {
     CTX_OR_FAIL();
+    c.vbulk_fresh = false;
     TRY(prepare_field(c, field));
     double* d = field_ptr(c, field, l);
     if (!d || !host) { set_error("field %d (l=%d) cannot be uploaded", field, l); return ICLS_ERR_PARAM; }
@@ -850,7 +851,7 @@ int32_t icls_init_program(void) {
     // init_vel_p (init.f90:148-210): fields start at zero; 'cou' sets v
     double* zf[] = {c.unew, c.vnew, c.wnew, c.dudt, c.dvdt, c.dwdt};
     for (auto* f : zf) TRY(launch_fill(c, f, 0., c.fp32));
-    c.p_lazy = c.p_in_pnew = false;   // pnew and pold are stored values from here
+    c.p_lazy = c.p_in_pnew = c.vbulk_fresh = false;   // pnew and pold are stored values from here
     TRY(launch_fill(c, c.pnew, 0., 0));
     TRY(launch_init_velocity(c));
     for (int l = 0; l < c.lmax; l++) TRY(launch_init_levelset(c, l, c.lvset[l]));
@@ -928,7 +929,7 @@ int32_t icls_init_program_restart(const char* path, int32_t* begin) {
     double* zf[] = {c.unew, c.vnew, c.wnew, c.dudt, c.dvdt, c.dwdt};
     int rc = ICLS_OK;
     for (auto* f : zf) if (!rc) rc = launch_fill(c, f, 0., c.fp32);
-    c.p_lazy = c.p_in_pnew = false;
+    c.p_lazy = c.p_in_pnew = c.vbulk_fresh = false;
     if (!rc) rc = launch_fill(c, c.pnew, 0., 0);
     if (!rc) rc = launch_init_velocity(c);
     for (auto& fl : fld_fields(c)) {
@@ -957,7 +958,8 @@ int32_t icls_step(int32_t istep, int32_t begin) {
     if (!c.initialised) { set_error("icls_init_program has not run"); return ICLS_ERR_STATE; }
     c.time = c.time + c.dt;                                // main.f90:107
     TRY(ls_main_impl(c, istep));                           // main.f90:113
-    TRY(launch_sum_interior(c, c.vnew, DS_V_BULK));        // main.f90:124
+    // main.f90:124; correc already summed v unless v changed since
+    if (!c.vbulk_fresh) TRY(launch_sum_interior(c, c.vnew, DS_V_BULK));
     TRY(launch_momentum(c));                               // main.f90:130
     TRY(launch_bc_uvw(c, c.dudt, c.dvdt, c.dwdt));
     TRY(gfm_impl(c, true, false));                         // main.f90:139-143

@@ -19,7 +19,6 @@ static inline dim3 red_grid(const Grid& g, int k_lo, int k_hi) {
     return dim3((g.nx + BX - 1) / BX, (g.ny + BY - 1) / BY, (nk + KCH - 1) / KCH);
 }
 static inline int red_nblocks(dim3 d) { return (int)(d.x * d.y * d.z); }
-__device__ __forceinline__ int block_linear() { return blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z); }
 
 // ---------------------------------------------------------------- update_materials
 // misc.f90:100-147 for every cell of the C-halo range (x/y by wrap, z = 0..k1):

@@ -407,34 +407,60 @@ int launch_bc_p_t_pred(Ctx& c, double* p, const double* flag) {
 }
 
 // ---------------------------------------------------------------- correc
-// correc.f90:33-35; the pressure difference p(+1) - p - p_x is formed in double
+// correc.f90:33-35; the pressure difference p(+1) - p - p_x is formed in double.
+// A thread takes KCH layers of its column.  The block also sums the new v over its
+// cells into part[block] (the interior sum of average_stream_vel, misc.f90:631-656,
+// which the next step reads before anything changes v).
 template <class T>
 __global__ void k_correc(Grid g, T dt, T rho0, const double* __restrict__ p, const T* __restrict__ px,
                          const T* __restrict__ py, const T* __restrict__ pz, const T* __restrict__ du,
                          const T* __restrict__ dv, const T* __restrict__ dw, T* __restrict__ u, T* __restrict__ v,
-                         T* __restrict__ w) {
-    int i0 = blockIdx.x * BX + threadIdx.x, j0 = blockIdx.y * BY + threadIdx.y, k = blockIdx.z + 1;
-    if (i0 >= g.nx || j0 >= g.ny) return;
-    long q = gidx(g, i0, j0, k);
-    double pq = p[q];
-    u[q] = du[q] - dt * T(g.dxi) * T(p[gidx(g, wrap(i0 + 1, g.nx), j0, k)] - pq - (double)px[q]) / rho0;
-    v[q] = dv[q] - dt * T(g.dyi) * T(p[gidx(g, i0, wrap(j0 + 1, g.ny), k)] - pq - (double)py[q]) / rho0;
-    w[q] = dw[q] - dt * T(g.dzi) * T(p[gidx(g, i0, j0, k + 1)] - pq - (double)pz[q]) / rho0;
+                         T* __restrict__ w, double* __restrict__ part) {
+    __shared__ double smem[16];
+    const int i0 = blockIdx.x * BX + threadIdx.x, j0 = blockIdx.y * BY + threadIdx.y;
+    const bool valid = i0 < g.nx && j0 < g.ny;
+    double vs = 0.;
+    if (valid) {
+        const int ip = wrap(i0 + 1, g.nx), jp = wrap(j0 + 1, g.ny);
+#pragma unroll
+        for (int kk = 0; kk < KCH; kk++) {
+            const int k = 1 + blockIdx.z * KCH + kk;
+            if (k > g.nz) break;
+            const long q = gidx(g, i0, j0, k);
+            const double pq = p[q];
+            u[q] = du[q] - dt * T(g.dxi) * T(p[gidx(g, ip, j0, k)] - pq - (double)px[q]) / rho0;
+            const T vq = dv[q] - dt * T(g.dyi) * T(p[gidx(g, i0, jp, k)] - pq - (double)py[q]) / rho0;
+            v[q] = vq;
+            vs += (double)vq;
+            w[q] = dw[q] - dt * T(g.dzi) * T(p[gidx(g, i0, j0, k + 1)] - pq - (double)pz[q]) / rho0;
+        }
+    }
+    const double r = block_reduce(vs, OpSum(), smem);
+    if (threadIdx.x == 0 && threadIdx.y == 0) part[block_linear()] = r;
 }
 
 int launch_correc(Ctx& c) {
-    ProfScope ps(c, "k_correc");
-    dim3 blk(BX, BY, 1);
-    dim3 grd = grid3(c.g, c.g.nz, blk);
-    if (c.fp32)
-        hipLaunchKernelGGL(k_correc<float>, grd, blk, 0, c.stream, c.g, (float)c.dt, (float)c.rho0, (const double*)c.p,
-                           as<float>(c.p_x), as<float>(c.p_y), as<float>(c.p_z), as<float>(c.dudt), as<float>(c.dvdt),
-                           as<float>(c.dwdt), as<float>(c.unew), as<float>(c.vnew), as<float>(c.wnew));
-    else
-        hipLaunchKernelGGL(k_correc<double>, grd, blk, 0, c.stream, c.g, c.dt, c.rho0, (const double*)c.p,
-                           (const double*)c.p_x, (const double*)c.p_y, (const double*)c.p_z, (const double*)c.dudt,
-                           (const double*)c.dvdt, (const double*)c.dwdt, c.unew, c.vnew, c.wnew);
-    HIP_OK(hipGetLastError());
+    {
+        ProfScope ps(c, "k_correc");
+        dim3 blk(BX, BY, 1);
+        dim3 grd((c.g.nx + BX - 1) / BX, (c.g.ny + BY - 1) / BY, (c.g.nz + KCH - 1) / KCH);
+        if (c.fp32)
+            hipLaunchKernelGGL(k_correc<float>, grd, blk, 0, c.stream, c.g, (float)c.dt, (float)c.rho0,
+                               (const double*)c.p, as<float>(c.p_x), as<float>(c.p_y), as<float>(c.p_z),
+                               as<float>(c.dudt), as<float>(c.dvdt), as<float>(c.dwdt), as<float>(c.unew),
+                               as<float>(c.vnew), as<float>(c.wnew), c.dpart);
+        else
+            hipLaunchKernelGGL(k_correc<double>, grd, blk, 0, c.stream, c.g, c.dt, c.rho0, (const double*)c.p,
+                               (const double*)c.p_x, (const double*)c.p_y, (const double*)c.p_z,
+                               (const double*)c.dudt, (const double*)c.dvdt, (const double*)c.dwdt, c.unew, c.vnew,
+                               c.wnew, c.dpart);
+        HIP_OK(hipGetLastError());
+        const int nb = (int)(grd.x * grd.y * grd.z);
+        hipLaunchKernelGGL(k_reduce_final<OpSum>, dim3(1), dim3(256), 0, c.stream, (const double*)c.dpart, nb, 0.,
+                           c.dscal + DS_V_BULK);
+        HIP_OK(hipGetLastError());
+    }
+    c.vbulk_fresh = true;
     return ICLS_OK;
 }
 
@@ -578,7 +604,6 @@ int launch_nan_check(Ctx& c) {
 static inline dim3 red_grid(const Grid& g, int nk) {
     return dim3((g.nx + BX - 1) / BX, (g.ny + BY - 1) / BY, (nk + KCH - 1) / KCH);
 }
-__device__ __forceinline__ int block_linear() { return blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z); }
 
 // chkdiv (chkdiv.f90:28-48): sum of div and max |div| with its first location in
 // (k, j, i) loop order; partials [sum | max | location].  The divergence is formed in
