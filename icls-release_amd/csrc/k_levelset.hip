@@ -56,8 +56,110 @@ __global__ void k_tube(Grid g, T NB2, const T* __restrict__ in, T* __restrict__ 
     mask[q] = m ? 1 : 0;
 }
 
+// The same construction marching in z over a 64 x 4 tile: the input passes through a
+// ring of five LDS planes (k-2 .. k+2) with a two-cell x/y halo, so the 13-point scan
+// reads LDS and every input element is fetched once per block.  Chunks of TZ layers;
+// the first and last chunk also write the ghost layers.  TPD planes are loaded ahead
+// (unconditional loads, clamped to the chunk, as in the re-initialisation kernel).
+constexpr int TZ = 32, TH = 2, TTW = BX + 2 * TH, TTH = BY + 2 * TH, TTP = TTW * TTH;
+constexpr int TNLD = (TTP + BX * BY - 1) / (BX * BY), TNS = 2 * TH + 1;
+
+template <class T, int TPD>
+__global__ void __launch_bounds__(BX * BY)
+k_tube_zm(Grid g, T NB2, const T* __restrict__ in, T* __restrict__ out, uint8_t* __restrict__ mask) {
+    __shared__ T ring[TNS][TTP];
+    const int ntx = (g.nx + BX - 1) / BX, nty = (g.ny + BY - 1) / BY, ntz = (g.nz + TZ - 1) / TZ;
+    const int tile = xcd_tile(blockIdx.x, ntx * nty * ntz);
+    const int bx = tile % ntx, by = (tile / ntx) % nty, bz = tile / (ntx * nty);
+    const int x0 = bx * BX, y0 = by * BY, kb = 1 + bz * TZ;
+    const int ke = kb + TZ - 1 < g.nz ? kb + TZ - 1 : g.nz;
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = tx + BX * ty;
+    const int xm = g.nx - 1, ym = g.ny - 1;
+    // a thread past the end of x or y takes the wrapped cell (same values, same stores)
+    const int i = (x0 + tx) & xm, j = (y0 + ty) & ym;
+    long poff[TNLD];
+#pragma unroll
+    for (int r = 0; r < TNLD; r++) {
+        int e = tid + r * BX * BY;
+        if (e >= TTP) e = TTP - 1;
+        const int lx = e % TTW - TH, ly = e / TTW - TH;
+        poff[r] = (long)((x0 + lx) & xm) + g.sy * (long)((y0 + ly) & ym);
+    }
+    T nxt[TPD][TNLD];
+    auto fetch = [&](int k, T* dst) {
+        const long base = g.sz * (long)(k + 2);
+#pragma unroll
+        for (int r = 0; r < TNLD; r++) dst[r] = in[base + poff[r]];
+    };
+    auto slot_of = [&](int k) { return (k - kb + 2 * TNS) % TNS; };
+    auto stash_plane = [&](int k, const T* src) {
+        T* dst = ring[slot_of(k)];
+#pragma unroll
+        for (int r = 0; r < TNLD; r++) {
+            const int e = tid + r * BX * BY;
+            if (e < TTP) dst[e] = src[r];
+        }
+    };
+    auto clampv = [&](T v) { return v > NB2 ? NB2 : (v < -NB2 ? -NB2 : v); };
+    auto col = [&](int k) { return in[gidx(g, i, j, k)]; };
+    // ghost layers below the first chunk: clamped, layer 0 extrapolated
+    if (bz == 0) {
+        out[gidx(g, i, j, -2)] = clampv(col(-2));
+        out[gidx(g, i, j, -1)] = clampv(col(-1));
+        out[gidx(g, i, j, 0)] = T(2.) * clampv(col(1)) - clampv(col(2));
+    }
+    for (int k = kb - TH; k < kb + TH; k++) {
+        fetch(k, nxt[0]);
+        stash_plane(k, nxt[0]);
+    }
+#pragma unroll
+    for (int u = 0; u < TPD; u++) fetch(min(kb + u, ke) + TH, nxt[u]);
+    const int c0 = (ty + TH) * TTW + tx + TH;
+    for (int k0 = kb; k0 <= ke; k0 += TPD) {
+#pragma unroll
+        for (int u = 0; u < TPD; u++) {
+            const int k = k0 + u;
+            stash_plane(k + TH, nxt[u]);
+            __syncthreads();
+            fetch(min(k + TPD, ke) + TH, nxt[u]);
+            const T* P = ring[slot_of(k)];
+            const T v = P[c0];
+            bool m = fabs(v) < NB2;
+#pragma unroll
+            for (int d = 1; d <= TH; d++) {
+                m = m || fabs(P[c0 - d]) < NB2 || fabs(P[c0 + d]) < NB2;
+                m = m || fabs(P[c0 - d * TTW]) < NB2 || fabs(P[c0 + d * TTW]) < NB2;
+                m = m || fabs(ring[slot_of(k - d)][c0]) < NB2 || fabs(ring[slot_of(k + d)][c0]) < NB2;
+            }
+            const long q = gidx(g, i, j, k);
+            out[q] = clampv(v);
+            mask[q] = m ? 1 : 0;
+            __syncthreads();
+        }
+    }
+    if (bz == ntz - 1) {
+        out[gidx(g, i, j, g.nz + 1)] = T(2.) * clampv(col(g.nz)) - clampv(col(g.nz - 1));
+        out[gidx(g, i, j, g.nz + 2)] = clampv(col(g.nz + 2));
+        out[gidx(g, i, j, g.nz + 3)] = clampv(col(g.nz + 3));
+    }
+}
+
+template <class T>
+static void tube_dispatch(Ctx& c, const double* phi_in, double* phi_out) {
+    dim3 blk(BX, BY, 1);
+    const int ntiles = ((c.g.nx + BX - 1) / BX) * ((c.g.ny + BY - 1) / BY) * ((c.g.nz + TZ - 1) / TZ);
+    // groups of 2 steps when every chunk length is even
+    if (TZ % 2 == 0 && (c.g.nz % TZ) % 2 == 0)
+        hipLaunchKernelGGL((k_tube_zm<T, 2>), dim3(ntiles), blk, 0, c.stream, c.g, (T)c.NB2, as<T>(phi_in),
+                           as<T>(phi_out), c.mask);
+    else
+        hipLaunchKernelGGL((k_tube_zm<T, 1>), dim3(ntiles), blk, 0, c.stream, c.g, (T)c.NB2, as<T>(phi_in),
+                           as<T>(phi_out), c.mask);
+}
+
 int launch_tube(Ctx& c, const double* phi_in, double* phi_out) {
     ProfScope ps(c, "k_tube");
+#ifdef ICLS_TUBE_POINT
     dim3 blk(BX, BY, 1);
     dim3 grd = grid3(c.g, c.g.nz + 6, blk);
     if (c.fp32)
@@ -65,6 +167,10 @@ int launch_tube(Ctx& c, const double* phi_in, double* phi_out) {
                            c.mask);
     else
         hipLaunchKernelGGL(k_tube<double>, grd, blk, 0, c.stream, c.g, c.NB2, phi_in, phi_out, c.mask);
+#else
+    if (c.fp32) tube_dispatch<float>(c, phi_in, phi_out);
+    else tube_dispatch<double>(c, phi_in, phi_out);
+#endif
     HIP_OK(hipGetLastError());
     return ICLS_OK;
 }

@@ -815,7 +815,7 @@ __global__ void k_thomas_seed(Grid g, const double* __restrict__ xyrt, double* _
 // holds two segments of 32 columns (fewer registers per lane at S = 8, so that two
 // blocks fit a CU).  A block is P segments x CPW columns.
 #ifndef ICLS_THOMAS_CPW
-#define ICLS_THOMAS_CPW 64
+#define ICLS_THOMAS_CPW 32
 #endif
 constexpr int TCPW = ICLS_THOMAS_CPW;
 constexpr int TMAXSEG = 16 * (64 / TCPW);   // segments per block at 1024 threads
@@ -903,7 +903,7 @@ k_thomas_seg(Grid g, const double* __restrict__ xyrt, const double* __restrict__
 // Segment length of the partitioned solve for nz layers, 0 where the serial kernel
 // runs (a block holds at most TMAXSEG segments).
 int thomas_seg_len(int nz) {
-    if (TCPW == 64 && nz % 16 == 0 && nz / 16 <= TMAXSEG) return 16;
+    if (nz % 16 == 0 && nz / 16 <= TMAXSEG && nz / 16 >= 64 / TCPW) return 16;
     if (nz % 8 == 0 && nz / 8 <= TMAXSEG) return 8;
     return 0;
 }
