@@ -34,27 +34,43 @@ __device__ __forceinline__ double block_reduce(double v, Op op, double* smem /* 
 // linear id of the block in its 3D grid (x fastest)
 __device__ __forceinline__ int block_linear() { return blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z); }
 
-// Fold n partials (in order of index) into out[0] with one 256-thread block.
+// threads of the single block that folds the partials
+constexpr int RED_FINAL_THREADS = 1024;
+
+// Thread t folds partials t, t + T, t + 2T, ... (T threads) in that order, four loads
+// in flight at a time; the block then folds the threads' values.  The order is fixed,
+// so the result does not change from run to run.
 template <class Op>
-__global__ void k_reduce_final(const double* __restrict__ part, int n, double init, double* out) {
-    __shared__ double smem[16];
-    double v = init;
+__device__ __forceinline__ double fold_partials(const double* __restrict__ part, int n, double init) {
     Op op;
-    for (int q = threadIdx.x; q < n; q += blockDim.x) v = op(v, part[q]);
-    double r = block_reduce(v, op, smem);
+    double v = init;
+    const int T = blockDim.x;
+    int q = threadIdx.x;
+    for (; q + 3 * T < n; q += 4 * T) {
+        const double a = part[q], b = part[q + T], c = part[q + 2 * T], d = part[q + 3 * T];
+        v = op(op(op(op(v, a), b), c), d);
+    }
+    for (; q < n; q += T) v = op(v, part[q]);
+    return v;
+}
+
+// Fold n partials into out[0] with one block of RED_FINAL_THREADS threads.
+template <class Op>
+__global__ void __launch_bounds__(RED_FINAL_THREADS)
+k_reduce_final(const double* __restrict__ part, int n, double init, double* out) {
+    __shared__ double smem[16];
+    double r = block_reduce(fold_partials<Op>(part, n, init), Op(), smem);
     if (threadIdx.x == 0) out[0] = r;
 }
 
 // as k_reduce_final, skipped when the device switch flag[0] is zero
 template <class Op>
-__global__ void k_reduce_final_pred(const double* __restrict__ part, int n, double init, double* out,
-                                    const double* __restrict__ flag) {
+__global__ void __launch_bounds__(RED_FINAL_THREADS)
+k_reduce_final_pred(const double* __restrict__ part, int n, double init, double* out,
+                    const double* __restrict__ flag) {
     if (flag[0] == 0.) return;
     __shared__ double smem[16];
-    double v = init;
-    Op op;
-    for (int q = threadIdx.x; q < n; q += blockDim.x) v = op(v, part[q]);
-    double r = block_reduce(v, op, smem);
+    double r = block_reduce(fold_partials<Op>(part, n, init), Op(), smem);
     if (threadIdx.x == 0) out[0] = r;
 }
 

@@ -90,7 +90,7 @@ int launch_materials(Ctx& c) {
     HIP_OK(hipGetLastError());
     double* sums = c.dscal + DS_COUNT_FIXED + 2 * c.lmax;   // lmax scratch sums after volume/init_volume
     for (int l = 0; l < c.lmax; l++)
-        hipLaunchKernelGGL(k_reduce_final<OpSum>, dim3(1), dim3(256), 0, c.stream, c.dpart + (long)l * nb, nb, 0.,
+        hipLaunchKernelGGL(k_reduce_final<OpSum>, dim3(1), dim3(RED_FINAL_THREADS), 0, c.stream, c.dpart + (long)l * nb, nb, 0.,
                            sums + l);
     double ncell = 1. * c.g.nx * c.g.ny * c.g.nz;
     hipLaunchKernelGGL(k_volume_final, dim3(1), dim3(64), 0, c.stream, c.lmax, ncell, P.rho1, P.rho2,
@@ -147,7 +147,7 @@ int launch_sum_interior(Ctx& c, const double* f, int slot) {
     int nb = red_nblocks(grd);
     if (c.fp32) hipLaunchKernelGGL(k_sum_interior<float>, grd, blk, 0, c.stream, c.g, as<float>(f), c.dpart);
     else hipLaunchKernelGGL(k_sum_interior<double>, grd, blk, 0, c.stream, c.g, f, c.dpart);
-    hipLaunchKernelGGL(k_reduce_final<OpSum>, dim3(1), dim3(256), 0, c.stream, (const double*)c.dpart, nb, 0.,
+    hipLaunchKernelGGL(k_reduce_final<OpSum>, dim3(1), dim3(RED_FINAL_THREADS), 0, c.stream, (const double*)c.dpart, nb, 0.,
                        c.dscal + slot);
     HIP_OK(hipGetLastError());
     return ICLS_OK;
@@ -196,7 +196,7 @@ int launch_ipot(Ctx& c, double* dtemp) {
                                c.g.dz, c.NB2, c.dpart, nb);
         for (int n = 0; n < c.lmax; n++) {
             if (n == m) continue;
-            hipLaunchKernelGGL(k_reduce_final<OpMin>, dim3(1), dim3(256), 0, c.stream, (const double*)(c.dpart + (long)n * nb),
+            hipLaunchKernelGGL(k_reduce_final<OpMin>, dim3(1), dim3(RED_FINAL_THREADS), 0, c.stream, (const double*)(c.dpart + (long)n * nb),
                                nb, 1e3, dtemp + m + (long)c.lmax * n);
         }
     }
@@ -308,9 +308,9 @@ static int inflation_velocity(Ctx& c, int l) {
     hipLaunchKernelGGL(k_infl_sums<T>, grd, blk, 0, c.stream, c.g, flag, as<T>(c.lvset[l]), as<T>(c.curv2),
                        as<T>(c.normal_m), (T)s1, (T)c.pi, as<T>(c.dheav), c.dpart, nb);
     double* sums = c.dscal + DS_COUNT_FIXED + 2 * c.lmax;
-    hipLaunchKernelGGL(k_reduce_final_pred<OpSum>, dim3(1), dim3(256), 0, c.stream, (const double*)c.dpart, nb, 0.,
+    hipLaunchKernelGGL(k_reduce_final_pred<OpSum>, dim3(1), dim3(RED_FINAL_THREADS), 0, c.stream, (const double*)c.dpart, nb, 0.,
                        sums, flag);
-    hipLaunchKernelGGL(k_reduce_final_pred<OpSum>, dim3(1), dim3(256), 0, c.stream, (const double*)(c.dpart + nb), nb,
+    hipLaunchKernelGGL(k_reduce_final_pred<OpSum>, dim3(1), dim3(RED_FINAL_THREADS), 0, c.stream, (const double*)(c.dpart + nb), nb,
                        0., sums + 1, flag);
     double ncell = 1. * c.g.nx * c.g.ny * c.g.nz;
     double lxyz = c.P.lx * c.P.ly * c.P.lz;

@@ -456,7 +456,7 @@ int launch_correc(Ctx& c) {
                                c.wnew, c.dpart);
         HIP_OK(hipGetLastError());
         const int nb = (int)(grd.x * grd.y * grd.z);
-        hipLaunchKernelGGL(k_reduce_final<OpSum>, dim3(1), dim3(256), 0, c.stream, (const double*)c.dpart, nb, 0.,
+        hipLaunchKernelGGL(k_reduce_final<OpSum>, dim3(1), dim3(RED_FINAL_THREADS), 0, c.stream, (const double*)c.dpart, nb, 0.,
                            c.dscal + DS_V_BULK);
         HIP_OK(hipGetLastError());
     }
@@ -713,7 +713,7 @@ int launch_chkdt_umax(Ctx& c) {
         hipLaunchKernelGGL(k_umax<double>, grd, blk, 0, c.stream, c.g, (const double*)c.unew, (const double*)c.vnew,
                            (const double*)c.wnew, c.dpart, nb);
     for (int t = 0; t < 3; t++)
-        hipLaunchKernelGGL(k_reduce_final<OpMax>, dim3(1), dim3(256), 0, c.stream, (const double*)(c.dpart + t * nb), nb,
+        hipLaunchKernelGGL(k_reduce_final<OpMax>, dim3(1), dim3(RED_FINAL_THREADS), 0, c.stream, (const double*)(c.dpart + t * nb), nb,
                            0., c.dscal + DS_UMAX + t);
     HIP_OK(hipGetLastError());
     return ICLS_OK;
@@ -748,7 +748,7 @@ int launch_max_vel(Ctx& c) {
     else
         hipLaunchKernelGGL(k_vmag_max<double>, grd, blk, 0, c.stream, c.g, (const double*)c.ucc, (const double*)c.vcc,
                            (const double*)c.wcc, c.dpart);
-    hipLaunchKernelGGL(k_reduce_final<OpMax>, dim3(1), dim3(256), 0, c.stream, (const double*)c.dpart, nb, 0.,
+    hipLaunchKernelGGL(k_reduce_final<OpMax>, dim3(1), dim3(RED_FINAL_THREADS), 0, c.stream, (const double*)c.dpart, nb, 0.,
                        c.dscal + DS_VMAX);
     HIP_OK(hipGetLastError());
     return ICLS_OK;
