@@ -62,12 +62,15 @@ def cpu_baseline(p, nsteps=1):
                       (nsteps, p.itot, p.jtot, p.ktot, el)}
 
 
-def traffic_for(kernel, workload):
+def traffic_for(kernel, workload, fp32):
+    """Measured HBM bytes per launch of the kernel a profiler scope times, from
+    profiles/traffic.json (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
-    if not os.path.exists(path):
+    key = R.traffic_key(kernel, fp32)
+    if key is None or not os.path.exists(path):
         return None
     try:
-        rec = json.load(open(path)).get(workload, {}).get(kernel)
+        rec = json.load(open(path)).get(workload, {}).get(key)
         return None if rec is None else rec["bytes_per_launch"]
     except (OSError, ValueError, KeyError):
         return None
@@ -126,7 +129,7 @@ def run_mode(mode, n, steps, warmup, show_kernels):
     achieved = bytes_per_launch / avg_s / 1e9
     roof = {"kernel": dominant, "bound": "hbm", "achieved": round(achieved, 1), "peak": R.HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / R.HBM_PEAK_GBS, 4),
-            "traffic": traffic_for(dominant, workload), "algorithmic_bytes_per_launch": bytes_per_launch,
+            "traffic": traffic_for(dominant, workload, fp32), "algorithmic_bytes_per_launch": bytes_per_launch,
             "avg_launch_ms": round(avg_s * 1e3, 4)}
     return {"p": p, "workload": workload, "value": ncell * steps / el / 1e6, "ms_per_step": 1e3 * el / steps,
             "roofline": roof, "init_s": t_init, "listed_cells": listed}

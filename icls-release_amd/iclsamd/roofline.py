@@ -63,3 +63,25 @@ def step_bytes_per_cell(calls, mode_fp32=False):
     """Algorithmic bytes of one step given {kernel: launches per step}."""
     kb = kernel_bytes(mode_fp32)
     return sum(kb.get(k, 0) * n for k, n in calls.items())
+
+
+# profiler scope name -> the kernel symbol (template arguments as the profiler prints
+# them; {T} is float or double) whose HBM traffic profiles/traffic.json records
+_TRAFFIC_KERNEL = {
+    "k_momentum": "k_momentum_zm<{T}, 0, true>",
+    "k_correc": "k_correc<{T}>",
+    "k_gfm": "k_gfm<{T}, 2>",
+    "k_tube": "k_tube_zm<{T}, 2>",
+    "k_reinit_stage1": "k_reinit_zm<{T}, 1, true, 4, true>",
+    "k_reinit_stage23": "k_reinit_zm<{T}, 3, true, 4, true>",
+    "k_materials": "k_materials<{T}>",
+    "k_curvature": "k_curvature<{T}>",
+    "k_thomas": "k_thomas_seg<{T}, 16>",
+}
+
+
+def traffic_key(scope, fp32):
+    """The traffic.json key of the kernel a profiler scope times, or None."""
+    k = _TRAFFIC_KERNEL.get(scope)
+    return None if k is None else k.format(T="float" if fp32 else "double")
+
