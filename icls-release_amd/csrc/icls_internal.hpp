@@ -120,6 +120,12 @@ struct Ctx {
     bool p_first = false;
     bool p_in_pnew = false;               // the reference's p is the pnew buffer
     bool vbulk_fresh = false;             // DS_V_BULK is the interior sum of the current v
+    // Face-jump masks: bit 0/1/2 of jmask says p_x/p_y/p_z at the cell may be nonzero
+    // (jmask_old likewise for the old jumps).  Readers take a jump value only where its
+    // bit is set; elsewhere the jump is zero whatever the array holds.
+    uint8_t* jmask = nullptr;
+    uint8_t* jmask_old = nullptr;
+    double* zplane = nullptr;             // nx * ny zeros, never written: the masked-off jump reads
     double* thomas_d = nullptr;           // serial Thomas: multipliers (interior)
     double* thomas_y = nullptr;           // serial Thomas, mixed: forward-sweep values
     int thomas_seg = 0;                   // partitioned Thomas: segment length (0: serial)
@@ -220,6 +226,7 @@ int launch_add_pjump(Ctx& c);
 // poisson: solve L p = frhs into p (solver2d; in the mixed modes with one refinement)
 int launch_solve(Ctx& c);
 int launch_pressure_materialize(Ctx& c);
+int launch_jump_densify(Ctx& c, bool old);   // zero the unmasked jumps, mask every cell
 int launch_pressure_release_p(Ctx& c);   // the work array p takes the raw solution back
 int thomas_seg_len(int nz);
 int launch_thomas_seed(Ctx& c);

@@ -146,6 +146,9 @@ static void free_ctx(Ctx* c) {
                                c->xyrt, c->tw_x, c->tw_y, c->dls, c->stash, c->scratch, c->pf, c->thomas_y,
                                c->thomas_seed, c->pinvA_f, c->tw_x_f, c->tw_y_f};
     if (c->frhs != c->p) ptrs.push_back(c->frhs);   // in FP64 the right-hand side is p itself
+    ptrs.push_back(c->jmask);
+    ptrs.push_back(c->jmask_old);
+    ptrs.push_back(c->zplane);
     for (auto* p : ptrs) if (p) hipFree(p);
     for (auto* p : c->lvset) if (p) hipFree(p);
     if (c->hscal) hipHostFree(c->hscal);
@@ -250,6 +253,13 @@ static int build_ctx(const icls_params* prm) {
     }
     HIP_OK(hipMalloc((void**)&C.mask, C.nfield));
     HIP_OK(hipMemset(C.mask, 0, C.nfield));
+    // face-jump masks start with every bit set: the (zero) arrays are read as they are
+    HIP_OK(hipMalloc((void**)&C.jmask, C.nfield));
+    HIP_OK(hipMemset(C.jmask, 7, C.nfield));
+    HIP_OK(hipMalloc((void**)&C.jmask_old, C.nfield));
+    HIP_OK(hipMemset(C.jmask_old, 7, C.nfield));
+    HIP_OK(hipMalloc((void**)&C.zplane, sizeof(double) * g.nx * g.ny));
+    HIP_OK(hipMemset(C.zplane, 0, sizeof(double) * g.nx * g.ny));
     HIP_OK(hipMalloc((void**)&C.stash, sizeof(double) * 2 * g.sz));
     const int nscal = DS_COUNT_FIXED + 3 * C.lmax + 16;
     HIP_OK(hipMalloc((void**)&C.dscal, sizeof(double) * nscal));
@@ -418,6 +428,8 @@ int32_t icls_sync(void) {
 // the pressures of a lazy rotation are made stored values before they are accessed
 static int prepare_field(Ctx& c, int field) {
     if (field == ICLS_F_PNEW || field == ICLS_F_POLD) return launch_pressure_materialize(c);
+    if (field == ICLS_F_PX || field == ICLS_F_PY || field == ICLS_F_PZ) return launch_jump_densify(c, false);
+    if (field == ICLS_F_PXOLD || field == ICLS_F_PYOLD || field == ICLS_F_PZOLD) return launch_jump_densify(c, true);
     if (field == ICLS_F_P) return launch_pressure_release_p(c);
     return ICLS_OK;
 }
@@ -644,6 +656,7 @@ static void rotate_jumps(Ctx& c) {
     std::swap(c.p_x, c.p_xold);
     std::swap(c.p_y, c.p_yold);
     std::swap(c.p_z, c.p_zold);
+    std::swap(c.jmask, c.jmask_old);
 }
 
 static int gfm_impl(Ctx& c, bool fuse_fillps, bool store_pjump) {
@@ -835,6 +848,7 @@ static int init_derived_impl(Ctx& c) {
     TRY(launch_copy_field(c, c.p_x, c.p_xold));
     TRY(launch_copy_field(c, c.p_y, c.p_yold));
     TRY(launch_copy_field(c, c.p_z, c.p_zold));
+    HIP_OK(hipMemcpyAsync(c.jmask_old, c.jmask, c.nfield, hipMemcpyDeviceToDevice, c.stream));
     HIP_OK(hipStreamSynchronize(c.stream));
     c.initialised = true;
     return ICLS_OK;

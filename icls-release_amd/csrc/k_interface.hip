@@ -167,7 +167,7 @@ template <class T, int MODE>
 __global__ void k_gfm(Grid g, T nondi, T coef, const T* __restrict__ ls, const T* __restrict__ cvf,
                       T* __restrict__ pjump, T* __restrict__ px, T* __restrict__ py, T* __restrict__ pz,
                       const T* __restrict__ du, const T* __restrict__ dv, const T* __restrict__ dw,
-                      T* __restrict__ rhs) {
+                      T* __restrict__ rhs, uint8_t* __restrict__ jm) {
     int i0 = blockIdx.x * BX + threadIdx.x, j0 = blockIdx.y * BY + threadIdx.y, k = blockIdx.z + 1;
     if (i0 >= g.nx || j0 >= g.ny) return;
     long q = gidx(g, i0, j0, k);
@@ -180,7 +180,18 @@ __global__ void k_gfm(Grid g, T nondi, T coef, const T* __restrict__ ls, const T
         }
         return;
     }
-    px[q] = pxs; py[q] = pys; pz[q] = pzs;
+    if (MODE == 2) {
+        // the face jumps are zero outside a few cells of the interface: only the nonzero
+        // ones are stored, and the mask says which
+        const unsigned bits = (pxs != T(0.) ? 1u : 0u) | (pys != T(0.) ? 2u : 0u) | (pzs != T(0.) ? 4u : 0u);
+        jm[q] = (uint8_t)bits;
+        if (bits & 1u) px[q] = pxs;
+        if (bits & 2u) py[q] = pys;
+        if (bits & 4u) pz[q] = pzs;
+    } else {
+        px[q] = pxs; py[q] = pys; pz[q] = pzs;
+        jm[q] = 7;
+    }
     if (pjump) pjump[q] = pjs;
     if (MODE == 2) rhs[q] = fillps_rhs<T>(g, coef, du, dv, dw, i0, j0, k) + pjs;
 }
@@ -196,13 +207,13 @@ static void gfm_dispatch(Ctx& c, int l, const double* kappa2, int mode, bool sto
     if (mode == 2)
         hipLaunchKernelGGL((k_gfm<T, 2>), grd, blk, 0, c.stream, c.g, nondi, coef, ls, k2,
                            store_pjump ? as<T>(c.p_jump) : (T*)nullptr, px, py, pz, as<T>(c.dudt), as<T>(c.dvdt),
-                           as<T>(c.dwdt), as<T>(c.frhs));
+                           as<T>(c.dwdt), as<T>(c.frhs), c.jmask);
     else if (mode == 0)
         hipLaunchKernelGGL((k_gfm<T, 0>), grd, blk, 0, c.stream, c.g, nondi, coef, ls, k2, as<T>(c.p_jump), px, py, pz,
-                           (const T*)nullptr, (const T*)nullptr, (const T*)nullptr, (T*)nullptr);
+                           (const T*)nullptr, (const T*)nullptr, (const T*)nullptr, (T*)nullptr, c.jmask);
     else
         hipLaunchKernelGGL((k_gfm<T, 1>), grd, blk, 0, c.stream, c.g, nondi, coef, ls, k2, as<T>(c.p_jump), px, py, pz,
-                           (const T*)nullptr, (const T*)nullptr, (const T*)nullptr, (T*)nullptr);
+                           (const T*)nullptr, (const T*)nullptr, (const T*)nullptr, (T*)nullptr, c.jmask);
 }
 
 // fuse_fillps: the single-level-set step (MODE 2); otherwise MODE 0 for l = 0 and
@@ -212,6 +223,34 @@ int launch_gfm(Ctx& c, int l, const double* kappa2, bool fuse_fillps, bool store
     const int mode = fuse_fillps ? 2 : (l == 0 ? 0 : 1);
     if (c.fp32) gfm_dispatch<float>(c, l, kappa2, mode, store_pjump);
     else gfm_dispatch<double>(c, l, kappa2, mode, store_pjump);
+    HIP_OK(hipGetLastError());
+    return ICLS_OK;
+}
+
+// every stored cell: jumps whose mask bit is clear become zero, every bit is set
+template <class T>
+__global__ void k_jump_densify(Grid g, uint8_t* __restrict__ jm, T* __restrict__ px, T* __restrict__ py,
+                               T* __restrict__ pz) {
+    int i0 = blockIdx.x * BX + threadIdx.x, j0 = blockIdx.y * BY + threadIdx.y, k = (int)blockIdx.z - 2;
+    if (i0 >= g.nx || j0 >= g.ny) return;
+    const long q = gidx(g, i0, j0, k);
+    const unsigned b = jm[q];
+    if (!(b & 1u)) px[q] = T(0.);
+    if (!(b & 2u)) py[q] = T(0.);
+    if (!(b & 4u)) pz[q] = T(0.);
+    jm[q] = 7;
+}
+
+int launch_jump_densify(Ctx& c, bool old) {
+    dim3 blk(BX, BY, 1);
+    dim3 grd = grid3(c.g, c.g.nz + 6, blk);
+    uint8_t* jm = old ? c.jmask_old : c.jmask;
+    double *px = old ? c.p_xold : c.p_x, *py = old ? c.p_yold : c.p_y, *pz = old ? c.p_zold : c.p_z;
+    if (c.fp32)
+        hipLaunchKernelGGL(k_jump_densify<float>, grd, blk, 0, c.stream, c.g, jm, as<float>(px), as<float>(py),
+                           as<float>(pz));
+    else
+        hipLaunchKernelGGL(k_jump_densify<double>, grd, blk, 0, c.stream, c.g, jm, px, py, pz);
     HIP_OK(hipGetLastError());
     return ICLS_OK;
 }

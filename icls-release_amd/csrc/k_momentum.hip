@@ -49,7 +49,8 @@ k_momentum_zm(Grid g, MomParams m, const double* __restrict__ dscal, const T* __
               T* __restrict__ dvold, T* __restrict__ dwold, const double* __restrict__ pnew,
               const double* __restrict__ pold, const T* __restrict__ p_x, const T* __restrict__ p_y,
               const T* __restrict__ p_z, const T* __restrict__ p_xold, const T* __restrict__ p_yold,
-              const T* __restrict__ p_zold, T* __restrict__ dudt, T* __restrict__ dvdt, T* __restrict__ dwdt) {
+              const T* __restrict__ p_zold, T* __restrict__ dudt, T* __restrict__ dvdt, T* __restrict__ dwdt,
+              const uint8_t* __restrict__ jm, const uint8_t* __restrict__ jmo, const T* __restrict__ zp) {
     __shared__ T ring[5][4][TP];   // u, v, w, miu, rho x four planes
     const int ntx = (g.nx + BX - 1) / BX, nty = (g.ny + BY - 1) / BY, ntz = (g.nz + MZ - 1) / MZ;
     const int tile = xcd_tile(blockIdx.x, ntx * nty * ntz);
@@ -103,15 +104,31 @@ k_momentum_zm(Grid g, MomParams m, const double* __restrict__ dscal, const T* __
     // again.  Every load of the loop is unconditional, its layer clamped to the last
     // one the chunk reads: a conditional load would be merged into its register by a
     // copy, and the copy waits for every load in flight.
+    // The face jumps are read only where the GFM masks say they may be nonzero; the
+    // mask bytes travel one step further ahead (jn, jo: the layer fetch_point loads
+    // next), so that the jump loads do not wait for them.  A masked-off lane loads a
+    // zero of the zero plane zp instead (cached, the same words every layer): the load
+    // stays unconditional, with no branch and no select after it.
     T pd[9];
     double pp[8];
+    unsigned jn = 7u, jo = 7u;
+    auto fetch_mask = [&](int k) {
+        if (RHS_ONLY) return;
+        const long q = gidx(g, i, j, k);
+        jn = jm[q];
+        jo = jmo[q];
+    };
     auto fetch_point = [&](int k) {
         if (RHS_ONLY) return;
         const long q = gidx(g, i, j, k), qx = gidx(g, ip, j, k), qy = gidx(g, i, jp, k), qz = q + g.sz;
         pd[0] = duold[q]; pd[1] = dvold[q]; pd[2] = dwold[q];
-        pd[3] = p_x[q]; pd[4] = p_xold[q];
-        pd[5] = p_y[q]; pd[6] = p_yold[q];
-        pd[7] = p_z[q]; pd[8] = p_zold[q];
+        const T* z0 = zp + (i + g.sy * (long)j);
+        pd[3] = *((jn & 1u) ? p_x + q : z0);
+        pd[4] = *((jo & 1u) ? p_xold + q : z0);
+        pd[5] = *((jn & 2u) ? p_y + q : z0);
+        pd[6] = *((jo & 2u) ? p_yold + q : z0);
+        pd[7] = *((jn & 4u) ? p_z + q : z0);
+        pd[8] = *((jo & 4u) ? p_zold + q : z0);
         pp[0] = pnew[q]; pp[1] = pold[q];
         pp[2] = pnew[qx]; pp[3] = pold[qx];
         pp[4] = pnew[qy]; pp[5] = pold[qy];
@@ -144,8 +161,10 @@ k_momentum_zm(Grid g, MomParams m, const double* __restrict__ dscal, const T* __
         gpy = phy - (double)pjy - phq;
         gpz = phz - (double)pjz - phq;
     };
+    fetch_mask(kb);
     fetch_point(kb);
     derive();
+    fetch_mask(min(kb + 1, ke));
     __syncthreads();
     const Sp<T, P2> sx{T(g.dx), T(g.dxi)}, sy{T(g.dy), T(g.dyi)}, sz{T(g.dz), T(g.dzi)};
     const T qt = T(0.25), two = T(2.), one = T(1.);
@@ -157,6 +176,7 @@ k_momentum_zm(Grid g, MomParams m, const double* __restrict__ dscal, const T* __
         const long q = gidx(g, i, j, k);
         fetch(min(k + 2, ke + 1), nxt);
         fetch_point(min(k + 1, ke));
+        fetch_mask(min(k + 2, ke));
         const T* U[3] = {ring[0][slot_of(k - 1)], ring[0][slot_of(k)], ring[0][slot_of(k + 1)]};
         const T* V[3] = {ring[1][slot_of(k - 1)], ring[1][slot_of(k)], ring[1][slot_of(k + 1)]};
         const T* W[3] = {ring[2][slot_of(k - 1)], ring[2][slot_of(k)], ring[2][slot_of(k + 1)]};
@@ -310,7 +330,8 @@ static void momentum_dispatch(Ctx& c, const MomParams& m, int rhs_only) {
                        (const double*)c.dscal, as<T>(c.unew), as<T>(c.vnew), as<T>(c.wnew), as<T>(c.miu),          \
                        as<T>(c.rho), as<T>(c.duold), as<T>(c.dvold), as<T>(c.dwold), (const double*)c.pnew,        \
                        (const double*)c.pold, as<T>(c.p_x), as<T>(c.p_y), as<T>(c.p_z), as<T>(c.p_xold),          \
-                       as<T>(c.p_yold), as<T>(c.p_zold), as<T>(c.dudt), as<T>(c.dvdt), as<T>(c.dwdt))
+                       as<T>(c.p_yold), as<T>(c.p_zold), as<T>(c.dudt), as<T>(c.dvdt), as<T>(c.dwdt),        \
+                       (const uint8_t*)c.jmask, (const uint8_t*)c.jmask_old, as<T>(c.zplane))
     if (c.g.pow2) {
         if (rhs_only) MOM(1, true); else MOM(0, true);
     } else {
@@ -415,7 +436,8 @@ template <class T>
 __global__ void k_correc(Grid g, T dt, T rho0, const double* __restrict__ p, const T* __restrict__ px,
                          const T* __restrict__ py, const T* __restrict__ pz, const T* __restrict__ du,
                          const T* __restrict__ dv, const T* __restrict__ dw, T* __restrict__ u, T* __restrict__ v,
-                         T* __restrict__ w, double* __restrict__ part) {
+                         T* __restrict__ w, double* __restrict__ part, const uint8_t* __restrict__ jm,
+                         const T* __restrict__ zp) {
     __shared__ double smem[16];
     const int i0 = blockIdx.x * BX + threadIdx.x, j0 = blockIdx.y * BY + threadIdx.y;
     const bool valid = i0 < g.nx && j0 < g.ny;
@@ -428,11 +450,15 @@ __global__ void k_correc(Grid g, T dt, T rho0, const double* __restrict__ p, con
             if (k > g.nz) break;
             const long q = gidx(g, i0, j0, k);
             const double pq = p[q];
-            u[q] = du[q] - dt * T(g.dxi) * T(p[gidx(g, ip, j0, k)] - pq - (double)px[q]) / rho0;
-            const T vq = dv[q] - dt * T(g.dyi) * T(p[gidx(g, i0, jp, k)] - pq - (double)py[q]) / rho0;
+            // face jumps only where the GFM mask says they may be nonzero
+            const unsigned b = jm[q];
+            const T* z0 = zp + (i0 + g.sy * (long)j0);
+            const T jx = *((b & 1u) ? px + q : z0), jy = *((b & 2u) ? py + q : z0), jz = *((b & 4u) ? pz + q : z0);
+            u[q] = du[q] - dt * T(g.dxi) * T(p[gidx(g, ip, j0, k)] - pq - (double)jx) / rho0;
+            const T vq = dv[q] - dt * T(g.dyi) * T(p[gidx(g, i0, jp, k)] - pq - (double)jy) / rho0;
             v[q] = vq;
             vs += (double)vq;
-            w[q] = dw[q] - dt * T(g.dzi) * T(p[gidx(g, i0, j0, k + 1)] - pq - (double)pz[q]) / rho0;
+            w[q] = dw[q] - dt * T(g.dzi) * T(p[gidx(g, i0, j0, k + 1)] - pq - (double)jz) / rho0;
         }
     }
     const double r = block_reduce(vs, OpSum(), smem);
@@ -448,12 +474,13 @@ int launch_correc(Ctx& c) {
             hipLaunchKernelGGL(k_correc<float>, grd, blk, 0, c.stream, c.g, (float)c.dt, (float)c.rho0,
                                (const double*)c.p, as<float>(c.p_x), as<float>(c.p_y), as<float>(c.p_z),
                                as<float>(c.dudt), as<float>(c.dvdt), as<float>(c.dwdt), as<float>(c.unew),
-                               as<float>(c.vnew), as<float>(c.wnew), c.dpart);
+                               as<float>(c.vnew), as<float>(c.wnew), c.dpart, (const uint8_t*)c.jmask,
+                               as<float>(c.zplane));
         else
             hipLaunchKernelGGL(k_correc<double>, grd, blk, 0, c.stream, c.g, c.dt, c.rho0, (const double*)c.p,
                                (const double*)c.p_x, (const double*)c.p_y, (const double*)c.p_z,
                                (const double*)c.dudt, (const double*)c.dvdt, (const double*)c.dwdt, c.unew, c.vnew,
-                               c.wnew, c.dpart);
+                               c.wnew, c.dpart, (const uint8_t*)c.jmask, (const double*)c.zplane);
         HIP_OK(hipGetLastError());
         const int nb = (int)(grd.x * grd.y * grd.z);
         hipLaunchKernelGGL(k_reduce_final<OpSum>, dim3(1), dim3(RED_FINAL_THREADS), 0, c.stream, (const double*)c.dpart, nb, 0.,

@@ -10,6 +10,9 @@ partitioned one does not fit) is not an input or output and is not counted.
 
 Arrays are of three classes: "T" field arrays (8 bytes in the FP64 mode, 4 in the
 mixed modes), "P" pressure arrays (8 bytes in every mode) and "B" byte masks.
+The face-jump fields p_x .. p_zold are read and written only where their byte mask
+says they may be nonzero (a few cells around the interface); by the rule above they
+still count every cell, so GFM, momentum and correc move fewer bytes than counted.
 """
 
 HBM_PEAK_GBS = 8000.0       # MI355X HBM3E, vendor figure
@@ -25,9 +28,9 @@ KERNEL_ARRAYS = {
     "k_materials": (3, 0, 0),          # R phi; W miu, rho
     "k_curvature": (2, 0, 0),          # R phi; W kappa
     "k_recurvature": (3, 0, 0),        # R phi, kappa; W kappa'
-    "k_gfm": (9, 0, 0),                # R phi, kappa', u*, v*, w*; W rhs, p_x, p_y, p_z
-    "k_momentum": (20, 2, 0),          # R u,v,w,miu,rho,duold x3,p_xyz,p_xyz_old; R pnew,pold; W dudt x3, duold x3
-    "k_correc": (9, 1, 0),             # R u* x3, p_x, p_y, p_z; R p; W u, v, w
+    "k_gfm": (9, 0, 1),                # R phi, kappa', u*, v*, w*; W rhs, p_x, p_y, p_z, jump mask
+    "k_momentum": (20, 2, 2),          # R u,v,w,miu,rho,duold x3,p_xyz,p_xyz_old; R pnew,pold; W dudt x3, duold x3; R 2 jump masks
+    "k_correc": (9, 1, 1),             # R u* x3, p_x, p_y, p_z; R p; W u, v, w; R jump mask
     "k_pressure_rotate": (0, 4, 0),    # R p, pnew; W pnew, pold
     "k_avg_vel": (6, 0, 0),            # R u, v, w; W ucc, vcc, wcc
     "k_residual": (2, 1, 0),           # R rhs, p; W r (mixed modes only)
