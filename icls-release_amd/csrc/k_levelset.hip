@@ -407,9 +407,14 @@ int launch_adv_stage(Ctx& c, int scheme, int stage, const double* phi0, const do
 #ifndef ICLS_REINIT_PD
 #define ICLS_REINIT_PD 4
 #endif
+#ifndef ICLS_REINIT_RR
+#define ICLS_REINIT_RR 1
+#endif
 constexpr int RZ = ICLS_REINIT_RZ;                        // layers per block
 constexpr int RH = 3;                                    // stencil radius
-constexpr int RTW = BX + 2 * RH, RTH = BY + 2 * RH, RTP = RTW * RTH;
+constexpr int RR = ICLS_REINIT_RR;                       // rows per thread
+constexpr int RTY = BY * RR;                             // tile rows
+constexpr int RTW = BX + 2 * RH, RTH = RTY + 2 * RH, RTP = RTW * RTH;
 constexpr int RNLD = (RTP + BX * BY - 1) / (BX * BY);
 constexpr int RNS = 2 * RH + 1;                           // planes in the ring
 constexpr int RPD_MAX = ICLS_REINIT_PD;                   // planes in flight ahead of the ring
@@ -432,14 +437,14 @@ template <class T, int STAGE, bool P2, int RPD, bool WALL>
 __global__ void __launch_bounds__(BX * BY, ICLS_REINIT_WPE)
 k_reinit_zm(Grid g, T NB2, double* res, const T* phi0, const T* __restrict__ phis, T* out, T* __restrict__ stash) {
     __shared__ T ring[RNS][RTP];
-    __shared__ T p0s[BX * BY];
+    __shared__ T p0s[RR][BX * BY];
     __shared__ int wok[RNS][BX * BY / 64];   // per plane and wavefront: all values equal wv
     __shared__ T wv[RNS][BX * BY / 64];
     __shared__ double smem[16];
-    const int ntx = (g.nx + BX - 1) / BX, nty = (g.ny + BY - 1) / BY, ntz = (g.nz + RZ - 1) / RZ;
+    const int ntx = (g.nx + BX - 1) / BX, nty = (g.ny + RTY - 1) / RTY, ntz = (g.nz + RZ - 1) / RZ;
     const int tile = xcd_tile(blockIdx.x, ntx * nty * ntz);
     const int bx = tile % ntx, by = (tile / ntx) % nty, bz = tile / (ntx * nty);
-    const int x0 = bx * BX, y0 = by * BY, kb = 1 + bz * RZ;
+    const int x0 = bx * BX, y0 = by * RTY, kb = 1 + bz * RZ;
     const int ke = kb + RZ - 1 < g.nz ? kb + RZ - 1 : g.nz;
     const int tx = threadIdx.x, ty = threadIdx.y, tid = tx + BX * ty;
     if (STAGE == 1 && tile == 0 && tid == 0) res[0] = 0.;
@@ -502,20 +507,28 @@ k_reinit_zm(Grid g, T NB2, double* res, const T* phi0, const T* __restrict__ phi
     __syncthreads();
     for (int k = kb - RH; k < kb + RH; k++) push(k);
 
-    // A thread past the end of x or y (nx < 64 or ny < 4) takes the wrapped cell and
-    // computes and stores the very values of the thread that owns it (its ring
-    // positions and loads wrap the same way); no branch guards the arithmetic and the
-    // stores, which keeps the memory-counter waits of the loop exact.
-    const int i = (x0 + tx) & xm, j = (y0 + ty) & ym;
+    // Thread (tx, ty) computes rows ty, ty + 4, .. (RR of them) of the tile.  A thread
+    // past the end of x or y (nx < 64 or ny < 8) takes the wrapped cell and computes
+    // and stores the very values of the thread that owns it (its ring positions and
+    // loads wrap the same way); no branch guards the arithmetic and the stores, which
+    // keeps the memory-counter waits of the loop exact.
+    const int i = (x0 + tx) & xm;
+    int jr[RR], cr[RR];
+#pragma unroll
+    for (int rr = 0; rr < RR; rr++) {
+        jr[rr] = (y0 + ty + rr * BY) & ym;
+        cr[rr] = (ty + rr * BY + RH) * RTW + tx + RH;
+    }
     const T dz = T(g.dz), dtau = T(0.5) * dz;
     const Sp<T, P2> sx{T(g.dx), T(g.dxi)}, sz{T(g.dz), T(g.dzi)};
-    const int c0 = (ty + RH) * RTW + tx + RH;
     T dmax = T(0.);
 
     // ghost layers (no right-hand side): below the first chunk, above the last; with
     // WALL, layers 0 and k1 get their final values from the wall extrapolation
-    T r_lo = T(0.), r_hi = T(0.);
-    auto ghost = [&](int k) {
+    T r_lo[RR], r_hi[RR];
+#pragma unroll
+    for (int rr = 0; rr < RR; rr++) r_lo[rr] = r_hi[rr] = T(0.);
+    auto ghost = [&](int k, int j) {
         const long q = gidx(g, i, j, k);
         const T p0 = phi0[q];
         const T r = reinit_combine<T, STAGE, P2>(p0, phis[q], T(0.), dtau);
@@ -527,7 +540,9 @@ k_reinit_zm(Grid g, T NB2, double* res, const T* phi0, const T* __restrict__ phi
         out[q] = r;
     };
     if (bz == 0)
-        for (int k = -2; k <= (WALL ? -1 : 0); k++) ghost(k);
+        for (int k = -2; k <= (WALL ? -1 : 0); k++)
+#pragma unroll
+            for (int rr = 0; rr < RR; rr++) ghost(k, jr[rr]);
 
     // The queues: planes of phi_s and the cell's phi0, RPD z steps ahead.  Every queue
     // load is unconditional, its plane clamped to the last one the chunk reads (a
@@ -536,24 +551,29 @@ k_reinit_zm(Grid g, T NB2, double* res, const T* phi0, const T* __restrict__ phi
     // the loop, so that the waits on the loop's entry path are no tighter than on its
     // back edge.  Each step moves its phi0 to the thread's own LDS slot before the
     // queue register is loaded again, for the same reason.
-    T p0q[RPD];
+    T p0q[RPD][RR];
 #pragma unroll
     for (int u = 0; u < RPD; u++) {
         fetch(min(kb + u, ke) + RH, nxt[u]);
-        p0q[u] = phi0[gidx(g, i, j, min(kb + u, ke))];
+#pragma unroll
+        for (int rr = 0; rr < RR; rr++) p0q[u][rr] = phi0[gidx(g, i, jr[rr], min(kb + u, ke))];
     }
     for (int k0 = kb; k0 <= ke; k0 += RPD) {
 #pragma unroll
         for (int u = 0; u < RPD; u++) {
             const int k = k0 + u;
             stash_plane(k + RH, nxt[u]);
-            p0s[tid] = p0q[u];
+#pragma unroll
+            for (int rr = 0; rr < RR; rr++) p0s[rr][tid] = p0q[u][rr];
             __syncthreads();
             push(k + RH);
             fetch(min(k + RPD, ke) + RH, nxt[u]);
-            p0q[u] = phi0[gidx(g, i, j, min(k + RPD, ke))];
-            const T p0 = p0s[tid];
-            {
+#pragma unroll
+            for (int rr = 0; rr < RR; rr++) p0q[u][rr] = phi0[gidx(g, i, jr[rr], min(k + RPD, ke))];
+#pragma unroll
+            for (int rr = 0; rr < RR; rr++) {
+                const int j = jr[rr], c0 = cr[rr];
+                const T p0 = p0s[rr][tid];
                 const long q = gidx(g, i, j, k);
                 const T* P0 = ring[slot_of(k)];
                 const bool flat = run >= RNS;   // planes k - 3 .. k + 3 are all rval
@@ -609,17 +629,17 @@ k_reinit_zm(Grid g, T NB2, double* res, const T* phi0, const T* __restrict__ phi
                 if (WALL) {
                     // layer 0 from layers 1 and 2, layer k1 from kmax - 1 and kmax (k is
                     // uniform over the block)
-                    if (k == 1) r_lo = r;
+                    if (k == 1) r_lo[rr] = r;
                     else if (k == 2) {
                         // phi0 of layer 0 is read before the store (stage 3 may run in place)
                         const long qw = gidx(g, i, j, 0);
-                        const T lo = T(2.) * r_lo - r;
+                        const T lo = T(2.) * r_lo[rr] - r;
                         if (STAGE == 3) dmax = fmax(dmax, fabs(lo - phi0[qw]));
                         out[qw] = lo;
                     }
-                    if (k == g.nz - 1) r_hi = r;
+                    if (k == g.nz - 1) r_hi[rr] = r;
                     else if (k == g.nz) {
-                        const T hi = T(2.) * r - r_hi;
+                        const T hi = T(2.) * r - r_hi[rr];
                         const long qw = gidx(g, i, j, g.nz + 1);
                         if (STAGE == 3) dmax = fmax(dmax, fabs(hi - phi0[qw]));
                         out[qw] = hi;
@@ -630,7 +650,9 @@ k_reinit_zm(Grid g, T NB2, double* res, const T* phi0, const T* __restrict__ phi
         }
     }
     if (bz == ntz - 1)
-        for (int k = g.nz + (WALL ? 2 : 1); k <= g.nz + 3; k++) ghost(k);
+        for (int k = g.nz + (WALL ? 2 : 1); k <= g.nz + 3; k++)
+#pragma unroll
+            for (int rr = 0; rr < RR; rr++) ghost(k, jr[rr]);
     if (STAGE == 3) {
         double b = block_reduce((double)dmax, OpMax(), smem);
         if (tid == 0) atomic_max_nonneg(res, b);
@@ -640,7 +662,7 @@ k_reinit_zm(Grid g, T NB2, double* res, const T* phi0, const T* __restrict__ phi
 template <class T, int RPD, bool WALL>
 static void reinit_dispatch_pd(Ctx& c, int stage, const double* phi0, const double* phis, double* out) {
     dim3 blk(BX, BY, 1);
-    const int ntiles = ((c.g.nx + BX - 1) / BX) * ((c.g.ny + BY - 1) / BY) * ((c.g.nz + RZ - 1) / RZ);
+    const int ntiles = ((c.g.nx + BX - 1) / BX) * ((c.g.ny + RTY - 1) / RTY) * ((c.g.nz + RZ - 1) / RZ);
     double* res = c.dscal + DS_REINIT_RES;
 #define RS(S, P)                                                                                                  \
     hipLaunchKernelGGL((k_reinit_zm<T, S, P, RPD, WALL>), dim3(ntiles), blk, 0, c.stream, c.g, (T)c.NB2, res,     \
