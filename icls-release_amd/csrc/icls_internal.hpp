@@ -87,6 +87,7 @@ struct Ctx {
     bool fp32 = false;     // field arrays hold float (mixed modes)
     size_t esz = 8;        // bytes per field element
     double NB2 = 0, rho0 = 0, visc = 0, pi = 0, dt = 0, time = 0;
+    double dt_old = 0;                    // dt of the previous step (variable-step AB2)
     hipStream_t stream = nullptr;
     int device = 0;
     bool initialised = false;

@@ -186,6 +186,8 @@ static int check_params(const icls_params* p) {
     if (p->ktot < 8) { set_error("ktot must be >= 8 (got %d)", p->ktot); return ICLS_ERR_PARAM; }
     if (p->lmax < 1 || p->lmax > ICLS_MAX_DROPS) { set_error("lmax out of range (%d)", p->lmax); return ICLS_ERR_PARAM; }
     if (!(p->lx > 0 && p->ly > 0 && p->lz > 0 && p->timestep > 0)) { set_error("domain and dt must be positive"); return ICLS_ERR_PARAM; }
+    if (!(p->cfl >= 0. && p->cfl < 1e3)) { set_error("cfl must be in [0, 1e3) (got %g)", p->cfl); return ICLS_ERR_PARAM; }
+    if (p->cfl > 0. && p->ioutchk < 1) { set_error("an adaptive dt (cfl > 0) needs ioutchk >= 1"); return ICLS_ERR_PARAM; }
     if (p->ls_adv_space != ICLS_ADV_HOUC5 && p->ls_adv_space != ICLS_ADV_WENO5) { set_error("unknown ls_adv_space"); return ICLS_ERR_PARAM; }
     if (p->reinit_step < 1 || p->inflate_step < 1) { set_error("reinit_step and inflate_step must be >= 1"); return ICLS_ERR_PARAM; }
     if (p->re_initialization && p->reinit_iter_max != 1) {
@@ -791,9 +793,8 @@ int32_t icls_chkdiv(int32_t istep, double* divtot, double* divmax, int32_t* ijk)
     return ICLS_OK;
 }
 
-// chkdt (chkdt.f90:11-105) from the device maxima
-int32_t icls_chkdt(double* dtmax) {
-    CTX_OR_FAIL();
+// chkdt's combined speed (chkdt.f90:11-104) from the device maxima; dtmax = 0.5 / speed
+static int chkdt_speed(Ctx& c, double& speed_out) {
     TRY(launch_chkdt_umax(c));
     TRY(pull_scalars(c));
     const Grid& g = c.g;
@@ -814,8 +815,28 @@ int32_t icls_chkdt(double* dtmax) {
     double curv_max = 2. / h_min;
     double speed_surf = sqrt(curv_max / (P.We * c.rho0 * (h_min * h_min)));
     double s1 = speed_conv + speed_visc;
-    double speed = (s1 + sqrt(s1 * s1 + 4. * (speed_grav * speed_grav) + 4. * (speed_surf * speed_surf))) / 2.;
-    if (dtmax) *dtmax = 0.5 / speed;
+    speed_out = (s1 + sqrt(s1 * s1 + 4. * (speed_grav * speed_grav) + 4. * (speed_surf * speed_surf))) / 2.;
+    return ICLS_OK;
+}
+
+int32_t icls_chkdt(double* dtmax) {
+    CTX_OR_FAIL();
+    double speed = 0.;
+    TRY(chkdt_speed(c, speed));
+    if (dtmax) *dtmax = 0.5 / speed;              // chkdt.f90:104
+    return ICLS_OK;
+}
+
+// dt = cfl / speed (an extension: the reference keeps dt = timestep, main.f90:216);
+// after the first setting dt grows at most by DT_GROWTH per update
+static constexpr double DT_GROWTH = 1.2;
+static int adapt_dt(Ctx& c, bool limit_growth) {
+    double speed = 0.;
+    TRY(chkdt_speed(c, speed));
+    double dt = c.P.cfl / speed;
+    if (limit_growth && dt > DT_GROWTH * c.dt) dt = DT_GROWTH * c.dt;
+    if (!(dt > 0.) || !std::isfinite(dt)) { set_error("chkdt gave dt = %g", dt); return ICLS_ERR_NAN; }
+    c.dt = dt;
     return ICLS_OK;
 }
 
@@ -838,6 +859,8 @@ static int init_derived_impl(Ctx& c) {
     TRY(launch_bc_p(c, c.pnew));
     TRY(launch_avg_vel(c));                                // init.f90:79
     TRY(launch_materials(c));                              // init.f90:85
+    if (c.P.cfl > 0.) TRY(adapt_dt(c, false));             // needs rho_avr
+    c.dt_old = c.dt;
     // init_volume = volume (init.f90:86)
     HIP_OK(hipMemcpyAsync(c.dscal + DS_COUNT_FIXED + c.lmax, c.dscal + DS_COUNT_FIXED, sizeof(double) * c.lmax,
                           hipMemcpyDeviceToDevice, c.stream));
@@ -986,6 +1009,8 @@ int32_t icls_step(int32_t istep, int32_t begin) {
     // main.f90:190: the cell-centre velocities are averaged from unew/vnew/wnew where
     // they are consumed (the next step's advection), and on download
     TRY(launch_nan_check(c));                              // main.f90:207 (sticky flag)
+    c.dt_old = c.dt;
+    if (c.P.cfl > 0. && istep % c.P.ioutchk == 0) TRY(adapt_dt(c, true));   // main.f90:214
     return ICLS_OK;
 }
 

@@ -17,6 +17,9 @@ static constexpr int KCH = 8;
 
 struct MomParams {
     double dt, visc, iFr, g0, g1, g2, rho0, inv_rho0, rho1, forceytot;
+    // variable-step AB2 and p_hat extrapolation with r = dt / dt_old:
+    // ab_n = 1 + r/2, ab_o = r/2, ex_n = 1 + r, ex_o = r (1.5, 0.5, 2, 1 at r = 1)
+    double ab_n, ab_o, ex_n, ex_o;
     int rho_equal;
     int plazy, pprev, pfirst;   // the lazy pressure rotation (Ctx::p_lazy ..)
 };
@@ -136,7 +139,8 @@ k_momentum_zm(Grid g, MomParams m, const double* __restrict__ dscal, const T* __
     };
     T du0 = T(0.), dv0 = T(0.), dw0 = T(0.);
     double gpx = 0., gpy = 0., gpz = 0.;
-    const T two_ = T(2.);
+    const double exn = m.ex_n, exo = m.ex_o;
+    const T exn_ = T(m.ex_n), exo_ = T(m.ex_o);
     // pnew and pold from the raw buffers of a lazy rotation, with the operations of
     // the rotation itself (main.f90:164-183)
     const double SA = dscal[DS_PN_A], SB = dscal[DS_PN_B];
@@ -150,13 +154,14 @@ k_momentum_zm(Grid g, MomParams m, const double* __restrict__ dscal, const T* __
         if (RHS_ONLY) return;
         du0 = pd[0]; dv0 = pd[1]; dw0 = pd[2];
         // split pressure (ab2.f90:79-101): p_hat = 2 p^n - p^(n-1), p_hat_j = 2 p_j - p_j,old
-        const double phq = 2. * pn_of(pp[0]) - po_of(pp[1]);
-        const double phx = 2. * pn_of(pp[2]) - po_of(pp[3]);
-        const double phy = 2. * pn_of(pp[4]) - po_of(pp[5]);
-        const double phz = 2. * pn_of(pp[6]) - po_of(pp[7]);
-        const T pjx = two_ * pd[3] - pd[4];
-        const T pjy = two_ * pd[5] - pd[6];
-        const T pjz = two_ * pd[7] - pd[8];
+        // (ex_n = 2, ex_o = 1 at a constant dt)
+        const double phq = exn * pn_of(pp[0]) - exo * po_of(pp[1]);
+        const double phx = exn * pn_of(pp[2]) - exo * po_of(pp[3]);
+        const double phy = exn * pn_of(pp[4]) - exo * po_of(pp[5]);
+        const double phz = exn * pn_of(pp[6]) - exo * po_of(pp[7]);
+        const T pjx = exn_ * pd[3] - exo_ * pd[4];
+        const T pjy = exn_ * pd[5] - exo_ * pd[6];
+        const T pjz = exn_ * pd[7] - exo_ * pd[8];
         gpx = phx - (double)pjx - phq;
         gpy = phy - (double)pjy - phq;
         gpz = phz - (double)pjz - phq;
@@ -169,7 +174,7 @@ k_momentum_zm(Grid g, MomParams m, const double* __restrict__ dscal, const T* __
     const Sp<T, P2> sx{T(g.dx), T(g.dxi)}, sy{T(g.dy), T(g.dyi)}, sz{T(g.dz), T(g.dzi)};
     const T qt = T(0.25), two = T(2.), one = T(1.);
     const T rho_avr = T(dscal[DS_RHO_AVR]);
-    const T iFr = T(m.iFr), visc = T(m.visc), dt = T(m.dt), c15 = T(1.5), c05 = T(0.5), irho0 = T(m.inv_rho0);
+    const T iFr = T(m.iFr), visc = T(m.visc), dt = T(m.dt), c15 = T(m.ab_n), c05 = T(m.ab_o), irho0 = T(m.inv_rho0);
     const int c0 = (ty + 1) * TW + tx + 1;   // own cell in the tile plane
 
     for (int k = kb; k <= ke; k++) {
@@ -344,6 +349,9 @@ static int momentum_launch(Ctx& c, int rhs_only) {
     ProfScope ps(c, "k_momentum");
     MomParams m;
     m.dt = c.dt; m.visc = c.visc; m.iFr = c.P.iFr;
+    const double r = c.dt_old > 0. ? c.dt / c.dt_old : 1.;
+    m.ab_n = 1. + r / 2.; m.ab_o = r / 2.;
+    m.ex_n = 1. + r; m.ex_o = r;
     m.g0 = c.P.g_unit[0]; m.g1 = c.P.g_unit[1]; m.g2 = c.P.g_unit[2];
     m.rho0 = c.rho0; m.inv_rho0 = 1. / c.rho0; m.rho1 = c.P.rho1;
     m.forceytot = c.P.constPoi ? -12. * c.visc / (c.P.lz * c.P.lz) : 0.;

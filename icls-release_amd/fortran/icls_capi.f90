@@ -28,7 +28,8 @@ module icls_capi
                            mass_correction, inflate_step, ns_np
      real(c_double)     :: wall_speed
      integer(c_int32_t) :: constPoi, iniu, mode, ioutchk
-     integer(c_int32_t) :: reserved(8)
+     real(c_double)     :: cfl
+     integer(c_int32_t) :: reserved(6)
      real(c_double)     :: drop_xc(ICLS_MAX_DROPS), drop_yc(ICLS_MAX_DROPS), drop_zc(ICLS_MAX_DROPS), &
                            drop_r(ICLS_MAX_DROPS)
   end type icls_params

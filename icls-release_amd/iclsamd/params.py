@@ -12,7 +12,9 @@ src/param.f90:58-60 and src/misc.f90:124,146):
                   mass correction every step, dt = 1e-4.
 * ``bubble256`` -- 256^3 rising bubble in a 1 x 1 x 2 box (dz = 2 dx), R = 0.125 at
                   z = 0.5, density ratio 100:1, viscosity ratio 50:1, gravity -z,
-                  Eotvos ~ 10, WENO5 advection.
+                  Eotvos ~ 10, WENO5 advection, dt = 0.3 / (chkdt's combined speed),
+                  updated every ioutchk = 20 steps (``cfl``; the reference keeps dt
+                  fixed, so this is an extension).
 * ``drops256`` -- 256^3 box with seeded random non-overlapping static drops, ratios 1:1.
 """
 import ctypes
@@ -81,7 +83,8 @@ class IclsParams(ctypes.Structure):
         ("iniu", ctypes.c_int32),
         ("mode", ctypes.c_int32),
         ("ioutchk", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 8),
+        ("cfl", ctypes.c_double),
+        ("reserved", ctypes.c_int32 * 6),
         ("drop_xc", ctypes.c_double * MAX_DROPS),
         ("drop_yc", ctypes.c_double * MAX_DROPS),
         ("drop_zc", ctypes.c_double * MAX_DROPS),
@@ -145,7 +148,8 @@ def bubble256(n=256, mode=MODE_FP64):
     sigma = (p.rho1 - p.rho2) * 1.0 * (2 * 0.125) ** 2 / 10.0
     p.We = 1.0 / sigma
     p.ls_adv_space = ADV_WENO5
-    p.timestep = 2.0e-5
+    p.timestep = 2.0e-5        # unused while cfl > 0
+    p.cfl = 0.3
     p.mode = mode
     p.set_drops([(0.5, 0.5, 0.5, 0.125)])
     return p

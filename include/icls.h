@@ -83,7 +83,15 @@ typedef struct icls_params {
     int32_t iniu;                    /* ICLS_INIU_*                                      */
     int32_t mode;                    /* ICLS_MODE_*                                      */
     int32_t ioutchk;                 /* diagnostics cadence (param.f90:85)               */
-    int32_t reserved[8];
+    double  cfl;                     /* 0: dt = timestep for the whole run, as the reference
+                                        (main.f90:216).  > 0: dt = cfl / speed, with chkdt's
+                                        combined speed (chkdt.f90:62-104), set at init and
+                                        after every ioutchk-th step, growing at most 1.2x per
+                                        update; AB2 and the split-pressure extrapolation then
+                                        take their variable-step coefficients (icls_step).
+                                        The reference's CFL is timestep/dz (param.f90:47) and
+                                        its dt never adapts: this is an extension.          */
+    int32_t reserved[6];
     double  drop_xc[ICLS_MAX_DROPS]; /* drop centres and radii, one level set each       */
     double  drop_yc[ICLS_MAX_DROPS];
     double  drop_zc[ICLS_MAX_DROPS];
@@ -182,7 +190,12 @@ int32_t icls_write_fld(const char* path, int32_t nr);
  * and init_volume to the restarted volume, as the reference does).  *begin receives nr. */
 int32_t icls_init_program_restart(const char* path, int32_t* begin);
 /* One whole time step: main.f90:105-210 in the reference order, with the kernels
- * fused where the arithmetic per element is unchanged. */
+ * fused where the arithmetic per element is unchanged.  AB2 (ab2.f90:55-57) and the
+ * pressure extrapolation p_hat (ab2.f90:79-97) use the variable-step forms with
+ * r = dt / dt_old:  u* = u + dt ((1 + r/2) RU^n - (r/2) RU^(n-1)),
+ * p_hat = (1 + r) p^n - r p^(n-1); at r = 1 these are the reference's 1.5, 0.5, 2, 1
+ * exactly.  With cfl > 0 and istep a multiple of ioutchk, the step ends by setting
+ * the next step's dt from chkdt (one host synchronisation). */
 int32_t icls_step(int32_t istep, int32_t begin);
 
 /* ---- instrumentation ---- */

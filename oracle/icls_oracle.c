@@ -35,7 +35,7 @@ struct oracle_state {
     long nls, nc, ni;
     double lx, ly, lz, dxi, dyi, dzi, dx, dy, dz;
     double NB2, rho0, visc, pi;
-    double dt, time, rho_avr, v_bulk, maxchk_glb, norm_old;
+    double dt, dt_old, time, rho_avr, v_bulk, maxchk_glb, norm_old;
     double reinit_residual;
     /* C-halo arrays (0:i1,0:j1,0:k1) */
     double *unew, *vnew, *wnew, *pnew, *dudt, *dvdt, *dwdt;
@@ -1086,29 +1086,54 @@ int oracle_mom_main(oracle_state* S, double* RHSu, double* RHSv, double* RHSw) {
     return 0;
 }
 
+/* Coefficients of ab2 for a step dt after a step dt_old, r = dt / dt_old:
+ * c[0] RU^n - c[1] RU^(n-1) is the linear extrapolation of RU to t^n + dt/2 and
+ * c[2] p^n - c[3] p^(n-1) that of p to t^n + dt.  At r = 1 they are 1.5, 0.5, 2 and 1,
+ * the constants of ab2.f90:55-57 and 79-97; a variable dt is not in the reference. */
+void oracle_ab2_coeffs(double dt, double dt_old, double c[4]) {
+    const double r = dt_old > 0. ? dt / dt_old : 1.;
+    c[0] = 1. + r / 2.;
+    c[1] = r / 2.;
+    c[2] = 1. + r;
+    c[3] = r;
+}
+
+static double oracle_chkdt_speed(oracle_state* S);
+
+/* dt from chkdt's speed for cfl > 0 (not in the reference, whose dt is fixed,
+ * main.f90:216): dt = cfl / speed = 2 cfl dtmax, growing at most 1.2x per update */
+static void oracle_adapt_dt(oracle_state* S, int limit_growth) {
+    double dt = S->P.cfl / oracle_chkdt_speed(S);
+    if (limit_growth && dt > 1.2 * S->dt) dt = 1.2 * S->dt;
+    S->dt = dt;
+}
+
 /* ab2 with surface_tension_method = 'GFM' (ab2.f90:21-138), momxp/yp/zp (mom.f90:779-851) */
 int oracle_ab2(oracle_state* S) {
     long ni = S->ni, nc = S->nc;
     double *dunew = zalloc(ni), *dvnew = zalloc(ni), *dwnew = zalloc(ni);
     double *phat = zalloc(nc), *pjx = zalloc(nc), *pjy = zalloc(nc), *pjz = zalloc(nc);
     const double dt = S->dt;
+    double cf[4];
+    oracle_ab2_coeffs(dt, S->dt_old, cf);
+    const double abn = cf[0], abo = cf[1], exn = cf[2], exo = cf[3];
     oracle_mom_main(S, dunew, dvnew, dwnew);
     for (int k = 1; k <= S->kmax; k++)
         for (int j = 1; j <= S->jmax; j++)
             for (int i = 1; i <= S->imax; i++) {
                 long c = II(i, j, k), q = CC(i, j, k);
-                S->dudt[q] = S->unew[q] + dt * (1.5 * dunew[c] - 0.5 * S->duold[c]);
-                S->dvdt[q] = S->vnew[q] + dt * (1.5 * dvnew[c] - 0.5 * S->dvold[c]);
-                S->dwdt[q] = S->wnew[q] + dt * (1.5 * dwnew[c] - 0.5 * S->dwold[c]);
+                S->dudt[q] = S->unew[q] + dt * (abn * dunew[c] - abo * S->duold[c]);
+                S->dvdt[q] = S->vnew[q] + dt * (abn * dvnew[c] - abo * S->dvold[c]);
+                S->dwdt[q] = S->wnew[q] + dt * (abn * dwnew[c] - abo * S->dwold[c]);
             }
     memcpy(S->duold, dunew, sizeof(double) * ni);
     memcpy(S->dvold, dvnew, sizeof(double) * ni);
     memcpy(S->dwold, dwnew, sizeof(double) * ni);
     for (long q = 0; q < nc; q++) {
-        pjx[q] = 2. * S->p_x[q] - S->p_xold[q];
-        pjy[q] = 2. * S->p_y[q] - S->p_yold[q];
-        pjz[q] = 2. * S->p_z[q] - S->p_zold[q];
-        phat[q] = 2. * S->pnew[q] - S->pold[q];
+        pjx[q] = exn * S->p_x[q] - exo * S->p_xold[q];
+        pjy[q] = exn * S->p_y[q] - exo * S->p_yold[q];
+        pjz[q] = exn * S->p_z[q] - exo * S->p_zold[q];
+        phat[q] = exn * S->pnew[q] - exo * S->pold[q];
     }
     const double rho0 = S->rho0;
     for (int k = 1; k <= S->kmax; k++)
@@ -1445,8 +1470,8 @@ int oracle_chkdiv(oracle_state* S, int istep, double* divtot_o, double* divmax_o
     return (divmax > 1e-12 && istep != 0) ? 1 : 0;
 }
 
-/* chkdt (chkdt.f90:11-105) */
-double oracle_chkdt(oracle_state* S) {
+/* chkdt's combined speed (chkdt.f90:11-104) */
+static double oracle_chkdt_speed(oracle_state* S) {
     double u_max = 0., v_max = 0., w_max = 0.;
     for (int k = 1; k <= S->k1; k++)
         for (int j = 1; j <= S->j1; j++)
@@ -1473,9 +1498,11 @@ double oracle_chkdt(oracle_state* S) {
     double curv_max = 2. / h_min;
     double speed_surf = sqrt(curv_max / (S->P.We * S->rho0 * (h_min * h_min)));
     double s1 = speed_conv + speed_visc;
-    double speed = (s1 + sqrt(s1 * s1 + 4. * (speed_grav * speed_grav) + 4. * (speed_surf * speed_surf))) / 2.;
-    return 0.5 / speed;
+    return (s1 + sqrt(s1 * s1 + 4. * (speed_grav * speed_grav) + 4. * (speed_surf * speed_surf))) / 2.;
 }
+
+/* chkdt (chkdt.f90:11-105) */
+double oracle_chkdt(oracle_state* S) { return 0.5 / oracle_chkdt_speed(S); }
 
 /* max_vel_at_center (misc.f90:610-636) */
 double oracle_max_vel_at_center(oracle_state* S) {
@@ -1499,6 +1526,8 @@ int oracle_init_derived(oracle_state* S) {
     boundp_arr(S, S->pnew);
     oracle_average_vel_at_center(S);
     oracle_update_materials(S);
+    if (S->P.cfl > 0.) oracle_adapt_dt(S, 0);
+    S->dt_old = S->dt;
     for (int l = 0; l < S->lmax; l++) S->init_volume[l] = S->volume[l];
     oracle_mom_main(S, S->duold, S->dvold, S->dwold);
     memcpy(S->pold, S->pnew, sizeof(double) * S->nc);
@@ -1531,6 +1560,8 @@ int oracle_step(oracle_state* S, int istep, int begin) {
     oracle_update_pressure(S, istep == begin + 1);
     oracle_average_vel_at_center(S);
     if (isnan(S->vnew[CC(1, 1, 1)])) return ICLS_ERR_NAN;
+    S->dt_old = S->dt;
+    if (S->P.cfl > 0. && S->P.ioutchk > 0 && istep % S->P.ioutchk == 0) oracle_adapt_dt(S, 1);
     return 0;
 }
 

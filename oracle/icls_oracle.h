@@ -40,6 +40,7 @@ int    oracle_update_pressure(oracle_state* S, int first_step);
 int    oracle_average_vel_at_center(oracle_state* S);
 int    oracle_chkdiv(oracle_state* S, int istep, double* divtot, double* divmax, int* ijk);
 double oracle_chkdt(oracle_state* S);
+void   oracle_ab2_coeffs(double dt, double dt_old, double c[4]);
 double oracle_max_vel_at_center(oracle_state* S);
 
 /* sub-steps, exposed for unit pins */

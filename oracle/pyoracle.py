@@ -55,6 +55,8 @@ def lib():
         L.oracle_chkdiv.restype = i32
         L.oracle_chkdt.argtypes = [vp]
         L.oracle_chkdt.restype = dbl
+        L.oracle_ab2_coeffs.argtypes = [dbl, dbl, dp]
+        L.oracle_ab2_coeffs.restype = None
         L.oracle_max_vel_at_center.argtypes = [vp]
         L.oracle_max_vel_at_center.restype = dbl
         L.oracle_mom_main.argtypes = [vp, dp, dp, dp]

@@ -100,6 +100,7 @@ def test_mixed_steps_bubble():
     p = P.bubble256(32)
     p.ktot = 64
     p.timestep = 2e-4
+    p.cfl = 0.0
     p.mass_correction = 0
     errs, dv = _run_pair(p, 5)
     print("mixed vs fp64, bubble 32x32x64 x5:", {k: "%.2e" % v for k, v in errs.items()}, "dV %.2e" % dv)

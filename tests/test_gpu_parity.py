@@ -86,6 +86,7 @@ def test_steps_parity_anisotropic_bubble():
     p = P.bubble256(32)
     p.ktot = 64
     p.timestep = 2e-4
+    p.cfl = 0.0
     p.mass_correction = 0
     o, g = _pair(p)
     try:
@@ -95,6 +96,33 @@ def test_steps_parity_anisotropic_bubble():
             o.step(s, 0)
             g.step(s, 0)
         _compare(o, g, FIELDS, TOL, "bubble 5 steps")
+    finally:
+        g.close()
+
+
+def test_steps_parity_adaptive_dt_bubble():
+    """cfl = 0.3 with dt set from chkdt at init and after every second step, so that
+    steps 3 and 5 run the variable-step AB2 and pressure extrapolation (r != 1).
+    dt follows rho_avr, a device reduction, so it agrees to rounding, not bitwise."""
+    p = P.bubble256(32)
+    p.ktot = 64
+    p.mass_correction = 0
+    p.cfl = 0.3
+    p.ioutchk = 2
+    o, g = _pair(p)
+    try:
+        o.init_program()
+        g.init_program()
+        dts = []
+        for s in range(0, 7):
+            if s:
+                o.step(s, 0)
+                g.step(s, 0)
+            dg, do = g.scalar(P.S_DT), o.scalar(P.S_DT)
+            assert abs(dg - do) <= 1e-13 * do, (s, dg, do)
+            dts.append(do)
+        assert dts[0] != p.timestep and len(set(dts)) >= 2
+        _compare(o, g, FIELDS, TOL, "adaptive dt, 6 steps")
     finally:
         g.close()
 

@@ -337,3 +337,79 @@ def test_momentum_polynomial_fields():
     o.set(P.F_WNEW, np.zeros(shape))
     ru, rv, rw = o.mom_main()
     assert np.abs(rv[:, :, 1:-1]).max() < 1e-9
+
+
+# ---------------------------------------------------------------- chkdt and the adaptive dt
+def test_chkdt_viscous_and_convective_limits():
+    """chkdt (chkdt.f90:62-104) in two textbook limits: without gravity, capillarity or
+    flow it is the explicit-diffusion bound 0.5 / (nu (2/dx^2 + 2/dy^2 + 2/dz^2)),
+    dx^2 / (12 nu) on a cubic grid; without viscosity, gravity or capillarity and with
+    a uniform velocity U along x it is the Courant limit 0.5 dx / U."""
+    p = small_drop(16)
+    p.iFr = 0.0
+    p.We = 1e40
+    p.Re = 4.0
+    p.miu1, p.miu2 = 1.0, 3.0
+    p.rho1, p.rho2 = 1.0, 2.0
+    o = O.Oracle(p)
+    o.init_program()
+    dx = 1.0 / 16
+    nu = max(1.0 / 1.0, 3.0 / 2.0) / 4.0
+    assert abs(o.chkdt() - dx * dx / (12 * nu)) <= 1e-12 * dx * dx / (12 * nu)
+    p.Re = 1e40
+    o = O.Oracle(p)
+    o.init_program()
+    shape = P.field_dims(p, P.F_UNEW)
+    o.set(P.F_UNEW, np.full(shape, 2.5))
+    assert abs(o.chkdt() - 0.5 * dx / 2.5) <= 1e-12 * 0.5 * dx / 2.5
+
+
+@pytest.mark.parametrize("r", [1.0, 0.5, 1.2, 0.137])
+def test_ab2_coefficients_extrapolate_linearly(r):
+    """The variable-step AB2 weights are those of the linear extrapolation of the
+    right-hand side to t^n + dt/2, and the split-pressure weights that of p to
+    t^n + dt; at r = 1 they are exactly ab2.f90's 1.5, 0.5, 2 and 1."""
+    rng = np.random.default_rng(7)
+    c = np.zeros(4)
+    dt_old = 3e-3
+    dt = r * dt_old
+    O.lib().oracle_ab2_coeffs(dt, dt_old, c.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    if r == 1.0:
+        assert list(c) == [1.5, 0.5, 2.0, 1.0]
+    tn = 0.4
+    for _ in range(5):
+        a, b = rng.normal(size=2)
+        f = lambda t: a + b * t
+        assert abs(c[0] * f(tn) - c[1] * f(tn - dt_old) - f(tn + dt / 2)) <= 1e-13 * (1 + abs(f(tn)))
+        assert abs(c[2] * f(tn) - c[3] * f(tn - dt_old) - f(tn + dt)) <= 1e-13 * (1 + abs(f(tn)))
+
+
+def test_adaptive_dt_cadence_and_growth_cap():
+    """cfl > 0: dt is cfl / speed at init (with cfl = 0.5 that is chkdt's dtmax), and
+    changes only after steps that are multiples of ioutchk, to min(cfl / speed,
+    1.2 dt); cfl = 0 keeps dt = timestep."""
+    p = P.bubble256(16)
+    p.ktot = 32
+    p.mass_correction = 0
+    p.cfl = 0.5
+    p.ioutchk = 2
+    o = O.Oracle(p)
+    o.init_program()
+    dt = o.scalar(P.S_DT)
+    assert dt == o.chkdt() and dt != p.timestep
+    changed = 0
+    for s in range(1, 7):
+        o.step(s, 0)
+        new = o.scalar(P.S_DT)
+        if s % 2:
+            assert new == dt
+        else:
+            assert new == min(o.chkdt(), 1.2 * dt)
+            changed += new != dt
+        dt = new
+    assert changed >= 1
+    p.cfl = 0.0
+    o = O.Oracle(p)
+    o.init_program()
+    o.run(2)
+    assert o.scalar(P.S_DT) == p.timestep
