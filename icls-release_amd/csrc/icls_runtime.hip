@@ -989,6 +989,149 @@ int32_t icls_init_program_restart(const char* path, int32_t* begin) {
     return init_derived_impl(c);   // sets dt = timestep (init.f90:68)
 }
 
+// ---------------------------------------------------------------- output files
+// The Fortran edit descriptor Ew.dE2 of the reference's writers: 0.d1..dd E+xx,
+// right-justified in w columns (w asterisks when it does not fit).
+static void fort_e(char* out, double v, int w, int d) {
+    char m[64], f[64];
+    if (v == 0.) {
+        snprintf(f, sizeof f, "0.%0*dE+00", d, 0);
+    } else {
+        snprintf(m, sizeof m, "%.*E", d - 1, v);             // [-]D.DDDE[+-]XX
+        const char* s = m;
+        const bool neg = *s == '-';
+        if (neg) s++;
+        const char* e = strchr(s, 'E');
+        const int ex = atoi(e + 1) + 1;
+        char digits[40];
+        int nd = 0;
+        for (const char* t = s; t < e; t++)
+            if (*t != '.') digits[nd++] = *t;
+        digits[nd] = 0;
+        snprintf(f, sizeof f, "%s0.%sE%c%02d", neg ? "-" : "", digits, ex < 0 ? '-' : '+', ex < 0 ? -ex : ex);
+        if ((ex < 0 ? -ex : ex) > 99) f[0] = 0;
+    }
+    const int n = (int)strlen(f);
+    if (n == 0 || n > w) { memset(out, '*', w); out[w] = 0; return; }
+    snprintf(out, w + 1, "%*s", w, f);
+}
+
+// interior (1:itot,1:jtot,1:ktot) of a downloaded field, x fastest
+static int interior_of(Ctx& c, int field, int l, std::vector<double>& in) {
+    std::vector<double> full((size_t)host_count(c, field));
+    TRY(icls_download(field, l, full.data()));
+    in.resize((size_t)c.g.nx * c.g.ny * c.g.nz);
+    copy_interior(c, field, full.data(), in.data(), true);
+    return ICLS_OK;
+}
+
+static bool put_be(FILE* fp, const std::vector<double>& a) {
+    std::vector<uint64_t> b(a.size());
+    for (size_t q = 0; q < a.size(); q++) {
+        uint64_t u;
+        memcpy(&u, &a[q], sizeof u);
+        b[q] = __builtin_bswap64(u);
+    }
+    return fwrite(b.data(), sizeof(uint64_t), b.size(), fp) == b.size();
+}
+
+// vtk (write_vtk.f90:214-432) of the fields globe4vtk gathers (write_vtk.f90:45-210):
+// legacy binary VTK, big-endian FP64 STRUCTURED_POINTS at the cell centres, with
+// U, V, W (uccnew, vccnew, wccnew), abs(Vel), V_r = v - 6 v_bulk_desired z (1 - z)
+// (v_bulk_desired = 1, param.f90:49), Pressure (pnew) and Interface (jiemian).
+int32_t icls_write_vtk(const char* path) {
+    CTX_OR_FAIL();
+    if (!path) { set_error("null path"); return ICLS_ERR_PARAM; }
+    const Grid& g = c.g;
+    const size_t n = (size_t)g.nx * g.ny * g.nz;
+    std::vector<double> u, v, w, p, jam;
+    TRY(interior_of(c, ICLS_F_UCC, 0, u));
+    TRY(interior_of(c, ICLS_F_VCC, 0, v));
+    TRY(interior_of(c, ICLS_F_WCC, 0, w));
+    TRY(interior_of(c, ICLS_F_PNEW, 0, p));
+    TRY(interior_of(c, ICLS_F_JIEMIAN, 0, jam));
+    FILE* fp = fopen(path, "wb");
+    if (!fp) { set_error("cannot open %s for writing", path); return ICLS_ERR_PARAM; }
+    std::string hdr = "# vtk DataFile Version 3.0\ntest file\nBINARY\n\nDATASET STRUCTURED_POINTS\n";
+    char line[160], a[32], b[32], d[32];
+    snprintf(line, sizeof line, "DIMENSIONS%12d%12d%12d\n", g.nx, g.ny, g.nz);
+    hdr += line;
+    fort_e(a, g.dx / 2., 14, 6); fort_e(b, g.dy / 2., 14, 6); fort_e(d, g.dz / 2., 14, 6);
+    hdr += std::string("ORIGIN") + a + b + d + "\n";
+    fort_e(a, g.dx, 14, 6); fort_e(b, g.dy, 14, 6); fort_e(d, g.dz, 14, 6);
+    hdr += std::string("SPACING") + a + b + d + "\n\n";
+    snprintf(line, sizeof line, "POINT_DATA%12lld\n", (long long)n);
+    hdr += line;
+    bool ok = fwrite(hdr.data(), 1, hdr.size(), fp) == hdr.size();
+    auto block = [&](const char* name, const std::vector<double>& x) {
+        char h[160];
+        snprintf(h, sizeof h, "SCALARS %s double%12d\nLOOKUP_TABLE default\n", name, 1);
+        ok = ok && fwrite(h, 1, strlen(h), fp) == strlen(h) && put_be(fp, x);
+    };
+    block("U", u);
+    block("V", v);
+    block("W", w);
+    std::vector<double> t(n);
+    for (size_t q = 0; q < n; q++) t[q] = sqrt(u[q] * u[q] + v[q] * v[q] + w[q] * w[q]);
+    block("abs(Vel)", t);
+    const double v_bulk_desired = 1.;
+    for (int k = 1; k <= g.nz; k++) {
+        const double coorz = (k - 0.5) * g.dz / c.P.lz;
+        const double vu = 6. * v_bulk_desired * coorz * (1. - coorz);
+        for (size_t q = (size_t)(k - 1) * g.nx * g.ny; q < (size_t)k * g.nx * g.ny; q++) t[q] = v[q] - vu;
+    }
+    block("V_r", t);
+    block("Pressure", p);
+    block("Interface", jam);
+    ok = ok && fputc('\n', fp) != EOF;
+    if (fclose(fp) != 0) ok = false;
+    if (!ok) { set_error("short write to %s", path); return ICLS_ERR_PARAM; }
+    return ICLS_OK;
+}
+
+// post2d (output.f90:179-427): the Tecplot ASCII mid-yz plane i = itot/2 with the
+// columns y, z, u (x-centred), v and w (centred in y and z, w = 0 below k = 1),
+// p, the first level set, jiemian and fixhv.  fixhv is the Heaviside of the static
+// level set (init.f90:1578), which this build does not carry: its column is 0.
+// The header lines are list-directed writes, spaced as flang writes them.
+int32_t icls_post2d(const char* path) {
+    CTX_OR_FAIL();
+    if (!path) { set_error("null path"); return ICLS_ERR_PARAM; }
+    const Grid& g = c.g;
+    const int fid[6] = {ICLS_F_UNEW, ICLS_F_VNEW, ICLS_F_WNEW, ICLS_F_PNEW, ICLS_F_LVSET, ICLS_F_JIEMIAN};
+    std::vector<double> full[6];
+    int lb[6], hx[6], hy[6];
+    for (int f = 0; f < 6; f++) {
+        int hz;
+        host_dims(c, shape_of(fid[f]), hx[f], hy[f], hz, lb[f]);
+        full[f].resize((size_t)host_count(c, fid[f]));
+        TRY(icls_download(fid[f], 0, full[f].data()));
+    }
+    auto at = [&](int f, int i, int j, int k) {
+        return full[f][(size_t)(i - lb[f]) + (size_t)hx[f] * ((size_t)(j - lb[f]) + (size_t)hy[f] * (size_t)(k - lb[f]))];
+    };
+    FILE* fp = fopen(path, "w");
+    if (!fp) { set_error("cannot open %s for writing", path); return ICLS_ERR_PARAM; }
+    const int il = g.nx / 2, npoints = 0;
+    bool ok = fprintf(fp, " VARIABLES = \"y\",\"z\",\"u\",\"v\",\"w\",\"p\",\"s\",\"indie\",\"fixhv\"\n") > 0;
+    ok = ok && fprintf(fp, " ZONE T=\"Zone1\" I= %d  J= %d , F=POINT\n", g.ny + 2 * npoints, g.nz) > 0;
+    char e[9][32];
+    for (int k = 1; k <= g.nz && ok; k++) {
+        for (int j = 1; j <= g.ny && ok; j++) {
+            const int jm = j - 1 < 1 ? j - 1 + g.ny : j - 1;
+            const double wkm = k - 1 >= 1 ? at(2, il, j, k - 1) : 0.;
+            const double col[9] = {(j - 0.5) / g.dyi, (k - 0.5) / g.dzi, 0.5 * (at(0, il, j, k) + at(0, il - 1, j, k)),
+                                   0.5 * (at(1, il, j, k) + at(1, il, jm, k)), 0.5 * (at(2, il, j, k) + wkm),
+                                   at(3, il, j, k), at(4, il, j, k), at(5, il, j, k), 0.};
+            for (int q = 0; q < 9; q++) fort_e(e[q], col[q], 15, 7);
+            ok = fprintf(fp, "%s%s%s%s%s%s%s%s%s\n", e[0], e[1], e[2], e[3], e[4], e[5], e[6], e[7], e[8]) > 0;
+        }
+    }
+    if (fclose(fp) != 0) ok = false;
+    if (!ok) { set_error("short write to %s", path); return ICLS_ERR_PARAM; }
+    return ICLS_OK;
+}
+
 // ---------------------------------------------------------------- one time step
 int32_t icls_step(int32_t istep, int32_t begin) {
     CTX_OR_FAIL();
@@ -1009,7 +1152,6 @@ int32_t icls_step(int32_t istep, int32_t begin) {
     // main.f90:190: the cell-centre velocities are averaged from unew/vnew/wnew where
     // they are consumed (the next step's advection), and on download
     TRY(launch_nan_check(c));                              // main.f90:207 (sticky flag)
-    c.dt_old = c.dt;
     if (c.P.cfl > 0. && istep % c.P.ioutchk == 0) TRY(adapt_dt(c, true));   // main.f90:214
     return ICLS_OK;
 }

@@ -352,6 +352,7 @@ static int momentum_launch(Ctx& c, int rhs_only) {
     const double r = c.dt_old > 0. ? c.dt / c.dt_old : 1.;
     m.ab_n = 1. + r / 2.; m.ab_o = r / 2.;
     m.ex_n = 1. + r; m.ex_o = r;
+    if (!rhs_only) c.dt_old = c.dt;      // the next ab2 extrapolates from this step
     m.g0 = c.P.g_unit[0]; m.g1 = c.P.g_unit[1]; m.g2 = c.P.g_unit[2];
     m.rho0 = c.rho0; m.inv_rho0 = 1. / c.rho0; m.rho1 = c.P.rho1;
     m.forceytot = c.P.constPoi ? -12. * c.visc / (c.P.lz * c.P.lz) : 0.;

@@ -64,6 +64,11 @@ module icls_capi
        integer(c_int32_t), value :: which
        real(c_double), intent(out) :: v
      end function
+     integer(c_int32_t) function icls_set_scalar(which, v) bind(c, name='icls_set_scalar')
+       import :: c_int32_t, c_double
+       integer(c_int32_t), value :: which
+       real(c_double), value :: v
+     end function
      integer(c_int32_t) function icls_get_volume(l, v, v0) bind(c, name='icls_get_volume')
        import :: c_int32_t, c_double
        integer(c_int32_t), value :: l
@@ -133,6 +138,19 @@ module icls_capi
      integer(c_int32_t) function icls_step(istep, begin) bind(c, name='icls_step')
        import :: c_int32_t
        integer(c_int32_t), value :: istep, begin
+     end function
+     integer(c_int32_t) function icls_write_fld(path, nr) bind(c, name='icls_write_fld')
+       import :: c_int32_t, c_char
+       character(kind=c_char), dimension(*), intent(in) :: path
+       integer(c_int32_t), value :: nr
+     end function
+     integer(c_int32_t) function icls_write_vtk(path) bind(c, name='icls_write_vtk')
+       import :: c_int32_t, c_char
+       character(kind=c_char), dimension(*), intent(in) :: path
+     end function
+     integer(c_int32_t) function icls_post2d(path) bind(c, name='icls_post2d')
+       import :: c_int32_t, c_char
+       character(kind=c_char), dimension(*), intent(in) :: path
      end function
   end interface
 

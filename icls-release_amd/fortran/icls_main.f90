@@ -3,7 +3,8 @@
 ! namelist /icls/ in the file named as the first command-line argument (the
 ! reference compiles these in, src/param.f90).  Optional: out_file names a raw
 ! stream file receiving unew, vnew, wnew, pnew (0:i1,0:j1,0:k1) and lvset(:,:,:,1)
-! (-2:i1+2,...) in Fortran order after the last step.
+! (-2:i1+2,...) in Fortran order after the last step; iout2d, iout3d and ioutfld
+! (param.f90:86-87) write yz_mid, para .vtk and fld files into data/ (0 = off).
 program ICLS
   use iso_c_binding
   use icls_capi
@@ -20,6 +21,9 @@ program ICLS
   use mod_chkdiv
   use mod_chkdt
   use mod_icls_main_ops
+  use mod_write_vtk
+  use mod_output
+  use mod_loadd
   implicit none
 
   integer :: itot = 64, jtot = 64, ktot = 64, nstep = 100, begin = 0, ioutchk = 20, ndrops = 1
@@ -29,11 +33,12 @@ program ICLS
   integer :: mass_correction = 1, inflate_step = 1, ns_np = 1, constPoi = 0, iniu = 0, mode = 0
   real :: drop_x(ICLS_MAX_DROPS) = 0.5, drop_y(ICLS_MAX_DROPS) = 0.5, drop_z(ICLS_MAX_DROPS) = 0.5
   real :: drop_r(ICLS_MAX_DROPS) = 0.2
+  integer :: iout2d = 0, iout3d = 0, ioutfld = 0
   character(len=256) :: out_file = ''
   namelist /icls/ itot, jtot, ktot, nstep, begin, ioutchk, lx, ly, lz, timestep, rho1, rho2, miu1, miu2, &
        Re, We, iFr, g_unit, wall_speed, ls_adv_space, re_initialization, reinit_step, reinit_iter_max, &
        mass_correction, inflate_step, ns_np, constPoi, iniu, mode, ndrops, drop_x, drop_y, drop_z, drop_r, &
-       out_file
+       iout2d, iout3d, ioutfld, out_file
 
   type(icls_params) :: p0
   integer :: istep, u, l
@@ -76,6 +81,7 @@ program ICLS
   !---- main loop (main.f90:103-291)
   do istep = begin + 1, nstep
      time = time + timestep
+     call icls_check(icls_set_scalar(S_TIME, time), 'time')   ! common time (main.f90:107)
 
      call ls_main(istep)
 
@@ -109,6 +115,17 @@ program ICLS
         end do
         write(6, '(A,I8,A,1ES12.4,A,1ES12.4,A,1ES12.4)') '! istep', istep, '  time =', time, &
              '  massloss =', massloss, '  max |u_c| =', vel_max
+     end if
+
+     !---- output files (main.f90:262-277); 0 switches a kind off
+     if (iout2d > 0) then
+        if (mod(istep, iout2d) == 0) call post2d(istep)
+     end if
+     if (iout3d > 0) then
+        if (mod(istep, iout3d) == 0) call write_vtk(istep)
+     end if
+     if (ioutfld > 0) then
+        if (mod(istep, ioutfld) == 0) call loadflds(1, istep)
      end if
   end do
 

@@ -262,3 +262,53 @@ contains
     call icls_check(rc, 'check_nan')
   end subroutine check_nan
 end module mod_icls_main_ops
+
+! write_vtk (src/write_vtk.f90:10-39): datadir//'para'//istep(i7.7)//'.vtk'
+module mod_write_vtk
+  use mod_icls_state
+  implicit none
+  private
+  public write_vtk
+contains
+  subroutine write_vtk(istep)
+    integer, intent(in) :: istep
+    character(len=7) :: istepchar
+    write(istepchar, '(i7.7)') istep
+    call icls_check(icls_write_vtk('data/para'//istepchar//'.vtk'//c_null_char), 'write_vtk')
+  end subroutine write_vtk
+end module mod_write_vtk
+
+! post2d (src/output.f90:179-427): datadir//'yz_mid'//istep(i7.7)//'.dat'
+module mod_output
+  use mod_icls_state
+  implicit none
+  private
+  public post2d
+contains
+  subroutine post2d(istep)
+    integer, intent(in) :: istep
+    character(len=7) :: filenumber
+    write(filenumber, '(i7.7)') istep
+    call icls_check(icls_post2d('data/yz_mid'//filenumber//'.dat'//c_null_char), 'post2d')
+  end subroutine post2d
+end module mod_output
+
+! loadflds (src/loadd.f90:16-108), writing direction (in = 1): datadir//'fld'//nr(i7.7).
+! Reading back is init_program's restart branch (icls_init_program_restart).
+module mod_loadd
+  use mod_icls_state
+  implicit none
+  private
+  public loadflds
+contains
+  subroutine loadflds(in, nr)
+    integer, intent(in) :: in, nr
+    character(len=7) :: istepchar
+    if (in /= 1) then
+       write(6, *) 'loadflds: reading is done by init_program (begin > 0)'
+       error stop 2
+    end if
+    write(istepchar, '(i7.7)') nr
+    call icls_check(icls_write_fld('data/fld'//istepchar//c_null_char, int(nr, c_int32_t)), 'loadflds')
+  end subroutine loadflds
+end module mod_loadd

@@ -25,7 +25,7 @@ EXPORTS = [
     "icls_sync", "icls_ls_main", "icls_average_stream_vel", "icls_ab2", "icls_fillps", "icls_gfm4p",
     "icls_add_pjump", "icls_solver2d", "icls_boundp", "icls_correc", "icls_bounduvw", "icls_update_pressure",
     "icls_average_vel_at_center", "icls_check_nan", "icls_chkdiv", "icls_chkdt", "icls_max_vel_at_center",
-    "icls_ipot", "icls_write_fld", "icls_init_program_restart",
+    "icls_ipot", "icls_write_fld", "icls_init_program_restart", "icls_write_vtk", "icls_post2d",
     "icls_step", "icls_profile_enable", "icls_profile_only", "icls_profile_reset", "icls_kernel_times",
     "icls_mlp_defaults", "icls_mlp_train", "icls_mlp_set_weights", "icls_mlp_get_weights", "icls_mlp_set_gate",
     "icls_mlp_forward", "icls_mlp_gradient", "icls_mlp_samples", "icls_mlp_listed",
@@ -81,6 +81,8 @@ def load():
     L.icls_ipot.argtypes = [dp]
     L.icls_write_fld.argtypes = [ctypes.c_char_p, i32]
     L.icls_init_program_restart.argtypes = [ctypes.c_char_p, ctypes.POINTER(i32)]
+    L.icls_write_vtk.argtypes = [ctypes.c_char_p]
+    L.icls_post2d.argtypes = [ctypes.c_char_p]
     L.icls_step.argtypes = [i32, i32]
     L.icls_profile_only.argtypes = [ctypes.c_char_p]
     L.icls_kernel_times.argtypes = [i32, ctypes.c_char_p, dp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(i32)]
@@ -257,6 +259,13 @@ class Solver:
         b = ctypes.c_int32()
         self._call("icls_init_program_restart", os.fsencode(path), ctypes.byref(b))
         return b.value
+
+    # --- output files (src/write_vtk.f90, src/output.f90)
+    def write_vtk(self, path):
+        self._call("icls_write_vtk", os.fsencode(path))
+
+    def post2d(self, path):
+        self._call("icls_post2d", os.fsencode(path))
 
     # --- learned curvature closure
     def mlp_train(self, **kw):

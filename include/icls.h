@@ -189,10 +189,23 @@ int32_t icls_write_fld(const char* path, int32_t nr);
  * the file, then the derived state as icls_init_derived (which resets dt to timestep
  * and init_volume to the restarted volume, as the reference does).  *begin receives nr. */
 int32_t icls_init_program_restart(const char* path, int32_t* begin);
+
+/* ---- output files ----
+ * write_vtk / vtk (src/write_vtk.f90:10-39, 214-432): legacy binary VTK, big-endian FP64
+ * STRUCTURED_POINTS at the cell centres (DIMENSIONS itot jtot ktot, ORIGIN dx/2.., SPACING
+ * dx..) with the scalars U, V, W (cell-centre velocities), abs(Vel), V_r (v minus the
+ * laminar profile 6 z (1 - z)), Pressure (pnew) and Interface (jiemian).  The reference
+ * names the file data/para<istep:7>.vtk; the Fortran wrapper write_vtk(istep) does too. */
+int32_t icls_write_vtk(const char* path);
+/* post2d (src/output.f90:179-427): Tecplot ASCII of the mid-yz plane i = itot/2, nine
+ * columns (9E15.7): y, z, u, v, w, p, level set 1, jiemian, fixhv (0: no static level set).
+ * The reference names the file data/yz_mid<istep:7>.dat. */
+int32_t icls_post2d(const char* path);
 /* One whole time step: main.f90:105-210 in the reference order, with the kernels
  * fused where the arithmetic per element is unchanged.  AB2 (ab2.f90:55-57) and the
  * pressure extrapolation p_hat (ab2.f90:79-97) use the variable-step forms with
- * r = dt / dt_old:  u* = u + dt ((1 + r/2) RU^n - (r/2) RU^(n-1)),
+ * r = dt / dt_old, dt_old the dt of the previous ab2 (icls_ab2 or icls_step; at
+ * init, dt itself):  u* = u + dt ((1 + r/2) RU^n - (r/2) RU^(n-1)),
  * p_hat = (1 + r) p^n - r p^(n-1); at r = 1 these are the reference's 1.5, 0.5, 2, 1
  * exactly.  With cfl > 0 and istep a multiple of ioutchk, the step ends by setting
  * the next step's dt from chkdt (one host synchronisation). */

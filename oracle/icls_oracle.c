@@ -1117,6 +1117,7 @@ int oracle_ab2(oracle_state* S) {
     double cf[4];
     oracle_ab2_coeffs(dt, S->dt_old, cf);
     const double abn = cf[0], abo = cf[1], exn = cf[2], exo = cf[3];
+    S->dt_old = dt;
     oracle_mom_main(S, dunew, dvnew, dwnew);
     for (int k = 1; k <= S->kmax; k++)
         for (int j = 1; j <= S->jmax; j++)
@@ -1560,7 +1561,6 @@ int oracle_step(oracle_state* S, int istep, int begin) {
     oracle_update_pressure(S, istep == begin + 1);
     oracle_average_vel_at_center(S);
     if (isnan(S->vnew[CC(1, 1, 1)])) return ICLS_ERR_NAN;
-    S->dt_old = S->dt;
     if (S->P.cfl > 0. && S->P.ioutchk > 0 && istep % S->P.ioutchk == 0) oracle_adapt_dt(S, 1);
     return 0;
 }
