@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 #include <cstdio>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -90,6 +91,8 @@ struct Ctx {
     double dt_old = 0;                    // dt of the previous step (variable-step AB2)
     hipStream_t stream = nullptr;
     int device = 0;
+    int ncu = 0;                          // compute units of the device
+    std::map<const void*, int> zchunk;    // z-chunk length per z-marching kernel
     bool initialised = false;
 
     // field class (T)
@@ -165,6 +168,12 @@ inline const T* as(const double* p) { return reinterpret_cast<const T*>(p); }
 
 // ---- error handling
 void set_error(const char* fmt, ...);
+// Layers per block for a z-marching kernel `fn` launched with `threads` threads per block
+// over tiles_xy (x, y) tiles: a multiple of `mult` chosen so that the launch comes as
+// close as possible to whole rounds of resident blocks (occupancy x CUs at once; a
+// partly empty last round costs a full one), each block costing its layers plus `halo`
+// prologue layers.  Cached per kernel.
+int zchunk_for(Ctx& c, const void* fn, int threads, int tiles_xy, int mult, int halo);
 #define HIP_OK(call)                                                                    \
     do {                                                                                \
         hipError_t e_ = (call);                                                         \

@@ -209,6 +209,7 @@ static int build_ctx(const icls_params* prm) {
     Ctx& C = *c;
     C.P = *prm;
     HIP_OK(hipGetDevice(&C.device));
+    HIP_OK(hipDeviceGetAttribute(&C.ncu, hipDeviceAttributeMultiprocessorCount, C.device));
     HIP_OK(hipStreamCreateWithFlags(&C.stream, hipStreamNonBlocking));
     Grid& g = C.g;
     g.nx = prm->itot; g.ny = prm->jtot; g.nz = prm->ktot;
@@ -307,6 +308,29 @@ static int build_ctx(const icls_params* prm) {
     TRY(mlp_alloc(C));
     HIP_OK(hipDeviceSynchronize());
     return ICLS_OK;
+}
+
+int zchunk_for(Ctx& c, const void* fn, int threads, int tiles_xy, int mult, int halo) {
+    auto it = c.zchunk.find(fn);
+    if (it != c.zchunk.end()) return it->second;
+    const int nz = c.g.nz;
+    int occ = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, threads, 0) != hipSuccess || occ < 1) occ = 1;
+    const long slots = (long)occ * (c.ncu > 0 ? c.ncu : 1);
+    if (mult < 1 || nz % mult != 0) mult = 1;
+    int best = nz;
+    long best_cost = -1;
+    for (int ntz = 1; ntz <= nz; ntz++) {
+        int ch = (nz + ntz - 1) / ntz;
+        ch = (ch + mult - 1) / mult * mult;
+        if ((nz + ch - 1) / ch != ntz) continue;           // each chunk count once
+        if (ch < (nz < 16 ? nz : 16)) break;               // chunks of 16 layers at least
+        const long blocks = (long)tiles_xy * ntz;
+        const long cost = (blocks + slots - 1) / slots * (ch + halo);
+        if (best_cost < 0 || cost < best_cost) { best_cost = cost; best = ch; }
+    }
+    c.zchunk[fn] = best;
+    return best;
 }
 
 static Ctx* need_ctx() {

@@ -399,18 +399,15 @@ int launch_adv_stage(Ctx& c, int scheme, int stage, const double* phi0, const do
 // clamped far field, most of the domain) every stencil is flat, all six WENO5
 // derivatives are +0, and RHS = s exactly wherever |v| < NarrowBand_2 + dz; the step
 // then skips the stencils.
-// Interior layers are split into chunks of RZ; the first and the last chunk also
-// write the three ghost layers below and above.  Tiles are taken in XCD order.
-#ifndef ICLS_REINIT_RZ
-#define ICLS_REINIT_RZ 32
-#endif
+// Interior layers are split into chunks of rz (zchunk_for); the first and the last
+// chunk also write the three ghost layers below and above.  Tiles are taken in XCD
+// order.
 #ifndef ICLS_REINIT_PD
 #define ICLS_REINIT_PD 4
 #endif
 #ifndef ICLS_REINIT_RR
 #define ICLS_REINIT_RR 1
 #endif
-constexpr int RZ = ICLS_REINIT_RZ;                        // layers per block
 constexpr int RH = 3;                                    // stencil radius
 constexpr int RR = ICLS_REINIT_RR;                       // rows per thread
 constexpr int RTY = BY * RR;                             // tile rows
@@ -435,7 +432,8 @@ __device__ __forceinline__ T reinit_combine(T p0, T ps, T rhs, T dtau) {
 #endif
 template <class T, int STAGE, bool P2, int RPD, bool WALL>
 __global__ void __launch_bounds__(BX * BY, ICLS_REINIT_WPE)
-k_reinit_zm(Grid g, T NB2, double* res, const T* phi0, const T* __restrict__ phis, T* out, T* __restrict__ stash) {
+k_reinit_zm(Grid g, int RZ, T NB2, double* res, const T* phi0, const T* __restrict__ phis, T* out,
+            T* __restrict__ stash) {
     __shared__ T ring[RNS][RTP];
     __shared__ T p0s[RR][BX * BY];
     __shared__ int wok[RNS][BX * BY / 64];   // per plane and wavefront: all values equal wv
@@ -660,12 +658,12 @@ k_reinit_zm(Grid g, T NB2, double* res, const T* phi0, const T* __restrict__ phi
 }
 
 template <class T, int RPD, bool WALL>
-static void reinit_dispatch_pd(Ctx& c, int stage, const double* phi0, const double* phis, double* out) {
+static void reinit_dispatch_pd(Ctx& c, int rz, int stage, const double* phi0, const double* phis, double* out) {
     dim3 blk(BX, BY, 1);
-    const int ntiles = ((c.g.nx + BX - 1) / BX) * ((c.g.ny + RTY - 1) / RTY) * ((c.g.nz + RZ - 1) / RZ);
+    const int ntiles = ((c.g.nx + BX - 1) / BX) * ((c.g.ny + RTY - 1) / RTY) * ((c.g.nz + rz - 1) / rz);
     double* res = c.dscal + DS_REINIT_RES;
 #define RS(S, P)                                                                                                  \
-    hipLaunchKernelGGL((k_reinit_zm<T, S, P, RPD, WALL>), dim3(ntiles), blk, 0, c.stream, c.g, (T)c.NB2, res,     \
+    hipLaunchKernelGGL((k_reinit_zm<T, S, P, RPD, WALL>), dim3(ntiles), blk, 0, c.stream, c.g, rz, (T)c.NB2, res, \
                        as<T>(phi0), as<T>(phis), as<T>(out), as<T>(c.stash))
     if (c.g.pow2) {
         if (stage == 1) RS(1, true); else if (stage == 2) RS(2, true); else RS(3, true);
@@ -675,20 +673,34 @@ static void reinit_dispatch_pd(Ctx& c, int stage, const double* phi0, const doub
 #undef RS
 }
 
+// Layers per block of the re-init stages: 32.  At 256^3 (MI355X) chunks of 8, 12, 16,
+// 24 and 32 layers ran stages 1 / 2-3 in 0.122 / 0.138, 0.130 / 0.140, 0.114 / 0.121,
+// 0.139 / 0.151 and 0.098 / 0.105 ms (FP32), and chunks filling whole rounds of
+// resident blocks (52 layers) in 0.131 / 0.137 ms: far-field blocks end early (the
+// uniform-window path), so block count matters more than rounds, and 8 chunks put one
+// z slab of tiles on each XCD.
+constexpr int REINIT_RZ = 32;
+template <class T>
+static int reinit_rz(Ctx& c) { return c.g.nz < REINIT_RZ ? c.g.nz : REINIT_RZ; }
+
 // the wall extrapolation folds into the stage when the last chunk has two layers or
 // more (a chunk of one layer would need layer kmax - 1 from another block)
-bool reinit_fuses_wall(const Grid& g) { return g.nz >= 2 && (g.nz % RZ == 0 || g.nz % RZ >= 2); }
+static bool fuses_wall(const Grid& g, int rz) { return g.nz >= 2 && (g.nz % rz == 0 || g.nz % rz >= 2); }
+static bool reinit_fuses_wall(Ctx& c) {
+    return fuses_wall(c.g, c.fp32 ? reinit_rz<float>(c) : reinit_rz<double>(c));
+}
 
 template <class T>
 static void reinit_dispatch(Ctx& c, int stage, const double* phi0, const double* phis, double* out) {
-    // every chunk (RZ layers, the last one nz % RZ) a multiple of RPD_MAX
-    const bool deep = RZ % RPD_MAX == 0 && (c.g.nz % RZ) % RPD_MAX == 0;
-    if (reinit_fuses_wall(c.g)) {
-        if (deep) reinit_dispatch_pd<T, RPD_MAX, true>(c, stage, phi0, phis, out);
-        else reinit_dispatch_pd<T, 1, true>(c, stage, phi0, phis, out);
+    const int rz = reinit_rz<T>(c);
+    // every chunk (rz layers, the last one nz % rz) a multiple of RPD_MAX
+    const bool deep = rz % RPD_MAX == 0 && (c.g.nz % rz) % RPD_MAX == 0;
+    if (fuses_wall(c.g, rz)) {
+        if (deep) reinit_dispatch_pd<T, RPD_MAX, true>(c, rz, stage, phi0, phis, out);
+        else reinit_dispatch_pd<T, 1, true>(c, rz, stage, phi0, phis, out);
     } else {
-        if (deep) reinit_dispatch_pd<T, RPD_MAX, false>(c, stage, phi0, phis, out);
-        else reinit_dispatch_pd<T, 1, false>(c, stage, phi0, phis, out);
+        if (deep) reinit_dispatch_pd<T, RPD_MAX, false>(c, rz, stage, phi0, phis, out);
+        else reinit_dispatch_pd<T, 1, false>(c, rz, stage, phi0, phis, out);
     }
 }
 
@@ -701,7 +713,7 @@ int launch_reinit_stage(Ctx& c, int stage, const double* phi0, const double* phi
         else reinit_dispatch<double>(c, stage, phi0, phis, out);
         HIP_OK(hipGetLastError());
     }
-    if (reinit_fuses_wall(c.g)) return ICLS_OK;
+    if (reinit_fuses_wall(c)) return ICLS_OK;
     if (stage == 3) return bc_ls(c, out, c.stash, c.dscal + DS_REINIT_RES, nullptr);
     return bc_ls(c, out, nullptr, nullptr, nullptr);
 }

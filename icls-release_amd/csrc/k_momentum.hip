@@ -21,6 +21,7 @@ struct MomParams {
     // ab_n = 1 + r/2, ab_o = r/2, ex_n = 1 + r, ex_o = r (1.5, 0.5, 2, 1 at r = 1)
     double ab_n, ab_o, ex_n, ex_o;
     int rho_equal;
+    int mz;                     // layers per block (zchunk_for)
     int plazy, pprev, pfirst;   // the lazy pressure rotation (Ctx::p_lazy ..)
 };
 
@@ -29,7 +30,7 @@ struct MomParams {
 // (mom.f90:779-851).  Face densities are the arithmetic means of update_materials
 // (misc.f90:151-172), formed here from rho.
 //
-// The kernel marches in z.  A block owns a 64 x 4 tile of (i,j) columns over MZ
+// The kernel marches in z.  A block owns a 64 x 4 tile of (i,j) columns over m.mz
 // consecutive layers.  u, v, w, miu and rho -- the arrays read at neighbouring cells
 // -- pass through a ring of four LDS planes of the tile with a one-cell x/y halo, so
 // every element of them is fetched once per block.  Step k reads planes k-1 .. k+1;
@@ -38,10 +39,6 @@ struct MomParams {
 // stores plane k+2 into the fourth slot (that of plane k-2, which no thread reads any
 // more) before the one barrier of the step.  Tiles are taken in XCD order
 // (xcd_tile).  x and y wrap with a mask: itot and jtot are powers of two.
-#ifndef ICLS_MOM_MZ
-#define ICLS_MOM_MZ 32
-#endif
-constexpr int MZ = ICLS_MOM_MZ;                          // layers per block
 constexpr int TW = BX + 2, TH = BY + 2, TP = TW * TH;   // tile plane with halo
 constexpr int NLD = (TP + BX * BY - 1) / (BX * BY);      // plane elements per thread
 
@@ -55,6 +52,7 @@ k_momentum_zm(Grid g, MomParams m, const double* __restrict__ dscal, const T* __
               const T* __restrict__ p_zold, T* __restrict__ dudt, T* __restrict__ dvdt, T* __restrict__ dwdt,
               const uint8_t* __restrict__ jm, const uint8_t* __restrict__ jmo, const T* __restrict__ zp) {
     __shared__ T ring[5][4][TP];   // u, v, w, miu, rho x four planes
+    const int MZ = m.mz;
     const int ntx = (g.nx + BX - 1) / BX, nty = (g.ny + BY - 1) / BY, ntz = (g.nz + MZ - 1) / MZ;
     const int tile = xcd_tile(blockIdx.x, ntx * nty * ntz);
     const int bx = tile % ntx, by = (tile / ntx) % nty, bz = tile / (ntx * nty);
@@ -327,16 +325,21 @@ k_momentum_zm(Grid g, MomParams m, const double* __restrict__ dscal, const T* __
 }
 
 template <class T>
-static void momentum_dispatch(Ctx& c, const MomParams& m, int rhs_only) {
+static void momentum_dispatch(Ctx& c, MomParams m, int rhs_only) {
     dim3 blk(BX, BY, 1);
-    const int ntiles = ((c.g.nx + BX - 1) / BX) * ((c.g.ny + BY - 1) / BY) * ((c.g.nz + MZ - 1) / MZ);
+    const int txy = ((c.g.nx + BX - 1) / BX) * ((c.g.ny + BY - 1) / BY);
+    int ntiles = 0;
 #define MOM(R, P)                                                                                                  \
-    hipLaunchKernelGGL((k_momentum_zm<T, R, P>), dim3(ntiles), blk, 0, c.stream, c.g, m,                            \
-                       (const double*)c.dscal, as<T>(c.unew), as<T>(c.vnew), as<T>(c.wnew), as<T>(c.miu),          \
-                       as<T>(c.rho), as<T>(c.duold), as<T>(c.dvold), as<T>(c.dwold), (const double*)c.pnew,        \
-                       (const double*)c.pold, as<T>(c.p_x), as<T>(c.p_y), as<T>(c.p_z), as<T>(c.p_xold),          \
-                       as<T>(c.p_yold), as<T>(c.p_zold), as<T>(c.dudt), as<T>(c.dvdt), as<T>(c.dwdt),        \
-                       (const uint8_t*)c.jmask, (const uint8_t*)c.jmask_old, as<T>(c.zplane))
+    do {                                                                                                           \
+        m.mz = zchunk_for(c, (const void*)k_momentum_zm<T, R, P>, BX * BY, txy, 1, 2);                           \
+        ntiles = txy * ((c.g.nz + m.mz - 1) / m.mz);                                                               \
+        hipLaunchKernelGGL((k_momentum_zm<T, R, P>), dim3(ntiles), blk, 0, c.stream, c.g, m,                        \
+                           (const double*)c.dscal, as<T>(c.unew), as<T>(c.vnew), as<T>(c.wnew), as<T>(c.miu),      \
+                           as<T>(c.rho), as<T>(c.duold), as<T>(c.dvold), as<T>(c.dwold), (const double*)c.pnew,    \
+                           (const double*)c.pold, as<T>(c.p_x), as<T>(c.p_y), as<T>(c.p_z), as<T>(c.p_xold),      \
+                           as<T>(c.p_yold), as<T>(c.p_zold), as<T>(c.dudt), as<T>(c.dvdt), as<T>(c.dwdt),    \
+                           (const uint8_t*)c.jmask, (const uint8_t*)c.jmask_old, as<T>(c.zplane));                 \
+    } while (0)
     if (c.g.pow2) {
         if (rhs_only) MOM(1, true); else MOM(0, true);
     } else {
