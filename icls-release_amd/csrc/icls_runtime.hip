@@ -189,6 +189,7 @@ static int check_params(const icls_params* p) {
     if (!(p->cfl >= 0. && p->cfl < 1e3)) { set_error("cfl must be in [0, 1e3) (got %g)", p->cfl); return ICLS_ERR_PARAM; }
     if (p->cfl > 0. && p->ioutchk < 1) { set_error("an adaptive dt (cfl > 0) needs ioutchk >= 1"); return ICLS_ERR_PARAM; }
     if (p->ls_adv_space != ICLS_ADV_HOUC5 && p->ls_adv_space != ICLS_ADV_WENO5) { set_error("unknown ls_adv_space"); return ICLS_ERR_PARAM; }
+    if (p->mass_correction < 0 || p->mass_correction > 2) { set_error("mass_correction must be 0, 1 or 2"); return ICLS_ERR_PARAM; }
     if (p->reinit_step < 1 || p->inflate_step < 1) { set_error("reinit_step and inflate_step must be >= 1"); return ICLS_ERR_PARAM; }
     if (p->re_initialization && p->reinit_iter_max != 1) {
         set_error("reinit_iter_max other than 1 is not supported by this build");
@@ -870,6 +871,16 @@ int32_t icls_max_vel_at_center(double* vmax) {
     TRY(launch_max_vel(c));
     TRY(pull_scalars(c));
     if (vmax) *vmax = c.hscal[DS_VMAX];
+    return ICLS_OK;
+}
+
+// spurious_currents (misc.f90:802-843): Ca = mu max|u_c| / sigma with the reference's
+// fixed mu = 0.1 and sigma = 1; its max|u_c| is the one of max_vel_at_center
+int32_t icls_spurious_currents(double* Ca) {
+    double vmax = 0.;
+    TRY(icls_max_vel_at_center(&vmax));
+    const double mu = 0.1, sigma = 1.;
+    if (Ca) *Ca = mu * vmax / sigma;
     return ICLS_OK;
 }
 

@@ -215,8 +215,9 @@ __global__ void k_infl_flag(int l, int lmax, int due, double* dscal) {
     dscal[DS_INFL_FLAG] = (due && fabs(massloss) > 1e-5) ? 1. : 0.;
 }
 
-// partial sums of delta*kappa*|grad phi| (levelset.f90:499) and of the Heaviside
-// (levelset.f90:505) over the interior; dheav over the C-halo range
+// partial sums of delta*kappa*|grad phi| (levelset.f90:499), of the Heaviside
+// (levelset.f90:505) and of delta^2*kappa*|grad phi|^2 (the normalisation of
+// mass_correction = 2) over the interior; dheav over the C-halo range
 template <class T>
 __global__ void k_infl_sums(Grid g, const double* __restrict__ flag, const T* __restrict__ phi,
                             const T* __restrict__ kappa, const T* __restrict__ nm, T s1, T pi, T* __restrict__ dheav,
@@ -225,7 +226,7 @@ __global__ void k_infl_sums(Grid g, const double* __restrict__ flag, const T* __
     __shared__ double smem[16];
     int i0 = blockIdx.x * BX + threadIdx.x, j0 = blockIdx.y * BY + threadIdx.y;
     bool valid = i0 < g.nx && j0 < g.ny;
-    double sa = 0., sv = 0.;
+    double sa = 0., sv = 0., s2 = 0.;
     for (int kk = 0; kk < KCH; kk++) {
         int k = blockIdx.z * KCH + kk;
         if (!valid || k > g.nz + 1) continue;
@@ -235,19 +236,31 @@ __global__ void k_infl_sums(Grid g, const double* __restrict__ flag, const T* __
         if (k >= 1 && k <= g.nz) {
             T del = dirac_trig<T>(phi[q], s1, pi);
             sa += (double)(del * kappa[q] * nm[q]);
+            s2 += (double)(del * del * kappa[q] * (nm[q] * nm[q]));
             sv += (double)h;
         }
     }
     double ra = block_reduce(sa, OpSum(), smem);
     double rv = block_reduce(sv, OpSum(), smem);
-    if (threadIdx.x == 0 && threadIdx.y == 0) { part[block_linear()] = ra; part[nblocks + block_linear()] = rv; }
+    double r2 = block_reduce(s2, OpSum(), smem);
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+        part[block_linear()] = ra;
+        part[nblocks + block_linear()] = rv;
+        part[2 * nblocks + block_linear()] = r2;
+    }
 }
 
-// levelset.f90:499-521 after the two global sums
-__global__ void k_infl_scalars(int l, int lmax, double ncell, double dx, double dy, double dz, double lxyz,
-                               double inflate_dt, double s1, const double* __restrict__ sums, double* dscal) {
+// levelset.f90:499-521 after the global sums.  mass_correction = 2 divides the loss
+// rate by sum(delta^2 kappa |grad phi|^2) dV instead of the area: the correction
+// velocity -temp kappa grad H moves phi by temp kappa delta |grad phi|^2 per unit time,
+// which changes the H-measured volume at temp times that sum, so one correction
+// restores the loss.  The reference's normalisation (mass_correction = 1) restores
+// about int(delta^2) dphi = 3/(4 eps) = 1/(2 dz) times the loss.
+__global__ void k_infl_scalars(int l, int lmax, int norm, double ncell, double dx, double dy, double dz,
+                               double lxyz, double inflate_dt, double s1, const double* __restrict__ sums,
+                               double* dscal) {
     if (threadIdx.x != 0 || dscal[DS_INFL_FLAG] == 0.) return;
-    double area = sums[0] * dx * dy * dz;
+    double area = (norm ? sums[2] : sums[0]) * dx * dy * dz;
     double vol = 1. - sums[1] / ncell;
     dscal[DS_COUNT_FIXED + l] = vol;
     double loss_rate = lxyz * (vol - dscal[DS_COUNT_FIXED + lmax + l]) / inflate_dt;
@@ -305,6 +318,7 @@ static int inflation_velocity(Ctx& c, int l) {
     dim3 blk(BX, BY, 1);
     dim3 grd = red_grid(c.g, 0, c.g.nz + 1);
     int nb = red_nblocks(grd);
+    if (3L * nb > c.npart_cap) { set_error("reduction scratch too small"); return ICLS_ERR_STATE; }
     hipLaunchKernelGGL(k_infl_sums<T>, grd, blk, 0, c.stream, c.g, flag, as<T>(c.lvset[l]), as<T>(c.curv2),
                        as<T>(c.normal_m), (T)s1, (T)c.pi, as<T>(c.dheav), c.dpart, nb);
     double* sums = c.dscal + DS_COUNT_FIXED + 2 * c.lmax;
@@ -312,11 +326,13 @@ static int inflation_velocity(Ctx& c, int l) {
                        sums, flag);
     hipLaunchKernelGGL(k_reduce_final_pred<OpSum>, dim3(1), dim3(RED_FINAL_THREADS), 0, c.stream, (const double*)(c.dpart + nb), nb,
                        0., sums + 1, flag);
+    hipLaunchKernelGGL(k_reduce_final_pred<OpSum>, dim3(1), dim3(RED_FINAL_THREADS), 0, c.stream,
+                       (const double*)(c.dpart + 2 * nb), nb, 0., sums + 2, flag);
     double ncell = 1. * c.g.nx * c.g.ny * c.g.nz;
     double lxyz = c.P.lx * c.P.ly * c.P.lz;
     double inflate_dt = c.P.inflate_step * c.dt;
-    hipLaunchKernelGGL(k_infl_scalars, dim3(1), dim3(64), 0, c.stream, l, c.lmax, ncell, c.g.dx, c.g.dy, c.g.dz, lxyz,
-                       inflate_dt, s1, (const double*)sums, c.dscal);
+    hipLaunchKernelGGL(k_infl_scalars, dim3(1), dim3(64), 0, c.stream, l, c.lmax, c.P.mass_correction == 2 ? 1 : 0,
+                       ncell, c.g.dx, c.g.dy, c.g.dz, lxyz, inflate_dt, s1, (const double*)sums, c.dscal);
     HIP_OK(hipGetLastError());
     int rc = launch_bc_p_t_pred(c, c.dheav, flag);
     if (rc) return rc;

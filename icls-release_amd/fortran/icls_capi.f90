@@ -135,6 +135,10 @@ module icls_capi
        import :: c_int32_t, c_double
        real(c_double), intent(out) :: vmax
      end function
+     integer(c_int32_t) function icls_spurious_currents(Ca) bind(c, name='icls_spurious_currents')
+       import :: c_int32_t, c_double
+       real(c_double), intent(out) :: Ca
+     end function
      integer(c_int32_t) function icls_step(istep, begin) bind(c, name='icls_step')
        import :: c_int32_t
        integer(c_int32_t), value :: istep, begin

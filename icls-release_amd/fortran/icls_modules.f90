@@ -83,7 +83,7 @@ module mod_misc
   use mod_icls_state
   implicit none
   private
-  public average_vel_at_center, average_stream_vel, max_vel_at_center
+  public average_vel_at_center, average_stream_vel, max_vel_at_center, spurious_currents
 contains
   subroutine average_vel_at_center
     call icls_check(icls_average_vel_at_center(), 'average_vel_at_center')
@@ -96,6 +96,10 @@ contains
     real, intent(out) :: vel_mag_max_all
     call icls_check(icls_max_vel_at_center(vel_mag_max_all), 'max_vel_at_center')
   end subroutine max_vel_at_center
+  subroutine spurious_currents(Ca)
+    real, intent(out) :: Ca
+    call icls_check(icls_spurious_currents(Ca), 'spurious_currents')
+  end subroutine spurious_currents
 end module mod_misc
 
 ! ab2 (src/ab2.f90:21-138)

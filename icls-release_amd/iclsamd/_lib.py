@@ -25,6 +25,7 @@ EXPORTS = [
     "icls_sync", "icls_ls_main", "icls_average_stream_vel", "icls_ab2", "icls_fillps", "icls_gfm4p",
     "icls_add_pjump", "icls_solver2d", "icls_boundp", "icls_correc", "icls_bounduvw", "icls_update_pressure",
     "icls_average_vel_at_center", "icls_check_nan", "icls_chkdiv", "icls_chkdt", "icls_max_vel_at_center",
+    "icls_spurious_currents",
     "icls_ipot", "icls_write_fld", "icls_init_program_restart", "icls_write_vtk", "icls_post2d",
     "icls_step", "icls_profile_enable", "icls_profile_only", "icls_profile_reset", "icls_kernel_times",
     "icls_mlp_defaults", "icls_mlp_train", "icls_mlp_set_weights", "icls_mlp_get_weights", "icls_mlp_set_gate",
@@ -78,6 +79,7 @@ def load():
     L.icls_chkdiv.argtypes = [i32, dp, dp, ctypes.POINTER(i32)]
     L.icls_chkdt.argtypes = [dp]
     L.icls_max_vel_at_center.argtypes = [dp]
+    L.icls_spurious_currents.argtypes = [dp]
     L.icls_ipot.argtypes = [dp]
     L.icls_write_fld.argtypes = [ctypes.c_char_p, i32]
     L.icls_init_program_restart.argtypes = [ctypes.c_char_p, ctypes.POINTER(i32)]
@@ -241,6 +243,12 @@ class Solver:
     def max_vel_at_center(self):
         v = ctypes.c_double()
         self._call("icls_max_vel_at_center", ctypes.byref(v))
+        return v.value
+
+    def spurious_currents(self):
+        """spurious_currents (misc.f90:802-843): Ca = 0.1 max|u_c| / 1."""
+        v = ctypes.c_double()
+        self._call("icls_spurious_currents", ctypes.byref(v))
         return v.value
 
     def ipot(self):

@@ -74,7 +74,13 @@ typedef struct icls_params {
     int32_t re_initialization;       /* param.f90:164                                    */
     int32_t reinit_step;             /* param.f90:230                                    */
     int32_t reinit_iter_max;         /* param.f90:233                                    */
-    int32_t mass_correction;         /* param.f90:165                                    */
+    int32_t mass_correction;         /* param.f90:165.  1: the reference's inflation
+                                        (levelset.f90:377-582), which restores about
+                                        1/(2 dz) times the volume loss it targets and so
+                                        overshoots and oscillates once it fires; 2: the
+                                        same correction normalised by sum(delta^2 kappa
+                                        |grad phi|^2) dV, which restores the loss itself
+                                        (an extension); 0: off                            */
     int32_t inflate_step;            /* param.f90:258                                    */
     int32_t ns_np;                   /* no-slip walls at rest (param.f90:95)             */
     double  wall_speed;              /* |v| of the z walls when ns_np == 0; the reference
@@ -175,6 +181,9 @@ int32_t icls_check_nan(void);                            /* main.f90:207-210    
 int32_t icls_chkdiv(int32_t istep, double* divtot, double* divmax, int32_t* ijk); /* chkdiv.f90:10-90 */
 int32_t icls_chkdt(double* dtmax);                       /* chkdt.f90:11-105            */
 int32_t icls_max_vel_at_center(double* vmax);            /* misc.f90:610-636            */
+/* spurious_currents (misc.f90:802-843): the static-drop capillary number
+ * Ca = mu max|u_c| / sigma with the reference's fixed mu = 0.1, sigma = 1. */
+int32_t icls_spurious_currents(double* Ca);
 /* ipot (interface.f90:225-277): minimal distances between interfaces, dmin(lmax,lmax)
  * column-major, on request.  In the reference only exjump (the 'attractive' option,
  * not built) and the per-output diagnostic read dmin, so it is not part of icls_step. */

@@ -756,6 +756,18 @@ int oracle_inflation(oracle_state* S, int l) {
         for (int j = 1; j <= S->jmax; j++)
             for (int i = 1; i <= S->imax; i++) temp = temp + delta[II(i, j, k)] * kappa[CC(i, j, k)] * S->nrm_m[II(i, j, k)];
     double area = temp * S->dx * S->dy * S->dz;
+    if (S->P.mass_correction == 2) {
+        /* normalised variant (not in the reference): the H-measured volume changes at
+         * temp * sum(delta^2 kappa |grad phi|^2) dV under the correction velocity */
+        temp = 0.;
+        for (int k = 1; k <= S->kmax; k++)
+            for (int j = 1; j <= S->jmax; j++)
+                for (int i = 1; i <= S->imax; i++) {
+                    const double d = delta[II(i, j, k)], m = S->nrm_m[II(i, j, k)];
+                    temp = temp + d * d * kappa[CC(i, j, k)] * (m * m);
+                }
+        area = temp * S->dx * S->dy * S->dz;
+    }
     temp = 0.;
     for (int k = 1; k <= S->kmax; k++)
         for (int j = 1; j <= S->jmax; j++)

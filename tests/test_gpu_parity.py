@@ -127,6 +127,25 @@ def test_steps_parity_adaptive_dt_bubble():
         g.close()
 
 
+def test_spurious_currents_matches_oracle():
+    """spurious_currents (misc.f90:802-843) after 5 steps of the static drop: the
+    reference's Ca = 0.1 max|u_c| / 1, with max|u_c| against the oracle's."""
+    p = small_drop(32, mass_correction=0)
+    o, g = _pair(p)
+    try:
+        o.init_program()
+        g.init_program()
+        o.run(5)
+        for s in range(1, 6):
+            g.step(s, 0)
+        ca, vm = g.spurious_currents(), g.max_vel_at_center()
+        assert ca == 0.1 * vm
+        want = o.max_vel_at_center()
+        assert want > 0 and abs(vm - want) <= TOL * want, (vm, want)
+    finally:
+        g.close()
+
+
 def test_steps_parity_three_drops():
     """lmax = 3 ('mdp', init.f90:901-986): one level set per drop, materials from the
     minimum Heaviside (misc.f90:102-120), GFM jumps accumulated over the drops
@@ -206,9 +225,11 @@ def test_restart_from_fld_matches_oracle(tmp_path):
         g.close()
 
 
-def test_inflation_parity():
-    """One step with the mass correction firing (|V - V0|/V0 = 1e-3)."""
-    p = small_drop(32)
+@pytest.mark.parametrize("mc", [1, 2])
+def test_inflation_parity(mc):
+    """One step with the mass correction firing (|V - V0|/V0 = 1e-3): the reference's
+    normalisation (1) and the normalised variant (2)."""
+    p = small_drop(32, mass_correction=mc)
     o, g = _pair(p)
     try:
         o.init_program()

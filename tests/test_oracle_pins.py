@@ -413,3 +413,40 @@ def test_adaptive_dt_cadence_and_growth_cap():
     o.init_program()
     o.run(2)
     assert o.scalar(P.S_DT) == p.timestep
+
+
+def _inflation_gain(n, deficit, mc=1):
+    """Volume restored by one inflation (levelset.f90:405-582) over the deficit it targets."""
+    p = P.drop64(n)
+    p.mass_correction = mc
+    o = O.Oracle(p)
+    o.init_program()
+    o.tube_construction(0)
+    v, _ = o.volume(0)
+    o.set_volume(0, v, v * (1 + deficit))
+    o.inflation(0)
+    o.update_materials()
+    return (o.volume(0)[0] - v) / (v * deficit)
+
+
+def test_inflation_gain_closed_form():
+    """One inflation moves the interface with u = -temp kappa grad H, so the H-measured
+    volume changes by temp |kappa| A int(delta^2) dphi = 3/(4 eps) x the loss that
+    `area` = sum(delta kappa |grad phi|) dV normalises for (int delta dphi = 1).  The
+    restored volume is therefore linear in the deficit and 3/(4 eps) = N/2 times it
+    (eps = 1.5 dz; 0.876 of that on the grid: sampled delta and H differences), which
+    is why the reference's mass correction overshoots and oscillates once it fires."""
+    g32a, g32b, g64 = _inflation_gain(32, 1e-6), _inflation_gain(32, 1e-5), _inflation_gain(64, 1e-6)
+    assert abs(g32a / g32b - 1) < 1e-3                    # linear in the deficit
+    assert abs(g64 / g32a - 2) < 0.02                     # proportional to 1/dz
+    assert abs(g32a / 16.0 - 0.876) < 0.01                # 3/(4 eps) with eps = 1.5/32
+
+
+def test_normalised_inflation_restores_the_loss():
+    """mass_correction = 2 divides by sum(delta^2 kappa |grad phi|^2) dV, the rate at
+    which the correction velocity changes the H-measured volume: one correction then
+    restores the loss itself, at any resolution (0.89 and 0.88 of it on the grid), so
+    repeated corrections converge instead of oscillating."""
+    g32, g64 = _inflation_gain(32, 1e-5, 2), _inflation_gain(64, 1e-5, 2)
+    assert 0.85 < g32 < 0.95 and 0.85 < g64 < 0.95
+    assert abs(g64 / g32 - 1) < 0.03
