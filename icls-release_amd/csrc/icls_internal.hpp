@@ -80,9 +80,22 @@ struct Mlp {
     float tol = 0.5f;                   // accepted |kappa_mlp - kappa_fd| dz
 };
 
+// The part of the grid one level set is stepped on: cells ox .. ox+wx-1 in x (and
+// likewise y), layers oz+1 .. oz+wz in z.  wx and wy are powers of two, and either the
+// window does not cross the periodic boundary or it spans the whole grid.
+struct LsWindow {
+    int ox = 0, oy = 0, oz = 0, wx = 0, wy = 0, wz = 0;
+};
+
 struct Ctx {
     icls_params P{};
     Grid g{};
+    Grid gfull{};                         // the whole grid (g is a window's inside a WindowScope)
+    long ncell_all = 0;                   // interior cells of the whole grid
+    std::vector<LsWindow> win;            // per level set; empty: every level set on the whole grid
+    int win_age = 0;                      // ls_main passes since the windows were set
+    int* dext = nullptr;                  // device extents, 6 ints per level set
+    int* hext = nullptr;                  // pinned host copy
     int lmax = 1;
     long nfield = 0;       // elements per device 3D field
     bool fp32 = false;     // field arrays hold float (mixed modes)
@@ -214,6 +227,7 @@ __device__ __forceinline__ int xcd_tile(int b, int n) {
 // Field-class pointers are reinterpreted as the mode's element type.
 // level set
 int launch_tube(Ctx& c, const double* phi_in, double* phi_out);
+int launch_ls_extent(Ctx& c, const double* phi, int* ext);
 int launch_adv_stage(Ctx& c, int scheme, int stage, const double* phi0, const double* phis, double* out,
                      const double* vu, const double* vv, const double* vw, int velf, const double* flag);
 int launch_reinit_stage(Ctx& c, int stage, const double* phi0, const double* phis, double* out);

@@ -153,6 +153,8 @@ static void free_ctx(Ctx* c) {
     for (auto* p : c->lvset) if (p) hipFree(p);
     if (c->hscal) hipHostFree(c->hscal);
     if (c->hvol) hipHostFree(c->hvol);
+    if (c->dext) hipFree(c->dext);
+    if (c->hext) hipHostFree(c->hext);
     if (c->ev_vol) hipEventDestroy(c->ev_vol);
     if (c->stream) hipStreamDestroy(c->stream);
     mlp_free(*c);
@@ -221,6 +223,8 @@ static int build_ctx(const icls_params* prm) {
     auto p2 = [](double v) { int e; return std::frexp(v, &e) == 0.5; };
     g.pow2 = p2(g.dxi) && p2(g.dyi) && p2(g.dzi) && g.dx * g.dxi == 1. && g.dy * g.dyi == 1. && g.dz * g.dzi == 1.;
     C.lmax = prm->lmax;
+    C.gfull = g;
+    C.ncell_all = (long)g.nx * g.ny * g.nz;
     C.nfield = g.sz * (long)(g.nz + 6);
     C.NB2 = 6. * g.dz;
     C.rho0 = prm->rho1 < prm->rho2 ? prm->rho1 : prm->rho2;
@@ -274,6 +278,10 @@ static int build_ctx(const icls_params* prm) {
     HIP_OK(hipHostMalloc((void**)&C.hvol, sizeof(double) * 2 * C.lmax));
     memset(C.hvol, 0, sizeof(double) * 2 * C.lmax);
     HIP_OK(hipEventCreateWithFlags(&C.ev_vol, hipEventDisableTiming));
+    if (C.lmax > 1 && !prm->ls_full) {
+        HIP_OK(hipMalloc((void**)&C.dext, sizeof(int) * 6 * C.lmax));
+        HIP_OK(hipHostMalloc((void**)&C.hext, sizeof(int) * 6 * C.lmax));
+    }
     // reduction partials: enough for lmax per-block sums of any sweep
     // (the chkdt sweep covers 1..i1 x 1..j1 x 1..k1, one row and column more than the others)
     long nb = (long)((g.nx + 2 + 63) / 64 + 1) * ((g.ny + 2 + 3) / 4 + 1) * ((g.nz + 2 + 7) / 8 + 1);
@@ -461,6 +469,8 @@ static int prepare_field(Ctx& c, int field) {
     return ICLS_OK;
 }
 
+static int init_windows(Ctx& c);
+
 int32_t icls_upload(int32_t field, int32_t l, const double* host) {
     CTX_OR_FAIL();
     c.vbulk_fresh = false;
@@ -484,6 +494,8 @@ int32_t icls_upload(int32_t field, int32_t l, const double* host) {
     TRY(store_array(c, d, f32, stage));
     // in the mixed modes the solve reads its right-hand side from a float array
     if (field == ICLS_F_P && c.frhs != c.p) TRY(store_array(c, c.frhs, true, stage));
+    // a new level set: its window is found again from the whole grid
+    if (field == ICLS_F_LVSET && !c.win.empty()) TRY(init_windows(c));
     return ICLS_OK;
 }
 
@@ -503,13 +515,19 @@ int32_t icls_download(int32_t field, int32_t l, double* host) {
     TRY(fetch_array(c, d, field_is_fp32(c, field), stage));
     const bool face = field == ICLS_F_RHO_U || field == ICLS_F_RHO_V || field == ICLS_F_RHO_W;
     const bool req = c.P.rho1 == c.P.rho2;
+    // gfm4p zeroes p_jump and the face jumps and fills the interior only (gfm.f90:71-79,
+    // 219-233); nothing exchanges their halos, which stay zero in the reference
+    const bool jump = field == ICLS_F_PJUMP || field == ICLS_F_PX || field == ICLS_F_PY || field == ICLS_F_PZ ||
+                      field == ICLS_F_PXOLD || field == ICLS_F_PYOLD || field == ICLS_F_PZOLD;
     for (int kh = 0; kh < hz; kh++) {
         int k = kh + lb;
         for (int jh = 0; jh < hy; jh++) {
             int j0 = wrap(jh + lb - 1, g.ny);
+            const bool jhalo = jh + lb - 1 < 0 || jh + lb - 1 >= g.ny;
             for (int ih = 0; ih < hx; ih++) {
                 int i0 = wrap(ih + lb - 1, g.nx);
                 double v = stage[(size_t)gidx(g, i0, j0, k)];
+                if (jump && (jhalo || ih + lb - 1 < 0 || ih + lb - 1 >= g.nx)) v = 0.;
                 if (face) {
                     // arithmetic face means (misc.f90:151-172)
                     if (req) v = c.P.rho1;
@@ -612,8 +630,145 @@ static int mass_correction_due(Ctx& c, int l, int istep, bool& due) {
 }
 
 // ---------------------------------------------------------------- level set
+// ---------------------------------------------------------------- level-set windows
+// With several level sets each one is stepped inside a window around its drop
+// (LsWindow): the launches see a grid of the window's size whose strides are the whole
+// grid's and whose arrays start at the window's first cell.  x and y wrap inside the
+// window, so a window is a small periodic domain; its edges lie in the clamped far
+// field (phi = NarrowBand_2 after the tube), where every stencil is flat, and the
+// stepping inside equals the whole-grid stepping wherever phi is below NarrowBand_2.
+// The arrays keep their whole-grid size: cells outside a window keep the values they
+// had when they left it (far field, phi >= NarrowBand_2, Heaviside 1).
+constexpr int WIN_REFRESH = 4;     // ls_main passes between window updates
+constexpr int WIN_REACH = 7;       // stencil reach of one step beyond the band: tube scan,
+                                   // RK stages, re-init
+constexpr int WIN_DRIFT = 2;       // band motion allowed between updates (cells)
+
+struct WindowScope {
+    Ctx& c;
+    bool on;
+    int l;
+    Grid g{};
+    double* saved[21];
+    double* ls = nullptr;
+    uint8_t* mask = nullptr;
+    double* stash = nullptr;
+    double** fld(int i) {
+        double** f[21] = {&c.unew, &c.vnew, &c.wnew, &c.curv, &c.curv2, &c.ucc_l, &c.vcc_l, &c.wcc_l, &c.u_l,
+                          &c.v_l, &c.w_l, &c.dheav, &c.normal_m, &c.ls_a, &c.ls_b, &c.ls_c, &c.p_jump, &c.p_x,
+                          &c.p_y, &c.p_z, &c.ucc};
+        return f[i];
+    }
+    static double* shift(double* p, long off, size_t esz) { return (double*)((char*)p + off * (long)esz); }
+    WindowScope(Ctx& c_, int l_) : c(c_), on(!c_.win.empty()), l(l_) {
+        if (!on) return;
+        const LsWindow& w = c.win[l];
+        g = c.g;
+        const long poff = w.ox + c.g.sy * (long)w.oy, off = poff + c.g.sz * (long)w.oz;
+        c.g.nx = w.wx; c.g.ny = w.wy; c.g.nz = w.wz;
+        for (int i = 0; i < 21; i++) { saved[i] = *fld(i); *fld(i) = shift(saved[i], off, c.esz); }
+        ls = c.lvset[l]; c.lvset[l] = shift(ls, off, c.esz);
+        mask = c.mask; c.mask = mask + off;
+        stash = c.stash; c.stash = shift(stash, poff, c.esz);
+    }
+    ~WindowScope() {
+        if (!on) return;
+        c.g = g;
+        for (int i = 0; i < 21; i++) *fld(i) = saved[i];
+        c.lvset[l] = ls;
+        c.mask = mask;
+        c.stash = stash;
+    }
+};
+
+// smallest power of two >= v
+static int pow2_at_least(int v) { int p = 1; while (p < v) p <<= 1; return p; }
+
+// One periodic axis: cover global cells lo .. hi (lo may be < 0 or hi >= n when the
+// drop crosses the boundary) with a power-of-two range that does not cross it, else
+// with the whole axis.
+static void window_axis_periodic(int lo, int hi, int n, int& o, int& w) {
+    const int need = hi - lo + 1;
+    w = pow2_at_least(need);
+    if (lo < 0 || hi > n - 1 || w >= n) { o = 0; w = n; return; }
+    o = lo - (w - need) / 2;
+    if (o < 0) o = 0;
+    if (o > n - w) o = n - w;
+}
+
+// z (walls): layers lo .. hi clamped to 1 .. nz, wz a multiple of 4 where nz allows
+static void window_axis_z(int lo, int hi, int n, int& o, int& w) {
+    if (lo < 1) lo = 1;
+    if (hi > n) hi = n;
+    w = hi - lo + 1;
+    if (n % 4 == 0) {
+        w = (w + 3) / 4 * 4;
+        if (lo - 1 + w > n) lo = n - w + 1;
+    }
+    if (w >= n) { o = 0; w = n; return; }
+    o = lo - 1;
+}
+
+// Windows from the current level sets: the bounding box of phi < NarrowBand_2 inside
+// each window, widened by WIN_REACH + WIN_DRIFT cells.  Fails when a drop's band came
+// closer than WIN_REACH cells to an edge of its window that is not a wall or the whole
+// periodic axis.
+static int update_windows(Ctx& c) {
+    const int L = c.lmax;
+    HIP_OK(hipMemsetAsync(c.dext, 0x7f, sizeof(int) * 6 * L, c.stream));
+    for (int l = 0; l < L; l++) {
+        WindowScope ws(c, l);
+        TRY(launch_ls_extent(c, c.lvset[l], c.dext + 6 * l));
+    }
+    HIP_OK(hipMemcpyAsync(c.hext, c.dext, sizeof(int) * 6 * L, hipMemcpyDeviceToHost, c.stream));
+    HIP_OK(hipStreamSynchronize(c.stream));
+    const Grid& G = c.gfull;
+    const int m = WIN_REACH + WIN_DRIFT;
+    for (int l = 0; l < L; l++) {
+        LsWindow& w = c.win[l];
+        const int* e = c.hext + 6 * l;
+        if (e[0] == 0x7f7f7f7f) continue;          // no cell below the band limit: keep it
+        const int lo[3] = {e[0], e[1], e[2]}, hi[3] = {-e[3], -e[4], -e[5]};
+        const int ext[3] = {w.wx, w.wy, w.wz}, full[3] = {G.nx, G.ny, G.nz};
+        for (int a = 0; a < 3; a++) {
+            const bool whole = ext[a] == full[a];
+            const int first = a == 2 ? 1 : 0, last = a == 2 ? ext[a] : ext[a] - 1;
+            const bool wall_lo = a == 2 && w.oz == 0, wall_hi = a == 2 && w.oz + w.wz == G.nz;
+            if (!whole && ((!wall_lo && lo[a] - first < WIN_REACH) || (!wall_hi && last - hi[a] < WIN_REACH))) {
+                set_error("level set %d came within %d cells of its window edge; the band moves faster than "
+                          "%d cells per %d steps", l, WIN_REACH, WIN_DRIFT, WIN_REFRESH);
+                return ICLS_ERR_STATE;
+            }
+        }
+        LsWindow n;
+        if (w.wx == G.nx && (lo[0] < m || hi[0] > G.nx - 1 - m)) { n.ox = 0; n.wx = G.nx; }
+        else window_axis_periodic(w.ox + lo[0] - m, w.ox + hi[0] + m, G.nx, n.ox, n.wx);
+        if (w.wy == G.ny && (lo[1] < m || hi[1] > G.ny - 1 - m)) { n.oy = 0; n.wy = G.ny; }
+        else window_axis_periodic(w.oy + lo[1] - m, w.oy + hi[1] + m, G.ny, n.oy, n.wy);
+        window_axis_z(w.oz + lo[2] - m, w.oz + hi[2] + m, G.nz, n.oz, n.wz);
+        w = n;
+    }
+    return ICLS_OK;
+}
+
+// windows on (several level sets, not ls_full): start from the whole grid
+static int init_windows(Ctx& c) {
+    c.win.clear();
+    if (!c.dext) return ICLS_OK;
+    LsWindow all;
+    all.wx = c.gfull.nx; all.wy = c.gfull.ny; all.wz = c.gfull.nz;
+    c.win.assign(c.lmax, all);
+    c.win_age = 0;
+    return update_windows(c);
+}
+
 static int ls_main_impl(Ctx& c, int istep) {
+    if (!c.win.empty() && ++c.win_age >= WIN_REFRESH) {
+        TRY(update_windows(c));
+        c.win_age = 0;
+    }
     for (int l = 0; l < c.lmax; l++) {
+        WindowScope ws(c, l);
         double* ph = c.lvset[l];
         // tube_construction (levelset.f90:87-197); every level-set kernel below folds
         // the boundsls that follows it (bound.f90:404-411) into its own launch
@@ -691,16 +846,23 @@ static int gfm_impl(Ctx& c, bool fuse_fillps, bool store_pjump) {
     c.p_in_pnew = false;   // fillps writes p
     const bool learned = c.learned && c.mlp.ready;
     for (int l = 0; l < c.lmax; l++) {
-        if (learned) {
-            // get_curvature with the learned closure on the under-resolved cells
-            TRY(launch_mlp_reset_list(c));
-            TRY(launch_curvature(c, c.lvset[l], c.curv, nullptr, nullptr, true));
-            TRY(launch_mlp_curv(c, c.lvset[l], c.curv));
-        } else {
-            TRY(launch_curvature(c, c.lvset[l], c.curv, nullptr, nullptr));
+        {
+            // inside the level set's window; the first level set's jumps are written
+            // over the whole grid (they clear the previous step's), the others added
+            // inside their windows
+            WindowScope ws(c, l);
+            if (learned) {
+                // get_curvature with the learned closure on the under-resolved cells
+                TRY(launch_mlp_reset_list(c));
+                TRY(launch_curvature(c, c.lvset[l], c.curv, nullptr, nullptr, true));
+                TRY(launch_mlp_curv(c, c.lvset[l], c.curv));
+            } else {
+                TRY(launch_curvature(c, c.lvset[l], c.curv, nullptr, nullptr));
+            }
+            TRY(launch_recurvature(c, c.lvset[l], c.curv, c.curv2));
+            if (l > 0) TRY(launch_gfm(c, l, c.curv2, false, true));
         }
-        TRY(launch_recurvature(c, c.lvset[l], c.curv, c.curv2));
-        TRY(launch_gfm(c, l, c.curv2, fuse_fillps && c.lmax == 1, store_pjump || c.lmax > 1));
+        if (l == 0) TRY(launch_gfm(c, 0, c.curv2, fuse_fillps && c.lmax == 1, store_pjump || c.lmax > 1));
     }
     if (fuse_fillps && c.lmax > 1) {
         TRY(launch_fillps(c));
@@ -891,6 +1053,7 @@ static int init_derived_impl(Ctx& c) {
     TRY(launch_bc_uvw(c, c.unew, c.vnew, c.wnew));         // init.f90:72-76
     TRY(launch_bc_uvw(c, c.dudt, c.dvdt, c.dwdt));
     for (int l = 0; l < c.lmax; l++) TRY(launch_boundmls(c, c.lvset[l]));
+    TRY(init_windows(c));
     TRY(launch_bc_p(c, c.pnew));
     TRY(launch_avg_vel(c));                                // init.f90:79
     TRY(launch_materials(c));                              // init.f90:85

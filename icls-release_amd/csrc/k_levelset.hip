@@ -718,6 +718,36 @@ int launch_reinit_stage(Ctx& c, int stage, const double* phi0, const double* phi
     return bc_ls(c, out, nullptr, nullptr, nullptr);
 }
 
+// ---------------------------------------------------------------- level-set windows
+// Bounding box, in the launch grid's own indices, of the cells with phi < lim (the
+// inside and the narrow band of a drop): ext = {min i, min j, min k, -max i, -max j,
+// -max k}, folded in with atomicMin (the caller fills ext with a large value).
+template <class T>
+__global__ void k_ls_extent(Grid g, const T* __restrict__ phi, T lim, int* __restrict__ ext) {
+    __shared__ int s[6];
+    const int tid = threadIdx.x + BX * threadIdx.y;
+    if (tid < 6) s[tid] = 0x7f7f7f7f;
+    __syncthreads();
+    const int i0 = blockIdx.x * BX + threadIdx.x, j0 = blockIdx.y * BY + threadIdx.y, k = blockIdx.z + 1;
+    if (i0 < g.nx && j0 < g.ny && phi[gidx(g, i0, j0, k)] < lim) {
+        atomicMin(&s[0], i0); atomicMin(&s[1], j0); atomicMin(&s[2], k);
+        atomicMin(&s[3], -i0); atomicMin(&s[4], -j0); atomicMin(&s[5], -k);
+    }
+    __syncthreads();
+    if (tid < 6 && s[tid] != 0x7f7f7f7f) atomicMin(&ext[tid], s[tid]);
+}
+
+int launch_ls_extent(Ctx& c, const double* phi, int* ext) {
+    dim3 blk(BX, BY, 1);
+    dim3 grd((c.g.nx + BX - 1) / BX, (c.g.ny + BY - 1) / BY, c.g.nz);
+    if (c.fp32)
+        hipLaunchKernelGGL(k_ls_extent<float>, grd, blk, 0, c.stream, c.g, as<float>(phi), (float)c.NB2, ext);
+    else
+        hipLaunchKernelGGL(k_ls_extent<double>, grd, blk, 0, c.stream, c.g, (const double*)phi, c.NB2, ext);
+    HIP_OK(hipGetLastError());
+    return ICLS_OK;
+}
+
 // ---------------------------------------------------------------- initial level sets
 // Sphere SDF with the +lx and +ly periodic images (init.f90:945-980; 'drp' is the
 // lmax = 1 case, init.f90:717-753), evaluated in double at every stored cell and

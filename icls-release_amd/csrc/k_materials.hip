@@ -256,12 +256,14 @@ __global__ void k_infl_sums(Grid g, const double* __restrict__ flag, const T* __
 // which changes the H-measured volume at temp times that sum, so one correction
 // restores the loss.  The reference's normalisation (mass_correction = 1) restores
 // about int(delta^2) dphi = 3/(4 eps) = 1/(2 dz) times the loss.
-__global__ void k_infl_scalars(int l, int lmax, int norm, double ncell, double dx, double dy, double dz,
-                               double lxyz, double inflate_dt, double s1, const double* __restrict__ sums,
-                               double* dscal) {
+// hoff: cells of the grid outside the swept window (Heaviside 1 there), 0 for a
+// whole-grid sweep.
+__global__ void k_infl_scalars(int l, int lmax, int norm, double ncell, double hoff, double dx, double dy,
+                               double dz, double lxyz, double inflate_dt, double s1,
+                               const double* __restrict__ sums, double* dscal) {
     if (threadIdx.x != 0 || dscal[DS_INFL_FLAG] == 0.) return;
     double area = (norm ? sums[2] : sums[0]) * dx * dy * dz;
-    double vol = 1. - sums[1] / ncell;
+    double vol = 1. - (sums[1] + hoff) / ncell;
     dscal[DS_COUNT_FIXED + l] = vol;
     double loss_rate = lxyz * (vol - dscal[DS_COUNT_FIXED + lmax + l]) / inflate_dt;
     double temp = loss_rate / area;
@@ -328,11 +330,12 @@ static int inflation_velocity(Ctx& c, int l) {
                        0., sums + 1, flag);
     hipLaunchKernelGGL(k_reduce_final_pred<OpSum>, dim3(1), dim3(RED_FINAL_THREADS), 0, c.stream,
                        (const double*)(c.dpart + 2 * nb), nb, 0., sums + 2, flag);
-    double ncell = 1. * c.g.nx * c.g.ny * c.g.nz;
+    double ncell = 1. * c.ncell_all;
+    double hoff = (double)(c.ncell_all - (long)c.g.nx * c.g.ny * c.g.nz);
     double lxyz = c.P.lx * c.P.ly * c.P.lz;
     double inflate_dt = c.P.inflate_step * c.dt;
     hipLaunchKernelGGL(k_infl_scalars, dim3(1), dim3(64), 0, c.stream, l, c.lmax, c.P.mass_correction == 2 ? 1 : 0,
-                       ncell, c.g.dx, c.g.dy, c.g.dz, lxyz, inflate_dt, s1, (const double*)sums, c.dscal);
+                       ncell, hoff, c.g.dx, c.g.dy, c.g.dz, lxyz, inflate_dt, s1, (const double*)sums, c.dscal);
     HIP_OK(hipGetLastError());
     int rc = launch_bc_p_t_pred(c, c.dheav, flag);
     if (rc) return rc;

@@ -408,7 +408,9 @@ k_mlp_curv(Grid g, const int* __restrict__ list, const int* __restrict__ count, 
             for (int k = 0; k < MLP_NPAD; k++) f[k] = 0.f;
             if (e < cnt) {
                 const int c = list[e];
-                const int i0 = c % g.nx, j0 = (c / g.nx) % g.ny, k = c / (g.nx * g.ny) + 1;
+                // the list holds i0 + sy j0 + sz (k - 1) (k_curvature): decode with the
+                // strides, which differ from nx and ny inside a level-set window
+                const int i0 = (int)(c % g.sy), j0 = (int)((c / g.sy) % (g.sz / g.sy)), k = (int)(c / g.sz) + 1;
                 q = gidx(g, i0, j0, k);
                 T v[27];
                 int qq = 0;

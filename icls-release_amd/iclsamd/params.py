@@ -84,7 +84,8 @@ class IclsParams(ctypes.Structure):
         ("mode", ctypes.c_int32),
         ("ioutchk", ctypes.c_int32),
         ("cfl", ctypes.c_double),
-        ("reserved", ctypes.c_int32 * 6),
+        ("ls_full", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 5),
         ("drop_xc", ctypes.c_double * MAX_DROPS),
         ("drop_yc", ctypes.c_double * MAX_DROPS),
         ("drop_zc", ctypes.c_double * MAX_DROPS),

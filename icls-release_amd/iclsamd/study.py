@@ -12,7 +12,8 @@ prints one JSON line per mode with
 * dv_max/dv_mean -- max and mean over the drops of |V_l / V_l(0) - 1| at the end;
 * closure_cells  -- cells the learned closure was consulted on in the last step.
 
-Every drop is its own level set ('mdp', init.f90:901-986), held on the full grid.
+Every drop is its own level set ('mdp', init.f90:901-986), stepped inside a window
+around the drop (--ls-full 1: over the whole grid).
 """
 import argparse
 import json
@@ -27,8 +28,9 @@ from iclsamd import _lib, params as P  # noqa: E402
 MODES = {"fp64": P.MODE_FP64, "mixed": P.MODE_MIXED, "learned": P.MODE_MIXED_LEARNED}
 
 
-def run(n, drops, steps, mode, mass_correction, every, seed, cfl):
+def run(n, drops, steps, mode, mass_correction, every, seed, cfl, ls_full=0):
     p = P.drops256(n, drops, mode=MODES[mode], seed=seed)
+    p.ls_full = ls_full
     p.cfl = cfl
     p.ioutchk = every
     p.mass_correction = mass_correction
@@ -52,6 +54,7 @@ def run(n, drops, steps, mode, mass_correction, every, seed, cfl):
         g.sync()
         dv = [abs(v / v0 - 1.0) for v, v0 in (g.volume(l) for l in range(drops))]
         return {"mode": mode, "n": n, "drops": drops, "steps": steps, "mass_correction": mass_correction,
+                "ls_full": ls_full,
                 "ms_per_step": round(elapsed / steps * 1e3, 3), "init_s": round(t_init, 3),
                 "ca": [(s, t, float("%.4e" % c)) for s, t, c in ca],
                 "dv_max": max(dv), "dv_mean": sum(dv) / len(dv),
@@ -70,9 +73,12 @@ def main():
     ap.add_argument("--every", type=int, default=20)
     ap.add_argument("--seed", type=int, default=20261202)
     ap.add_argument("--cfl", type=float, default=0.3)
+    ap.add_argument("--ls-full", type=int, default=0, choices=[0, 1],
+                    help="1: every level set over the whole grid instead of a window around its drop")
     a = ap.parse_args()
     for m in a.modes.split(","):
-        print(json.dumps(run(a.n, a.drops, a.steps, m, a.mass_correction, a.every, a.seed, a.cfl)), flush=True)
+        print(json.dumps(run(a.n, a.drops, a.steps, m, a.mass_correction, a.every, a.seed, a.cfl, a.ls_full)),
+              flush=True)
 
 
 if __name__ == "__main__":

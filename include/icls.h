@@ -97,7 +97,12 @@ typedef struct icls_params {
                                         take their variable-step coefficients (icls_step).
                                         The reference's CFL is timestep/dz (param.f90:47) and
                                         its dt never adapts: this is an extension.          */
-    int32_t reserved[6];
+    int32_t ls_full;                 /* with lmax > 1 each level set is stepped inside a
+                                        window around its drop (band + 9 cells, refreshed
+                                        every 4 steps; the level-set arrays stay full-grid,
+                                        values beyond the band outside the window are not
+                                        updated); 1 = every level set over the whole grid */
+    int32_t reserved[5];
     double  drop_xc[ICLS_MAX_DROPS]; /* drop centres and radii, one level set each       */
     double  drop_yc[ICLS_MAX_DROPS];
     double  drop_zc[ICLS_MAX_DROPS];

@@ -175,6 +175,95 @@ def test_steps_parity_three_drops():
         g.close()
 
 
+def _window_drops(mode=P.MODE_FP64, mc=0):
+    """Three drops of radius 4 cells on 128 x 128 x 32: the first crosses the periodic
+    x boundary and keeps the whole x axis, the other two get 64-cell windows in x and y."""
+    p = P.base_params(128, nz=32)
+    p.lz = 0.25
+    p.rho1 = p.rho2 = 1.0
+    p.miu1 = p.miu2 = 1.0
+    p.timestep = 1e-5
+    p.mass_correction = mc
+    p.mode = mode
+    r = 4.0 / 128
+    p.set_drops([(0.01, 0.45, 0.13, r), (0.25, 0.26, 0.125, r), (0.70, 0.66, 0.12, r)])
+    return p
+
+
+def _clip_ls(a, p):
+    nb2 = 6.0 * p.lz / p.ktot
+    return np.clip(a, -nb2, nb2)
+
+
+def test_windows_match_oracle():
+    """Level sets stepped inside per-drop windows (two window refreshes in 9 steps)
+    against the oracle's whole-grid level sets: every field at 1e-10, the level sets
+    after clipping to the narrow band (cells beyond it keep their old values outside
+    a window)."""
+    p = _window_drops()
+    o, g = _pair(p)
+    try:
+        o.init_program()
+        g.init_program()
+        for s in range(1, 10):
+            o.step(s, 0)
+            g.step(s, 0)
+        _compare(o, g, [f for f in FIELDS if f != P.F_LVSET], TOL, "windows, 9 steps")
+        for l in range(3):
+            e = rel_linf(_clip_ls(g.download(P.F_LVSET, l), p), _clip_ls(o.get(P.F_LVSET, l), p))
+            assert e <= TOL, (l, e)
+    finally:
+        g.close()
+
+
+def test_windowed_inflation_matches_oracle():
+    """The normalised mass correction firing for the last drop inside its window (maxchk
+    is the last level set's, levelset.f90:380): the volume counts the cells outside the
+    window as Heaviside 1.  The drop covers ~270 of 524288 cells (V ~ 5e-4), so the
+    rounding of V = 1 - sum(H)/N (~1e-16, summed over the window here and over the grid
+    in the oracle) is ~2e-11 of V - V0 at a deficit of 1e-2."""
+    p = _window_drops(mc=2)
+    o, g = _pair(p)
+    try:
+        o.init_program()
+        g.init_program()
+        v, _ = o.volume(2)
+        o.set_volume(2, v, v * (1 + 1e-2))
+        g.set_volume(2, v, v * (1 + 1e-2))
+        o.step(1, 0)
+        g.step(1, 0)
+        mo, mg = o.scalar(P.S_MAXCHK), g.scalar(P.S_MAXCHK)
+        assert mo != 0.0 and abs(mg - mo) <= 1e-10 * abs(mo), (mg, mo)
+        _compare(o, g, [f for f in FIELDS if f != P.F_LVSET], TOL, "windowed inflation")
+        e = rel_linf(_clip_ls(g.download(P.F_LVSET, 2), p), _clip_ls(o.get(P.F_LVSET, 2), p))
+        assert e <= TOL, e
+    finally:
+        g.close()
+
+
+def test_windows_equal_whole_grid_learned():
+    """MIXED_LEARNED (FP32 fields, learned closure): windowed and whole-grid level
+    sets give the same fields after 9 steps."""
+    from iclsamd import _lib
+    out = []
+    for full in (0, 1):
+        p = _window_drops(mode=P.MODE_MIXED_LEARNED)
+        p.ls_full = full
+        g = _lib.Solver(p)
+        try:
+            g.init_program()
+            for s in range(1, 10):
+                g.step(s, 0)
+            out.append({f: g.download(f) for f in (P.F_UNEW, P.F_VNEW, P.F_WNEW, P.F_PNEW)})
+            out[-1]["ls"] = [_clip_ls(g.download(P.F_LVSET, l), p) for l in range(3)]
+        finally:
+            g.close()
+    for f in (P.F_UNEW, P.F_VNEW, P.F_WNEW, P.F_PNEW):
+        assert rel_linf(out[0][f], out[1][f]) <= 1e-12, P.FIELD_NAMES[f]
+    for l in range(3):
+        assert rel_linf(out[0]["ls"][l], out[1]["ls"][l]) <= 1e-12, l
+
+
 def test_restart_from_fld_matches_oracle(tmp_path):
     """Checkpoint in the reference's fld format (loadd.f90:16-108) and restart
     (init.f90:44-51): the file holds u, v, w, p, each level set and jiemian (interiors,
