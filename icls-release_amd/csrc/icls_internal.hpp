@@ -96,6 +96,8 @@ struct Ctx {
     int win_age = 0;                      // ls_main passes since the windows were set
     int* dext = nullptr;                  // device extents, 6 ints per level set
     int* hext = nullptr;                  // pinned host copy
+    int* dwin = nullptr;                  // device copy of the windows, 6 ints per level set
+    int* hwin = nullptr;                  // pinned staging of it
     int lmax = 1;
     long nfield = 0;       // elements per device 3D field
     bool fp32 = false;     // field arrays hold float (mixed modes)
@@ -228,6 +230,8 @@ __device__ __forceinline__ int xcd_tile(int b, int n) {
 // level set
 int launch_tube(Ctx& c, const double* phi_in, double* phi_out);
 int launch_ls_extent(Ctx& c, const double* phi, int* ext);
+// the launch grid is a level-set window, not the whole grid
+inline bool in_window(const Ctx& c) { return c.g.nx != c.gfull.nx || c.g.ny != c.gfull.ny || c.g.nz != c.gfull.nz; }
 int launch_adv_stage(Ctx& c, int scheme, int stage, const double* phi0, const double* phis, double* out,
                      const double* vu, const double* vv, const double* vw, int velf, const double* flag);
 int launch_reinit_stage(Ctx& c, int stage, const double* phi0, const double* phis, double* out);

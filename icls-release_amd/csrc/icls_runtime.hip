@@ -155,6 +155,8 @@ static void free_ctx(Ctx* c) {
     if (c->hvol) hipHostFree(c->hvol);
     if (c->dext) hipFree(c->dext);
     if (c->hext) hipHostFree(c->hext);
+    if (c->dwin) hipFree(c->dwin);
+    if (c->hwin) hipHostFree(c->hwin);
     if (c->ev_vol) hipEventDestroy(c->ev_vol);
     if (c->stream) hipStreamDestroy(c->stream);
     mlp_free(*c);
@@ -281,6 +283,8 @@ static int build_ctx(const icls_params* prm) {
     if (C.lmax > 1 && !prm->ls_full) {
         HIP_OK(hipMalloc((void**)&C.dext, sizeof(int) * 6 * C.lmax));
         HIP_OK(hipHostMalloc((void**)&C.hext, sizeof(int) * 6 * C.lmax));
+        HIP_OK(hipMalloc((void**)&C.dwin, sizeof(int) * 6 * C.lmax));
+        HIP_OK(hipHostMalloc((void**)&C.hwin, sizeof(int) * 6 * C.lmax));
     }
     // reduction partials: enough for lmax per-block sums of any sweep
     // (the chkdt sweep covers 1..i1 x 1..j1 x 1..k1, one row and column more than the others)
@@ -748,6 +752,14 @@ static int update_windows(Ctx& c) {
         window_axis_z(w.oz + lo[2] - m, w.oz + hi[2] + m, G.nz, n.oz, n.wz);
         w = n;
     }
+    // the table k_materials reads (the copy completes before the next update writes hwin:
+    // the update synchronises the stream first)
+    for (int l = 0; l < L; l++) {
+        const LsWindow& w = c.win[l];
+        int* h = c.hwin + 6 * l;
+        h[0] = w.ox; h[1] = w.oy; h[2] = w.oz; h[3] = w.wx; h[4] = w.wy; h[5] = w.wz;
+    }
+    HIP_OK(hipMemcpyAsync(c.dwin, c.hwin, sizeof(int) * 6 * L, hipMemcpyHostToDevice, c.stream));
     return ICLS_OK;
 }
 

@@ -61,12 +61,13 @@ __global__ void k_tube(Grid g, T NB2, const T* __restrict__ in, T* __restrict__ 
 // reads LDS and every input element is fetched once per block.  Chunks of TZ layers;
 // the first and last chunk also write the ghost layers.  TPD planes are loaded ahead
 // (unconditional loads, clamped to the chunk, as in the re-initialisation kernel).
-constexpr int TZ = 32, TH = 2, TTW = BX + 2 * TH, TTH = BY + 2 * TH, TTP = TTW * TTH;
+// Chunks of TZ_GRID layers on the whole grid, of TZ_WINDOW inside a level-set window.
+constexpr int TZ_GRID = 32, TZ_WINDOW = 8, TH = 2, TTW = BX + 2 * TH, TTH = BY + 2 * TH, TTP = TTW * TTH;
 constexpr int TNLD = (TTP + BX * BY - 1) / (BX * BY), TNS = 2 * TH + 1;
 
 template <class T, int TPD>
 __global__ void __launch_bounds__(BX * BY)
-k_tube_zm(Grid g, T NB2, const T* __restrict__ in, T* __restrict__ out, uint8_t* __restrict__ mask) {
+k_tube_zm(Grid g, int TZ, T NB2, const T* __restrict__ in, T* __restrict__ out, uint8_t* __restrict__ mask) {
     __shared__ T ring[TNS][TTP];
     const int ntx = (g.nx + BX - 1) / BX, nty = (g.ny + BY - 1) / BY, ntz = (g.nz + TZ - 1) / TZ;
     const int tile = xcd_tile(blockIdx.x, ntx * nty * ntz);
@@ -147,13 +148,14 @@ k_tube_zm(Grid g, T NB2, const T* __restrict__ in, T* __restrict__ out, uint8_t*
 template <class T>
 static void tube_dispatch(Ctx& c, const double* phi_in, double* phi_out) {
     dim3 blk(BX, BY, 1);
-    const int ntiles = ((c.g.nx + BX - 1) / BX) * ((c.g.ny + BY - 1) / BY) * ((c.g.nz + TZ - 1) / TZ);
+    const int tz = in_window(c) ? TZ_WINDOW : TZ_GRID;
+    const int ntiles = ((c.g.nx + BX - 1) / BX) * ((c.g.ny + BY - 1) / BY) * ((c.g.nz + tz - 1) / tz);
     // groups of 2 steps when every chunk length is even
-    if (TZ % 2 == 0 && (c.g.nz % TZ) % 2 == 0)
-        hipLaunchKernelGGL((k_tube_zm<T, 2>), dim3(ntiles), blk, 0, c.stream, c.g, (T)c.NB2, as<T>(phi_in),
+    if (tz % 2 == 0 && (c.g.nz % tz) % 2 == 0)
+        hipLaunchKernelGGL((k_tube_zm<T, 2>), dim3(ntiles), blk, 0, c.stream, c.g, tz, (T)c.NB2, as<T>(phi_in),
                            as<T>(phi_out), c.mask);
     else
-        hipLaunchKernelGGL((k_tube_zm<T, 1>), dim3(ntiles), blk, 0, c.stream, c.g, (T)c.NB2, as<T>(phi_in),
+        hipLaunchKernelGGL((k_tube_zm<T, 1>), dim3(ntiles), blk, 0, c.stream, c.g, tz, (T)c.NB2, as<T>(phi_in),
                            as<T>(phi_out), c.mask);
 }
 
@@ -679,9 +681,14 @@ static void reinit_dispatch_pd(Ctx& c, int rz, int stage, const double* phi0, co
 // resident blocks (52 layers) in 0.131 / 0.137 ms: far-field blocks end early (the
 // uniform-window path), so block count matters more than rounds, and 8 chunks put one
 // z slab of tiles on each XCD.
-constexpr int REINIT_RZ = 32;
+// Inside a level-set window (a few tens of thousands of cells) 8-layer chunks: four
+// times the blocks of 32-layer chunks, and a quarter of each block's serial march.
+constexpr int REINIT_RZ = 32, REINIT_RZ_WINDOW = 8;
 template <class T>
-static int reinit_rz(Ctx& c) { return c.g.nz < REINIT_RZ ? c.g.nz : REINIT_RZ; }
+static int reinit_rz(Ctx& c) {
+    const int rz = in_window(c) ? REINIT_RZ_WINDOW : REINIT_RZ;
+    return c.g.nz < rz ? c.g.nz : rz;
+}
 
 // the wall extrapolation folds into the stage when the last chunk has two layers or
 // more (a chunk of one layer would need layer kmax - 1 from another block)

@@ -25,10 +25,26 @@ static inline int red_nblocks(dim3 d) { return (int)(d.x * d.y * d.z); }
 //   H_l  = Heaviside('0','trig',l,1.5 dz),  H = min_l H_l   (jiemian and h_rho)
 //   miu  = miu1 H + miu2 (1 - H),  rho = rho1 H + rho2 (1 - H)
 // and the partial interior sums of every H_l for volume(l) (misc.f90:128-134).
+// With level-set windows (win: ox, oy, oz, wx, wy, wz per level set) a cell outside
+// window l -- and its three ghost layers in z -- takes H_l = 1 without reading phi_l:
+// the level set is in its far field there (phi_l >= NarrowBand_2 > eps), where the
+// Heaviside is exactly 1.
+__device__ __forceinline__ bool in_win(const int* __restrict__ w, int i0, int j0, int k) {
+    return (unsigned)(i0 - w[0]) < (unsigned)w[3] && (unsigned)(j0 - w[1]) < (unsigned)w[4] && k >= w[2] - 2 &&
+           k <= w[2] + w[5] + 3;
+}
+
 template <class T>
-__global__ void k_materials(Grid g, int lmax, const T* const* __restrict__ ls, T eps, T pi, T miu1, T miu2, T rho1,
-                            T rho2, T* __restrict__ miu, T* __restrict__ rho, T* __restrict__ jiemian,
-                            double* __restrict__ part, int nblocks) {
+__device__ __forceinline__ T heav_l(const T* const* __restrict__ ls, const int* __restrict__ win, int l, long q,
+                                    int i0, int j0, int k, T eps, T pi) {
+    if (win && !in_win(win + 6 * l, i0, j0, k)) return T(1.);
+    return heaviside_trig<T>(ls[l][q], eps, pi);
+}
+
+template <class T>
+__global__ void k_materials(Grid g, int lmax, const T* const* __restrict__ ls, const int* __restrict__ win, T eps,
+                            T pi, T miu1, T miu2, T rho1, T rho2, T* __restrict__ miu, T* __restrict__ rho,
+                            T* __restrict__ jiemian, double* __restrict__ part, int nblocks) {
     __shared__ double smem[16];
     int i0 = blockIdx.x * BX + threadIdx.x, j0 = blockIdx.y * BY + threadIdx.y;
     bool valid = i0 < g.nx && j0 < g.ny;
@@ -41,21 +57,85 @@ __global__ void k_materials(Grid g, int lmax, const T* const* __restrict__ ls, T
             long q = gidx(g, i0, j0, k);
             T H = T(0.);
             if (l0 == 0) {
-                H = heaviside_trig<T>(ls[0][q], eps, pi);
-                for (int l = 1; l < lmax; l++) H = fmin(H, heaviside_trig<T>(ls[l][q], eps, pi));
+                H = heav_l<T>(ls, win, 0, q, i0, j0, k, eps, pi);
+                for (int l = 1; l < lmax; l++) H = fmin(H, heav_l<T>(ls, win, l, q, i0, j0, k, eps, pi));
                 miu[q] = miu1 * H + miu2 * (T(1.) - H);
                 rho[q] = rho1 * H + rho2 * (T(1.) - H);
                 if (jiemian) jiemian[q] = H;
             }
             if (k >= 1 && k <= g.nz) {
                 for (int t = 0; t < 4 && l0 + t < lmax; t++)
-                    acc[t] += (double)((l0 == 0 && t == 0 && lmax == 1) ? H : heaviside_trig<T>(ls[l0 + t][q], eps, pi));
+                    acc[t] += (double)((l0 == 0 && t == 0 && lmax == 1) ? H
+                                                                        : heav_l<T>(ls, win, l0 + t, q, i0, j0, k, eps, pi));
             }
         }
         for (int t = 0; t < 4 && l0 + t < lmax; t++) {
             double r = block_reduce(acc[t], OpSum(), smem);
             if (threadIdx.x == 0 && threadIdx.y == 0) part[(long)(l0 + t) * nblocks + block_linear()] = r;
         }
+    }
+}
+
+// The same with level-set windows.  The block first lists (in LDS) the windows its
+// cells can lie in; H is the minimum over those, and only their Heaviside sums are
+// reduced.  Every other level set is 1 on all the block's cells: its partial sum is
+// the block's interior cell count, the value the reduction of those 1s gives exactly.
+// A block that meets more than MAT_LIST windows takes every level set.
+constexpr int MAT_LIST = 128;
+template <class T>
+__global__ void k_materials_win(Grid g, int lmax, const T* const* __restrict__ ls, const int* __restrict__ win,
+                                T eps, T pi, T miu1, T miu2, T rho1, T rho2, T* __restrict__ miu, T* __restrict__ rho,
+                                double* __restrict__ part, int nblocks) {
+    __shared__ double smem[16];
+    __shared__ int list[MAT_LIST];
+    __shared__ int nlist;
+    const int tid = threadIdx.x + BX * threadIdx.y;
+    const int i0 = blockIdx.x * BX + threadIdx.x, j0 = blockIdx.y * BY + threadIdx.y;
+    const bool valid = i0 < g.nx && j0 < g.ny;
+    const int xlo = blockIdx.x * BX, ylo = blockIdx.y * BY, klo = blockIdx.z * KCH;
+    const int xhi = min(xlo + BX, g.nx) - 1, yhi = min(ylo + BY, g.ny) - 1, khi = min(klo + KCH - 1, g.nz + 1);
+    if (tid == 0) nlist = 0;
+    __syncthreads();
+    for (int l = tid; l < lmax; l += BX * BY) {
+        const int* w = win + 6 * l;
+        if (w[0] <= xhi && w[0] + w[3] - 1 >= xlo && w[1] <= yhi && w[1] + w[4] - 1 >= ylo && w[2] - 2 <= khi &&
+            w[2] + w[5] + 3 >= klo) {
+            const int s = atomicAdd(&nlist, 1);
+            if (s < MAT_LIST) list[s] = l;
+        }
+    }
+    __syncthreads();
+    const bool all = nlist > MAT_LIST;
+    const int nl = all ? lmax : nlist;
+    auto lid = [&](int e) { return all ? e : list[e]; };
+    double cnt = 0.;
+    for (int kk = 0; kk < KCH; kk++) {
+        const int k = klo + kk;
+        if (!valid || k > g.nz + 1) continue;
+        const long q = gidx(g, i0, j0, k);
+        T H = T(1.);
+        for (int e = 0; e < nl; e++) H = fmin(H, heav_l<T>(ls, win, lid(e), q, i0, j0, k, eps, pi));
+        miu[q] = miu1 * H + miu2 * (T(1.) - H);
+        rho[q] = rho1 * H + rho2 * (T(1.) - H);
+        if (k >= 1 && k <= g.nz) cnt += 1.;
+    }
+    __shared__ double count_s;
+    const double c0 = block_reduce(cnt, OpSum(), smem);   // valid in thread 0
+    if (tid == 0) count_s = c0;
+    __syncthreads();
+    const double count = count_s;
+    for (int l = tid; l < lmax; l += BX * BY) part[(long)l * nblocks + block_linear()] = count;
+    __syncthreads();   // the listed level sets' partials overwrite the counts below
+    for (int e = 0; e < nl; e++) {
+        const int l = lid(e);
+        double acc = 0.;
+        for (int kk = 0; kk < KCH; kk++) {
+            const int k = klo + kk;
+            if (!valid || k < 1 || k > g.nz) continue;
+            acc += (double)heav_l<T>(ls, win, l, gidx(g, i0, j0, k), i0, j0, k, eps, pi);
+        }
+        const double r = block_reduce(acc, OpSum(), smem);
+        if (tid == 0) part[(long)l * nblocks + block_linear()] = r;
     }
 }
 
@@ -79,13 +159,22 @@ int launch_materials(Ctx& c) {
     int nb = red_nblocks(grd);
     if ((long)nb * c.lmax > c.npart_cap) { set_error("reduction scratch too small"); return ICLS_ERR_STATE; }
     const icls_params& P = c.P;
-    if (c.fp32)
+    if (!c.win.empty()) {
+        if (c.fp32)
+            hipLaunchKernelGGL(k_materials_win<float>, grd, blk, 0, c.stream, c.g, c.lmax, (const float* const*)c.dls,
+                               (const int*)c.dwin, (float)(1.5 * c.g.dz), (float)c.pi, (float)P.miu1, (float)P.miu2,
+                               (float)P.rho1, (float)P.rho2, as<float>(c.miu), as<float>(c.rho), c.dpart, nb);
+        else
+            hipLaunchKernelGGL(k_materials_win<double>, grd, blk, 0, c.stream, c.g, c.lmax, (const double* const*)c.dls,
+                               (const int*)c.dwin, 1.5 * c.g.dz, c.pi, P.miu1, P.miu2, P.rho1, P.rho2, c.miu, c.rho,
+                               c.dpart, nb);
+    } else if (c.fp32)
         hipLaunchKernelGGL(k_materials<float>, grd, blk, 0, c.stream, c.g, c.lmax, (const float* const*)c.dls,
-                           (float)(1.5 * c.g.dz), (float)c.pi, (float)P.miu1, (float)P.miu2, (float)P.rho1,
+                           (const int*)(c.win.empty() ? nullptr : c.dwin), (float)(1.5 * c.g.dz), (float)c.pi, (float)P.miu1, (float)P.miu2, (float)P.rho1,
                            (float)P.rho2, as<float>(c.miu), as<float>(c.rho), (float*)nullptr, c.dpart, nb);
     else
         hipLaunchKernelGGL(k_materials<double>, grd, blk, 0, c.stream, c.g, c.lmax, (const double* const*)c.dls,
-                           1.5 * c.g.dz, c.pi, P.miu1, P.miu2, P.rho1, P.rho2, c.miu, c.rho, (double*)nullptr, c.dpart,
+                           (const int*)(c.win.empty() ? nullptr : c.dwin), 1.5 * c.g.dz, c.pi, P.miu1, P.miu2, P.rho1, P.rho2, c.miu, c.rho, (double*)nullptr, c.dpart,
                            nb);
     HIP_OK(hipGetLastError());
     double* sums = c.dscal + DS_COUNT_FIXED + 2 * c.lmax;   // lmax scratch sums after volume/init_volume

@@ -243,7 +243,9 @@ def test_windowed_inflation_matches_oracle():
 
 def test_windows_equal_whole_grid_learned():
     """MIXED_LEARNED (FP32 fields, learned closure): windowed and whole-grid level
-    sets give the same fields after 9 steps."""
+    sets give the same fields after 9 steps, and the same volumes bit for bit (the
+    windowed materials kernel sums each drop's Heaviside over the blocks its window
+    meets and counts the cells of the others)."""
     from iclsamd import _lib
     out = []
     for full in (0, 1):
@@ -256,12 +258,14 @@ def test_windows_equal_whole_grid_learned():
                 g.step(s, 0)
             out.append({f: g.download(f) for f in (P.F_UNEW, P.F_VNEW, P.F_WNEW, P.F_PNEW)})
             out[-1]["ls"] = [_clip_ls(g.download(P.F_LVSET, l), p) for l in range(3)]
+            out[-1]["vol"] = [g.volume(l) for l in range(3)]
         finally:
             g.close()
     for f in (P.F_UNEW, P.F_VNEW, P.F_WNEW, P.F_PNEW):
         assert rel_linf(out[0][f], out[1][f]) <= 1e-12, P.FIELD_NAMES[f]
     for l in range(3):
         assert rel_linf(out[0]["ls"][l], out[1]["ls"][l]) <= 1e-12, l
+    assert out[0]["vol"] == out[1]["vol"], (out[0]["vol"], out[1]["vol"])
 
 
 def test_restart_from_fld_matches_oracle(tmp_path):
