@@ -16,6 +16,8 @@ src/param.f90:58-60 and src/misc.f90:124,146):
                   updated every ioutchk = 20 steps (``cfl``; the reference keeps dt
                   fixed, so this is an extension).
 * ``drops256`` -- 256^3 box with seeded random non-overlapping static drops, ratios 1:1.
+* ``emulsion512`` -- 512 x 512 x 256 plane Couette flow (walls at -1 and +1) with 200
+                  seeded drops of radius 4 .. 16 cells, ratios 1:1, Ca = 0.2.
 """
 import ctypes
 import math
@@ -205,4 +207,28 @@ def drops256(n=256, ndrops=64, mode=MODE_FP64, seed=20261202):
     return p
 
 
-PRESETS = {"drop64": drop64, "bubble256": bubble256, "drops256": drops256}
+def emulsion512(n=512, nz=256, ndrops=200, mode=MODE_MIXED_LEARNED, seed=20261202, ca=0.2):
+    """Drops in plane Couette flow: n x n x nz cells over 1 x 1 x nz/n, z walls moving at
+    -1 and +1 along y (ns_np = 0, wall_speed = 1, Couette start), seeded drops of radius
+    4 .. 16 cells, density and viscosity ratio 1:1, Re = 10, capillary number
+    Ca = nu gamma R_ref / sigma = ca with gamma = 2 / lz and R_ref = 10 cells, dt from chkdt
+    at CFL 0.3, the normalised mass correction."""
+    p = base_params(n, 1.0, 1.0, nz / n, nz=nz)
+    p.rho1 = p.rho2 = 1.0
+    p.miu1 = p.miu2 = 1.0
+    p.Re = 10.0
+    p.ns_np = 0
+    p.wall_speed = 1.0
+    p.iniu = INIU_COU
+    gamma = 2.0 / p.lz
+    sigma = (1.0 / p.Re) * gamma * (10.0 / n) / ca
+    p.We = 1.0 / sigma
+    p.cfl = 0.3
+    p.ioutchk = 20
+    p.mass_correction = 2
+    p.mode = mode
+    p.set_drops(random_drops(ndrops, n, 4.0, 16.0, seed=seed, lx=1.0, ly=1.0, lz=p.lz))
+    return p
+
+
+PRESETS = {"drop64": drop64, "bubble256": bubble256, "drops256": drops256, "emulsion512": emulsion512}

@@ -1,10 +1,12 @@
-"""Static many-drop study: spurious currents and per-drop volume, mode by mode.
+"""Many-drop study: spurious currents and per-drop volume, mode by mode.
 
     python -m iclsamd.study --n 256 --drops 64 --steps 200 --mass-correction 2
+    python -m iclsamd.study --workload emulsion512 --steps 1000
 
-runs the seeded static-drop workload (drops256: radii uniform in [3, 20] cells,
-density and viscosity ratio 1:1, sigma = 1, fluid at rest) in each precision mode and
-prints one JSON line per mode with
+runs a seeded many-drop workload in each precision mode -- drops256: static drops,
+radii uniform in [3, 20] cells, ratios 1:1, sigma = 1, fluid at rest; emulsion512:
+512 x 512 x 256 plane Couette flow with 200 drops of radius 4 .. 16 cells, Ca = 0.2
+(params.emulsion512) -- and prints one JSON line per mode with
 
 * ms_per_step    -- wall time of the stepping loop per step (dt adaptive, cfl 0.3);
 * ca             -- spurious_currents (misc.f90:802-843), Ca = 0.1 max|u_c| / 1, every
@@ -28,8 +30,11 @@ from iclsamd import _lib, params as P  # noqa: E402
 MODES = {"fp64": P.MODE_FP64, "mixed": P.MODE_MIXED, "learned": P.MODE_MIXED_LEARNED}
 
 
-def run(n, drops, steps, mode, mass_correction, every, seed, cfl, ls_full=0):
-    p = P.drops256(n, drops, mode=MODES[mode], seed=seed)
+def run(n, drops, steps, mode, mass_correction, every, seed, cfl, ls_full=0, workload="drops256", nz=None):
+    if workload == "emulsion512":
+        p = P.emulsion512(n, nz or n // 2, drops, mode=MODES[mode], seed=seed)
+    else:
+        p = P.drops256(n, drops, mode=MODES[mode], seed=seed)
     p.ls_full = ls_full
     p.cfl = cfl
     p.ioutchk = every
@@ -53,7 +58,8 @@ def run(n, drops, steps, mode, mass_correction, every, seed, cfl, ls_full=0):
                 elapsed += time.perf_counter() - t0
         g.sync()
         dv = [abs(v / v0 - 1.0) for v, v0 in (g.volume(l) for l in range(drops))]
-        return {"mode": mode, "n": n, "drops": drops, "steps": steps, "mass_correction": mass_correction,
+        return {"workload": workload, "grid": [p.itot, p.jtot, p.ktot], "mode": mode, "drops": drops,
+                "steps": steps, "mass_correction": mass_correction,
                 "ls_full": ls_full,
                 "ms_per_step": round(elapsed / steps * 1e3, 3), "init_s": round(t_init, 3),
                 "ca": [(s, t, float("%.4e" % c)) for s, t, c in ca],
@@ -65,8 +71,10 @@ def run(n, drops, steps, mode, mass_correction, every, seed, cfl, ls_full=0):
 
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
-    ap.add_argument("--n", type=int, default=256)
-    ap.add_argument("--drops", type=int, default=64)
+    ap.add_argument("--workload", default="drops256", choices=["drops256", "emulsion512"])
+    ap.add_argument("--n", type=int, default=None, help="cells along x and y (256; 512 for emulsion512)")
+    ap.add_argument("--nz", type=int, default=None, help="cells along z (emulsion512: n / 2)")
+    ap.add_argument("--drops", type=int, default=None, help="64; 200 for emulsion512")
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--modes", default="fp64,learned")
     ap.add_argument("--mass-correction", type=int, default=2, choices=[0, 1, 2])
@@ -76,9 +84,12 @@ def main():
     ap.add_argument("--ls-full", type=int, default=0, choices=[0, 1],
                     help="1: every level set over the whole grid instead of a window around its drop")
     a = ap.parse_args()
+    emu = a.workload == "emulsion512"
+    n = a.n or (512 if emu else 256)
+    drops = a.drops or (200 if emu else 64)
     for m in a.modes.split(","):
-        print(json.dumps(run(a.n, a.drops, a.steps, m, a.mass_correction, a.every, a.seed, a.cfl, a.ls_full)),
-              flush=True)
+        print(json.dumps(run(n, drops, a.steps, m, a.mass_correction, a.every, a.seed, a.cfl, a.ls_full,
+                             a.workload, a.nz)), flush=True)
 
 
 if __name__ == "__main__":
