@@ -71,9 +71,11 @@ struct Mlp {
     float* gpart = nullptr;             // MLP_GBLOCKS x (MLP_NPARAM + 1) gradient partials
     float* grad = nullptr;              // reduced gradient (MLP_NPARAM) and loss
     double* pinvA_fine = nullptr;       // least-squares matrix at the 4x finer spacing
-    int* list = nullptr;                // under-resolved interface cells of one step
+    int* list = nullptr;                // under-resolved interface cells of one level set
     int* count = nullptr;
     int list_cap = 0;
+    int* total = nullptr;               // cells listed over all level sets in the last gfm pass
+    std::vector<int*> list_s, count_s;  // one list per side stream (concurrent level-set windows)
     long adam_t = 0;
     bool ready = false;                 // weights set or trained
     float gate = 0.4f;                  // |kappa| dz at which the closure is consulted
@@ -98,6 +100,16 @@ struct Ctx {
     int* hext = nullptr;                  // pinned host copy
     int* dwin = nullptr;                  // device copy of the windows, 6 ints per level set
     int* hwin = nullptr;                  // pinned staging of it
+    // Windows of one colour do not overlap (ghost layers included): their level sets run
+    // at once on the side streams, sharing the stage buffers.
+    std::vector<int> colour;              // per level set
+    int ncolour = 0;
+    std::vector<hipStream_t> lstream;     // side streams (default priority)
+    std::vector<hipEvent_t> lev;          // one per side stream, for the joins
+    hipEvent_t fork_ev = nullptr;
+    int res_slot = 0;                     // dscal slot of the re-init residual (DS_REINIT_RES)
+    int res_slot0 = 0;                    // first of the side streams' residual slots
+    std::vector<double*> stash_s;         // the re-init stash planes of each side stream
     int lmax = 1;
     long nfield = 0;       // elements per device 3D field
     bool fp32 = false;     // field arrays hold float (mixed modes)
@@ -183,6 +195,8 @@ inline const T* as(const double* p) { return reinterpret_cast<const T*>(p); }
 
 // ---- error handling
 void set_error(const char* fmt, ...);
+constexpr int LSTREAMS = 4;               // side streams for concurrent level-set windows
+
 // Layers per block for a z-marching kernel `fn` launched with `threads` threads per block
 // over tiles_xy (x, y) tiles: a multiple of `mult` chosen so that the launch comes as
 // close as possible to whole rounds of resident blocks (occupancy x CUs at once; a
@@ -268,6 +282,7 @@ int mlp_adam(Ctx& c, float lr);                                            // af
 int mlp_samples(Ctx& c, long n, unsigned long long seed, float r_min, float r_max, float gate, uint16_t* x_bf16,
                 float* t, float* fd);
 int launch_mlp_reset_list(Ctx& c);
+int launch_mlp_reset_total(Ctx& c);
 int launch_mlp_curv(Ctx& c, const double* phi, double* kappa);
 // momentum / correction
 int launch_momentum(Ctx& c);

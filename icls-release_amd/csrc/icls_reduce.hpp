@@ -63,6 +63,16 @@ k_reduce_final(const double* __restrict__ part, int n, double init, double* out)
     if (threadIdx.x == 0) out[0] = r;
 }
 
+// Block b folds the n partials at part + b * n into out[b], as k_reduce_final does
+// for one array (same order, same result): one launch for several sums.
+template <class Op>
+__global__ void __launch_bounds__(RED_FINAL_THREADS)
+k_reduce_final_rows(const double* __restrict__ part, int n, double init, double* out) {
+    __shared__ double smem[16];
+    double r = block_reduce(fold_partials<Op>(part + (long)blockIdx.x * n, n, init), Op(), smem);
+    if (threadIdx.x == 0) out[blockIdx.x] = r;
+}
+
 // as k_reduce_final, skipped when the device switch flag[0] is zero
 template <class Op>
 __global__ void __launch_bounds__(RED_FINAL_THREADS)

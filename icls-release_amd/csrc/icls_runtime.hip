@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <string>
 #include <vector>
@@ -157,6 +158,10 @@ static void free_ctx(Ctx* c) {
     if (c->hext) hipHostFree(c->hext);
     if (c->dwin) hipFree(c->dwin);
     if (c->hwin) hipHostFree(c->hwin);
+    for (auto* p : c->stash_s) if (p) hipFree(p);
+    for (auto e : c->lev) if (e) hipEventDestroy(e);
+    if (c->fork_ev) hipEventDestroy(c->fork_ev);
+    for (auto st : c->lstream) if (st) { hipStreamSynchronize(st); hipStreamDestroy(st); }
     if (c->ev_vol) hipEventDestroy(c->ev_vol);
     if (c->stream) hipStreamDestroy(c->stream);
     mlp_free(*c);
@@ -271,7 +276,11 @@ static int build_ctx(const icls_params* prm) {
     HIP_OK(hipMalloc((void**)&C.zplane, sizeof(double) * g.nx * g.ny));
     HIP_OK(hipMemset(C.zplane, 0, sizeof(double) * g.nx * g.ny));
     HIP_OK(hipMalloc((void**)&C.stash, sizeof(double) * 2 * g.sz));
-    const int nscal = DS_COUNT_FIXED + 3 * C.lmax + 16;
+    // volumes, initial volumes and scratch sums per level set, then the side streams'
+    // re-init residual slots
+    const int nscal = DS_COUNT_FIXED + 3 * C.lmax + 16 + LSTREAMS;
+    C.res_slot = DS_REINIT_RES;
+    C.res_slot0 = DS_COUNT_FIXED + 3 * C.lmax + 16;
     HIP_OK(hipMalloc((void**)&C.dscal, sizeof(double) * nscal));
     HIP_OK(hipMemset(C.dscal, 0, sizeof(double) * nscal));
     HIP_OK(hipHostMalloc((void**)&C.hscal, sizeof(double) * nscal));
@@ -285,6 +294,15 @@ static int build_ctx(const icls_params* prm) {
         HIP_OK(hipHostMalloc((void**)&C.hext, sizeof(int) * 6 * C.lmax));
         HIP_OK(hipMalloc((void**)&C.dwin, sizeof(int) * 6 * C.lmax));
         HIP_OK(hipHostMalloc((void**)&C.hwin, sizeof(int) * 6 * C.lmax));
+        C.lstream.assign(LSTREAMS, nullptr);
+        C.lev.assign(LSTREAMS, nullptr);
+        for (int s = 0; s < LSTREAMS; s++) {
+            HIP_OK(hipStreamCreateWithFlags(&C.lstream[s], hipStreamNonBlocking));
+            HIP_OK(hipEventCreateWithFlags(&C.lev[s], hipEventDisableTiming));
+        }
+        HIP_OK(hipEventCreateWithFlags(&C.fork_ev, hipEventDisableTiming));
+        C.stash_s.assign(LSTREAMS, nullptr);
+        for (int s = 0; s < LSTREAMS; s++) HIP_OK(hipMalloc((void**)&C.stash_s[s], sizeof(double) * 2 * g.sz));
     }
     // reduction partials: enough for lmax per-block sums of any sweep
     // (the chkdt sweep covers 1..i1 x 1..j1 x 1..k1, one row and column more than the others)
@@ -652,11 +670,15 @@ struct WindowScope {
     Ctx& c;
     bool on;
     int l;
+    int s;                     // side stream, or -1 for the library stream
     Grid g{};
     double* saved[21];
     double* ls = nullptr;
     uint8_t* mask = nullptr;
     double* stash = nullptr;
+    hipStream_t stream0 = nullptr;
+    int res0 = 0;
+    int *list0 = nullptr, *count0 = nullptr;
     double** fld(int i) {
         double** f[21] = {&c.unew, &c.vnew, &c.wnew, &c.curv, &c.curv2, &c.ucc_l, &c.vcc_l, &c.wcc_l, &c.u_l,
                           &c.v_l, &c.w_l, &c.dheav, &c.normal_m, &c.ls_a, &c.ls_b, &c.ls_c, &c.p_jump, &c.p_x,
@@ -664,7 +686,7 @@ struct WindowScope {
         return f[i];
     }
     static double* shift(double* p, long off, size_t esz) { return (double*)((char*)p + off * (long)esz); }
-    WindowScope(Ctx& c_, int l_) : c(c_), on(!c_.win.empty()), l(l_) {
+    WindowScope(Ctx& c_, int l_, int s_ = -1) : c(c_), on(!c_.win.empty()), l(l_), s(on ? s_ : -1) {
         if (!on) return;
         const LsWindow& w = c.win[l];
         g = c.g;
@@ -673,7 +695,14 @@ struct WindowScope {
         for (int i = 0; i < 21; i++) { saved[i] = *fld(i); *fld(i) = shift(saved[i], off, c.esz); }
         ls = c.lvset[l]; c.lvset[l] = shift(ls, off, c.esz);
         mask = c.mask; c.mask = mask + off;
-        stash = c.stash; c.stash = shift(stash, poff, c.esz);
+        stash = c.stash; c.stash = shift(s >= 0 ? c.stash_s[s] : stash, poff, c.esz);
+        if (s >= 0) {
+            // the side stream, its residual slot and its closure cell list
+            stream0 = c.stream; c.stream = c.lstream[s];
+            res0 = c.res_slot; c.res_slot = c.res_slot0 + s;
+            list0 = c.mlp.list; count0 = c.mlp.count;
+            if (!c.mlp.list_s.empty()) { c.mlp.list = c.mlp.list_s[s]; c.mlp.count = c.mlp.count_s[s]; }
+        }
     }
     ~WindowScope() {
         if (!on) return;
@@ -682,8 +711,54 @@ struct WindowScope {
         c.lvset[l] = ls;
         c.mask = mask;
         c.stash = stash;
+        if (s >= 0) {
+            c.stream = stream0;
+            c.res_slot = res0;
+            c.mlp.list = list0; c.mlp.count = count0;
+        }
     }
 };
+
+// the side streams wait for the library stream's work so far
+static int fork_streams(Ctx& c) {
+    HIP_OK(hipEventRecord(c.fork_ev, c.stream));
+    for (auto st : c.lstream) HIP_OK(hipStreamWaitEvent(st, c.fork_ev, 0));
+    return ICLS_OK;
+}
+
+// the library stream waits for the side streams' work so far
+static int join_streams(Ctx& c) {
+    for (size_t s = 0; s < c.lstream.size(); s++) {
+        HIP_OK(hipEventRecord(c.lev[s], c.lstream[s]));
+        HIP_OK(hipStreamWaitEvent(c.stream, c.lev[s], 0));
+    }
+    return ICLS_OK;
+}
+
+// fn(l) for every level set (except `skip`) inside its window: colour by colour, the
+// level sets of a colour spread over the side streams, the colours one after another.
+// With last_residual the re-init residual of the last level set (the reference reports
+// the last ls_reinit's, levelset.f90:831) is copied to its slot after its colour.
+static int run_colours(Ctx& c, const std::function<int(int)>& fn, bool last_residual, int skip = -1) {
+    for (int col = 0; col < c.ncolour; col++) {
+        TRY(fork_streams(c));
+        int idx = 0, last_s = -1;
+        for (int l = 0; l < c.lmax; l++) {
+            if (c.colour[l] != col || l == skip) continue;
+            const int s = idx++ % (int)c.lstream.size();
+            {
+                WindowScope ws(c, l, s);
+                TRY(fn(l));
+            }
+            if (l == c.lmax - 1) last_s = s;
+        }
+        TRY(join_streams(c));
+        if (last_residual && last_s >= 0)
+            HIP_OK(hipMemcpyAsync(c.dscal + DS_REINIT_RES, c.dscal + c.res_slot0 + last_s, sizeof(double),
+                                  hipMemcpyDeviceToDevice, c.stream));
+    }
+    return ICLS_OK;
+}
 
 // smallest power of two >= v
 static int pow2_at_least(int v) { int p = 1; while (p < v) p <<= 1; return p; }
@@ -752,6 +827,24 @@ static int update_windows(Ctx& c) {
         window_axis_z(w.oz + lo[2] - m, w.oz + hi[2] + m, G.nz, n.oz, n.wz);
         w = n;
     }
+    // colours: windows of one colour share no cell they read or write -- x and y inside
+    // the window, z from 2 layers below to 3 above it (the ghost layers)
+    auto meets = [](const LsWindow& a, const LsWindow& b) {
+        return a.ox < b.ox + b.wx && b.ox < a.ox + a.wx && a.oy < b.oy + b.wy && b.oy < a.oy + a.wy &&
+               a.oz - 2 <= b.oz + b.wz + 3 && b.oz - 2 <= a.oz + a.wz + 3;
+    };
+    c.colour.assign(L, 0);
+    c.ncolour = 0;
+    std::vector<char> used;
+    for (int l = 0; l < L; l++) {
+        used.assign(c.ncolour + 1, 0);
+        for (int m = 0; m < l; m++)
+            if (meets(c.win[l], c.win[m])) used[c.colour[m]] = 1;
+        int k = 0;
+        while (used[k]) k++;
+        c.colour[l] = k;
+        if (k + 1 > c.ncolour) c.ncolour = k + 1;
+    }
     // the table k_materials reads (the copy completes before the next update writes hwin:
     // the update synchronises the stream first)
     for (int l = 0; l < L; l++) {
@@ -779,38 +872,58 @@ static int ls_main_impl(Ctx& c, int istep) {
         TRY(update_windows(c));
         c.win_age = 0;
     }
-    for (int l = 0; l < c.lmax; l++) {
-        WindowScope ws(c, l);
+    const int sch = c.P.ls_adv_space;
+    // tube_construction (levelset.f90:87-197) and ls_advection, SSP-RK3
+    // (levelset.f90:235-243 / 268-276), cell-centre velocity averaged from
+    // unew/vnew/wnew inside the stencil (misc.f90:595-597); every level-set kernel
+    // folds the boundsls that follows it (bound.f90:404-411) into its own launch
+    auto advect = [&](int l) -> int {
         double* ph = c.lvset[l];
-        // tube_construction (levelset.f90:87-197); every level-set kernel below folds
-        // the boundsls that follows it (bound.f90:404-411) into its own launch
         TRY(launch_tube(c, ph, c.ls_a));
-        // ls_advection, SSP-RK3 (levelset.f90:235-243 / 268-276), cell-centre velocity
-        // averaged from unew/vnew/wnew inside the stencil (misc.f90:595-597)
-        const int sch = c.P.ls_adv_space;
         TRY(launch_adv_stage(c, sch, 1, c.ls_a, c.ls_a, c.ls_b, c.unew, c.vnew, c.wnew, 1, nullptr));
         TRY(launch_adv_stage(c, sch, 2, c.ls_a, c.ls_b, c.ls_c, c.unew, c.vnew, c.wnew, 1, nullptr));
-        TRY(launch_adv_stage(c, sch, 3, c.ls_a, c.ls_c, ph, c.unew, c.vnew, c.wnew, 1, nullptr));
-        // ls_mass_correction (levelset.f90:377-582)
-        if (c.P.mass_correction) {
-            bool due = false;
-            TRY(mass_correction_due(c, l, istep, due));
-            TRY(launch_inflation_prepare(c, l, istep));   // maxchk_glb = 0 and the device flag
-            if (due) {
-                const double* flag = c.dscal + DS_INFL_FLAG;
-                TRY(launch_curvature(c, ph, c.curv2, c.normal_m, flag));
-                TRY(launch_inflation_velocity(c, l));
-                TRY(launch_adv_stage(c, ICLS_ADV_HOUC5, 1, ph, ph, c.ls_a, c.ucc_l, c.vcc_l, c.wcc_l, 0, flag));
-                TRY(launch_adv_stage(c, ICLS_ADV_HOUC5, 2, ph, c.ls_a, c.ls_b, c.ucc_l, c.vcc_l, c.wcc_l, 0, flag));
-                TRY(launch_adv_stage(c, ICLS_ADV_HOUC5, 3, ph, c.ls_b, ph, c.ucc_l, c.vcc_l, c.wcc_l, 0, flag));
-            }
+        return launch_adv_stage(c, sch, 3, c.ls_a, c.ls_c, ph, c.unew, c.vnew, c.wnew, 1, nullptr);
+    };
+    // ls_mass_correction (levelset.f90:377-582)
+    auto correct = [&](int l) -> int {
+        if (!c.P.mass_correction) return ICLS_OK;
+        double* ph = c.lvset[l];
+        bool due = false;
+        TRY(mass_correction_due(c, l, istep, due));
+        // maxchk_glb = 0 and the device flag; when the correction is not due the call
+        // only resets maxchk, which the last level set's call leaves as the value
+        if (due || l == c.lmax - 1) TRY(launch_inflation_prepare(c, l, istep));
+        if (!due) return ICLS_OK;
+        const double* flag = c.dscal + DS_INFL_FLAG;
+        TRY(launch_curvature(c, ph, c.curv2, c.normal_m, flag));
+        TRY(launch_inflation_velocity(c, l));
+        TRY(launch_adv_stage(c, ICLS_ADV_HOUC5, 1, ph, ph, c.ls_a, c.ucc_l, c.vcc_l, c.wcc_l, 0, flag));
+        TRY(launch_adv_stage(c, ICLS_ADV_HOUC5, 2, ph, c.ls_a, c.ls_b, c.ucc_l, c.vcc_l, c.wcc_l, 0, flag));
+        return launch_adv_stage(c, ICLS_ADV_HOUC5, 3, ph, c.ls_b, ph, c.ucc_l, c.vcc_l, c.wcc_l, 0, flag);
+    };
+    // ls_reinit_main, WENO5 + RK3 (levelset.f90:592-882)
+    const bool reinit = c.P.re_initialization && istep % c.P.reinit_step == 0;
+    auto reinitialise = [&](int l) -> int {
+        double* ph = c.lvset[l];
+        TRY(launch_reinit_stage(c, 1, ph, ph, c.ls_a));
+        TRY(launch_reinit_stage(c, 2, ph, c.ls_a, c.ls_b));
+        return launch_reinit_stage(c, 3, ph, c.ls_b, ph);    // in place: lvset = phi_new
+    };
+    if (c.win.empty()) {
+        for (int l = 0; l < c.lmax; l++) {
+            TRY(advect(l));
+            TRY(correct(l));
+            if (reinit) TRY(reinitialise(l));
         }
-        // ls_reinit_main, WENO5 + RK3 (levelset.f90:592-882)
-        if (c.P.re_initialization && istep % c.P.reinit_step == 0) {
-            TRY(launch_reinit_stage(c, 1, ph, ph, c.ls_a));
-            TRY(launch_reinit_stage(c, 2, ph, c.ls_a, c.ls_b));
-            TRY(launch_reinit_stage(c, 3, ph, c.ls_b, ph));    // in place: lvset = phi_new
+    } else {
+        // the level sets are independent here: advection and re-initialisation run
+        // colour by colour on the side streams, the mass corrections one at a time
+        TRY(run_colours(c, advect, false));
+        for (int l = 0; l < c.lmax; l++) {
+            WindowScope ws(c, l);
+            TRY(correct(l));
         }
+        if (reinit) TRY(run_colours(c, reinitialise, true));
     }
     // update_materials (misc.f90:79-290)
     TRY(launch_materials(c));
@@ -857,24 +970,32 @@ static int gfm_impl(Ctx& c, bool fuse_fillps, bool store_pjump) {
     rotate_jumps(c);
     c.p_in_pnew = false;   // fillps writes p
     const bool learned = c.learned && c.mlp.ready;
-    for (int l = 0; l < c.lmax; l++) {
-        {
-            // inside the level set's window; the first level set's jumps are written
-            // over the whole grid (they clear the previous step's), the others added
-            // inside their windows
-            WindowScope ws(c, l);
-            if (learned) {
-                // get_curvature with the learned closure on the under-resolved cells
-                TRY(launch_mlp_reset_list(c));
-                TRY(launch_curvature(c, c.lvset[l], c.curv, nullptr, nullptr, true));
-                TRY(launch_mlp_curv(c, c.lvset[l], c.curv));
-            } else {
-                TRY(launch_curvature(c, c.lvset[l], c.curv, nullptr, nullptr));
-            }
-            TRY(launch_recurvature(c, c.lvset[l], c.curv, c.curv2));
-            if (l > 0) TRY(launch_gfm(c, l, c.curv2, false, true));
+    if (learned) TRY(launch_mlp_reset_total(c));
+    // get_curvature (with the learned closure on the under-resolved cells), re_curvature,
+    // and for the level sets after the first their jumps added inside their windows
+    auto curvature = [&](int l) -> int {
+        if (learned) {
+            TRY(launch_mlp_reset_list(c));
+            TRY(launch_curvature(c, c.lvset[l], c.curv, nullptr, nullptr, true));
+            TRY(launch_mlp_curv(c, c.lvset[l], c.curv));
+        } else {
+            TRY(launch_curvature(c, c.lvset[l], c.curv, nullptr, nullptr));
         }
-        if (l == 0) TRY(launch_gfm(c, 0, c.curv2, fuse_fillps && c.lmax == 1, store_pjump || c.lmax > 1));
+        TRY(launch_recurvature(c, c.lvset[l], c.curv, c.curv2));
+        return l > 0 ? launch_gfm(c, l, c.curv2, false, true) : ICLS_OK;
+    };
+    // the first level set's jumps are written over the whole grid (they clear the
+    // previous step's)
+    {
+        WindowScope ws(c, 0);
+        TRY(curvature(0));
+    }
+    TRY(launch_gfm(c, 0, c.curv2, fuse_fillps && c.lmax == 1, store_pjump || c.lmax > 1));
+    if (c.win.empty()) {
+        for (int l = 1; l < c.lmax; l++) TRY(curvature(l));
+    } else {
+        // windows of one colour do not overlap: their jumps are added at once
+        TRY(run_colours(c, curvature, false, 0));
     }
     if (fuse_fillps && c.lmax > 1) {
         TRY(launch_fillps(c));
@@ -1520,9 +1641,10 @@ int32_t icls_mlp_listed(int32_t* n) {
     CTX_OR_FAIL();
     if (!n) return ICLS_ERR_PARAM;
     int v = 0;
-    HIP_OK(hipMemcpyAsync(&v, c.mlp.count, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    if (!c.mlp.total) { *n = 0; return ICLS_OK; }
+    HIP_OK(hipMemcpyAsync(&v, c.mlp.total, sizeof(int), hipMemcpyDeviceToHost, c.stream));
     HIP_OK(hipStreamSynchronize(c.stream));
-    *n = v < c.mlp.list_cap ? v : c.mlp.list_cap;
+    *n = v;
     return ICLS_OK;
 }
 

@@ -663,7 +663,7 @@ template <class T, int RPD, bool WALL>
 static void reinit_dispatch_pd(Ctx& c, int rz, int stage, const double* phi0, const double* phis, double* out) {
     dim3 blk(BX, BY, 1);
     const int ntiles = ((c.g.nx + BX - 1) / BX) * ((c.g.ny + RTY - 1) / RTY) * ((c.g.nz + rz - 1) / rz);
-    double* res = c.dscal + DS_REINIT_RES;
+    double* res = c.dscal + c.res_slot;
 #define RS(S, P)                                                                                                  \
     hipLaunchKernelGGL((k_reinit_zm<T, S, P, RPD, WALL>), dim3(ntiles), blk, 0, c.stream, c.g, rz, (T)c.NB2, res, \
                        as<T>(phi0), as<T>(phis), as<T>(out), as<T>(c.stash))
@@ -721,7 +721,7 @@ int launch_reinit_stage(Ctx& c, int stage, const double* phi0, const double* phi
         HIP_OK(hipGetLastError());
     }
     if (reinit_fuses_wall(c)) return ICLS_OK;
-    if (stage == 3) return bc_ls(c, out, c.stash, c.dscal + DS_REINIT_RES, nullptr);
+    if (stage == 3) return bc_ls(c, out, c.stash, c.dscal + c.res_slot, nullptr);
     return bc_ls(c, out, nullptr, nullptr, nullptr);
 }
 

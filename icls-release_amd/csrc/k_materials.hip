@@ -178,9 +178,8 @@ int launch_materials(Ctx& c) {
                            nb);
     HIP_OK(hipGetLastError());
     double* sums = c.dscal + DS_COUNT_FIXED + 2 * c.lmax;   // lmax scratch sums after volume/init_volume
-    for (int l = 0; l < c.lmax; l++)
-        hipLaunchKernelGGL(k_reduce_final<OpSum>, dim3(1), dim3(RED_FINAL_THREADS), 0, c.stream, c.dpart + (long)l * nb, nb, 0.,
-                           sums + l);
+    hipLaunchKernelGGL(k_reduce_final_rows<OpSum>, dim3(c.lmax), dim3(RED_FINAL_THREADS), 0, c.stream,
+                       (const double*)c.dpart, nb, 0., sums);
     double ncell = 1. * c.g.nx * c.g.ny * c.g.nz;
     hipLaunchKernelGGL(k_volume_final, dim3(1), dim3(64), 0, c.stream, c.lmax, ncell, P.rho1, P.rho2,
                        (const double*)sums, c.dscal);

@@ -391,13 +391,14 @@ template <class T>
 __global__ void __launch_bounds__(64)
 k_mlp_curv(Grid g, const int* __restrict__ list, const int* __restrict__ count, int cap, const T* __restrict__ A,
            const T* __restrict__ phi, T* __restrict__ kappa, const uint16_t* __restrict__ w1b,
-           const uint16_t* __restrict__ w2b, const float* __restrict__ w, float tol) {
+           const uint16_t* __restrict__ w2b, const float* __restrict__ w, float tol, int* __restrict__ total) {
     __shared__ __align__(16) uint16_t xs[16 * MLP_NPAD];
     __shared__ __align__(16) uint16_t hs[16 * MLP_NH];
     __shared__ long qs[16];
     const int l = threadIdx.x, lg = l >> 4, lr = l & 15;
     const int cnt = min(count[0], cap);
     const double dz = g.dz;
+    if (blockIdx.x == 0 && l == 0) atomicAdd(total, cnt);   // the step's listed cells, all level sets
     for (int grp = blockIdx.x; grp * 16 < cnt; grp += gridDim.x) {
         // lanes 0..15 build the feature rows of the group's cells
         if (l < 16) {
@@ -479,6 +480,16 @@ int mlp_alloc(Ctx& c) {
     HIP_OK(hipMalloc((void**)&m.list, sizeof(int) * m.list_cap));
     HIP_OK(hipMalloc((void**)&m.count, sizeof(int)));
     HIP_OK(hipMemset(m.count, 0, sizeof(int)));
+    HIP_OK(hipMalloc((void**)&m.total, sizeof(int)));
+    HIP_OK(hipMemset(m.total, 0, sizeof(int)));
+    for (size_t s = 0; s < c.lstream.size(); s++) {   // concurrent level-set windows
+        int *l = nullptr, *n = nullptr;
+        HIP_OK(hipMalloc((void**)&l, sizeof(int) * m.list_cap));
+        HIP_OK(hipMalloc((void**)&n, sizeof(int)));
+        HIP_OK(hipMemset(n, 0, sizeof(int)));
+        m.list_s.push_back(l);
+        m.count_s.push_back(n);
+    }
     // deterministic initial weights: uniform(-1, 1) / sqrt(fan_in), zero biases and
     // zero padding columns
     std::vector<float> w(MLP_NPARAM, 0.f);
@@ -499,8 +510,11 @@ int mlp_alloc(Ctx& c) {
 
 void mlp_free(Ctx& c) {
     Mlp& m = c.mlp;
-    void* ptrs[] = {m.w, m.adam_m, m.adam_v, m.w1b, m.w2b, m.w2tb, m.gpart, m.grad, m.pinvA_fine, m.list, m.count};
+    void* ptrs[] = {m.w, m.adam_m, m.adam_v, m.w1b, m.w2b, m.w2tb, m.gpart, m.grad, m.pinvA_fine, m.list, m.count,
+                    m.total};
     for (void* p : ptrs) if (p) hipFree(p);
+    for (int* p : m.list_s) if (p) hipFree(p);
+    for (int* p : m.count_s) if (p) hipFree(p);
     m = Mlp();
 }
 
@@ -580,11 +594,17 @@ int launch_mlp_curv(Ctx& c, const double* phi, double* kappa) {
     if (c.fp32)
         hipLaunchKernelGGL(k_mlp_curv<float>, dim3(nb), dim3(64), 0, c.stream, c.g, (const int*)m.list,
                            (const int*)m.count, m.list_cap, as<float>(c.pinvA_f), as<float>(phi), as<float>(kappa),
-                           (const uint16_t*)m.w1b, (const uint16_t*)m.w2b, (const float*)m.w, m.tol);
+                           (const uint16_t*)m.w1b, (const uint16_t*)m.w2b, (const float*)m.w, m.tol, m.total);
     else
         hipLaunchKernelGGL(k_mlp_curv<double>, dim3(nb), dim3(64), 0, c.stream, c.g, (const int*)m.list,
                            (const int*)m.count, m.list_cap, (const double*)c.pinvA, phi, kappa,
-                           (const uint16_t*)m.w1b, (const uint16_t*)m.w2b, (const float*)m.w, m.tol);
+                           (const uint16_t*)m.w1b, (const uint16_t*)m.w2b, (const float*)m.w, m.tol, m.total);
+    HIP_OK(hipGetLastError());
+    return ICLS_OK;
+}
+
+int launch_mlp_reset_total(Ctx& c) {
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, c.stream, c.mlp.total);
     HIP_OK(hipGetLastError());
     return ICLS_OK;
 }

@@ -273,7 +273,8 @@ int32_t icls_mlp_gradient(const uint16_t* x_bf16, const float* t, int32_t n, flo
 /* The training generator's samples: features, targets (the correction times dz) and,
  * if fd is not NULL, the coarse least-squares kappa * dz. */
 int32_t icls_mlp_samples(int64_t n, uint64_t seed, float r_min, float r_max, uint16_t* x_bf16, float* t, float* fd);
-/* Cells the closure was consulted on in the last step (capped at the list size). */
+/* Cells the closure was consulted on in the last gfm4p pass, summed over the level sets
+ * (each level set's list capped at the list size). */
 int32_t icls_mlp_listed(int32_t* n);
 
 #ifdef __cplusplus
