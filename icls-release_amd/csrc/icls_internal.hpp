@@ -178,6 +178,7 @@ struct Ctx {
     double* xyrt = nullptr;               // nx * ny eigenvalues
     double* tw_x = nullptr;               // FFT twiddles along x (nx/2 complex)
     double* tw_y = nullptr;               // along y
+    double* cos_tab = nullptr;            // solver2d_cos: z transform matrices, xrt, zrt (on first use)
     double* pinvA_f = nullptr;            // float copies (raw float arrays)
     double* tw_x_f = nullptr;
     double* tw_y_f = nullptr;
@@ -267,6 +268,8 @@ int launch_fillps(Ctx& c);
 int launch_add_pjump(Ctx& c);
 // poisson: solve L p = frhs into p (solver2d; in the mixed modes with one refinement)
 int launch_solve(Ctx& c);
+// solver2d_cos: p -> solution with y rule 1 Walls, 2 InOut, 3 OutOut, in double in every mode
+int launch_solve_cos(Ctx& c, int bcy);
 int launch_pressure_materialize(Ctx& c);
 int launch_jump_densify(Ctx& c, bool old);   // zero the unmasked jumps, mask every cell
 int launch_pressure_release_p(Ctx& c);   // the work array p takes the raw solution back

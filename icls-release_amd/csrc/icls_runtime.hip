@@ -144,7 +144,7 @@ static void free_ctx(Ctx* c) {
                                c->p_yold, c->p_zold, c->miu, c->rho, c->curv, c->curv2, c->ucc, c->vcc, c->wcc,
                                c->u_l, c->v_l, c->w_l, c->dheav, c->ucc_l, c->vcc_l, c->wcc_l, c->normal_m,
                                c->ls_a, c->ls_b, c->ls_c, c->mask, c->thomas_d, c->dscal, c->dpart, c->pinvA,
-                               c->xyrt, c->tw_x, c->tw_y, c->dls, c->stash, c->scratch, c->pf, c->thomas_y,
+                               c->xyrt, c->tw_x, c->tw_y, c->cos_tab, c->dls, c->stash, c->scratch, c->pf, c->thomas_y,
                                c->thomas_seed, c->pinvA_f, c->tw_x_f, c->tw_y_f};
     if (c->frhs != c->p) ptrs.push_back(c->frhs);   // in FP64 the right-hand side is p itself
     ptrs.push_back(c->jmask);
@@ -1023,6 +1023,16 @@ static int solver_impl(Ctx& c) {
 int32_t icls_solver2d(void) {
     CTX_OR_FAIL();
     return solver_impl(c);
+}
+
+int32_t icls_solver2d_cos(int32_t bc_y) {
+    CTX_OR_FAIL();
+    if (bc_y < ICLS_BCY_WALLS || bc_y > ICLS_BCY_OUTOUT) {
+        set_error("solver2d_cos: bc_y must be 1 (Walls), 2 (InOut) or 3 (OutOut)");
+        return ICLS_ERR_PARAM;
+    }
+    c.p_in_pnew = false;
+    return launch_solve_cos(c, bc_y);
 }
 
 int32_t icls_boundp(int32_t field) {

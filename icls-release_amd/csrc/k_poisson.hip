@@ -14,6 +14,7 @@
 // iterative refinement follows: r = rhs - L p in double (L is the discrete operator
 // the transforms diagonalise), the same solve on r, p += the correction.
 #include <type_traits>
+#include <vector>
 
 #include "icls_internal.hpp"
 
@@ -1024,5 +1025,222 @@ static int solve_mixed(Ctx& c) {
 }
 
 int launch_solve(Ctx& c) { return c.fp32 ? solve_mixed(c) : solve_fp64(c); }
+
+// ---------------------------------------------------------------- solver2d_cos
+// solver.f90:202-317 with initsolver.f90:84-160: real FFT along x, the cosine transform
+// VCOSQB along z, Gauss elimination along y with the boundary rule of BC_in_y,
+// VCOSQF along z, inverse FFT along x.  On one GPU the transposes are identities.
+//
+// Both z transforms are dense products over the nz layers of every (i, j) column,
+// out(col, kk) = sum_k in(col, k) M(k, kk), with M the matrix of the transform (fft.f:
+// 116-120, 266-270; VFFTPACK's fast algorithm computes the same sums in another order).
+// A block computes 64 columns x 64 outputs, staging 16 layers of the input and of M in
+// LDS per pass; each thread holds 4 x 4 sums (columns tx + 16 r, outputs ty + 16 s).
+constexpr int ZT_COLS = 64, ZT_OUT = 64, ZT_K = 16;
+
+__global__ void __launch_bounds__(256) k_ztransform(Grid g, const double* __restrict__ M,
+                                                    const double* __restrict__ in, double* __restrict__ out) {
+    __shared__ double As[ZT_K][ZT_COLS];
+    __shared__ double Bs[ZT_K][ZT_OUT];
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    const long ncol = (long)g.nx * g.ny;
+    const long col0 = (long)blockIdx.x * ZT_COLS;
+    const int kk0 = blockIdx.y * ZT_OUT;
+    const int nz = g.nz;
+    const double* in0 = in + g.sz * 3;   // layer k = 1
+    double acc[4][4] = {};
+    for (int kb = 0; kb < nz; kb += ZT_K) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int e = threadIdx.x + 256 * q, kq = e >> 6, cc = e & 63;
+            const long col = col0 + cc;
+            As[kq][cc] = (kb + kq < nz && col < ncol) ? in0[col + g.sz * (kb + kq)] : 0.;
+            Bs[kq][cc] = (kb + kq < nz && kk0 + cc < nz) ? M[(long)(kb + kq) * nz + kk0 + cc] : 0.;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kq = 0; kq < ZT_K; kq++) {
+            double av[4], bv[4];
+#pragma unroll
+            for (int r = 0; r < 4; r++) av[r] = As[kq][tx + 16 * r];
+#pragma unroll
+            for (int s = 0; s < 4; s++) bv[s] = Bs[kq][ty + 16 * s];
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+#pragma unroll
+                for (int s = 0; s < 4; s++) acc[r][s] = fma(av[r], bv[s], acc[r][s]);
+        }
+        __syncthreads();
+    }
+    double* out0 = out + g.sz * 3;
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+        const int kk = kk0 + ty + 16 * s;
+        if (kk >= nz) continue;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const long col = col0 + tx + 16 * r;
+            if (col < ncol) out0[col + g.sz * kk] = acc[r][s];
+        }
+    }
+}
+
+// Gauss elimination along y per (i, k) line (solver.f90:253-290): a = c = dyi^2,
+// b = -(a + c), the end rows changed by BC_in_y (initsolver.f90:88-124; 1 Walls:
+// b(1) += a(1), b(jtot) += c(jtot); 2 InOut: b(1) += a(1), b(jtot) -= c(jtot);
+// 3 OutOut: b(1) -= a(1), b(jtot) -= c(jtot)), then a(1) = c(jtot) = 0, the diagonal
+// shifted by xrt(i) + zrt(k); a zero last pivot gives p(jtot) = 0.  The sweep runs in
+// place on q (the transformed right-hand side); the multipliers go to dst.
+__global__ void __launch_bounds__(64) k_thomas_ycos(Grid g, int bcy, const double* __restrict__ xrt,
+                                                    const double* __restrict__ zrt, double* q,
+                                                    double* __restrict__ dst) {
+    const int i0 = blockIdx.x * 64 + threadIdx.x, k = blockIdx.y + 1;
+    if (i0 >= g.nx) return;
+    const int jmax = g.ny;
+    const double a = g.dyi * g.dyi, cc = g.dyi * g.dyi;
+    const double bmid = -(a + cc);
+    const double b1 = bcy == 3 ? bmid - a : bmid + a;
+    const double bN = bcy == 1 ? bmid + cc : bmid - cc;
+    const double xz = xrt[i0 + 1] + zrt[k];
+    const long sy = g.sy;
+    const long q1 = gidx(g, i0, 0, k);
+    constexpr int CH = 32;
+    double dprev = 0., pprev = 0.;
+    auto fwd = [&](int j, double rj) {
+        double z;
+        if (j == 1) {
+            z = 1. / (b1 + xz);
+            dprev = cc * z;
+            pprev = rj * z;
+        } else if (j < jmax) {
+            double bb = bmid + xz;
+            z = 1. / (bb - a * dprev);
+            dprev = cc * z;
+            pprev = (rj - a * pprev) * z;
+        } else {
+            double bb = bN + xz;
+            z = bb - a * dprev;
+            if (z != 0.) pprev = (rj - a * pprev) / z;
+            else pprev = 0.;
+        }
+    };
+    int jb = 1;
+    for (; jb + CH - 1 <= jmax; jb += CH) {
+        double buf[CH], dbuf[CH];
+#pragma unroll
+        for (int t = 0; t < CH; t++) buf[t] = q[q1 + (long)(jb + t - 1) * sy];
+#pragma unroll
+        for (int t = 0; t < CH; t++) {
+            fwd(jb + t, buf[t]);
+            buf[t] = pprev;
+            dbuf[t] = dprev;
+        }
+#pragma unroll
+        for (int t = 0; t < CH; t++) {
+            const long o = q1 + (long)(jb + t - 1) * sy;
+            q[o] = buf[t];
+            dst[o] = dbuf[t];
+        }
+    }
+    for (; jb <= jmax; jb++) {
+        const long o = q1 + (long)(jb - 1) * sy;
+        fwd(jb, q[o]);
+        q[o] = pprev;
+        dst[o] = dprev;
+    }
+    // back substitution from j = jmax-1 down to 1; pprev holds p(jmax)
+    int jt = jmax - 1;
+    for (; jt - CH + 1 >= 1; jt -= CH) {
+        double buf[CH], dbuf[CH];
+#pragma unroll
+        for (int t = 0; t < CH; t++) {
+            const long o = q1 + (long)(jt - t - 1) * sy;
+            buf[t] = q[o];
+            dbuf[t] = dst[o];
+        }
+#pragma unroll
+        for (int t = 0; t < CH; t++) {
+            pprev = buf[t] - dbuf[t] * pprev;
+            buf[t] = pprev;
+        }
+#pragma unroll
+        for (int t = 0; t < CH; t++) q[q1 + (long)(jt - t - 1) * sy] = buf[t];
+    }
+    for (; jt >= 1; jt--) {
+        const long o = q1 + (long)(jt - 1) * sy;
+        pprev = q[o] - dst[o] * pprev;
+        q[o] = pprev;
+    }
+}
+
+// tables of solver2d_cos, built on the host once per context: the VCOSQB and VCOSQF
+// matrices (row = input layer, contiguous outputs) and xrt(1..nx), zrt(1..nz)
+// (initsolver.f90:132-145)
+static int cos_tables(Ctx& c) {
+    if (c.cos_tab) return ICLS_OK;
+    const int nx = c.g.nx, nz = c.g.nz;
+    const double pi = c.pi, s = sqrt(4. * nz);
+    std::vector<double> t((size_t)2 * nz * nz + (nx + 1) + (nz + 1), 0.);
+    double* mb = t.data();
+    double* mf = mb + (size_t)nz * nz;
+    double* xrt = mf + (size_t)nz * nz;
+    double* zrt = xrt + nx + 1;
+    for (int k = 1; k <= nz; k++)
+        for (int i = 1; i <= nz; i++) {
+            mb[(size_t)(k - 1) * nz + (i - 1)] = 4. * cos((double)((2 * k - 1) * (i - 1)) * pi / (2. * nz)) / s;
+            mf[(size_t)(k - 1) * nz + (i - 1)] =
+                (k == 1 ? 1. : 2. * cos((double)((2 * i - 1) * (k - 1)) * pi / (2. * nz))) / s;
+        }
+    for (int i = 3; i <= nx; i += 2) {
+        double sn = sin((double)(i - 1) * pi / (2. * nx));
+        xrt[i - 1] = -4. * c.g.dxi * c.g.dxi * (sn * sn);
+        xrt[i] = xrt[i - 1];
+    }
+    xrt[1] = 0.;
+    xrt[nx] = -4. * c.g.dxi * c.g.dxi;
+    for (int k = 1; k <= nz; k++) zrt[k] = (c.g.dzi * c.g.dzi) * (-2. + 2. * cos((double)(k - 1) * pi / nz));
+    HIP_OK(hipMalloc((void**)&c.cos_tab, sizeof(double) * t.size()));
+    HIP_OK(hipMemcpy(c.cos_tab, t.data(), sizeof(double) * t.size(), hipMemcpyHostToDevice));
+    return ICLS_OK;
+}
+
+int launch_solve_cos(Ctx& c, int bcy) {
+    if (bcy < 1 || bcy > 3) return ICLS_ERR_PARAM;
+    if (c.g.ny < 2) return ICLS_ERR_PARAM;
+    int rc = cos_tables(c);
+    if (rc) return rc;
+    const int nz = c.g.nz;
+    const double* mb = c.cos_tab;
+    const double* mf = mb + (size_t)nz * nz;
+    const double* xrt = mf + (size_t)nz * nz;
+    const double* zrt = xrt + c.g.nx + 1;
+    double* p = c.p;
+    double* w = c.scratch;
+    {
+        ProfScope ps(c, "k_rfft_x_fwd");
+        rc = rfft_launch<0, 0, double, double, 0>(c, c.tw_x, p, p);
+        if (rc) return rc;
+    }
+    const long ncol = (long)c.g.nx * c.g.ny;
+    dim3 zg((unsigned)((ncol + ZT_COLS - 1) / ZT_COLS), (unsigned)((nz + ZT_OUT - 1) / ZT_OUT));
+    {
+        ProfScope ps(c, "k_ztransform_fwd");
+        hipLaunchKernelGGL(k_ztransform, zg, dim3(256), 0, c.stream, c.g, mb, (const double*)p, w);
+        HIP_OK(hipGetLastError());
+    }
+    {
+        ProfScope ps(c, "k_thomas_ycos");
+        hipLaunchKernelGGL(k_thomas_ycos, dim3((c.g.nx + 63) / 64, nz), dim3(64), 0, c.stream, c.g, bcy, xrt, zrt,
+                           w, p);
+        HIP_OK(hipGetLastError());
+    }
+    {
+        ProfScope ps(c, "k_ztransform_inv");
+        hipLaunchKernelGGL(k_ztransform, zg, dim3(256), 0, c.stream, c.g, mf, (const double*)w, p);
+        HIP_OK(hipGetLastError());
+    }
+    ProfScope ps(c, "k_rfft_x_inv");
+    return rfft_launch<0, 1, double, double, 0>(c, c.tw_x, p, p);
+}
 
 }  // namespace icls

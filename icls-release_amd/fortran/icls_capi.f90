@@ -10,6 +10,8 @@ module icls_capi
                                    ICLS_ERR_DIVERGENCE = 4, ICLS_ERR_STATE = 5, ICLS_ERR_NODEVICE = 6, &
                                    ICLS_ERR_UNSUPPORTED = 7
   integer(c_int32_t), parameter :: ICLS_MAX_DROPS = 1024
+  ! BC_in_y of solver2d_cos (initsolver.f90:84-124)
+  integer(c_int32_t), parameter :: ICLS_BCY_WALLS = 1, ICLS_BCY_INOUT = 2, ICLS_BCY_OUTOUT = 3
 
   ! field ids (enum icls_field)
   integer(c_int32_t), parameter :: F_UNEW = 0, F_VNEW = 1, F_WNEW = 2, F_DUDT = 3, F_DVDT = 4, F_DWDT = 5, &
@@ -100,6 +102,10 @@ module icls_capi
      end function
      integer(c_int32_t) function icls_solver2d() bind(c, name='icls_solver2d')
        import :: c_int32_t
+     end function
+     integer(c_int32_t) function icls_solver2d_cos(bc_y) bind(c, name='icls_solver2d_cos')
+       import :: c_int32_t
+       integer(c_int32_t), value :: bc_y
      end function
      integer(c_int32_t) function icls_boundp(field) bind(c, name='icls_boundp')
        import :: c_int32_t

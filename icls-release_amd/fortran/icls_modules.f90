@@ -13,6 +13,8 @@ module mod_icls_state
   type(icls_params), save :: prm
   integer, save :: imax = 0, jmax = 0, kmax = 0, i1 = 0, j1 = 0, k1 = 0, lmax = 0
   logical, save :: sync_host = .false.
+  ! BC_in_y (param.f90:115): the y rule solver2d_cos applies ('Walls', 'InOut', 'OutOut')
+  character(len=8), save :: bc_in_y = 'Periodic'
 contains
   subroutine icls_setup(p)
     type(icls_params), intent(in) :: p
@@ -147,12 +149,13 @@ contains
   end subroutine gfm4p
 end module mod_gfm
 
-! solver2d (src/solver.f90:68-196); pz is the interior section of p
+! solver2d (src/solver.f90:68-196) and solver2d_cos (solver.f90:202-317); pz is the
+! interior section of p
 module mod_solver
   use mod_icls_state
   implicit none
   private
-  public solver2d
+  public solver2d, solver2d_cos
 contains
   subroutine solver2d(pz)
     real, intent(inout), dimension(1:, 1:, 1:) :: pz
@@ -165,6 +168,31 @@ contains
        deallocate(tmp)
     end if
   end subroutine solver2d
+
+  ! solver2d_cos (src/solver.f90:202-317) with the y rule bc_in_y
+  subroutine solver2d_cos(pz)
+    real, intent(inout), dimension(1:, 1:, 1:) :: pz
+    real, allocatable :: tmp(:, :, :)
+    integer(c_int32_t) :: rule
+    select case (trim(bc_in_y))
+    case ('Walls')
+       rule = ICLS_BCY_WALLS
+    case ('InOut')
+       rule = ICLS_BCY_INOUT
+    case ('OutOut')
+       rule = ICLS_BCY_OUTOUT
+    case default
+       write(6, *) 'solver2d_cos: BC_in_y must be Walls, InOut or OutOut, got ', bc_in_y
+       error stop 1
+    end select
+    call icls_check(icls_solver2d_cos(rule), 'solver2d_cos')
+    if (sync_host) then
+       allocate(tmp(0:i1, 0:j1, 0:k1))
+       call pull_c(F_P, tmp)
+       pz = tmp(1:imax, 1:jmax, 1:kmax)
+       deallocate(tmp)
+    end if
+  end subroutine solver2d_cos
 end module mod_solver
 
 ! boundp, bounduvw (src/bound.f90:15-287) for the arrays main.f90 passes

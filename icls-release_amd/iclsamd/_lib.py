@@ -23,7 +23,8 @@ EXPORTS = [
     "icls_init_derived", "icls_upload", "icls_download", "icls_get_scalar", "icls_set_scalar", "icls_get_volume",
     "icls_set_volume",
     "icls_sync", "icls_ls_main", "icls_average_stream_vel", "icls_ab2", "icls_fillps", "icls_gfm4p",
-    "icls_add_pjump", "icls_solver2d", "icls_boundp", "icls_correc", "icls_bounduvw", "icls_update_pressure",
+    "icls_add_pjump", "icls_solver2d", "icls_solver2d_cos", "icls_boundp", "icls_correc", "icls_bounduvw",
+    "icls_update_pressure",
     "icls_average_vel_at_center", "icls_check_nan", "icls_chkdiv", "icls_chkdt", "icls_max_vel_at_center",
     "icls_spurious_currents",
     "icls_ipot", "icls_write_fld", "icls_init_program_restart", "icls_write_vtk", "icls_post2d",
@@ -73,7 +74,8 @@ def load():
     L.icls_set_scalar.argtypes = [i32, dbl]
     L.icls_get_volume.argtypes = [i32, dp, dp]
     L.icls_set_volume.argtypes = [i32, dbl, dbl]
-    for name in ["icls_ls_main", "icls_boundp", "icls_bounduvw", "icls_update_pressure", "icls_profile_enable"]:
+    for name in ["icls_ls_main", "icls_solver2d_cos", "icls_boundp", "icls_bounduvw", "icls_update_pressure",
+                 "icls_profile_enable"]:
         getattr(L, name).argtypes = [i32]
     L.icls_average_stream_vel.argtypes = [dp]
     L.icls_chkdiv.argtypes = [i32, dp, dp, ctypes.POINTER(i32)]
@@ -208,6 +210,10 @@ class Solver:
 
     def solver2d(self):
         self._call("icls_solver2d")
+
+    def solver2d_cos(self, bc_y):
+        """solver2d_cos (solver.f90:202-317) on p with BC_in_y = bc_y (P.BCY_*)."""
+        self._call("icls_solver2d_cos", bc_y)
 
     def boundp(self, fid):
         self._call("icls_boundp", fid)

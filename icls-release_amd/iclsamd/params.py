@@ -26,6 +26,7 @@ MAX_DROPS = 1024
 
 MODE_FP64, MODE_MIXED, MODE_MIXED_LEARNED = 0, 1, 2
 ADV_HOUC5, ADV_WENO5 = 0, 1
+BCY_WALLS, BCY_INOUT, BCY_OUTOUT = 1, 2, 3   # BC_in_y of solver2d_cos (initsolver.f90:84-124)
 INIU_ZER, INIU_COU = 0, 1
 
 # field ids (enum icls_field)

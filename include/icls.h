@@ -40,6 +40,10 @@ extern "C" {
 #define ICLS_ERR_UNSUPPORTED 7   /* option the reference aborts on or this build omits */
 
 /* ---- precision modes ---- */
+/* BC_in_y of the cosine solver (initsolver.f90:84-124) */
+#define ICLS_BCY_WALLS          1
+#define ICLS_BCY_INOUT          2
+#define ICLS_BCY_OUTOUT         3
 #define ICLS_MODE_FP64          0   /* every field and kernel in FP64 (reference precision)   */
 #define ICLS_MODE_MIXED         1   /* FP32 level set, momentum, FFT; FP64 Thomas, residual,
                                        pressure accumulation and volume sums                 */
@@ -177,6 +181,13 @@ int32_t icls_fillps(void);                               /* fillps.f90:13-57 (in
 int32_t icls_gfm4p(void);                                /* gfm.f90:17-267 (p_jump)     */
 int32_t icls_add_pjump(void);                            /* main.f90:142  p = p+p_jump  */
 int32_t icls_solver2d(void);                             /* solver.f90:68-196 (on p)    */
+/* solver2d_cos (solver.f90:202-317) on p: FFT along x, VCOSQB along z, Gauss
+ * elimination along y with the y boundary rule bc_y (initsolver.f90:84-124, BC_in_y:
+ * ICLS_BCY_WALLS, ICLS_BCY_INOUT, ICLS_BCY_OUTOUT), VCOSQF along z, inverse FFT.
+ * Computed in FP64 in every precision mode, with the library's own scratch field as
+ * work space.  The other operators of this library keep y periodic: the entry serves a
+ * caller that builds its own y-wall step around it. */
+int32_t icls_solver2d_cos(int32_t bc_y);
 int32_t icls_boundp(int32_t field);                      /* bound.f90:182-287           */
 int32_t icls_correc(void);                               /* correc.f90:15-44            */
 int32_t icls_bounduvw(int32_t which);                    /* bound.f90:15-177; 0=unew,1=dudt */

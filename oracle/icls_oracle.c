@@ -1292,6 +1292,113 @@ int oracle_add_pjump(oracle_state* S) {
 
 /* solver2d (solver.f90:68-196) on p(1:imax,1:jmax,1:kmax); the 2DECOMP transposes
  * are identities on one rank */
+/* VCOSQB and VCOSQF (fft.f:218-370, 68-214) by the sums that define them
+ * (fft.f:266-270, 116-120), not by the fast algorithm:
+ *   VCOSQB: X(i) = sum_{k=1..n} 4 X(k) cos((2k-1)(i-1) pi/(2n)) / sqrt(4n)
+ *   VCOSQF: X(i) = (X(1) + sum_{k=2..n} 2 X(k) cos((2i-1)(k-1) pi/(2n))) / sqrt(4n) */
+void oracle_vcosqb(int n, double* x) {
+    const double pi = acos(-1.), s = sqrt(4. * n);
+    double* y = zalloc(n);
+    for (int i = 1; i <= n; i++) {
+        double acc = 0.;
+        for (int k = 1; k <= n; k++) acc += 4. * x[k - 1] * cos((double)((2 * k - 1) * (i - 1)) * pi / (2. * n));
+        y[i - 1] = acc / s;
+    }
+    memcpy(x, y, sizeof(double) * n);
+    free(y);
+}
+
+void oracle_vcosqf(int n, double* x) {
+    const double pi = acos(-1.), s = sqrt(4. * n);
+    double* y = zalloc(n);
+    for (int i = 1; i <= n; i++) {
+        double acc = x[0];
+        for (int k = 2; k <= n; k++) acc += 2. * x[k - 1] * cos((double)((2 * i - 1) * (k - 1)) * pi / (2. * n));
+        y[i - 1] = acc / s;
+    }
+    memcpy(x, y, sizeof(double) * n);
+    free(y);
+}
+
+/* solver2d_cos (solver.f90:202-317) on p's interior, with initsolver's branch for
+ * BC_in_y = 'Walls' (1), 'InOut' (2), 'OutOut' (3) (initsolver.f90:84-160): forward real
+ * FFT along x, VCOSQB along z, Gauss elimination along y with the eigenvalue
+ * xrt(i) + zrt(k), VCOSQF along z, backward FFT along x. */
+int oracle_solver2d_cos(oracle_state* S, int bcy) {
+    const int imax = S->imax, jmax = S->jmax, kmax = S->kmax;
+    const double pi = S->pi;
+    int nl = imax > jmax ? imax : jmax;
+    if (kmax > nl) nl = kmax;
+    double* line = zalloc(nl);
+    /* y tridiagonal (initsolver.f90:88-124) */
+    double* a = zalloc(jmax);
+    double* b = zalloc(jmax);
+    double* c = zalloc(jmax);
+    for (int j = 0; j < jmax; j++) { a[j] = S->dyi * S->dyi; c[j] = S->dyi * S->dyi; b[j] = -(a[j] + c[j]); }
+    if (bcy == 1) { b[0] = b[0] + a[0]; b[jmax - 1] = b[jmax - 1] + c[jmax - 1]; }
+    else if (bcy == 2) { b[0] = b[0] + a[0]; b[jmax - 1] = b[jmax - 1] - c[jmax - 1]; }
+    else { b[0] = b[0] - a[0]; b[jmax - 1] = b[jmax - 1] - c[jmax - 1]; }
+    a[0] = 0.; c[jmax - 1] = 0.;
+    /* eigenvalues (initsolver.f90:132-153) */
+    double* xrt = zalloc(imax + 1);
+    for (int i = 3; i <= imax; i += 2) {
+        double sn = sin((double)(i - 1) * pi / (2. * imax));
+        xrt[i - 1] = -4. * S->dxi * S->dxi * (sn * sn);
+        xrt[i] = xrt[i - 1];
+    }
+    xrt[1] = 0.; xrt[imax] = -4. * S->dxi * S->dxi;
+    double* zrt = zalloc(kmax + 1);
+    for (int k = 1; k <= kmax; k++) zrt[k] = (S->dzi * S->dzi) * (-2. + 2. * cos((double)(k - 1) * pi / kmax));
+#define PZ(i, j, k) S->p[CC(i, j, k)]
+    for (int k = 1; k <= kmax; k++)
+        for (int j = 1; j <= jmax; j++) {
+            for (int i = 1; i <= imax; i++) line[i - 1] = PZ(i, j, k);
+            oracle_rfft_forward(imax, line);
+            for (int i = 1; i <= imax; i++) PZ(i, j, k) = line[i - 1];
+        }
+    for (int j = 1; j <= jmax; j++)
+        for (int i = 1; i <= imax; i++) {
+            for (int k = 1; k <= kmax; k++) line[k - 1] = PZ(i, j, k);
+            oracle_vcosqb(kmax, line);
+            for (int k = 1; k <= kmax; k++) PZ(i, j, k) = line[k - 1];
+        }
+    /* Gauss elimination in y (solver.f90:253-290) */
+    double* d = zalloc(jmax);
+    for (int k = 1; k <= kmax; k++)
+        for (int i = 1; i <= imax; i++) {
+            const double xz = xrt[i] + zrt[k];
+            double z = 1. / (b[0] + xz);
+            d[0] = c[0] * z;
+            PZ(i, 1, k) = PZ(i, 1, k) * z;
+            for (int j = 2; j <= jmax - 1; j++) {
+                double bb = b[j - 1] + xz;
+                z = 1. / (bb - a[j - 1] * d[j - 2]);
+                d[j - 1] = c[j - 1] * z;
+                PZ(i, j, k) = (PZ(i, j, k) - a[j - 1] * PZ(i, j - 1, k)) * z;
+            }
+            double bb = b[jmax - 1] + xz;
+            z = bb - a[jmax - 1] * d[jmax - 2];
+            if (z != 0.) PZ(i, jmax, k) = (PZ(i, jmax, k) - a[jmax - 1] * PZ(i, jmax - 1, k)) / z;
+            else PZ(i, jmax, k) = 0.;
+            for (int j = jmax - 1; j >= 1; j--) PZ(i, j, k) = PZ(i, j, k) - d[j - 1] * PZ(i, j + 1, k);
+        }
+    for (int j = 1; j <= jmax; j++)
+        for (int i = 1; i <= imax; i++) {
+            for (int k = 1; k <= kmax; k++) line[k - 1] = PZ(i, j, k);
+            oracle_vcosqf(kmax, line);
+            for (int k = 1; k <= kmax; k++) PZ(i, j, k) = line[k - 1];
+        }
+    for (int k = 1; k <= kmax; k++)
+        for (int j = 1; j <= jmax; j++) {
+            for (int i = 1; i <= imax; i++) line[i - 1] = PZ(i, j, k);
+            oracle_rfft_backward(imax, line);
+            for (int i = 1; i <= imax; i++) PZ(i, j, k) = line[i - 1];
+        }
+#undef PZ
+    free(line); free(a); free(b); free(c); free(xrt); free(zrt); free(d);
+    return 0;
+}
+
 int oracle_solver2d(oracle_state* S) {
     int imax = S->imax, jmax = S->jmax, kmax = S->kmax;
     double* line = (double*)malloc(sizeof(double) * (imax > jmax ? (imax > kmax ? imax : kmax) : (jmax > kmax ? jmax : kmax)));

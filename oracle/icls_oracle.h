@@ -33,6 +33,9 @@ int    oracle_fillps(oracle_state* S);
 int    oracle_gfm4p(oracle_state* S);
 int    oracle_add_pjump(oracle_state* S);
 int    oracle_solver2d(oracle_state* S);
+int    oracle_solver2d_cos(oracle_state* S, int bcy);   /* 1 Walls, 2 InOut, 3 OutOut */
+void   oracle_vcosqb(int n, double* x);
+void   oracle_vcosqf(int n, double* x);
 int    oracle_boundp(oracle_state* S, int field);
 int    oracle_correc(oracle_state* S);
 int    oracle_bounduvw(oracle_state* S, int which);

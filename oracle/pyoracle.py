@@ -55,6 +55,10 @@ def lib():
         L.oracle_chkdiv.restype = i32
         L.oracle_chkdt.argtypes = [vp]
         L.oracle_chkdt.restype = dbl
+        L.oracle_solver2d_cos.argtypes = [vp, i32]
+        L.oracle_solver2d_cos.restype = i32
+        L.oracle_vcosqb.argtypes = [i32, dp]
+        L.oracle_vcosqf.argtypes = [i32, dp]
         L.oracle_ab2_coeffs.argtypes = [dbl, dbl, dp]
         L.oracle_ab2_coeffs.restype = None
         L.oracle_max_vel_at_center.argtypes = [vp]
@@ -218,6 +222,18 @@ def rfft_forward(x):
 def rfft_backward(x):
     r = np.array(x, dtype=np.float64, copy=True)
     lib().oracle_rfft_backward(len(r), _dp(r))
+    return r
+
+
+def vcosqb(x):
+    r = np.array(x, dtype=np.float64, copy=True)
+    lib().oracle_vcosqb(len(r), _dp(r))
+    return r
+
+
+def vcosqf(x):
+    r = np.array(x, dtype=np.float64, copy=True)
+    lib().oracle_vcosqf(len(r), _dp(r))
     return r
 
 

@@ -408,3 +408,51 @@ def test_solver2d_parity(n, nz):
         assert rel_linf(a, b) <= TOL
     finally:
         g.close()
+
+
+@pytest.mark.parametrize("bcy", [P.BCY_WALLS, P.BCY_INOUT, P.BCY_OUTOUT])
+@pytest.mark.parametrize("n,nz,mode", [(16, 8, P.MODE_FP64), (64, 24, P.MODE_FP64), (32, 80, P.MODE_FP64),
+                                       (256, 16, P.MODE_FP64), (32, 40, P.MODE_MIXED)])
+def test_solver2d_cos_parity(bcy, n, nz, mode):
+    """solver2d_cos (solver.f90:202-317) with each BC_in_y rule (initsolver.f90:84-124):
+    FFT x, VCOSQB z, Gauss elimination y, VCOSQF z, inverse FFT x.  nz = 24, 40 and 80
+    leave partial 16-layer passes and 64-output tiles in the z transform; the mixed
+    mode runs the same FP64 path on its double pressure."""
+    p = small_drop(n, nz=nz, mode=mode)
+    o, g = _pair(p)
+    try:
+        o.init_program()
+        g.init_program()
+        rng = np.random.default_rng(100 * bcy + n + nz)
+        f = np.zeros(P.field_dims(p, P.F_P), order="F")
+        f[1:-1, 1:-1, 1:-1] = rng.standard_normal((n, n, nz))
+        if bcy == P.BCY_WALLS:
+            f[1:-1, 1:-1, 1:-1] -= f[1:-1, 1:-1, 1:-1].mean()
+        o.set(P.F_P, f)
+        g.upload(P.F_P, f)
+        assert o.solver2d_cos(bcy) == 0
+        g.solver2d_cos(bcy)
+        a = g.download(P.F_P)[1:-1, 1:-1, 1:-1]
+        b = o.get(P.F_P)[1:-1, 1:-1, 1:-1]
+        e = rel_linf(a, b)
+        print("solver2d_cos bc_y=%d %dx%dx%d mode %d: rel Linf vs oracle %.3e" % (bcy, n, n, nz, mode, e))
+        assert e <= TOL
+        # twice on the same context: the tables built on the first call are reused
+        g.upload(P.F_P, f)
+        g.solver2d_cos(bcy)
+        assert np.array_equal(g.download(P.F_P)[1:-1, 1:-1, 1:-1], a)
+    finally:
+        g.close()
+
+
+def test_solver2d_cos_rejects_bad_rule():
+    from iclsamd import _lib
+    g = _lib.Solver(small_drop(16, nz=8))
+    try:
+        g.init_program()
+        with pytest.raises(_lib.IclsError):
+            g.solver2d_cos(0)
+        with pytest.raises(_lib.IclsError):
+            g.solver2d_cos(4)
+    finally:
+        g.close()

@@ -99,6 +99,93 @@ def test_rfft_single_modes():
     assert np.allclose(r, e, atol=1e-13)
 
 
+# ---------------------------------------------------------------- cosine transforms (VCOSQB / VCOSQF)
+def _vcosq(L, x, inverse):
+    n = len(x)
+    m = ctypes.c_int(1)
+    nn = ctypes.c_int(n)
+    wsave = np.zeros(2 * n + 15)
+    r = np.array(x, dtype=np.float64)
+    rt = np.zeros(n)
+    dp = ctypes.POINTER(ctypes.c_double)
+    L.vcosqi_(ctypes.byref(nn), wsave.ctypes.data_as(dp))
+    fn = L.vcosqf_ if inverse else L.vcosqb_
+    fn(ctypes.byref(m), ctypes.byref(nn), r.ctypes.data_as(dp), rt.ctypes.data_as(dp), ctypes.byref(m),
+       wsave.ctypes.data_as(dp))
+    return r
+
+
+@pytest.mark.parametrize("n", [4, 8, 16, 32, 64, 128])
+def test_vcosq_matches_vfftpack_source_build(n):
+    L = _vfftpack()
+    rng = np.random.default_rng(2000 + n)
+    x = rng.standard_normal(n)
+    ref = _vcosq(L, x, False)
+    assert np.max(np.abs(O.vcosqb(x) - ref)) <= 1e-13 * np.max(np.abs(ref))
+    back = _vcosq(L, x, True)
+    assert np.max(np.abs(O.vcosqf(x) - back)) <= 1e-13 * np.max(np.abs(back))
+
+
+@pytest.mark.parametrize("n", [4, 16, 64])
+def test_vcosq_inverse_pair_and_neumann_eigenvectors(n):
+    """VCOSQF inverts VCOSQB, and VCOSQB diagonalises the cell-centred second difference
+    with mirror ghosts (the z operator of the cosine solver): mode k of the transform
+    of L x is (-2 + 2 cos((k-1) pi / n)) times mode k of the transform of x."""
+    rng = np.random.default_rng(3000 + n)
+    x = rng.standard_normal(n)
+    assert np.allclose(O.vcosqf(O.vcosqb(x)), x, rtol=0, atol=1e-13)
+    assert np.allclose(O.vcosqb(O.vcosqf(x)), x, rtol=0, atol=1e-13)
+    g = np.concatenate([[x[0]], x, [x[-1]]])
+    lx = g[2:] + g[:-2] - 2 * g[1:-1]
+    lam = -2 + 2 * np.cos(np.arange(n) * np.pi / n)
+    assert np.allclose(O.vcosqb(lx), lam * O.vcosqb(x), rtol=0, atol=1e-12)
+    # a single cosine mode maps to one coefficient
+    k = np.arange(1, n + 1)
+    e = O.vcosqb(np.cos((2 * k - 1) * 2 * np.pi / (2 * n)))
+    want = np.zeros(n)
+    want[2] = 2 * n / np.sqrt(4 * n)
+    assert np.allclose(e, want, rtol=0, atol=1e-12)
+
+
+def _laplacian_cos(p, dxi, dyi, dzi, bcy):
+    """Cell-centred Laplacian: periodic x, mirror (Neumann) ghosts in z, in y mirror
+    ghosts at a 'Walls' end and odd (zero-value) ghosts at an outflow end
+    (initsolver.f90:88-124: b(1) +- a(1), b(jtot) +- c(jtot))."""
+    lo, hi = {1: (1, 1), 2: (1, -1), 3: (-1, -1)}[bcy]
+    lx = (np.roll(p, -1, 0) + np.roll(p, 1, 0) - 2 * p) * dxi * dxi
+    gy = np.concatenate([lo * p[:, :1], p, hi * p[:, -1:]], axis=1)
+    ly = (gy[:, 2:] + gy[:, :-2] - 2 * p) * dyi * dyi
+    gz = np.concatenate([p[:, :, :1], p, p[:, :, -1:]], axis=2)
+    lz = (gz[:, :, 2:] + gz[:, :, :-2] - 2 * p) * dzi * dzi
+    return lx + ly + lz
+
+
+@pytest.mark.parametrize("bcy", [1, 2, 3])
+@pytest.mark.parametrize("n,nz", [(16, 8), (32, 16)])
+def test_solver2d_cos_inverts_the_operator(bcy, n, nz):
+    """solver2d_cos (solver.f90:202-317) returns p with L p = f for each y boundary rule;
+    with walls on every side the operator is singular, f is made zero-mean and the
+    (1,1) mode's last pivot is the exact zero that sets p(jtot) to 0."""
+    p = small_drop(n, nz=nz)
+    o = O.Oracle(p)
+    o.init_program()
+    rng = np.random.default_rng(10 * bcy + n)
+    f = rng.standard_normal((n, n, nz))
+    if bcy == 1:
+        f -= f.mean()
+    a = np.zeros(P.field_dims(p, P.F_P), order="F")
+    a[1:-1, 1:-1, 1:-1] = f
+    o.set(P.F_P, a)
+    assert o.solver2d_cos(bcy) == 0
+    sol = o.get(P.F_P)[1:-1, 1:-1, 1:-1]
+    dxi, dyi, dzi = p.itot / p.lx, p.jtot / p.ly, p.ktot / p.lz
+    r = _laplacian_cos(sol, dxi, dyi, dzi, bcy)
+    assert np.max(np.abs(r - f)) <= 1e-11 * np.max(np.abs(f))
+    if bcy == 1:
+        # the constant is fixed by the zero pivot: the (x, z)-mean of the last y row is 0
+        assert abs(sol[:, -1, :].mean()) <= 1e-12 * np.max(np.abs(sol))
+
+
 # ---------------------------------------------------------------- Thomas (solver.f90:119-160)
 def _tridiag(kmax, dzi, lam):
     a = dzi * dzi
