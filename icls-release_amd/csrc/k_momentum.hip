@@ -6,6 +6,9 @@
 // T is the element type of the field arrays; the pressures pnew, pold and p are
 // double in every mode, and pressure differences are formed in double before they
 // meet a field value.
+#include <cstdint>
+#include <type_traits>
+
 #include "icls_internal.hpp"
 #include "icls_reduce.hpp"
 #include "icls_stencil.hpp"
@@ -42,7 +45,21 @@ struct MomParams {
 constexpr int TW = BX + 2, TH = BY + 2, TP = TW * TH;   // tile plane with halo
 constexpr int NLD = (TP + BX * BY - 1) / (BX * BY);      // plane elements per thread
 
-template <class T, int RHS_ONLY, bool P2>
+// Element q of a field.  With O32 the index and the byte offset are 32-bit, so an access
+// is the field's base (uniform, in scalar registers) plus one 32-bit vector offset and
+// needs no 64-bit address arithmetic; the host launches O32 only when every field is
+// smaller than 4 GiB.
+template <bool O32, class P, class I>
+__device__ __forceinline__ P& el(P* b, I q) {
+    if constexpr (O32) {
+        using C = typename std::conditional<std::is_const<P>::value, const char, char>::type;
+        return *reinterpret_cast<P*>(reinterpret_cast<C*>(b) + (uint32_t)q * (uint32_t)sizeof(P));
+    } else {
+        return b[q];
+    }
+}
+
+template <class T, int RHS_ONLY, bool P2, bool O32>
 __global__ void __launch_bounds__(BX * BY)
 k_momentum_zm(Grid g, MomParams m, const double* __restrict__ dscal, const T* __restrict__ u, const T* __restrict__ v,
               const T* __restrict__ w, const T* __restrict__ miu, const T* __restrict__ rho, T* __restrict__ duold,
@@ -52,6 +69,8 @@ k_momentum_zm(Grid g, MomParams m, const double* __restrict__ dscal, const T* __
               const T* __restrict__ p_zold, T* __restrict__ dudt, T* __restrict__ dvdt, T* __restrict__ dwdt,
               const uint8_t* __restrict__ jm, const uint8_t* __restrict__ jmo, const T* __restrict__ zp) {
     __shared__ T ring[5][4][TP];   // u, v, w, miu, rho x four planes
+    using Ix = typename std::conditional<O32, uint32_t, long>::type;
+    auto cell = [&](int i_, int j_, int k_) -> Ix { return (Ix)i_ + (Ix)g.sy * (Ix)j_ + (Ix)g.sz * (Ix)(k_ + 2); };
     const int MZ = m.mz;
     const int ntx = (g.nx + BX - 1) / BX, nty = (g.ny + BY - 1) / BY, ntz = (g.nz + MZ - 1) / MZ;
     const int tile = xcd_tile(blockIdx.x, ntx * nty * ntz);
@@ -62,22 +81,22 @@ k_momentum_zm(Grid g, MomParams m, const double* __restrict__ dscal, const T* __
     const int xm = g.nx - 1, ym = g.ny - 1;
     const T* src[5] = {u, v, w, miu, rho};
     // plane element e of the tile -> global offset within a z layer
-    long poff[NLD];
+    Ix poff[NLD];
 #pragma unroll
     for (int r = 0; r < NLD; r++) {
         int e = tid + r * BX * BY;
         if (e >= TP) e = TP - 1;
         const int lx = e % TW - 1, ly = e / TW - 1;
-        poff[r] = (long)((x0 + lx) & xm) + g.sy * (long)((y0 + ly) & ym);
+        poff[r] = (Ix)((x0 + lx) & xm) + (Ix)g.sy * (Ix)((y0 + ly) & ym);
     }
     // planes travel global -> registers -> LDS
     T nxt[5][NLD];
     auto fetch = [&](int k, T (*dst)[NLD]) {
-        const long base = g.sz * (long)(k + 2);
+        const Ix base = (Ix)g.sz * (Ix)(k + 2);
 #pragma unroll
         for (int f = 0; f < 5; f++)
 #pragma unroll
-            for (int r = 0; r < NLD; r++) dst[f][r] = src[f][base + poff[r]];
+            for (int r = 0; r < NLD; r++) dst[f][r] = el<O32>(src[f], base + poff[r]);
     };
     auto stash = [&](int slot, T (*from)[NLD]) {
 #pragma unroll
@@ -115,25 +134,25 @@ k_momentum_zm(Grid g, MomParams m, const double* __restrict__ dscal, const T* __
     unsigned jn = 7u, jo = 7u;
     auto fetch_mask = [&](int k) {
         if (RHS_ONLY) return;
-        const long q = gidx(g, i, j, k);
-        jn = jm[q];
-        jo = jmo[q];
+        const Ix q = cell(i, j, k);
+        jn = el<O32>(jm, q);
+        jo = el<O32>(jmo, q);
     };
     auto fetch_point = [&](int k) {
         if (RHS_ONLY) return;
-        const long q = gidx(g, i, j, k), qx = gidx(g, ip, j, k), qy = gidx(g, i, jp, k), qz = q + g.sz;
-        pd[0] = duold[q]; pd[1] = dvold[q]; pd[2] = dwold[q];
+        const Ix q = cell(i, j, k), qx = cell(ip, j, k), qy = cell(i, jp, k), qz = q + (Ix)g.sz;
+        pd[0] = el<O32>(duold, q); pd[1] = el<O32>(dvold, q); pd[2] = el<O32>(dwold, q);
         const T* z0 = zp + (i + g.sy * (long)j);
-        pd[3] = *((jn & 1u) ? p_x + q : z0);
-        pd[4] = *((jo & 1u) ? p_xold + q : z0);
-        pd[5] = *((jn & 2u) ? p_y + q : z0);
-        pd[6] = *((jo & 2u) ? p_yold + q : z0);
-        pd[7] = *((jn & 4u) ? p_z + q : z0);
-        pd[8] = *((jo & 4u) ? p_zold + q : z0);
-        pp[0] = pnew[q]; pp[1] = pold[q];
-        pp[2] = pnew[qx]; pp[3] = pold[qx];
-        pp[4] = pnew[qy]; pp[5] = pold[qy];
-        pp[6] = pnew[qz]; pp[7] = pold[qz];
+        pd[3] = *((jn & 1u) ? &el<O32>(p_x, q) : z0);
+        pd[4] = *((jo & 1u) ? &el<O32>(p_xold, q) : z0);
+        pd[5] = *((jn & 2u) ? &el<O32>(p_y, q) : z0);
+        pd[6] = *((jo & 2u) ? &el<O32>(p_yold, q) : z0);
+        pd[7] = *((jn & 4u) ? &el<O32>(p_z, q) : z0);
+        pd[8] = *((jo & 4u) ? &el<O32>(p_zold, q) : z0);
+        pp[0] = el<O32>(pnew, q); pp[1] = el<O32>(pold, q);
+        pp[2] = el<O32>(pnew, qx); pp[3] = el<O32>(pold, qx);
+        pp[4] = el<O32>(pnew, qy); pp[5] = el<O32>(pold, qy);
+        pp[6] = el<O32>(pnew, qz); pp[7] = el<O32>(pold, qz);
     };
     T du0 = T(0.), dv0 = T(0.), dw0 = T(0.);
     double gpx = 0., gpy = 0., gpz = 0.;
@@ -153,16 +172,26 @@ k_momentum_zm(Grid g, MomParams m, const double* __restrict__ dscal, const T* __
         du0 = pd[0]; dv0 = pd[1]; dw0 = pd[2];
         // split pressure (ab2.f90:79-101): p_hat = 2 p^n - p^(n-1), p_hat_j = 2 p_j - p_j,old
         // (ex_n = 2, ex_o = 1 at a constant dt)
-        const double phq = exn * pn_of(pp[0]) - exo * po_of(pp[1]);
-        const double phx = exn * pn_of(pp[2]) - exo * po_of(pp[3]);
-        const double phy = exn * pn_of(pp[4]) - exo * po_of(pp[5]);
-        const double phz = exn * pn_of(pp[6]) - exo * po_of(pp[7]);
         const T pjx = exn_ * pd[3] - exo_ * pd[4];
         const T pjy = exn_ * pd[5] - exo_ * pd[6];
         const T pjz = exn_ * pd[7] - exo_ * pd[8];
-        gpx = phx - (double)pjx - phq;
-        gpy = phy - (double)pjy - phq;
-        gpz = phz - (double)pjz - phq;
+        if constexpr (std::is_same<T, double>::value) {
+            // the reference's operations: p_hat at both cells, then the difference
+            const double phq = exn * pn_of(pp[0]) - exo * po_of(pp[1]);
+            const double phx = exn * pn_of(pp[2]) - exo * po_of(pp[3]);
+            const double phy = exn * pn_of(pp[4]) - exo * po_of(pp[5]);
+            const double phz = exn * pn_of(pp[6]) - exo * po_of(pp[7]);
+            gpx = phx - (double)pjx - phq;
+            gpy = phy - (double)pjy - phq;
+            gpz = phz - (double)pjz - phq;
+        } else {
+            // float fields use only the differences p_hat(neighbour) - p_hat(cell), still
+            // formed in double; the lazy rotation's offsets are the same at every cell and
+            // cancel in them, so the raw buffers are differenced directly
+            gpx = (exn * (pp[2] - pp[0]) - exo * (pp[3] - pp[1])) - (double)pjx;
+            gpy = (exn * (pp[4] - pp[0]) - exo * (pp[5] - pp[1])) - (double)pjy;
+            gpz = (exn * (pp[6] - pp[0]) - exo * (pp[7] - pp[1])) - (double)pjz;
+        }
     };
     fetch_mask(kb);
     fetch_point(kb);
@@ -176,7 +205,7 @@ k_momentum_zm(Grid g, MomParams m, const double* __restrict__ dscal, const T* __
     const int c0 = (ty + 1) * TW + tx + 1;   // own cell in the tile plane
 
     for (int k = kb; k <= ke; k++) {
-        const long q = gidx(g, i, j, k);
+        const Ix q = cell(i, j, k);
         fetch(min(k + 2, ke + 1), nxt);
         fetch_point(min(k + 1, ke));
         fetch_mask(min(k + 2, ke));
@@ -301,20 +330,20 @@ k_momentum_zm(Grid g, MomParams m, const double* __restrict__ dscal, const T* __
             T dvn = RCv + RDv * visc + RGv;
             T dwn = RCw + RDw * visc + RGw;
             if (RHS_ONLY) {
-                duold[q] = dun; dvold[q] = dvn; dwold[q] = dwn;
+                el<O32>(duold, q) = dun; el<O32>(dvold, q) = dvn; el<O32>(dwold, q) = dwn;
             } else {
                 // ---- AB2 (ab2.f90:55-65)
                 T us = uq + dt * (c15 * dun - c05 * du0);
                 T vs = vq + dt * (c15 * dvn - c05 * dv0);
                 T ws = wq + dt * (c15 * dwn - c05 * dw0);
-                duold[q] = dun; dvold[q] = dvn; dwold[q] = dwn;
+                el<O32>(duold, q) = dun; el<O32>(dvold, q) = dvn; el<O32>(dwold, q) = dwn;
                 // ---- split-pressure terms (mom.f90:793,818,843)
                 T mpx = -(qdiv(one, rho_u) - irho0) * T(g.dxi) * T(gpx);
                 T mpy = -(qdiv(one, rho_v) - irho0) * T(g.dyi) * T(gpy);
                 T mpz = -(qdiv(one, rho_w) - irho0) * T(g.dzi) * T(gpz);
-                dudt[q] = us + dt * mpx;
-                dvdt[q] = vs + dt * mpy - qdiv(T(m.forceytot) * dt, rho_v);
-                dwdt[q] = ws + dt * mpz;
+                el<O32>(dudt, q) = us + dt * mpx;
+                el<O32>(dvdt, q) = vs + dt * mpy - qdiv(T(m.forceytot) * dt, rho_v);
+                el<O32>(dwdt, q) = ws + dt * mpz;
             }
         }
 #undef AT
@@ -329,21 +358,28 @@ static void momentum_dispatch(Ctx& c, MomParams m, int rhs_only) {
     dim3 blk(BX, BY, 1);
     const int txy = ((c.g.nx + BX - 1) / BX) * ((c.g.ny + BY - 1) / BY);
     int ntiles = 0;
-#define MOM(R, P)                                                                                                  \
+#define MOM(R, P, O)                                                                                               \
     do {                                                                                                           \
-        m.mz = zchunk_for(c, (const void*)k_momentum_zm<T, R, P>, BX * BY, txy, 1, 2);                           \
+        m.mz = zchunk_for(c, (const void*)k_momentum_zm<T, R, P, O>, BX * BY, txy, 1, 2);                        \
         ntiles = txy * ((c.g.nz + m.mz - 1) / m.mz);                                                               \
-        hipLaunchKernelGGL((k_momentum_zm<T, R, P>), dim3(ntiles), blk, 0, c.stream, c.g, m,                        \
+        hipLaunchKernelGGL((k_momentum_zm<T, R, P, O>), dim3(ntiles), blk, 0, c.stream, c.g, m,                     \
                            (const double*)c.dscal, as<T>(c.unew), as<T>(c.vnew), as<T>(c.wnew), as<T>(c.miu),      \
                            as<T>(c.rho), as<T>(c.duold), as<T>(c.dvold), as<T>(c.dwold), (const double*)c.pnew,    \
                            (const double*)c.pold, as<T>(c.p_x), as<T>(c.p_y), as<T>(c.p_z), as<T>(c.p_xold),      \
                            as<T>(c.p_yold), as<T>(c.p_zold), as<T>(c.dudt), as<T>(c.dvdt), as<T>(c.dwdt),    \
                            (const uint8_t*)c.jmask, (const uint8_t*)c.jmask_old, as<T>(c.zplane));                 \
     } while (0)
+    // 32-bit offsets when the largest field (a double pressure) is below 4 GiB; the
+    // right-hand-side-only launch (once, at init) keeps 64-bit offsets
+    const bool o32 = (unsigned long)c.nfield * sizeof(double) < (1ul << 32);
     if (c.g.pow2) {
-        if (rhs_only) MOM(1, true); else MOM(0, true);
+        if (rhs_only) MOM(1, true, false);
+        else if (o32) MOM(0, true, true);
+        else MOM(0, true, false);
     } else {
-        if (rhs_only) MOM(1, false); else MOM(0, false);
+        if (rhs_only) MOM(1, false, false);
+        else if (o32) MOM(0, false, true);
+        else MOM(0, false, false);
     }
 #undef MOM
 }
