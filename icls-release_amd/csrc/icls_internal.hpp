@@ -89,6 +89,19 @@ struct LsWindow {
     int ox = 0, oy = 0, oz = 0, wx = 0, wy = 0, wz = 0;
 };
 
+// Several level-set windows in one launch (the drops of one colour: their windows do
+// not overlap).  Window t owns blocks first[t] .. first[t+1]-1; a pointer argument
+// given as nullptr stands for window t's own level set ls[t].  Every pointer is moved
+// to the window's first cell, as a one-window launch sees it.
+constexpr int WB_MAX = 64;
+struct WinBatch {
+    int n = 0;
+    int first[WB_MAX + 1] = {};
+    int ox[WB_MAX] = {}, oy[WB_MAX] = {}, oz[WB_MAX] = {}, wx[WB_MAX] = {}, wy[WB_MAX] = {}, wz[WB_MAX] = {};
+    const void* ls[WB_MAX] = {};
+    int lidx[WB_MAX] = {};      // level-set index of window t (its residual slot)
+};
+
 struct Ctx {
     icls_params P{};
     Grid g{};
@@ -178,6 +191,9 @@ struct Ctx {
     double* xyrt = nullptr;               // nx * ny eigenvalues
     double* tw_x = nullptr;               // FFT twiddles along x (nx/2 complex)
     double* tw_y = nullptr;               // along y
+    double* stash_b = nullptr;            // batched windows: 2 stash planes per batch slot
+    int res_b0 = 0;                       // batched windows: first of lmax residual slots
+    int save0 = 0;                        // two scalar slots: maxchk and the inflation flag kept aside
     double* cos_tab = nullptr;            // solver2d_cos: z transform matrices, xrt, zrt (on first use)
     double* pinvA_f = nullptr;            // float copies (raw float arrays)
     double* tw_x_f = nullptr;
@@ -247,6 +263,34 @@ int launch_tube(Ctx& c, const double* phi_in, double* phi_out);
 int launch_ls_extent(Ctx& c, const double* phi, int* ext);
 // the launch grid is a level-set window, not the whole grid
 inline bool in_window(const Ctx& c) { return c.g.nx != c.gfull.nx || c.g.ny != c.gfull.ny || c.g.nz != c.gfull.nz; }
+// the window of block b of a batched launch: its grid, the offset of its first cell and
+// of its first (i, j) column, its index t and the block's index inside the window
+struct WinPick {
+    Grid g;
+    long off, poff;
+    int t, bid;
+};
+__device__ __forceinline__ WinPick win_pick(const Grid& g, const WinBatch& wb, int b) {
+    int t = 0;
+    while (t + 1 < wb.n && b >= wb.first[t + 1]) t++;
+    WinPick w;
+    w.g = g;
+    w.g.nx = wb.wx[t]; w.g.ny = wb.wy[t]; w.g.nz = wb.wz[t];
+    w.poff = (long)wb.ox[t] + g.sy * (long)wb.oy[t];
+    w.off = w.poff + g.sz * (long)wb.oz[t];
+    w.t = t;
+    w.bid = b - wb.first[t];
+    return w;
+}
+template <class P>
+__device__ __forceinline__ P* win_ptr(P* shared, const WinBatch& wb, const WinPick& w) {
+    return (shared ? shared : (P*)wb.ls[w.t]) + w.off;
+}
+// one colour's windows (level sets ls), batched: tube + the three advection stages, and
+// the three re-initialisation stages; the re-init residual of level set l goes to
+// dscal[res_b0 + l], and that of level set lmax - 1 also to DS_REINIT_RES
+int launch_advect_batch(Ctx& c, const std::vector<int>& ls);
+int launch_reinit_batch(Ctx& c, const std::vector<int>& ls);
 int launch_adv_stage(Ctx& c, int scheme, int stage, const double* phi0, const double* phis, double* out,
                      const double* vu, const double* vv, const double* vw, int velf, const double* flag);
 int launch_reinit_stage(Ctx& c, int stage, const double* phi0, const double* phis, double* out);

@@ -144,7 +144,7 @@ static void free_ctx(Ctx* c) {
                                c->p_yold, c->p_zold, c->miu, c->rho, c->curv, c->curv2, c->ucc, c->vcc, c->wcc,
                                c->u_l, c->v_l, c->w_l, c->dheav, c->ucc_l, c->vcc_l, c->wcc_l, c->normal_m,
                                c->ls_a, c->ls_b, c->ls_c, c->mask, c->thomas_d, c->dscal, c->dpart, c->pinvA,
-                               c->xyrt, c->tw_x, c->tw_y, c->cos_tab, c->dls, c->stash, c->scratch, c->pf, c->thomas_y,
+                               c->xyrt, c->tw_x, c->tw_y, c->cos_tab, c->stash_b, c->dls, c->stash, c->scratch, c->pf, c->thomas_y,
                                c->thomas_seed, c->pinvA_f, c->tw_x_f, c->tw_y_f};
     if (c->frhs != c->p) ptrs.push_back(c->frhs);   // in FP64 the right-hand side is p itself
     ptrs.push_back(c->jmask);
@@ -278,9 +278,11 @@ static int build_ctx(const icls_params* prm) {
     HIP_OK(hipMalloc((void**)&C.stash, sizeof(double) * 2 * g.sz));
     // volumes, initial volumes and scratch sums per level set, then the side streams'
     // re-init residual slots
-    const int nscal = DS_COUNT_FIXED + 3 * C.lmax + 16 + LSTREAMS;
+    const int nscal = DS_COUNT_FIXED + 3 * C.lmax + 16 + LSTREAMS + C.lmax + 2;
     C.res_slot = DS_REINIT_RES;
     C.res_slot0 = DS_COUNT_FIXED + 3 * C.lmax + 16;
+    C.res_b0 = C.res_slot0 + LSTREAMS;
+    C.save0 = C.res_b0 + C.lmax;
     HIP_OK(hipMalloc((void**)&C.dscal, sizeof(double) * nscal));
     HIP_OK(hipMemset(C.dscal, 0, sizeof(double) * nscal));
     HIP_OK(hipHostMalloc((void**)&C.hscal, sizeof(double) * nscal));
@@ -303,6 +305,7 @@ static int build_ctx(const icls_params* prm) {
         HIP_OK(hipEventCreateWithFlags(&C.fork_ev, hipEventDisableTiming));
         C.stash_s.assign(LSTREAMS, nullptr);
         for (int s = 0; s < LSTREAMS; s++) HIP_OK(hipMalloc((void**)&C.stash_s[s], sizeof(double) * 2 * g.sz));
+        HIP_OK(hipMalloc((void**)&C.stash_b, (size_t)C.esz * 2 * g.sz * WB_MAX));
     }
     // reduction partials: enough for lmax per-block sums of any sweep
     // (the chkdt sweep covers 1..i1 x 1..j1 x 1..k1, one row and column more than the others)
@@ -916,14 +919,47 @@ static int ls_main_impl(Ctx& c, int istep) {
             if (reinit) TRY(reinitialise(l));
         }
     } else {
-        // the level sets are independent here: advection and re-initialisation run
-        // colour by colour on the side streams, the mass corrections one at a time
-        TRY(run_colours(c, advect, false));
-        for (int l = 0; l < c.lmax; l++) {
-            WindowScope ws(c, l);
-            TRY(correct(l));
+        // Colour by colour, all windows of a colour in one launch per kernel: advection,
+        // then the colour's mass corrections one at a time, then (after every colour)
+        // the re-initialisation.  A correction advects inside the band mask of its own
+        // level set's tube, as the reference's loop does (levelset.f90:47-68); the mask
+        // array is shared, and a later colour's tube rewrites it where windows overlap,
+        // so each colour is corrected before the next colour's tubes run.
+        std::vector<std::vector<int>> by_colour(c.ncolour);
+        for (int l = 0; l < c.lmax; l++) by_colour[c.colour[l]].push_back(l);
+        // the reference ends the loop with maxchk and the flag of the last level set's
+        // correction: they are kept aside when a later colour's trigger overwrites them
+        bool saved = false, later = false;
+        const int slots[2] = {DS_MAXCHK, DS_INFL_FLAG};
+        for (const auto& ls : by_colour) {
+            if (ls.empty()) continue;
+            TRY(launch_advect_batch(c, ls));
+            if (!c.P.mass_correction) continue;
+            for (int l : ls) {
+                {
+                    WindowScope ws(c, l);
+                    TRY(correct(l));
+                }
+                if (l == c.lmax - 1) {
+                    for (int q = 0; q < 2; q++)
+                        HIP_OK(hipMemcpyAsync(c.dscal + c.save0 + q, c.dscal + slots[q], sizeof(double),
+                                              hipMemcpyDeviceToDevice, c.stream));
+                    saved = true;
+                    later = false;
+                } else {
+                    bool due = false;
+                    TRY(mass_correction_due(c, l, istep, due));
+                    later = later || due;
+                }
+            }
         }
-        if (reinit) TRY(run_colours(c, reinitialise, true));
+        if (saved && later)
+            for (int q = 0; q < 2; q++)
+                HIP_OK(hipMemcpyAsync(c.dscal + slots[q], c.dscal + c.save0 + q, sizeof(double),
+                                      hipMemcpyDeviceToDevice, c.stream));
+        if (reinit)
+            for (const auto& ls : by_colour)
+                if (!ls.empty()) TRY(launch_reinit_batch(c, ls));
     }
     // update_materials (misc.f90:79-290)
     TRY(launch_materials(c));

@@ -66,11 +66,11 @@ constexpr int TZ_GRID = 32, TZ_WINDOW = 8, TH = 2, TTW = BX + 2 * TH, TTH = BY +
 constexpr int TNLD = (TTP + BX * BY - 1) / (BX * BY), TNS = 2 * TH + 1;
 
 template <class T, int TPD>
-__global__ void __launch_bounds__(BX * BY)
-k_tube_zm(Grid g, int TZ, T NB2, const T* __restrict__ in, T* __restrict__ out, uint8_t* __restrict__ mask) {
+__device__ __forceinline__ void tube_zm_body(const Grid& g, int bid, int TZ, T NB2, const T* __restrict__ in,
+                                             T* __restrict__ out, uint8_t* __restrict__ mask) {
     __shared__ T ring[TNS][TTP];
     const int ntx = (g.nx + BX - 1) / BX, nty = (g.ny + BY - 1) / BY, ntz = (g.nz + TZ - 1) / TZ;
-    const int tile = xcd_tile(blockIdx.x, ntx * nty * ntz);
+    const int tile = xcd_tile(bid, ntx * nty * ntz);
     const int bx = tile % ntx, by = (tile / ntx) % nty, bz = tile / (ntx * nty);
     const int x0 = bx * BX, y0 = by * BY, kb = 1 + bz * TZ;
     const int ke = kb + TZ - 1 < g.nz ? kb + TZ - 1 : g.nz;
@@ -143,6 +143,20 @@ k_tube_zm(Grid g, int TZ, T NB2, const T* __restrict__ in, T* __restrict__ out, 
         out[gidx(g, i, j, g.nz + 2)] = clampv(col(g.nz + 2));
         out[gidx(g, i, j, g.nz + 3)] = clampv(col(g.nz + 3));
     }
+}
+
+template <class T, int TPD>
+__global__ void __launch_bounds__(BX * BY)
+k_tube_zm(Grid g, int TZ, T NB2, const T* __restrict__ in, T* __restrict__ out, uint8_t* __restrict__ mask) {
+    tube_zm_body<T, TPD>(g, blockIdx.x, TZ, NB2, in, out, mask);
+}
+
+// the same over the windows of a batch (g: the whole grid)
+template <class T, int TPD>
+__global__ void __launch_bounds__(BX * BY)
+k_tube_zm_batch(Grid g, WinBatch wb, int TZ, T NB2, const T* in, T* out, uint8_t* mask) {
+    const WinPick w = win_pick(g, wb, blockIdx.x);
+    tube_zm_body<T, TPD>(w.g, w.bid, TZ, NB2, win_ptr(in, wb, w), win_ptr(out, wb, w), mask + w.off);
 }
 
 template <class T>
@@ -307,11 +321,11 @@ __device__ __forceinline__ T adv_rhs(const Grid& g, const T* __restrict__ phi, i
 // (the same operations the threads of those layers perform) and store the linear
 // extrapolation.
 template <class T, int SCHEME, int STAGE, bool P2, bool VELF>
-__global__ void k_adv_stage(Grid g, T dt, const T* phi0, const T* __restrict__ phis, T* out,
-                            const uint8_t* __restrict__ mask, const T* __restrict__ vu, const T* __restrict__ vv,
-                            const T* __restrict__ vw, const double* __restrict__ flag) {
-    if (flag && flag[0] == 0.) return;
-    int i0 = blockIdx.x * BX + threadIdx.x, j0 = blockIdx.y * BY + threadIdx.y, ks = blockIdx.z;
+__device__ __forceinline__ void adv_stage_body(const Grid& g, int bx, int by, int ks, T dt, const T* phi0,
+                                               const T* __restrict__ phis, T* out, const uint8_t* __restrict__ mask,
+                                               const T* __restrict__ vu, const T* __restrict__ vv,
+                                               const T* __restrict__ vw) {
+    int i0 = bx * BX + threadIdx.x, j0 = by * BY + threadIdx.y;
     if (i0 >= g.nx || j0 >= g.ny) return;
     int k = ks - 2;
     auto value = [&](int kk) -> T {
@@ -340,6 +354,28 @@ __global__ void k_adv_stage(Grid g, T dt, const T* phi0, const T* __restrict__ p
     else if (k == g.nz + 1) r = T(2.) * value(g.nz) - value(g.nz - 1);
     else r = value(k);
     out[gidx(g, i0, j0, k)] = r;
+}
+
+template <class T, int SCHEME, int STAGE, bool P2, bool VELF>
+__global__ void k_adv_stage(Grid g, T dt, const T* phi0, const T* __restrict__ phis, T* out,
+                            const uint8_t* __restrict__ mask, const T* __restrict__ vu, const T* __restrict__ vv,
+                            const T* __restrict__ vw, const double* __restrict__ flag) {
+    if (flag && flag[0] == 0.) return;
+    adv_stage_body<T, SCHEME, STAGE, P2, VELF>(g, blockIdx.x, blockIdx.y, blockIdx.z, dt, phi0, phis, out, mask, vu,
+                                               vv, vw);
+}
+
+// the same over the windows of a batch (g: the whole grid); the blocks of a window
+// are its (x, y, layer) blocks of the one-window launch, x fastest
+template <class T, int SCHEME, int STAGE, bool P2, bool VELF>
+__global__ void __launch_bounds__(BX * BY)
+k_adv_stage_batch(Grid g, WinBatch wb, T dt, const T* phi0, const T* phis, T* out, const uint8_t* mask,
+                  const T* vu, const T* vv, const T* vw) {
+    const WinPick w = win_pick(g, wb, blockIdx.x);
+    const int nbx = (w.g.nx + BX - 1) / BX, nby = (w.g.ny + BY - 1) / BY;
+    adv_stage_body<T, SCHEME, STAGE, P2, VELF>(w.g, w.bid % nbx, (w.bid / nbx) % nby, w.bid / (nbx * nby), dt,
+                                               win_ptr(phi0, wb, w), win_ptr(phis, wb, w), win_ptr(out, wb, w),
+                                               mask + w.off, vu + w.off, vv + w.off, vw + w.off);
 }
 
 template <class T>
@@ -433,16 +469,15 @@ __device__ __forceinline__ T reinit_combine(T p0, T ps, T rhs, T dtau) {
 #define ICLS_REINIT_WPE 1
 #endif
 template <class T, int STAGE, bool P2, int RPD, bool WALL>
-__global__ void __launch_bounds__(BX * BY, ICLS_REINIT_WPE)
-k_reinit_zm(Grid g, int RZ, T NB2, double* res, const T* phi0, const T* __restrict__ phis, T* out,
-            T* __restrict__ stash) {
+__device__ __forceinline__ void reinit_zm_body(const Grid& g, int bid, int RZ, T NB2, double* res, const T* phi0,
+                                               const T* __restrict__ phis, T* out, T* __restrict__ stash) {
     __shared__ T ring[RNS][RTP];
     __shared__ T p0s[RR][BX * BY];
     __shared__ int wok[RNS][BX * BY / 64];   // per plane and wavefront: all values equal wv
     __shared__ T wv[RNS][BX * BY / 64];
     __shared__ double smem[16];
     const int ntx = (g.nx + BX - 1) / BX, nty = (g.ny + RTY - 1) / RTY, ntz = (g.nz + RZ - 1) / RZ;
-    const int tile = xcd_tile(blockIdx.x, ntx * nty * ntz);
+    const int tile = xcd_tile(bid, ntx * nty * ntz);
     const int bx = tile % ntx, by = (tile / ntx) % nty, bz = tile / (ntx * nty);
     const int x0 = bx * BX, y0 = by * RTY, kb = 1 + bz * RZ;
     const int ke = kb + RZ - 1 < g.nz ? kb + RZ - 1 : g.nz;
@@ -659,6 +694,25 @@ k_reinit_zm(Grid g, int RZ, T NB2, double* res, const T* phi0, const T* __restri
     }
 }
 
+template <class T, int STAGE, bool P2, int RPD, bool WALL>
+__global__ void __launch_bounds__(BX * BY, ICLS_REINIT_WPE)
+k_reinit_zm(Grid g, int RZ, T NB2, double* res, const T* phi0, const T* __restrict__ phis, T* out,
+            T* __restrict__ stash) {
+    reinit_zm_body<T, STAGE, P2, RPD, WALL>(g, blockIdx.x, RZ, NB2, res, phi0, phis, out, stash);
+}
+
+// the same over the windows of a batch (g: the whole grid); window t keeps its
+// residual in res[lidx[t]] and its two stash planes at stash + 2 t sz
+template <class T, int STAGE, bool P2, int RPD, bool WALL>
+__global__ void __launch_bounds__(BX * BY, ICLS_REINIT_WPE)
+k_reinit_zm_batch(Grid g, WinBatch wb, int RZ, T NB2, double* res, const T* phi0, const T* phis, T* out,
+                  T* stash) {
+    const WinPick w = win_pick(g, wb, blockIdx.x);
+    reinit_zm_body<T, STAGE, P2, RPD, WALL>(w.g, w.bid, RZ, NB2, res + wb.lidx[w.t], win_ptr(phi0, wb, w),
+                                            win_ptr(phis, wb, w), win_ptr(out, wb, w),
+                                            stash + 2 * g.sz * (long)w.t + w.poff);
+}
+
 template <class T, int RPD, bool WALL>
 static void reinit_dispatch_pd(Ctx& c, int rz, int stage, const double* phi0, const double* phis, double* out) {
     dim3 blk(BX, BY, 1);
@@ -723,6 +777,218 @@ int launch_reinit_stage(Ctx& c, int stage, const double* phi0, const double* phi
     if (reinit_fuses_wall(c)) return ICLS_OK;
     if (stage == 3) return bc_ls(c, out, c.stash, c.dscal + c.res_slot, nullptr);
     return bc_ls(c, out, nullptr, nullptr, nullptr);
+}
+
+// ---------------------------------------------------------------- batched windows
+// The windows of one colour in launches of at most WB_MAX windows, each launch holding
+// windows with the same key (template and chunk choices).  Chunks of 8 layers, as in a
+// one-window launch: on 64 drops at 256^3 (MI355X) chunks of 4, 8, 16 and 32 layers
+// gave 8.78, 8.69, 9.03 and 9.97 ms per learned step; the kernels are bound by the
+// serial march of each block, so shorter chains beat fewer halo planes.
+constexpr int REINIT_RZ_BATCH = 8, TZ_BATCH = 8;
+static bool win_full(const Ctx& c, const LsWindow& w) {
+    return w.wx == c.gfull.nx && w.wy == c.gfull.ny && w.wz == c.gfull.nz;
+}
+static Grid win_grid(const Ctx& c, const LsWindow& w) {
+    Grid g = c.g;
+    g.nx = w.wx; g.ny = w.wy; g.nz = w.wz;
+    return g;
+}
+template <class KeyFn, class BlocksFn, class LaunchFn>
+static int for_each_batch(Ctx& c, const std::vector<int>& ls, KeyFn key, BlocksFn blocks, LaunchFn launch) {
+    std::vector<int> keys;
+    for (int l : ls) {
+        const int k = key(c.win[l]);
+        bool seen = false;
+        for (int q : keys) seen = seen || q == k;
+        if (!seen) keys.push_back(k);
+    }
+    for (int k : keys) {
+        WinBatch wb;
+        std::vector<int> members;
+        auto flush = [&]() -> int {
+            if (wb.n == 0) return ICLS_OK;
+            int rc = launch(wb, k, members);
+            wb = WinBatch();
+            members.clear();
+            return rc;
+        };
+        for (int l : ls) {
+            const LsWindow& w = c.win[l];
+            if (key(w) != k) continue;
+            const long nb = blocks(w);
+            if (nb <= 0 || (long)wb.first[wb.n] + nb > (1L << 30)) {
+                set_error("window batch: bad block count");
+                return ICLS_ERR_STATE;
+            }
+            const int t = wb.n;
+            wb.ox[t] = w.ox; wb.oy[t] = w.oy; wb.oz[t] = w.oz;
+            wb.wx[t] = w.wx; wb.wy[t] = w.wy; wb.wz[t] = w.wz;
+            wb.ls[t] = c.lvset[l];
+            wb.lidx[t] = l;
+            wb.first[t + 1] = wb.first[t] + (int)nb;
+            wb.n = t + 1;
+            members.push_back(l);
+            if (wb.n == WB_MAX) { int rc = flush(); if (rc) return rc; }
+        }
+        int rc = flush();
+        if (rc) return rc;
+    }
+    return ICLS_OK;
+}
+
+// tube and the three advection stages of ls_main for the level sets ls, whose windows
+// do not overlap: the launches of advect() in icls_runtime, one per colour instead of
+// one per level set
+template <class T>
+static int advect_batch_t(Ctx& c, const std::vector<int>& ls) {
+    dim3 blk(BX, BY, 1);
+    const T NB2 = (T)c.NB2;
+    // tube: chunk length and planes ahead as tube_dispatch chooses them
+    auto tz_of = [&](const LsWindow& w) { return win_full(c, w) ? TZ_GRID : TZ_BATCH; };
+    auto tkey = [&](const LsWindow& w) {
+        const int tz = tz_of(w);
+        return tz * 2 + ((tz % 2 == 0 && (w.wz % tz) % 2 == 0) ? 1 : 0);
+    };
+    auto tblocks = [&](const LsWindow& w) -> long {
+        const int tz = tz_of(w);
+        return (long)((w.wx + BX - 1) / BX) * ((w.wy + BY - 1) / BY) * ((w.wz + tz - 1) / tz);
+    };
+    int rc;
+    {
+        ProfScope ps(c, "k_tube");
+        rc = for_each_batch(c, ls, tkey, tblocks, [&](const WinBatch& wb, int k, const std::vector<int>&) -> int {
+            const int tz = k / 2;
+            if (k & 1)
+                hipLaunchKernelGGL((k_tube_zm_batch<T, 2>), dim3(wb.first[wb.n]), blk, 0, c.stream, c.g, wb, tz, NB2,
+                                   (const T*)nullptr, as<T>(c.ls_a), c.mask);
+            else
+                hipLaunchKernelGGL((k_tube_zm_batch<T, 1>), dim3(wb.first[wb.n]), blk, 0, c.stream, c.g, wb, tz, NB2,
+                                   (const T*)nullptr, as<T>(c.ls_a), c.mask);
+            HIP_OK(hipGetLastError());
+            return ICLS_OK;
+        });
+        if (rc) return rc;
+    }
+    auto akey = [&](const LsWindow&) { return 0; };
+    auto ablocks = [&](const LsWindow& w) -> long {
+        return (long)((w.wx + BX - 1) / BX) * ((w.wy + BY - 1) / BY) * (w.wz + 6);
+    };
+    const T* sa = as<T>(c.ls_a);
+    const T* sb = as<T>(c.ls_b);
+    const T* sc = as<T>(c.ls_c);
+    const T* in[3][2] = {{sa, sa}, {sa, sb}, {sa, sc}};      // phi0, phis per stage
+    T* out[3] = {as<T>(c.ls_b), as<T>(c.ls_c), nullptr};     // stage 3 writes the level set
+    for (int st = 1; st <= 3; st++) {
+        ProfScope ps(c, st == 1 ? "k_adv_stage1" : "k_adv_stage23");
+        rc = for_each_batch(c, ls, akey, ablocks, [&](const WinBatch& wb, int, const std::vector<int>&) -> int {
+#define ADVB(S, ST, P)                                                                                            \
+    hipLaunchKernelGGL((k_adv_stage_batch<T, S, ST, P, true>), dim3(wb.first[wb.n]), blk, 0, c.stream, c.g, wb,     \
+                       (T)c.dt, in[ST - 1][0], in[ST - 1][1], out[ST - 1], (const uint8_t*)c.mask, as<T>(c.unew),  \
+                       as<T>(c.vnew), as<T>(c.wnew))
+#define ADVB_ST(S)                                                                                     \
+    do {                                                                                               \
+        if (c.g.pow2) { if (st == 1) ADVB(S, 1, true); else if (st == 2) ADVB(S, 2, true); else ADVB(S, 3, true); } \
+        else { if (st == 1) ADVB(S, 1, false); else if (st == 2) ADVB(S, 2, false); else ADVB(S, 3, false); }       \
+    } while (0)
+            if (c.P.ls_adv_space == ICLS_ADV_HOUC5) ADVB_ST(ICLS_ADV_HOUC5);
+            else ADVB_ST(ICLS_ADV_WENO5);
+#undef ADVB_ST
+#undef ADVB
+            HIP_OK(hipGetLastError());
+            return ICLS_OK;
+        });
+        if (rc) return rc;
+    }
+    return ICLS_OK;
+}
+
+int launch_advect_batch(Ctx& c, const std::vector<int>& ls) {
+    return c.fp32 ? advect_batch_t<float>(c, ls) : advect_batch_t<double>(c, ls);
+}
+
+// the three re-initialisation stages for the level sets ls (windows that do not
+// overlap), with the chunk length, planes ahead and wall folding of reinit_dispatch
+// per window; a window whose last chunk is one layer runs k_bc_ls after each stage
+template <class T>
+static int reinit_batch_t(Ctx& c, const std::vector<int>& ls) {
+    dim3 blk(BX, BY, 1);
+    const T NB2 = (T)c.NB2;
+    auto rz_of = [&](const LsWindow& w) {
+        const int rz = win_full(c, w) ? REINIT_RZ : REINIT_RZ_BATCH;
+        return w.wz < rz ? w.wz : rz;
+    };
+    // key: rz * 4 + 2 deep + wall
+    auto key = [&](const LsWindow& w) {
+        const int rz = rz_of(w);
+        const bool deep = rz % RPD_MAX == 0 && (w.wz % rz) % RPD_MAX == 0;
+        const bool wall = fuses_wall(win_grid(c, w), rz);
+        return rz * 4 + (deep ? 2 : 0) + (wall ? 1 : 0);
+    };
+    auto blocks = [&](const LsWindow& w) -> long {
+        const int rz = rz_of(w);
+        return (long)((w.wx + BX - 1) / BX) * ((w.wy + RTY - 1) / RTY) * ((w.wz + rz - 1) / rz);
+    };
+    double* res = c.dscal + c.res_b0;
+    T* stash = as<T>(c.stash_b);
+    const T* sa = as<T>(c.ls_a);
+    const T* sb = as<T>(c.ls_b);
+    const T* phis[3] = {nullptr, sa, sb};
+    T* out[3] = {as<T>(c.ls_a), as<T>(c.ls_b), nullptr};
+    for (int st = 1; st <= 3; st++) {
+        ProfScope ps(c, st == 1 ? "k_reinit_stage1" : "k_reinit_stage23");
+        int rc = for_each_batch(c, ls, key, blocks, [&](const WinBatch& wb, int k, const std::vector<int>& members) -> int {
+            const int rz = k / 4;
+            const bool deep = k & 2, wall = k & 1;
+#define RB(S, P, PD, W)                                                                                            \
+    hipLaunchKernelGGL((k_reinit_zm_batch<T, S, P, PD, W>), dim3(wb.first[wb.n]), blk, 0, c.stream, c.g, wb, rz,   \
+                       NB2, res, (const T*)nullptr, phis[S - 1], out[S - 1], stash)
+#define RB_S(P, PD, W)                                                                \
+    do {                                                                              \
+        if (st == 1) RB(1, P, PD, W); else if (st == 2) RB(2, P, PD, W); else RB(3, P, PD, W); \
+    } while (0)
+#define RB_W(P, PD)                            \
+    do {                                       \
+        if (wall) RB_S(P, PD, true);           \
+        else RB_S(P, PD, false);               \
+    } while (0)
+            if (c.g.pow2) { if (deep) RB_W(true, RPD_MAX); else RB_W(true, 1); }
+            else { if (deep) RB_W(false, RPD_MAX); else RB_W(false, 1); }
+#undef RB_W
+#undef RB_S
+#undef RB
+            HIP_OK(hipGetLastError());
+            if (!wall) {
+                // the wall layers of each window as launch_reinit_stage does without folding
+                const Grid g0 = c.g;
+                for (int t = 0; t < wb.n; t++) {
+                    const int l = members[t];
+                    const LsWindow& w = c.win[l];
+                    const long poff = w.ox + g0.sy * (long)w.oy, off = poff + g0.sz * (long)w.oz;
+                    double* o = st == 3 ? c.lvset[l] : (st == 1 ? c.ls_a : c.ls_b);
+                    o = (double*)((char*)o + off * (long)c.esz);
+                    const double* ref =
+                        st == 3 ? (const double*)((char*)c.stash_b + (2 * g0.sz * (long)t + poff) * (long)c.esz) : nullptr;
+                    c.g = win_grid(c, w);
+                    int rc2 = bc_ls(c, o, ref, st == 3 ? res + l : nullptr, nullptr);
+                    c.g = g0;
+                    if (rc2) return rc2;
+                }
+            }
+            if (st == 3)
+                for (int t = 0; t < wb.n; t++)
+                    if (members[t] == c.lmax - 1)
+                        HIP_OK(hipMemcpyAsync(c.dscal + DS_REINIT_RES, res + c.lmax - 1, sizeof(double),
+                                              hipMemcpyDeviceToDevice, c.stream));
+            return ICLS_OK;
+        });
+        if (rc) return rc;
+    }
+    return ICLS_OK;
+}
+
+int launch_reinit_batch(Ctx& c, const std::vector<int>& ls) {
+    return c.fp32 ? reinit_batch_t<float>(c, ls) : reinit_batch_t<double>(c, ls);
 }
 
 // ---------------------------------------------------------------- level-set windows

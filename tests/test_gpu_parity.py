@@ -241,6 +241,48 @@ def test_windowed_inflation_matches_oracle():
         g.close()
 
 
+def test_overlapping_windows_correct_in_own_band():
+    """Two drops whose windows overlap (different colours), both short of volume by 1%
+    so that both corrections fire: each correction advects inside its own tube's band
+    mask (levelset.f90:47-68), although the other drop's tube writes the shared mask
+    array where the windows overlap.  Against the oracle: every field and both level
+    sets after step 2 at 1e-10, and maxchk of the last level set after step 1, where
+    the 1% deficit sets it.  maxchk is proportional to V - V0 with V = 1 - sum(H)/N
+    (levelset.f90:505): the sum of N ~ 5e5 terms near 1 is taken in another order here
+    than in the oracle, which moves V by about sqrt(N) eps, so maxchk agrees to about
+    sqrt(N) eps / |V - V0| (4x that is the bound; ~6e-8 here)."""
+    p = P.base_params(128, nz=32)
+    p.lz = 0.25
+    p.rho1 = p.rho2 = 1.0
+    p.miu1 = p.miu2 = 1.0
+    p.timestep = 1e-5
+    p.mass_correction = 2
+    r = 5.0 / 128
+    p.set_drops([(0.40, 0.50, 0.125, r), (0.525, 0.52, 0.125, r)])
+    o, g = _pair(p)
+    try:
+        o.init_program()
+        g.init_program()
+        for l in range(2):
+            v, _ = o.volume(l)
+            o.set_volume(l, v, v * (1 + 1e-2))
+            g.set_volume(l, v, v * (1 + 1e-2))
+        o.step(1, 0)
+        g.step(1, 0)
+        mo, mg = o.scalar(P.S_MAXCHK), g.scalar(P.S_MAXCHK)
+        n = p.itot * p.jtot * p.ktot
+        bound = 4 * np.sqrt(n) * np.finfo(float).eps / (1e-2 * v)
+        assert mo != 0.0 and abs(mg - mo) <= bound * abs(mo), (mg, mo, bound)
+        o.step(2, 0)
+        g.step(2, 0)
+        _compare(o, g, [f for f in FIELDS if f != P.F_LVSET], TOL, "overlapping windows, corrections")
+        for l in range(2):
+            e = rel_linf(_clip_ls(g.download(P.F_LVSET, l), p), _clip_ls(o.get(P.F_LVSET, l), p))
+            assert e <= TOL, (l, e)
+    finally:
+        g.close()
+
+
 def test_windows_equal_whole_grid_learned():
     """MIXED_LEARNED (FP32 fields, learned closure): windowed and whole-grid level
     sets give the same fields after 9 steps, and the same volumes bit for bit (the
