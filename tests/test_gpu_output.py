@@ -141,7 +141,7 @@ def test_post2d_columns(tmp_path):
 
 
 def test_fortran_driver_writes_output_files(tmp_path):
-    """iout2d / iout3d / ioutfld of the driver (main.f90:262-277) write yz_mid, para
+    """iout2d / iout3d / ioutfld of the Fortran main program (main.f90:262-277) write yz_mid, para
     .vtk and fld files into data/ under the reference's names."""
     exe = os.path.join(ROOT, "icls-release_amd", "icls_fortran")
     if not os.path.exists(exe):

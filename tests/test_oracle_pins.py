@@ -25,7 +25,7 @@ REF_VFFT = os.path.join(ROOT, "oracle", "_ref", "libvfftpack.so")
 # ---------------------------------------------------------------- FFT (VRFFTF / VRFFTB)
 def _vfftpack():
     if not os.path.exists(REF_VFFT):
-        pytest.skip("VFFTPACK reference build absent (needs /root/reference; make -C oracle ref)")
+        pytest.skip("VFFTPACK reference build absent (make -C oracle ref builds it from the reference tree)")
     return ctypes.CDLL(REF_VFFT)
 
 
