@@ -1085,6 +1085,90 @@ __global__ void __launch_bounds__(256) k_ztransform(Grid g, const double* __rest
     }
 }
 
+// The same products with half the work for even nz = 2H, from the symmetry of the two
+// matrices (0-based indices): M_b(nz-1-k, i) = (-1)^i M_b(k, i) and M_f(k, nz-1-i) =
+// (-1)^k M_f(k, i).
+//   VCOSQB: out(2j)   = sum_{k<H} (x(k) + x(nz-1-k)) M_b(k, 2j)
+//           out(2j+1) = sum_{k<H} (x(k) - x(nz-1-k)) M_b(k, 2j+1)
+//   VCOSQF: E(i) = sum_{m<H} x(2m) M_f(2m, i),  O(i) = sum_{m<H} x(2m+1) M_f(2m+1, i),
+//           out(i) = E(i) + O(i),  out(nz-1-i) = E(i) - O(i)   (i < H)
+// P and Q are the two H x H matrices (row = m, contiguous j or i): M_b(m, 2j) and
+// M_b(m, 2j+1) forward, M_f(2m, i) and M_f(2m+1, i) inverse.  A block computes 64
+// columns x 64 indices j (or i) of both halves.
+template <int INV>
+__global__ void __launch_bounds__(256) k_ztransform_sym(Grid g, const double* __restrict__ P,
+                                                        const double* __restrict__ Q, const double* __restrict__ in,
+                                                        double* __restrict__ out) {
+    __shared__ double A0[ZT_K][ZT_COLS], A1[ZT_K][ZT_COLS];
+    __shared__ double B0[ZT_K][ZT_OUT], B1[ZT_K][ZT_OUT];
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    const long ncol = (long)g.nx * g.ny;
+    const long col0 = (long)blockIdx.x * ZT_COLS;
+    const int j0 = blockIdx.y * ZT_OUT;
+    const int nz = g.nz, H = nz / 2;
+    const double* in0 = in + g.sz * 3;   // layer k = 1
+    double e[4][4] = {}, o[4][4] = {};
+    for (int mb = 0; mb < H; mb += ZT_K) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int el = threadIdx.x + 256 * q, mq = el >> 6, cc = el & 63;
+            const int m = mb + mq;
+            const long col = col0 + cc;
+            double a0 = 0., a1 = 0.;
+            if (m < H && col < ncol) {
+                if (INV) {
+                    a0 = in0[col + g.sz * (2 * m)];
+                    a1 = in0[col + g.sz * (2 * m + 1)];
+                } else {
+                    const double xl = in0[col + g.sz * m], xh = in0[col + g.sz * (nz - 1 - m)];
+                    a0 = xl + xh;
+                    a1 = xl - xh;
+                }
+            }
+            A0[mq][cc] = a0;
+            A1[mq][cc] = a1;
+            const bool okb = m < H && j0 + cc < H;
+            B0[mq][cc] = okb ? P[(long)m * H + j0 + cc] : 0.;
+            B1[mq][cc] = okb ? Q[(long)m * H + j0 + cc] : 0.;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int mq = 0; mq < ZT_K; mq++) {
+            double a0[4], a1[4], b0[4], b1[4];
+#pragma unroll
+            for (int r = 0; r < 4; r++) { a0[r] = A0[mq][tx + 16 * r]; a1[r] = A1[mq][tx + 16 * r]; }
+#pragma unroll
+            for (int t = 0; t < 4; t++) { b0[t] = B0[mq][ty + 16 * t]; b1[t] = B1[mq][ty + 16 * t]; }
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+#pragma unroll
+                for (int t = 0; t < 4; t++) {
+                    e[r][t] = fma(a0[r], b0[t], e[r][t]);
+                    o[r][t] = fma(a1[r], b1[t], o[r][t]);
+                }
+        }
+        __syncthreads();
+    }
+    double* out0 = out + g.sz * 3;
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        const int j = j0 + ty + 16 * t;
+        if (j >= H) continue;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const long col = col0 + tx + 16 * r;
+            if (col >= ncol) continue;
+            if (INV) {
+                out0[col + g.sz * j] = e[r][t] + o[r][t];
+                out0[col + g.sz * (nz - 1 - j)] = e[r][t] - o[r][t];
+            } else {
+                out0[col + g.sz * (2 * j)] = e[r][t];
+                out0[col + g.sz * (2 * j + 1)] = o[r][t];
+            }
+        }
+    }
+}
+
 // Gauss elimination along y per (i, k) line (solver.f90:253-290): a = c = dyi^2,
 // b = -(a + c), the end rows changed by BC_in_y (initsolver.f90:88-124; 1 Walls:
 // b(1) += a(1), b(jtot) += c(jtot); 2 InOut: b(1) += a(1), b(jtot) -= c(jtot);
@@ -1178,13 +1262,14 @@ __global__ void __launch_bounds__(64) k_thomas_ycos(Grid g, int bcy, const doubl
 // (initsolver.f90:132-145)
 static int cos_tables(Ctx& c) {
     if (c.cos_tab) return ICLS_OK;
-    const int nx = c.g.nx, nz = c.g.nz;
+    const int nx = c.g.nx, nz = c.g.nz, H = nz / 2;
     const double pi = c.pi, s = sqrt(4. * nz);
-    std::vector<double> t((size_t)2 * nz * nz + (nx + 1) + (nz + 1), 0.);
+    std::vector<double> t((size_t)2 * nz * nz + (nx + 1) + (nz + 1) + (size_t)4 * H * H, 0.);
     double* mb = t.data();
     double* mf = mb + (size_t)nz * nz;
     double* xrt = mf + (size_t)nz * nz;
     double* zrt = xrt + nx + 1;
+    double* half = zrt + nz + 1;   // the halves of the even-nz products: P_b, Q_b, P_f, Q_f
     for (int k = 1; k <= nz; k++)
         for (int i = 1; i <= nz; i++) {
             mb[(size_t)(k - 1) * nz + (i - 1)] = 4. * cos((double)((2 * k - 1) * (i - 1)) * pi / (2. * nz)) / s;
@@ -1199,6 +1284,13 @@ static int cos_tables(Ctx& c) {
     xrt[1] = 0.;
     xrt[nx] = -4. * c.g.dxi * c.g.dxi;
     for (int k = 1; k <= nz; k++) zrt[k] = (c.g.dzi * c.g.dzi) * (-2. + 2. * cos((double)(k - 1) * pi / nz));
+    for (int m = 0; m < H; m++)
+        for (int j = 0; j < H; j++) {
+            half[(size_t)m * H + j] = mb[(size_t)m * nz + 2 * j];
+            half[(size_t)H * H + (size_t)m * H + j] = mb[(size_t)m * nz + 2 * j + 1];
+            half[(size_t)2 * H * H + (size_t)m * H + j] = mf[(size_t)(2 * m) * nz + j];
+            half[(size_t)3 * H * H + (size_t)m * H + j] = mf[(size_t)(2 * m + 1) * nz + j];
+        }
     HIP_OK(hipMalloc((void**)&c.cos_tab, sizeof(double) * t.size()));
     HIP_OK(hipMemcpy(c.cos_tab, t.data(), sizeof(double) * t.size(), hipMemcpyHostToDevice));
     return ICLS_OK;
@@ -1214,6 +1306,12 @@ int launch_solve_cos(Ctx& c, int bcy) {
     const double* mf = mb + (size_t)nz * nz;
     const double* xrt = mf + (size_t)nz * nz;
     const double* zrt = xrt + c.g.nx + 1;
+    const int H = nz / 2;
+    const double* half = zrt + nz + 1;
+    // the halved products from nz = 128 on: at 256^3 they took 196 / 227 us against
+    // 347 / 352 us; at nz = 64 (H = 32, half of each 64-wide tile idle) 106 / 126 us
+    // against 99 / 103 us
+    const bool sym = nz % 2 == 0 && nz >= 128;
     double* p = c.p;
     double* w = c.scratch;
     {
@@ -1223,9 +1321,14 @@ int launch_solve_cos(Ctx& c, int bcy) {
     }
     const long ncol = (long)c.g.nx * c.g.ny;
     dim3 zg((unsigned)((ncol + ZT_COLS - 1) / ZT_COLS), (unsigned)((nz + ZT_OUT - 1) / ZT_OUT));
+    dim3 zh((unsigned)((ncol + ZT_COLS - 1) / ZT_COLS), (unsigned)((H + ZT_OUT - 1) / ZT_OUT));
     {
         ProfScope ps(c, "k_ztransform_fwd");
-        hipLaunchKernelGGL(k_ztransform, zg, dim3(256), 0, c.stream, c.g, mb, (const double*)p, w);
+        if (sym)
+            hipLaunchKernelGGL(k_ztransform_sym<0>, zh, dim3(256), 0, c.stream, c.g, half, half + (size_t)H * H,
+                               (const double*)p, w);
+        else
+            hipLaunchKernelGGL(k_ztransform, zg, dim3(256), 0, c.stream, c.g, mb, (const double*)p, w);
         HIP_OK(hipGetLastError());
     }
     {
@@ -1236,7 +1339,11 @@ int launch_solve_cos(Ctx& c, int bcy) {
     }
     {
         ProfScope ps(c, "k_ztransform_inv");
-        hipLaunchKernelGGL(k_ztransform, zg, dim3(256), 0, c.stream, c.g, mf, (const double*)w, p);
+        if (sym)
+            hipLaunchKernelGGL(k_ztransform_sym<1>, zh, dim3(256), 0, c.stream, c.g, half + (size_t)2 * H * H,
+                               half + (size_t)3 * H * H, (const double*)w, p);
+        else
+            hipLaunchKernelGGL(k_ztransform, zg, dim3(256), 0, c.stream, c.g, mf, (const double*)w, p);
         HIP_OK(hipGetLastError());
     }
     ProfScope ps(c, "k_rfft_x_inv");

@@ -487,12 +487,14 @@ def test_solver2d_parity(n, nz):
 
 @pytest.mark.parametrize("bcy", [P.BCY_WALLS, P.BCY_INOUT, P.BCY_OUTOUT])
 @pytest.mark.parametrize("n,nz,mode", [(16, 8, P.MODE_FP64), (64, 24, P.MODE_FP64), (32, 80, P.MODE_FP64),
-                                       (256, 16, P.MODE_FP64), (32, 40, P.MODE_MIXED)])
+                                       (256, 16, P.MODE_FP64), (32, 40, P.MODE_MIXED), (16, 128, P.MODE_FP64),
+                                       (16, 160, P.MODE_FP64)])
 def test_solver2d_cos_parity(bcy, n, nz, mode):
     """solver2d_cos (solver.f90:202-317) with each BC_in_y rule (initsolver.f90:84-124):
     FFT x, VCOSQB z, Gauss elimination y, VCOSQF z, inverse FFT x.  nz = 24, 40 and 80
-    leave partial 16-layer passes and 64-output tiles in the z transform; the mixed
-    mode runs the same FP64 path on its double pressure."""
+    leave partial 16-layer passes and 64-output tiles in the dense z transform; nz = 128
+    and 160 take the halved (even/odd) products, 160 with partial tiles; the mixed mode
+    runs the same FP64 path on its double pressure."""
     p = small_drop(n, nz=nz, mode=mode)
     o, g = _pair(p)
     try:
