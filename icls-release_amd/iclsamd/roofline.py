@@ -71,7 +71,7 @@ def step_bytes_per_cell(calls, mode_fp32=False):
 # profiler scope name -> the kernel symbol (template arguments as the profiler prints
 # them; {T} is float or double) whose HBM traffic profiles/traffic.json records
 _TRAFFIC_KERNEL = {
-    "k_momentum": "k_momentum_zm<{T}, 0, true>",
+    "k_momentum": "k_momentum_zm<{T}, 0, true, true>",   # power-of-two grid, 32-bit offsets
     "k_correc": "k_correc<{T}>",
     "k_gfm": "k_gfm<{T}, 2>",
     "k_tube": "k_tube_zm<{T}, 2>",

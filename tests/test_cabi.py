@@ -99,3 +99,17 @@ def test_fortran_driver_links_the_library():
         pytest.skip("Fortran driver not built (make -C icls-release_amd fortran)")
     out = subprocess.run(["ldd", exe], capture_output=True, text=True).stdout
     assert "libicls.so" in out
+
+
+def test_bench_traffic_table_names_profiled_kernels():
+    """Every kernel the bench reports traffic for is a key of the committed
+    profiles/traffic.json for both bench workloads (a renamed template would otherwise
+    leave roofline.traffic empty)."""
+    import json
+    from iclsamd import roofline as R
+    table = json.load(open(os.path.join(ROOT, "profiles", "traffic.json")))
+    for workload, fp32 in (("bubble256_learned", True), ("bubble256", False)):
+        for scope in R._TRAFFIC_KERNEL:
+            key = R.traffic_key(scope, fp32)
+            assert key in table[workload], (workload, scope, key)
+            assert table[workload][key]["bytes_per_launch"] > 0
