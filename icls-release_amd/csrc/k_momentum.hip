@@ -44,6 +44,10 @@ struct MomParams {
 // (xcd_tile).  x and y wrap with a mask: itot and jtot are powers of two.
 constexpr int TW = BX + 2, TH = BY + 2, TP = TW * TH;   // tile plane with halo
 constexpr int NLD = (TP + BX * BY - 1) / (BX * BY);      // plane elements per thread
+// pressure planes: the tile with a +1 halo in x and y (the split-pressure term reads
+// the cell and its x+1, y+1 and z+1 neighbours)
+constexpr int PW = BX + 1, PP = PW * (BY + 1);
+constexpr int PLD = (PP + BX * BY - 1) / (BX * BY);
 
 // Element q of a field.  With O32 the index and the byte offset are 32-bit, so an access
 // is the field's base (uniform, in scalar registers) plus one 32-bit vector offset and
@@ -69,6 +73,9 @@ k_momentum_zm(Grid g, MomParams m, const double* __restrict__ dscal, const T* __
               const T* __restrict__ p_zold, T* __restrict__ dudt, T* __restrict__ dvdt, T* __restrict__ dwdt,
               const uint8_t* __restrict__ jm, const uint8_t* __restrict__ jmo, const T* __restrict__ zp) {
     __shared__ T ring[5][4][TP];   // u, v, w, miu, rho x four planes
+    // pnew and pold x three planes: at step k the derive for layer k+1 reads planes k+1
+    // and k+2 while plane k+3 is stored into the slot of plane k
+    __shared__ double pring[2][3][RHS_ONLY ? 1 : PP];
     using Ix = typename std::conditional<O32, uint32_t, long>::type;
     auto cell = [&](int i_, int j_, int k_) -> Ix { return (Ix)i_ + (Ix)g.sy * (Ix)j_ + (Ix)g.sz * (Ix)(k_ + 2); };
     const int MZ = m.mz;
@@ -89,6 +96,36 @@ k_momentum_zm(Grid g, MomParams m, const double* __restrict__ dscal, const T* __
         const int lx = e % TW - 1, ly = e / TW - 1;
         poff[r] = (Ix)((x0 + lx) & xm) + (Ix)g.sy * (Ix)((y0 + ly) & ym);
     }
+    Ix qoff[PLD];
+#pragma unroll
+    for (int r = 0; r < PLD; r++) {
+        int e = tid + r * BX * BY;
+        if (e >= PP) e = PP - 1;
+        qoff[r] = (Ix)((x0 + e % PW) & xm) + (Ix)g.sy * (Ix)((y0 + e / PW) & ym);
+    }
+    double pnx[2][PLD];
+    auto pfetch = [&](int k) {
+        if (RHS_ONLY) return;
+        const Ix base = (Ix)g.sz * (Ix)(k + 2);
+#pragma unroll
+        for (int r = 0; r < PLD; r++) {
+            pnx[0][r] = el<O32>(pnew, base + qoff[r]);
+            pnx[1][r] = el<O32>(pold, base + qoff[r]);
+        }
+    };
+    auto pslot = [&](int k) { return (k - kb + 3) % 3; };   // k >= kb
+    auto pstash = [&](int k) {
+        if (RHS_ONLY) return;
+        const int sl = pslot(k);
+#pragma unroll
+        for (int r = 0; r < PLD; r++) {
+            const int e = tid + r * BX * BY;
+            if (e < PP) {
+                pring[0][sl][e] = pnx[0][r];
+                pring[1][sl][e] = pnx[1][r];
+            }
+        }
+    };
     // planes travel global -> registers -> LDS
     T nxt[5][NLD];
     auto fetch = [&](int k, T (*dst)[NLD]) {
@@ -111,6 +148,11 @@ k_momentum_zm(Grid g, MomParams m, const double* __restrict__ dscal, const T* __
     for (int k = kb - 1; k <= kb + 1; k++) {
         fetch(k, nxt);
         stash(slot_of(k), nxt);
+    }
+    // pressure planes kb .. kb + 2 (no further than ke + 1, the last one read)
+    for (int k = kb; k <= kb + 2 && k <= ke + 1; k++) {
+        pfetch(k);
+        pstash(k);
     }
 
     // A thread past the end of x or y (nx < 64 or ny < 4) takes the wrapped cell: its
@@ -149,10 +191,7 @@ k_momentum_zm(Grid g, MomParams m, const double* __restrict__ dscal, const T* __
         pd[6] = *((jo & 2u) ? &el<O32>(p_yold, q) : z0);
         pd[7] = *((jn & 4u) ? &el<O32>(p_z, q) : z0);
         pd[8] = *((jo & 4u) ? &el<O32>(p_zold, q) : z0);
-        pp[0] = el<O32>(pnew, q); pp[1] = el<O32>(pold, q);
-        pp[2] = el<O32>(pnew, qx); pp[3] = el<O32>(pold, qx);
-        pp[4] = el<O32>(pnew, qy); pp[5] = el<O32>(pold, qy);
-        pp[6] = el<O32>(pnew, qz); pp[7] = el<O32>(pold, qz);
+        (void)qx; (void)qy; (void)qz;
     };
     T du0 = T(0.), dv0 = T(0.), dw0 = T(0.);
     double gpx = 0., gpy = 0., gpz = 0.;
@@ -167,8 +206,19 @@ k_momentum_zm(Grid g, MomParams m, const double* __restrict__ dscal, const T* __
         const double po = m.pprev ? b - SB : b;
         return m.pfirst ? po - SA : po + SB - SA;
     };
-    auto derive = [&]() {
+    const int cp = ty * PW + tx;   // own cell in the pressure plane
+    auto derive = [&](int L) {
         if (RHS_ONLY) return;
+        {
+            const double* N0 = pring[0][pslot(L)];
+            const double* O0 = pring[1][pslot(L)];
+            const double* N1 = pring[0][pslot(L + 1)];
+            const double* O1 = pring[1][pslot(L + 1)];
+            pp[0] = N0[cp]; pp[1] = O0[cp];
+            pp[2] = N0[cp + 1]; pp[3] = O0[cp + 1];
+            pp[4] = N0[cp + PW]; pp[5] = O0[cp + PW];
+            pp[6] = N1[cp]; pp[7] = O1[cp];
+        }
         du0 = pd[0]; dv0 = pd[1]; dw0 = pd[2];
         // split pressure (ab2.f90:79-101): p_hat = 2 p^n - p^(n-1), p_hat_j = 2 p_j - p_j,old
         // (ex_n = 2, ex_o = 1 at a constant dt)
@@ -195,9 +245,11 @@ k_momentum_zm(Grid g, MomParams m, const double* __restrict__ dscal, const T* __
     };
     fetch_mask(kb);
     fetch_point(kb);
-    derive();
-    fetch_mask(min(kb + 1, ke));
     __syncthreads();
+    derive(kb);
+    fetch_mask(min(kb + 1, ke));
+    // the first step stores plane kb + 3 into the slot of plane kb, read just above
+    if (!RHS_ONLY) __syncthreads();
     const Sp<T, P2> sx{T(g.dx), T(g.dxi)}, sy{T(g.dy), T(g.dyi)}, sz{T(g.dz), T(g.dzi)};
     const T qt = T(0.25), two = T(2.), one = T(1.);
     const T rho_avr = T(dscal[DS_RHO_AVR]);
@@ -207,6 +259,7 @@ k_momentum_zm(Grid g, MomParams m, const double* __restrict__ dscal, const T* __
     for (int k = kb; k <= ke; k++) {
         const Ix q = cell(i, j, k);
         fetch(min(k + 2, ke + 1), nxt);
+        pfetch(min(k + 3, ke + 1));
         fetch_point(min(k + 1, ke));
         fetch_mask(min(k + 2, ke));
         const T* U[3] = {ring[0][slot_of(k - 1)], ring[0][slot_of(k)], ring[0][slot_of(k + 1)]};
@@ -347,8 +400,9 @@ k_momentum_zm(Grid g, MomParams m, const double* __restrict__ dscal, const T* __
             }
         }
 #undef AT
-        derive();
+        derive(min(k + 1, ke));
         stash(slot_of(k + 2), nxt);
+        if (k + 3 <= ke + 1) pstash(k + 3);
         __syncthreads();
     }
 }
