@@ -101,6 +101,8 @@ struct WinBatch {
     const void* ls[WB_MAX] = {};
     int lidx[WB_MAX] = {};      // level-set index of window t (its residual slot)
 };
+// the table travels in the kernel arguments, which hold 4 KiB with the other arguments
+static_assert(sizeof(WinBatch) <= 3072, "WinBatch too large for the kernel argument segment");
 
 struct Ctx {
     icls_params P{};
