@@ -147,8 +147,8 @@ def main():
     a = ap.parse_args()
 
     if a.gpus != 1:
-        print("bench.py: this build runs the whole domain on one GPU; --gpus %d is not supported "
-              "(multi-GPU sharding is design-only)" % a.gpus, file=sys.stderr)
+        print("bench.py: the solver runs the whole domain on one GPU; --gpus %d is not supported "
+              "(multi-GPU sharding is not implemented)" % a.gpus, file=sys.stderr)
         sys.exit(2)
 
     first = "learned" if a.mode == "both" else a.mode

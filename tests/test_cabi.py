@@ -8,6 +8,7 @@ import ctypes
 import os
 import re
 import subprocess
+import sys
 import tempfile
 
 import pytest
@@ -113,3 +114,12 @@ def test_bench_traffic_table_names_profiled_kernels():
             key = R.traffic_key(scope, fp32)
             assert key in table[workload], (workload, scope, key)
             assert table[workload][key]["bytes_per_launch"] > 0
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py --gpus N with N > 1 exits with a message and a nonzero status (the
+    solver is single-GPU), before touching the library or the device."""
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120, cwd=ROOT)
+    assert r.returncode != 0
+    assert "one GPU" in r.stderr and "--gpus 2" in r.stderr
