@@ -201,11 +201,11 @@ static int check_params(const icls_params* p) {
     if (p->mass_correction < 0 || p->mass_correction > 2) { set_error("mass_correction must be 0, 1 or 2"); return ICLS_ERR_PARAM; }
     if (p->reinit_step < 1 || p->inflate_step < 1) { set_error("reinit_step and inflate_step must be >= 1"); return ICLS_ERR_PARAM; }
     if (p->re_initialization && p->reinit_iter_max != 1) {
-        set_error("reinit_iter_max other than 1 is not supported by this build");
+        set_error("reinit_iter_max other than 1 is not supported");
         return ICLS_ERR_UNSUPPORTED;
     }
     if (p->mode != ICLS_MODE_FP64 && p->mode != ICLS_MODE_MIXED && p->mode != ICLS_MODE_MIXED_LEARNED) {
-        set_error("precision mode %d not available in this build", p->mode);
+        set_error("precision mode %d is not available", p->mode);
         return ICLS_ERR_UNSUPPORTED;
     }
     return ICLS_OK;
@@ -1469,7 +1469,7 @@ int32_t icls_write_vtk(const char* path) {
 // post2d (output.f90:179-427): the Tecplot ASCII mid-yz plane i = itot/2 with the
 // columns y, z, u (x-centred), v and w (centred in y and z, w = 0 below k = 1),
 // p, the first level set, jiemian and fixhv.  fixhv is the Heaviside of the static
-// level set (init.f90:1578), which this build does not carry: its column is 0.
+// level set (init.f90:1578), which this library does not carry: its column is 0.
 // The header lines are list-directed writes, spaced as flang writes them.
 int32_t icls_post2d(const char* path) {
     CTX_OR_FAIL();

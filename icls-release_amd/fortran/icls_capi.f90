@@ -155,6 +155,11 @@ module icls_capi
        character(kind=c_char), dimension(*), intent(in) :: path
        integer(c_int32_t), value :: nr
      end function
+     integer(c_int32_t) function icls_init_program_restart(path, begin) bind(c, name='icls_init_program_restart')
+       import :: c_int32_t, c_char
+       character(kind=c_char), dimension(*), intent(in) :: path
+       integer(c_int32_t), intent(out) :: begin
+     end function
      integer(c_int32_t) function icls_write_vtk(path) bind(c, name='icls_write_vtk')
        import :: c_int32_t, c_char
        character(kind=c_char), dimension(*), intent(in) :: path

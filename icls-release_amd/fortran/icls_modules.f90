@@ -54,11 +54,17 @@ contains
     integer, intent(in) :: begin
     real, dimension(1:, 1:, 1:), intent(out) :: duold, dvold, dwold
     real, dimension(0:, 0:, 0:), intent(out) :: pold, curv_restart
-    if (begin /= 0) then
-       write(6, *) 'Restarting is not provided by this build.'
-       error stop 1
+    character(len=7) :: istepchar
+    integer(c_int32_t) :: nr
+    if (begin == 0) then
+       call icls_check(icls_init_program(), 'init_program')
+    else
+       ! init.f90:44-51: the fields of data/fld<begin> (loadflds(0,begin)), then the
+       ! derived state
+       write(istepchar, '(i7.7)') begin
+       call icls_check(icls_init_program_restart('data/fld'//istepchar//c_null_char, nr), 'init_program')
+       write(6, *) 'Restarting from step ', begin
     end if
-    call icls_check(icls_init_program(), 'init_program')
     curv_restart = 0.
     if (sync_host) then
        call pull_i(F_DUOLD, duold); call pull_i(F_DVOLD, dvold); call pull_i(F_DWOLD, dwold)

@@ -37,7 +37,7 @@ extern "C" {
 #define ICLS_ERR_DIVERGENCE  4   /* max |div u| > 1e-12: src/chkdiv.f90:60-86          */
 #define ICLS_ERR_STATE       5   /* call out of order (no context, not initialised)    */
 #define ICLS_ERR_NODEVICE    6   /* no GPU visible                                     */
-#define ICLS_ERR_UNSUPPORTED 7   /* option the reference aborts on or this build omits */
+#define ICLS_ERR_UNSUPPORTED 7   /* option the reference aborts on or this library omits */
 
 /* ---- precision modes ---- */
 /* BC_in_y of the cosine solver (initsolver.f90:84-124) */

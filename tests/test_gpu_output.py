@@ -164,3 +164,51 @@ def test_fortran_driver_writes_output_files(tmp_path):
     assert rel_linf(b["V"], o.get(P.F_VCC).ravel(order="F")) <= TOL
     raw = np.fromfile(str(tmp_path / "data" / "fld0000006"), dtype=np.float64)
     assert raw[-3] == pytest.approx(6e-4, rel=1e-12) and raw[-2] == 6.0
+
+
+def test_fortran_driver_restarts_from_fld(tmp_path):
+    """begin > 0 in the Fortran main program: init_program reads data/fld<begin>
+    (init.f90:44-51, loadflds(0, begin)) and re-derives the state; the restarted run's
+    fld after 4 more steps against the oracle loaded from the same file."""
+    exe = os.path.join(ROOT, "icls-release_amd", "icls_fortran")
+    if not os.path.exists(exe):
+        pytest.skip("Fortran driver not built")
+    import shutil
+    import pyoracle
+    n = 32
+    base = " itot=%d, jtot=%d, ktot=%d, ioutchk=2, drop_r(1)=0.2, mass_correction=0,\n" % (n, n, n)
+    a, b = tmp_path / "a", tmp_path / "b"
+    for d in (a, b):
+        (d / "data").mkdir(parents=True)
+    (a / "run.nml").write_text("&icls\n" + base + " nstep=4, ioutfld=4\n/\n")
+    r = subprocess.run([exe, str(a / "run.nml")], capture_output=True, text=True, timeout=120, cwd=str(a))
+    assert r.returncode == 0, r.stdout + r.stderr
+    shutil.copy(a / "data" / "fld0000004", b / "data" / "fld0000004")
+    (b / "run.nml").write_text("&icls\n" + base + " begin=4, nstep=8, ioutfld=8\n/\n")
+    r = subprocess.run([exe, str(b / "run.nml")], capture_output=True, text=True, timeout=120, cwd=str(b))
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "Restarting from step" in r.stdout
+    p = small_drop(n, mass_correction=0)
+    cells = n * n * n
+    raw = np.fromfile(str(a / "data" / "fld0000004"), dtype=np.float64)
+    blocks = [raw[q * cells:(q + 1) * cells].reshape((n, n, n), order="F") for q in range(6)]
+    o = pyoracle.Oracle(p)
+    o.init_program()
+    for q, f in enumerate([P.F_UNEW, P.F_VNEW, P.F_WNEW, P.F_PNEW]):
+        x = np.zeros(P.field_dims(p, f), order="F")
+        x[1:-1, 1:-1, 1:-1] = blocks[q]
+        o.set(f, x)
+    x = np.zeros(P.field_dims(p, P.F_LVSET), order="F")
+    x[3:-3, 3:-3, 3:-3] = blocks[4]
+    o.set(P.F_LVSET, x)
+    o.set_scalar(P.S_TIME, raw[-3])
+    o.init_derived()
+    for s in range(5, 9):
+        o.step(s, 4)
+    out = np.fromfile(str(b / "data" / "fld0000008"), dtype=np.float64)
+    assert out[-2] == 8.0
+    got = [out[q * cells:(q + 1) * cells].reshape((n, n, n), order="F") for q in range(5)]
+    want = [o.get(f)[1:-1, 1:-1, 1:-1] for f in (P.F_UNEW, P.F_VNEW, P.F_WNEW, P.F_PNEW)]
+    want.append(o.get(P.F_LVSET)[3:-3, 3:-3, 3:-3])
+    for q in range(5):
+        assert rel_linf(got[q], want[q]) <= TOL, q
