@@ -31,3 +31,15 @@ def small_drop(n=32, nz=None, r=0.2, **kw):
     for k, v in kw.items():
         setattr(p, k, v)
     return p
+
+
+def fortran_coords(p, field):
+    """Coordinates of the cell centres of a host array of ``field`` (Fortran bounds
+    0, -2 or 1; cell (i,j,k) is centred at ((i-1/2) dx, (j-1/2) dy, (k-1/2) dz))."""
+    dims = P.field_dims(p, field)
+    lb = {P.SHAPE_C: 0, P.SHAPE_LS: -2, P.SHAPE_I: 1}[P.field_shape_kind(field)]
+    dx, dy, dz = p.lx / p.itot, p.ly / p.jtot, p.lz / p.ktot
+    i = (np.arange(dims[0]) + lb - 0.5) * dx
+    j = (np.arange(dims[1]) + lb - 0.5) * dy
+    k = (np.arange(dims[2]) + lb - 0.5) * dz
+    return np.meshgrid(i, j, k, indexing="ij")

@@ -1,0 +1,394 @@
+"""Pins of the oracle's flow operators and step bookkeeping (CPU only).
+
+The reference ships no test vectors for this path, so each oracle routine below is
+checked against something its own formula does not contain:
+
+* ipot (interface.f90:225-277): for two signed-distance spheres phi_m + phi_n >= gap
+  everywhere (triangle inequality), with equality on the segment joining the centres.
+* bounduvw with moving walls (bound.f90:29-35) and the 'cou' start (init.f90:191-210):
+  plane Couette flow with matched fluids and no surface tension is an exact steady state.
+* delta_diff (mom.f90:134-273): on the staggered grid the second-order stress divergence
+  is exact for quadratic velocities and a linear viscosity; compared with
+  div(mu (grad u + grad u^T)) of the polynomials, worked out in closed form.
+* NS_grav (mom.f90:278-301) and the split-pressure term momxp/yp/zp (mom.f90:779-851,
+  called from ab2.f90:99-101): closed forms for linear pressures.
+* gfm4p (gfm.f90:17-267): on a sphere, the set of cut faces, the sign of each face jump,
+  its size 2 sigma / R, and the assembly of p_jump from the face jumps.
+* the pressure normalisation of main.f90:164-185 (the mean over the k = 1 plane with its
+  halo ring) and the renormalisation of pold.
+* inflation (levelset.f90:405-582): phi moves only where the Heaviside differs from a
+  neighbour's, in the direction that restores the volume, linearly in the deficit.
+* the step's operator order (main.f90:103-291), through the relations it leaves between
+  the fields after a step.
+"""
+import numpy as np
+import pytest
+
+import pyoracle as O
+from helpers import fortran_coords, small_drop
+from iclsamd import params as P
+
+
+# ---------------------------------------------------------------- ipot
+def _two_drops(c1, r1, c2, r2, n=32):
+    p = small_drop(n)
+    p.set_drops([tuple(c1) + (r1,), tuple(c2) + (r2,)])
+    o = O.Oracle(p)
+    o.init_program()
+    o.gfm4p()                        # runs ipot first (gfm.f90:52)
+    return o.ipot_dmin(0, 1)
+
+
+def test_ipot_aligned_spheres_gap_is_exact():
+    n = 32
+    h = 1.0 / n
+    c1 = np.array([9.5, 15.5, 15.5]) * h            # a cell centre
+    r1, r2, gap = 4.3 * h, 3.6 * h, 2.5 * h
+    c2 = c1 + np.array([r1 + gap + r2, 0.0, 0.0])
+    d = _two_drops(c1, r1, c2, r2, n)
+    assert abs(d - gap) <= 1e-13
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3, 4])
+def test_ipot_random_spheres_within_one_cell_of_the_gap(seed):
+    n = 32
+    h = 1.0 / n
+    rng = np.random.default_rng(seed)
+    r1, r2 = rng.uniform(3, 6) * h, rng.uniform(3, 6) * h
+    gap = rng.uniform(1.0, 4.0) * h
+    c1 = np.array([0.35, 0.5, 0.5]) + rng.uniform(-0.03, 0.03, 3)
+    e = rng.standard_normal(3)
+    e /= np.linalg.norm(e)
+    c2 = c1 + (r1 + r2 + gap) * e
+    d = _two_drops(c1, r1, c2, r2, n)
+    assert gap - 1e-13 <= d <= gap + h
+
+
+# ---------------------------------------------------------------- Couette walls and start
+def test_couette_start_and_moving_walls_are_steady():
+    p = small_drop(16, r=0.15)
+    p.ns_np = 0
+    p.iniu = P.INIU_COU
+    p.wall_speed = 1.0
+    p.rho1 = p.rho2 = 1.0
+    p.miu1 = p.miu2 = 1.0
+    p.We = 1e30                       # no surface tension: the drop is a passive tracer
+    o = O.Oracle(p)
+    o.init_program()
+    X, Y, Z = fortran_coords(p, P.F_VNEW)
+    want = -1.0 + 2.0 * Z / p.lz      # linear from -wall_speed to +wall_speed
+    v0 = o.get(P.F_VNEW)
+    assert np.max(np.abs(v0 - want)) <= 1e-14
+    assert np.max(np.abs(0.5 * (v0[:, :, 0] + v0[:, :, 1]) + 1.0)) <= 1e-14      # wall value -1
+    assert np.max(np.abs(0.5 * (v0[:, :, -1] + v0[:, :, -2]) - 1.0)) <= 1e-14    # wall value +1
+    o.run(5)
+    u, v, w = o.get(P.F_UNEW), o.get(P.F_VNEW), o.get(P.F_WNEW)
+    assert np.max(np.abs(u)) <= 1e-12 and np.max(np.abs(w)) <= 1e-12
+    assert np.max(np.abs(v - want)) <= 1e-12
+    # the drop has been carried along y by the shear
+    assert np.max(np.abs(o.get(P.F_LVSET) - 0)) > 0
+
+
+def test_couette_wall_speed_sign_matters():
+    """The same run with the walls at rest is not steady: the check above sees the walls."""
+    p = small_drop(16, r=0.15)
+    p.ns_np = 1
+    p.iniu = P.INIU_COU
+    p.wall_speed = 1.0
+    p.rho1 = p.rho2 = 1.0
+    p.miu1 = p.miu2 = 1.0
+    p.We = 1e30
+    o = O.Oracle(p)
+    o.init_program()
+    v0 = o.get(P.F_VNEW)
+    o.run(2)
+    assert np.max(np.abs(o.get(P.F_VNEW)[:, :, 1:-1] - v0[:, :, 1:-1])) > 1e-6
+
+
+# ---------------------------------------------------------------- momentum
+def _quad(rng):
+    g = rng.standard_normal(3)
+    Q = rng.standard_normal((3, 3))
+    return rng.standard_normal(), g, 0.5 * (Q + Q.T)
+
+
+def _eval_quad(c, X):
+    c0, g, Q = c
+    x = np.stack(X, axis=-1)
+    return c0 + x @ g + np.einsum("...i,ij,...j->...", x, Q, x)
+
+
+def _grad_quad(c, X):
+    c0, g, Q = c
+    x = np.stack(X, axis=-1)
+    return g + 2 * x @ Q
+
+
+def _stress_div(cs, m0, m, X, comp):
+    """div(mu (grad u + grad u^T)) component ``comp`` at points X, for quadratic u and
+    mu = m0 + m . x."""
+    x = np.stack(X, axis=-1)
+    mu = m0 + x @ m
+    grads = [_grad_quad(c, X) for c in cs]          # grads[i][..., j] = d u_i / d x_j
+    out = np.zeros(X[0].shape)
+    for j in range(3):
+        out += m[j] * (grads[comp][..., j] + grads[j][..., comp])
+        out += mu * (2 * cs[comp][2][j, j] + 2 * cs[j][2][comp, j])
+    return out
+
+
+def _face_coords(p, comp):
+    """Coordinates of the u (comp 0), v (1) or w (2) points of a C-halo array."""
+    X = list(fortran_coords(p, P.F_UNEW))
+    h = (p.lx / p.itot, p.ly / p.jtot, p.lz / p.ktot)[comp]
+    X[comp] = X[comp] + 0.5 * h
+    return X
+
+
+def test_delta_diff_exact_on_quadratic_velocity_and_linear_viscosity():
+    p = small_drop(12, nz=10)
+    p.lz = 1.25
+    p.iFr = 0.0
+    o = O.Oracle(p)
+    o.init_program()
+    rng = np.random.default_rng(21)
+    cs = [_quad(rng) for _ in range(3)]
+    m0, m = 2.0, rng.uniform(-0.8, 0.8, 3)
+    for comp, fid in enumerate((P.F_UNEW, P.F_VNEW, P.F_WNEW)):
+        o.set(fid, _eval_quad(cs[comp], _face_coords(p, comp)))
+    Xc = fortran_coords(p, P.F_MIU)
+    o.set(P.F_MIU, m0 + np.stack(Xc, -1) @ m)
+    rho = [rng.uniform(0.5, 2.0, (12, 12, 10)) for _ in range(3)]
+    for fid, r in zip((P.F_RHO_U, P.F_RHO_V, P.F_RHO_W), rho):
+        o.set(fid, r)
+    with_mu = o.mom_main()
+    o.set(P.F_MIU, np.zeros(P.field_dims(p, P.F_MIU)))
+    without = o.mom_main()
+    visc = 1.0 / p.Re
+    for comp in range(3):
+        got = (with_mu[comp] - without[comp]) * rho[comp] / visc
+        X = [x[1:-1, 1:-1, 1:-1] for x in _face_coords(p, comp)]
+        want = _stress_div(cs, m0, m, X, comp)
+        assert np.max(np.abs(got - want)) <= 1e-9 * np.max(np.abs(want)), comp
+
+
+def test_gravity_term_closed_form():
+    p = small_drop(8)
+    p.iFr = 0.7
+    p.g_unit[0], p.g_unit[1], p.g_unit[2] = 0.3, -0.5, 0.8
+    o = O.Oracle(p)
+    o.init_program()
+    shape = P.field_dims(p, P.F_UNEW)
+    for fid in (P.F_UNEW, P.F_VNEW, P.F_WNEW):
+        o.set(fid, np.zeros(shape))
+    rng = np.random.default_rng(3)
+    rho = [rng.uniform(0.5, 2.0, (8, 8, 8)) for _ in range(3)]
+    for fid, r in zip((P.F_RHO_U, P.F_RHO_V, P.F_RHO_W), rho):
+        o.set(fid, r)
+    o.set_scalar(P.S_RHO_AVR, 0.9)
+    rhs = o.mom_main()
+    for comp in range(3):
+        want = 0.7 * (1.0 - 0.9 / rho[comp]) * p.g_unit[comp]
+        assert np.max(np.abs(rhs[comp] - want)) <= 1e-15
+
+
+def test_split_pressure_term_closed_form():
+    """With no flow, no gravity and no old right-hand side, ab2 leaves
+    u* = -dt (1/rho_face - 1/rho0) (d phat / dn - phat_j / h) with phat = 2 pnew - pold
+    and phat_j = 2 p_x - p_xold (ab2.f90:83-101), exact for linear pressures."""
+    p = small_drop(10, nz=8)
+    p.lz = 1.6
+    p.iFr = 0.0
+    p.rho1, p.rho2 = 1.0, 4.0
+    o = O.Oracle(p)
+    o.init_program()
+    shape = P.field_dims(p, P.F_UNEW)
+    for fid in (P.F_UNEW, P.F_VNEW, P.F_WNEW):
+        o.set(fid, np.zeros(shape))
+    for fid in (P.F_DUOLD, P.F_DVOLD, P.F_DWOLD):
+        o.set(fid, np.zeros((10, 10, 8)))
+    rng = np.random.default_rng(8)
+    Xc = np.stack(fortran_coords(p, P.F_PNEW), -1)
+    gn, go = rng.standard_normal(3), rng.standard_normal(3)
+    o.set(P.F_PNEW, Xc @ gn + 0.3)
+    o.set(P.F_POLD, Xc @ go - 0.2)
+    jn, jo = rng.standard_normal(3), rng.standard_normal(3)
+    for q, fid in enumerate((P.F_PX, P.F_PY, P.F_PZ)):
+        o.set(fid, np.full(shape, jn[q]))
+    for q, fid in enumerate((P.F_PXOLD, P.F_PYOLD, P.F_PZOLD)):
+        o.set(fid, np.full(shape, jo[q]))
+    rho = [rng.uniform(1.0, 4.0, (10, 10, 8)) for _ in range(3)]
+    for fid, r in zip((P.F_RHO_U, P.F_RHO_V, P.F_RHO_W), rho):
+        o.set(fid, r)
+    o.ab2()
+    dt = p.timestep
+    h = (p.lx / p.itot, p.ly / p.jtot, p.lz / p.ktot)
+    for q, fid in enumerate((P.F_DUDT, P.F_DVDT, P.F_DWDT)):
+        got = o.get(fid)[1:-1, 1:-1, 1:-1]
+        want = -dt * (1.0 / rho[q] - 1.0 / 1.0) * ((2 * gn[q] - go[q]) - (2 * jn[q] - jo[q]) / h[q])
+        if q == 2:
+            got, want = got[:, :, :-1], want[:, :, :-1]       # w(kmax) is the wall: 0
+        assert np.max(np.abs(got - want)) <= 1e-12 * np.max(np.abs(want)), q
+
+
+# ---------------------------------------------------------------- gfm4p
+@pytest.mark.parametrize("lz", [1.0, 2.0])
+def test_gfm_face_jumps_on_a_sphere(lz):
+    n = 32
+    p = small_drop(n, nz=n)
+    p.lz = lz
+    R = 0.27
+    p.set_drops([(0.5, 0.5, 0.5 * lz, R)])
+    sigma = 0.5
+    p.We = 1.0 / sigma
+    o = O.Oracle(p)
+    o.init_program()
+    o.gfm4p()
+    ls = o.get(P.F_LVSET)
+    phi = ls[3:-3, 3:-3, 3:-3]
+    h = (p.lx / n, p.ly / n, p.lz / n)
+    band = np.abs(phi) <= 2 * h[2]
+    jumps = [o.get(f)[1:-1, 1:-1, 1:-1] for f in (P.F_PX, P.F_PY, P.F_PZ)]
+    nbr = [ls[4:-2, 3:-3, 3:-3], ls[3:-3, 4:-2, 3:-3], ls[3:-3, 3:-3, 4:-2]]
+    prev = [ls[2:-4, 3:-3, 3:-3], ls[3:-3, 2:-4, 3:-3], ls[3:-3, 3:-3, 2:-4]]
+    lap = np.zeros_like(phi)
+    for a in range(3):
+        cut = band & (phi * nbr[a] < 0)
+        assert np.array_equal(jumps[a] != 0, cut), a
+        assert np.all(np.sign(jumps[a][cut]) == np.sign(phi[cut]))
+        # |jump| = sigma * |kappa| at the interface = 2 sigma / R, to second order in h:
+        # within 3% at R = 8.6 h, 6% at R = 4.3 dz on the 1 x 1 x 2 grid
+        assert np.max(np.abs(np.abs(jumps[a][cut]) / (2 * sigma / R) - 1)) < (0.03 if lz == 1.0 else 0.06)
+        # the cell's jump sum: its own +face, plus its -face (the neighbour's +face)
+        cut_m = band & (phi * prev[a] < 0)
+        other = np.roll(np.abs(jumps[a]), 1, axis=a)
+        lap += (np.abs(jumps[a]) + np.where(cut_m, other, 0.0)) / h[a] ** 2
+    pj = o.get(P.F_PJUMP)[1:-1, 1:-1, 1:-1]
+    want = np.where(band, np.sign(phi) * lap, 0.0)
+    assert np.max(np.abs(pj - want)) <= 1e-12 * np.max(np.abs(want))
+
+
+# ---------------------------------------------------------------- pressure normalisation
+def _boundp(a):
+    b = a.copy()
+    b[:, :, 0] = b[:, :, 1]
+    b[:, :, -1] = b[:, :, -2]
+    b[0], b[-1] = b[-2].copy(), b[1].copy()
+    b[:, 0], b[:, -1] = b[:, -2].copy(), b[:, 1].copy()
+    return b
+
+
+@pytest.mark.parametrize("first", [1, 0])
+def test_pressure_normalisation_bookkeeping(first):
+    p = small_drop(12, nz=8)
+    o = O.Oracle(p)
+    o.init_program()
+    rng = np.random.default_rng(31)
+    shape = P.field_dims(p, P.F_P)
+    pn0 = rng.standard_normal(shape)
+    psol = rng.standard_normal(shape) + 0.4
+    o.set(P.F_PNEW, pn0)
+    o.set(P.F_P, psol)
+    o.set_scalar(P.S_NORM_OLD, 0.37)
+    o.update_pressure(first)
+    pb = _boundp(psol)
+    norm = pb[:, :, 1].sum() / (12 * 12)          # k = 1 plane, halo ring included
+    assert abs(norm - psol[1:-1, 1:-1, 1].mean()) > 1e-3   # not the interior mean
+    assert np.max(np.abs(o.get(P.F_PNEW) - (pb - norm))) <= 1e-14
+    want_old = pn0 - norm if first else pn0 + 0.37 - norm
+    assert np.max(np.abs(o.get(P.F_POLD) - want_old)) <= 1e-14
+    assert abs(o.scalar(P.S_NORM_OLD) - norm) <= 1e-15
+    # what the AB2 extrapolation sees is unchanged by the renormalisation
+    assert np.max(np.abs((o.get(P.F_POLD) - o.get(P.F_PNEW)) - (want_old - pb + norm))) <= 1e-14
+
+
+# ---------------------------------------------------------------- inflation, per cell
+def _inflate(deficit, n=32):
+    p = P.drop64(n)
+    p.mass_correction = 1
+    o = O.Oracle(p)
+    o.init_program()
+    o.tube_construction(0)
+    phi0 = o.get(P.F_LVSET)
+    v, _ = o.volume(0)
+    o.set_volume(0, v, v * (1 + deficit))
+    o.inflation(0)
+    return p, phi0, o.get(P.F_LVSET)
+
+
+def test_inflation_moves_phi_only_where_the_heaviside_varies():
+    p, phi0, phi1 = _inflate(1e-5)
+    dz = p.lz / p.ktot
+    s1 = 1.5 * dz
+    ph = phi0[3:-3, 3:-3, 3:-3]
+    d = (phi1 - phi0)[3:-3, 3:-3, 3:-3]
+    H = lambda f: np.where(f < -s1, 0.0, np.where(f < s1, 0.5 + 0.5 * f / s1 + np.sin(np.pi * f / s1) / (2 * np.pi), 1.0))
+    hp = H(phi0[2:-2, 2:-2, 2:-2])
+    hc = hp[1:-1, 1:-1, 1:-1]
+    varies = np.zeros(hc.shape, dtype=bool)
+    for a in range(3):
+        for s in (-1, 1):
+            sl = [slice(1, -1)] * 3
+            sl[a] = slice(1 + s, hp.shape[a] - 1 + s)
+            varies |= hp[tuple(sl)] != hc
+    # elsewhere the RK3 combinations (3/4 phi + 1/4 phi1 ...) leave only rounding
+    assert np.all(np.abs(d[~varies]) <= 4 * np.finfo(float).eps * np.abs(ph[~varies]))
+    assert np.min(np.abs(d[varies & (np.abs(ph) < s1)])) > 1e3 * np.finfo(float).eps
+    # the drop (phi < 0) grows: phi decreases wherever it moves
+    assert np.max(d) <= 1e-15 and np.min(d) < 0
+    # the drop is centred, so the correction keeps its mirror symmetries in x, y and z
+    for a in range(3):
+        assert np.max(np.abs(d - np.flip(d, axis=a))) <= 1e-9 * np.max(np.abs(d))
+    # on the sphere the interface moves uniformly: the cells next to it all shift alike
+    near = np.abs(ph) < 0.5 * dz
+    assert np.std(d[near]) < 0.15 * abs(np.mean(d[near]))
+
+
+def test_inflation_is_linear_in_the_deficit_cell_by_cell():
+    _, phi0, a = _inflate(1e-6)
+    _, _, b = _inflate(2e-6)
+    da, db = (a - phi0), (b - phi0)
+    big = np.abs(da) > 1e-3 * np.max(np.abs(da))
+    assert np.max(np.abs(db[big] / da[big] - 2)) < 1e-4
+
+
+# ---------------------------------------------------------------- the operator order of a step
+def test_step_leaves_the_fields_related_as_the_operator_order_implies():
+    """main.f90:103-291: ls_main (advect, correct, re-init, update_materials) before ab2,
+    then gfm4p (which moves p_x to p_xold), fillps, solver2d, correc, bounduvw, the
+    pressure update, average_vel_at_center.  So after a step: the materials are those of
+    the new phi; duold is mom_main of the old velocity with the new materials; p_xold is
+    the previous p_x; the cell-centre velocities average the new face velocities; and the
+    velocity is divergence free."""
+    p = small_drop(16, r=0.23)
+    p.rho2, p.miu2 = 3.0, 2.0
+    o = O.Oracle(p)
+    o.init_program()
+    o.run(2)
+    twin = O.Oracle(p)
+    twin.init_program()
+    twin.run(2)
+    pxs = o.get(P.F_PX)
+    o.step(3, 0)
+    # materials of the new phi
+    rho, miu = o.get(P.F_RHO), o.get(P.F_MIU)
+    o.update_materials()
+    assert np.array_equal(o.get(P.F_RHO), rho) and np.array_equal(o.get(P.F_MIU), miu)
+    # duold: the momentum right-hand side of the old velocity with the new materials
+    twin.ls_main(3)
+    ru, rv, rw = twin.mom_main()
+    assert np.max(np.abs(o.get(P.F_DUOLD) - ru)) <= 1e-13 * max(np.max(np.abs(ru)), 1e-30)
+    assert np.max(np.abs(o.get(P.F_DWOLD) - rw)) <= 1e-13 * max(np.max(np.abs(rw)), 1e-30)
+    # ... and not the one with the old materials
+    stale = O.Oracle(p)
+    stale.init_program()
+    stale.run(2)
+    su, sv, sw = stale.mom_main()
+    assert np.max(np.abs(sw - rw)) > 1e-6 * np.max(np.abs(rw))
+    # the jump history and the centre velocities
+    assert np.array_equal(o.get(P.F_PXOLD), pxs)
+    u, v, w = o.get(P.F_UNEW), o.get(P.F_VNEW), o.get(P.F_WNEW)
+    assert np.max(np.abs(o.get(P.F_UCC) - 0.5 * (u[:-2, 1:-1, 1:-1] + u[1:-1, 1:-1, 1:-1]))) <= 1e-15
+    assert np.max(np.abs(o.get(P.F_WCC) - 0.5 * (w[1:-1, 1:-1, :-2] + w[1:-1, 1:-1, 1:-1]))) <= 1e-15
+    rc, divtot, divmax, ijk = o.chkdiv(3)
+    assert divmax < 1e-12
