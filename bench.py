@@ -15,10 +15,17 @@ Modes (--mode):
   learned       mixed plus the learned curvature closure (trained on the device
                 before the timed steps)
 
+The bubble is resolved by 32 cells per radius, so the closure's gate finds no
+under-resolved interface cell there and the learned mode runs the mixed step plus the
+gate; the line says so.  A second workload, where the closure does work, is timed
+beside it by default (--drops, off with --drops 0): the 256^3 box with 64 static drops
+of radius 3 .. 20 cells (seeded), one level set per drop stepped in its own window,
+CFL 0.3, normalised mass correction -- learned and plain FP64.
+
 One JSON line on stdout:
   value       = itot*jtot*ktot*K / (seconds of K timed steps) / 1e6 of the first mode
-                (mixed+learned by default); "fp64" holds the plain FP64 run and
-                "speedup_vs_fp64" the ratio
+                (mixed+learned by default) on the bubble; "fp64" holds the plain FP64
+                run and "speedup_vs_fp64" the ratio; "drops256" the many-drop leg
   roofline    = that run's dominant kernel: algorithmic bytes per launch / its mean
                 launch time (HIP events on the library stream over the timed region)
                 vs the 8 TB/s HBM peak; traffic from rocprofv3 FETCH_SIZE + WRITE_SIZE
@@ -76,12 +83,25 @@ def traffic_for(kernel, workload, fp32):
         return None
 
 
-def run_mode(mode, n, steps, warmup, show_kernels):
+def drops_params(n, ndrops, mode):
+    """The many-drop workload: static drops of radius 3 .. 20 cells, ratios 1:1,
+    sigma = 1, dt from chkdt at CFL 0.3, normalised mass correction."""
+    p = P.drops256(n, ndrops, mode=MODES[mode])
+    p.cfl = 0.3
+    p.mass_correction = 2
+    return p
+
+
+def run_mode(mode, n, steps, warmup, show_kernels, drops=0):
     """Time `steps` steps of the workload in one precision mode; returns a dict."""
     from iclsamd import _lib
     fp32 = mode != "fp64"
-    p = P.bubble256(n, mode=MODES[mode])
-    workload = "bubble%d" % n if mode == "fp64" else "bubble%d_%s" % (n, mode)
+    if drops:
+        p = drops_params(n, drops, mode)
+        workload = "drops%d" % n if mode == "fp64" else "drops%d_%s" % (n, mode)
+    else:
+        p = P.bubble256(n, mode=MODES[mode])
+        workload = "bubble%d" % n if mode == "fp64" else "bubble%d_%s" % (n, mode)
     kbytes = R.kernel_bytes(fp32)
     ncell = p.itot * p.jtot * p.ktot
     g = _lib.Solver(p)
@@ -142,6 +162,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=256, help="cells per direction")
     ap.add_argument("--mode", choices=["both", "fp64", "mixed", "learned"], default="both")
+    ap.add_argument("--drops", type=int, default=64,
+                    help="drops of the many-drop leg (learned and FP64 at --n cells); 0 skips it")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--kernels", action="store_true", help="also print the per-kernel table to stderr")
     a = ap.parse_args()
@@ -167,11 +189,25 @@ def main():
     if first == "learned":
         line["config"]["closure_cells_last_step"] = r["listed_cells"]
         line["config"]["init_s_with_training"] = round(r["init_s"], 2)
+        if not r["listed_cells"]:
+            line["config"]["closure_note"] = ("no under-resolved interface cell on this workload: the learned "
+                                              "mode runs the mixed step plus the closure's gate")
     if a.mode == "both":
         f = run_mode("fp64", a.n, a.steps, a.warmup, a.kernels)
         line["fp64"] = {"value": round(f["value"], 2), "ms_per_step": round(f["ms_per_step"], 4),
                         "roofline": f["roofline"]}
         line["speedup_vs_fp64"] = round(r["value"] / f["value"], 3)
+    if a.drops > 0:
+        dl = run_mode("learned", a.n, a.steps, a.warmup, a.kernels, drops=a.drops)
+        df = run_mode("fp64", a.n, a.steps, a.warmup, a.kernels, drops=a.drops)
+        line["drops%d" % a.n] = {
+            "workload": "%dx%dx%d box, %d static drops of radius 3..20 cells (seeded), ratios 1:1, sigma 1, "
+                        "per-drop windows, CFL 0.3, normalised mass correction" % (a.n, a.n, a.n, a.drops),
+            "learned": {"value": round(dl["value"], 2), "ms_per_step": round(dl["ms_per_step"], 4),
+                        "closure_cells_last_step": dl["listed_cells"], "roofline": dl["roofline"]},
+            "fp64": {"value": round(df["value"], 2), "ms_per_step": round(df["ms_per_step"], 4),
+                     "roofline": df["roofline"]},
+            "speedup_vs_fp64": round(dl["value"] / df["value"], 3)}
     if not a.no_cpu:
         q = P.bubble256(a.n, mode=P.MODE_FP64)
         line["cpu_baseline"] = cpu_baseline(q)

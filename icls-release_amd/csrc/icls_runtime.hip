@@ -1227,6 +1227,13 @@ int32_t icls_spurious_currents(double* Ca) {
 
 // ---------------------------------------------------------------- init_program
 static int init_derived_impl(Ctx& c) {
+    if (c.learned && !c.mlp.ready) {
+        // the learned mode trains its closure once, with the default options, on every
+        // way into a run: a fresh start, a restart from an fld file, or uploaded fields
+        icls_mlp_train_opts o;
+        icls_mlp_defaults(&o);
+        TRY(train_impl(c, &o));
+    }
     TRY(launch_pressure_materialize(c));
     c.dt = c.P.timestep;                                   // init.f90:68
     TRY(launch_bc_uvw(c, c.unew, c.vnew, c.wnew));         // init.f90:72-76
@@ -1269,12 +1276,6 @@ int32_t icls_init_program(void) {
     TRY(launch_fill(c, c.pnew, 0., 0));
     TRY(launch_init_velocity(c));
     for (int l = 0; l < c.lmax; l++) TRY(launch_init_levelset(c, l, c.lvset[l]));
-    if (c.learned && !c.mlp.ready) {
-        // the learned mode trains its closure once, with the default options
-        icls_mlp_train_opts o;
-        icls_mlp_defaults(&o);
-        TRY(train_impl(c, &o));
-    }
     return init_derived_impl(c);
 }
 

@@ -24,39 +24,11 @@ static constexpr int BX = 64, BY = 4;
 // the band: a cell is in the band unless all 13 points of its +-2 axis scans sit at
 // |phi| == NarrowBand_2 (levelset.f90:125-156; a sum of non-positive terms is zero
 // only when every term is).  A clamped value has |phi| < NarrowBand_2 exactly when
-// the unclamped one has, so the scan tests the input directly; the twelve
-// neighbours are loaded together rather than behind an early exit.
+// the unclamped one has, so the scan tests the input directly.
 // The boundsls that follows (bound.f90:404-411) is folded in: layers 0 and k1 take
 // the linear extrapolation of the clamped layers next to them.
-template <class T>
-__global__ void k_tube(Grid g, T NB2, const T* __restrict__ in, T* __restrict__ out, uint8_t* __restrict__ mask) {
-    int i0 = blockIdx.x * BX + threadIdx.x, j0 = blockIdx.y * BY + threadIdx.y, ks = blockIdx.z;
-    if (i0 >= g.nx || j0 >= g.ny) return;
-    int k = ks - 2;
-    long q = gidx(g, i0, j0, k);
-    auto clampv = [&](T v) { return v > NB2 ? NB2 : (v < -NB2 ? -NB2 : v); };
-    T v = in[q];
-    T c;
-    if (k == 0) c = T(2.) * clampv(in[gidx(g, i0, j0, 1)]) - clampv(in[gidx(g, i0, j0, 2)]);
-    else if (k == g.nz + 1) c = T(2.) * clampv(in[gidx(g, i0, j0, g.nz)]) - clampv(in[gidx(g, i0, j0, g.nz - 1)]);
-    else c = clampv(v);
-    out[q] = c;
-    if (k < 1 || k > g.nz) return;
-    T nb[12];
-#pragma unroll
-    for (int d = -2, t = 0; d <= 2; d++) {
-        if (d == 0) continue;
-        nb[t++] = in[gidx(g, wrap(i0 + d, g.nx), j0, k)];
-        nb[t++] = in[gidx(g, i0, wrap(j0 + d, g.ny), k)];
-        nb[t++] = in[gidx(g, i0, j0, k + d)];
-    }
-    bool m = fabs(v) < NB2;
-#pragma unroll
-    for (int t = 0; t < 12; t++) m = m || fabs(nb[t]) < NB2;
-    mask[q] = m ? 1 : 0;
-}
-
-// The same construction marching in z over a 64 x 4 tile: the input passes through a
+//
+// The construction marches in z over a 64 x 4 tile: the input passes through a
 // ring of five LDS planes (k-2 .. k+2) with a two-cell x/y halo, so the 13-point scan
 // reads LDS and every input element is fetched once per block.  Chunks of TZ layers;
 // the first and last chunk also write the ghost layers.  TPD planes are loaded ahead
@@ -175,18 +147,8 @@ static void tube_dispatch(Ctx& c, const double* phi_in, double* phi_out) {
 
 int launch_tube(Ctx& c, const double* phi_in, double* phi_out) {
     ProfScope ps(c, "k_tube");
-#ifdef ICLS_TUBE_POINT
-    dim3 blk(BX, BY, 1);
-    dim3 grd = grid3(c.g, c.g.nz + 6, blk);
-    if (c.fp32)
-        hipLaunchKernelGGL(k_tube<float>, grd, blk, 0, c.stream, c.g, (float)c.NB2, as<float>(phi_in), as<float>(phi_out),
-                           c.mask);
-    else
-        hipLaunchKernelGGL(k_tube<double>, grd, blk, 0, c.stream, c.g, c.NB2, phi_in, phi_out, c.mask);
-#else
     if (c.fp32) tube_dispatch<float>(c, phi_in, phi_out);
     else tube_dispatch<double>(c, phi_in, phi_out);
-#endif
     HIP_OK(hipGetLastError());
     return ICLS_OK;
 }

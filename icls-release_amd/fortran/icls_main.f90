@@ -76,7 +76,8 @@ program ICLS
   !---- initialisation (main.f90:80-82)
   call init_program(begin, duold, dvold, dwold, pold, curv_restart)
   pp2 => p(1:imax, 1:jmax, 1:kmax)
-  time = 0.
+  ! 0 on a fresh start; on a restart the time read from data/fld<begin> (loadd.f90:61)
+  call icls_check(icls_get_scalar(S_TIME, time), 'time')
 
   !---- main loop (main.f90:103-291)
   do istep = begin + 1, nstep

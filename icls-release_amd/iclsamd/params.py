@@ -173,7 +173,13 @@ def random_drops(ndrops, n, rmin_cells, rmax_cells, seed=20261202, gap_cells=6, 
                  max_tries=200000):
     """Seeded non-overlapping drops (centres uniform in the box, radii uniform in
     [rmin, rmax] cells), rejection with a gap of ``gap_cells`` cells between
-    interfaces and from the z walls."""
+    interfaces and from the z walls.
+
+    The level sets are initialised with the +lx and +ly periodic images only
+    (init.f90:945-980), so a drop whose narrow band (radius + 7 cells) crosses the
+    x = lx (y = ly) seam is given the centre x - lx (y - ly), whose +lx image then
+    covers the far side; a drop crossing an x and a y seam at once would need the
+    diagonal image as well and is drawn again."""
     dx = lx / n
     rng = splitmix64(seed)
     drops = []
@@ -194,7 +200,12 @@ def random_drops(ndrops, n, rmin_cells, rmax_cells, seed=20261202, gap_cells=6, 
                 ok = False
                 break
         if ok:
-            drops.append((x, y, z, r))
+            m = r + 7 * dx
+            cross_x = x + m > lx or x - m < 0
+            cross_y = y + m > ly or y - m < 0
+            if cross_x and cross_y:
+                continue
+            drops.append((x - lx if x + m > lx else x, y - ly if y + m > ly else y, z, r))
     return drops
 
 

@@ -156,3 +156,37 @@ def test_learned_curvature_beats_finite_differences_on_small_drops(R_cells):
           (R_cells, fd_mean, fd_max, ml_mean, ml_max, listed))
     assert listed > 0
     assert ml_mean < fd_mean
+
+
+def test_learned_mode_restart_trains_and_runs_the_closure(tmp_path):
+    """A MIXED_LEARNED run started from an fld file (icls_init_program_restart) trains
+    its closure like a fresh start and consults it on the first step; so does a run
+    built from uploaded fields (icls_init_derived)."""
+    p = small_drop(32, r=2.5 / 32, mass_correction=0)
+    p.rho2 = p.rho1
+    p.mode = P.MODE_MIXED_LEARNED
+    path = str(tmp_path / "fld0000002")
+    g = _solver(p)
+    try:
+        g.init_program()
+        g.run(2)
+        g.write_fld(path, 2)
+        fields = {f: g.download(f) for f in (P.F_UNEW, P.F_VNEW, P.F_WNEW, P.F_PNEW, P.F_LVSET)}
+    finally:
+        g.close()
+    g = _solver(p)
+    try:
+        assert g.init_program_restart(path) == 2
+        g.step(3, 2)
+        assert g.mlp_listed() > 0
+    finally:
+        g.close()
+    g = _solver(p)
+    try:
+        for f, a in fields.items():
+            g.upload(f, a)
+        g.init_derived()
+        g.step(3, 2)
+        assert g.mlp_listed() > 0
+    finally:
+        g.close()

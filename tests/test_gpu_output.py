@@ -207,6 +207,8 @@ def test_fortran_driver_restarts_from_fld(tmp_path):
         o.step(s, 4)
     out = np.fromfile(str(b / "data" / "fld0000008"), dtype=np.float64)
     assert out[-2] == 8.0
+    # the restarted run keeps the time it read (loadd.f90:61) and adds one dt per step
+    assert abs(out[-3] - (raw[-3] + 4 * p.timestep)) <= 1e-15 and raw[-3] > 0
     got = [out[q * cells:(q + 1) * cells].reshape((n, n, n), order="F") for q in range(5)]
     want = [o.get(f)[1:-1, 1:-1, 1:-1] for f in (P.F_UNEW, P.F_VNEW, P.F_WNEW, P.F_PNEW)]
     want.append(o.get(P.F_LVSET)[3:-3, 3:-3, 3:-3])
