@@ -261,6 +261,9 @@ __device__ __forceinline__ int xcd_tile(int b, int n) {
 // ---- launchers (defined in the k_*.hip files); each returns ICLS_OK or an error.
 // Field-class pointers are reinterpreted as the mode's element type.
 // level set
+int noalloc_setup(int device);
+int chkdiv_noalloc(const void* u, const void* v, const void* w, int elem_bytes, const Grid& g, double* out,
+                   hipStream_t stream);
 int launch_tube(Ctx& c, const double* phi_in, double* phi_out);
 int launch_ls_extent(Ctx& c, const double* phi, int* ext);
 // the launch grid is a level-set window, not the whole grid

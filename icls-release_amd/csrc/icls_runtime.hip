@@ -221,6 +221,7 @@ static int build_ctx(const icls_params* prm) {
     HIP_OK(hipGetDevice(&C.device));
     HIP_OK(hipDeviceGetAttribute(&C.ncu, hipDeviceAttributeMultiprocessorCount, C.device));
     HIP_OK(hipStreamCreateWithFlags(&C.stream, hipStreamNonBlocking));
+    TRY(noalloc_setup(C.device));   // the allocation-free chkdiv's partials on this GPU
     Grid& g = C.g;
     g.nx = prm->itot; g.ny = prm->jtot; g.nz = prm->ktot;
     g.sy = g.nx; g.sz = (long)g.nx * g.ny;
@@ -1157,6 +1158,22 @@ int32_t icls_chkdiv(int32_t istep, double* divtot, double* divmax, int32_t* ijk)
         return ICLS_ERR_DIVERGENCE;
     }
     return ICLS_OK;
+}
+
+int32_t icls_chkdiv_noalloc(const void* u, const void* v, const void* w, int32_t elem_bytes, int32_t nx, int32_t ny,
+                            int32_t nz, double lx, double ly, double lz, double* out, void* stream) {
+    if (!u || !v || !w || !out) { set_error("null array"); return ICLS_ERR_PARAM; }
+    if (elem_bytes != 4 && elem_bytes != 8) { set_error("elem_bytes must be 4 or 8, not %d", elem_bytes); return ICLS_ERR_PARAM; }
+    if (nx < 1 || ny < 1 || nz < 1 || !(lx > 0.) || !(ly > 0.) || !(lz > 0.)) {
+        set_error("bad grid %d x %d x %d over %g x %g x %g", nx, ny, nz, lx, ly, lz);
+        return ICLS_ERR_PARAM;
+    }
+    Grid g{};
+    g.nx = nx; g.ny = ny; g.nz = nz;
+    g.sy = nx; g.sz = (long)nx * ny;
+    g.dxi = nx / lx; g.dyi = ny / ly; g.dzi = nz / lz;   // param.f90:37-38
+    g.dx = 1. / g.dxi; g.dy = 1. / g.dyi; g.dz = 1. / g.dzi;
+    return chkdiv_noalloc(u, v, w, elem_bytes, g, out, (hipStream_t)stream);
 }
 
 // chkdt's combined speed (chkdt.f90:11-104) from the device maxima; dtmax = 0.5 / speed

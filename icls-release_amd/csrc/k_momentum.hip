@@ -772,7 +772,7 @@ __global__ void k_chkdiv(Grid g, const T* __restrict__ u, const T* __restrict__ 
     }
 }
 
-__global__ void k_chkdiv_final(const double* __restrict__ part, int nb, double* dscal) {
+__global__ void k_chkdiv_final(const double* __restrict__ part, int nb, double* tot, double* mx, double* loc) {
     __shared__ double smem[16];
     double v = 0.;
     for (int q = threadIdx.x; q < nb; q += blockDim.x) v += part[q];
@@ -783,7 +783,7 @@ __global__ void k_chkdiv_final(const double* __restrict__ part, int nb, double* 
             double m = part[nb + b], l = part[2 * nb + b];
             if (m > bm || (m == bm && l < bl)) { bm = m; bl = l; }
         }
-        dscal[DS_DIV_TOT] = r; dscal[DS_DIV_MAX] = bm; dscal[DS_DIV_MAX + 16] = bl;
+        *tot = r; *mx = bm; *loc = bl;
     }
 }
 
@@ -798,7 +798,69 @@ int launch_chkdiv(Ctx& c) {
     else
         hipLaunchKernelGGL(k_chkdiv<double>, grd, blk, 0, c.stream, c.g, (const double*)c.unew, (const double*)c.vnew,
                            (const double*)c.wnew, c.dpart, nb);
-    hipLaunchKernelGGL(k_chkdiv_final, dim3(1), dim3(256), 0, c.stream, (const double*)c.dpart, nb, c.dscal);
+    hipLaunchKernelGGL(k_chkdiv_final, dim3(1), dim3(256), 0, c.stream, (const double*)c.dpart, nb,
+                       c.dscal + DS_DIV_TOT, c.dscal + DS_DIV_MAX, c.dscal + DS_DIV_MAX + 16);
+    HIP_OK(hipGetLastError());
+    return ICLS_OK;
+}
+
+// ---------------------------------------------------------------- chkdiv without allocation
+// The block partials [sum | max | location] of icls_chkdiv_noalloc live in one fixed
+// device array: up to CHKDIV_NOALLOC_BLOCKS reduction blocks, which a 512^3 grid fills
+// (8 x 128 x 64 blocks of 64 x 4 columns over 8 layers); 1.5 MiB per GPU, held until
+// the process ends.  Every call on a GPU shares that GPU's copy, so calls that overlap
+// on one GPU race.  Each block writes its own three partials before the final kernel
+// reads them, so nothing has to start at zero.
+constexpr int CHKDIV_NOALLOC_BLOCKS = 1 << 16;
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static __device__ __align__(256) double scratch[3 * CHKDIV_NOALLOC_BLOCKS];
+// The device address of `scratch` on each GPU (host code must not pass the symbol itself).
+static double* g_scratch_dev[NOALLOC_MAX_DEVICES];
+
+// Looks the array up once per GPU, with that GPU current.  Called by icls_init for its
+// device and by the first icls_chkdiv_noalloc call on any other device.
+int noalloc_setup(int device) {
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES) {
+        set_error("device %d beyond the %d the allocation-free entry supports", device, NOALLOC_MAX_DEVICES);
+        return ICLS_ERR_PARAM;
+    }
+    if (g_scratch_dev[device]) return ICLS_OK;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+        set_error("hipGetSymbolAddress(scratch) on device %d: %s", device, hipGetErrorString(e));
+        return ICLS_ERR_HIP;
+    }
+    g_scratch_dev[device] = (double*)ptr;
+    return ICLS_OK;
+}
+
+// chkdiv (chkdiv.f90:28-48) over caller-owned device arrays in the library's layout,
+// the same kernels as launch_chkdiv with the partials in `scratch`; out[0..2] =
+// divtot, max |div|, linear index i0 + nx (j0 + ny (k-1)) of its first cell.
+int chkdiv_noalloc(const void* u, const void* v, const void* w, int elem_bytes, const Grid& g, double* out,
+                   hipStream_t stream) {
+    int dev = -1;
+    HIP_OK(hipGetDevice(&dev));
+    const int rc = noalloc_setup(dev);
+    if (rc) return rc;
+    const dim3 blk(BX, BY, 1);
+    const dim3 grd = red_grid(g, g.nz);
+    const long nb = (long)grd.x * grd.y * grd.z;
+    if (nb > CHKDIV_NOALLOC_BLOCKS) {
+        set_error("grid %d x %d x %d needs %ld reduction blocks; the allocation-free chkdiv holds %d", g.nx, g.ny,
+                  g.nz, nb, CHKDIV_NOALLOC_BLOCKS);
+        return ICLS_ERR_PARAM;
+    }
+    double* part = g_scratch_dev[dev];
+    if (elem_bytes == 4)
+        hipLaunchKernelGGL(k_chkdiv<float>, grd, blk, 0, stream, g, (const float*)u, (const float*)v,
+                           (const float*)w, part, (int)nb);
+    else
+        hipLaunchKernelGGL(k_chkdiv<double>, grd, blk, 0, stream, g, (const double*)u, (const double*)v,
+                           (const double*)w, part, (int)nb);
+    hipLaunchKernelGGL(k_chkdiv_final, dim3(1), dim3(256), 0, stream, (const double*)part, (int)nb, out, out + 1,
+                       out + 2);
     HIP_OK(hipGetLastError());
     return ICLS_OK;
 }

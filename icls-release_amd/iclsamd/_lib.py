@@ -25,7 +25,8 @@ EXPORTS = [
     "icls_sync", "icls_ls_main", "icls_average_stream_vel", "icls_ab2", "icls_fillps", "icls_gfm4p",
     "icls_add_pjump", "icls_solver2d", "icls_solver2d_cos", "icls_boundp", "icls_correc", "icls_bounduvw",
     "icls_update_pressure",
-    "icls_average_vel_at_center", "icls_check_nan", "icls_chkdiv", "icls_chkdt", "icls_max_vel_at_center",
+    "icls_average_vel_at_center", "icls_check_nan", "icls_chkdiv", "icls_chkdiv_noalloc", "icls_chkdt",
+    "icls_max_vel_at_center",
     "icls_spurious_currents",
     "icls_ipot", "icls_write_fld", "icls_init_program_restart", "icls_write_vtk", "icls_post2d",
     "icls_step", "icls_profile_enable", "icls_profile_only", "icls_profile_reset", "icls_kernel_times",
@@ -79,6 +80,8 @@ def load():
         getattr(L, name).argtypes = [i32]
     L.icls_average_stream_vel.argtypes = [dp]
     L.icls_chkdiv.argtypes = [i32, dp, dp, ctypes.POINTER(i32)]
+    vp = ctypes.c_void_p
+    L.icls_chkdiv_noalloc.argtypes = [vp, vp, vp, i32, i32, i32, i32, dbl, dbl, dbl, vp, vp]
     L.icls_chkdt.argtypes = [dp]
     L.icls_max_vel_at_center.argtypes = [dp]
     L.icls_spurious_currents.argtypes = [dp]

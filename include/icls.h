@@ -195,6 +195,20 @@ int32_t icls_update_pressure(int32_t first_step);        /* main.f90:164-185    
 int32_t icls_average_vel_at_center(void);                /* misc.f90:587-604            */
 int32_t icls_check_nan(void);                            /* main.f90:207-210            */
 int32_t icls_chkdiv(int32_t istep, double* divtot, double* divmax, int32_t* ijk); /* chkdiv.f90:10-90 */
+/* No counterpart in the reference.  The divergence check of icls_chkdiv over caller-owned
+ * device arrays, for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc): it allocates nothing
+ * and needs no context.  u, v, w: nx * ny * (nz + 6) elements of elem_bytes (8 double,
+ * 4 float) in the library's layout, element (i0, j0, k) at i0 + nx (j0 + ny (k + 2)) for
+ * k = -2 .. nz + 3 (x and y periodic, no halo; z ghost layer k = 0 read).  out: device
+ * memory for 3 doubles, written on `stream` (a hipStream_t, NULL for the null stream):
+ * divtot, max |div| and the linear index i0 + nx (j0 + ny (k - 1)) of the first cell
+ * holding it.  The block partials live in one fixed 1.5 MiB device array per GPU, held
+ * until the process ends; grids needing more than 65536 reduction blocks (beyond 512^3)
+ * are rejected with ICLS_ERR_PARAM.  Calls that overlap on one GPU share that array and
+ * race.  Returns after queueing; synchronise `stream` before reading out. */
+int32_t icls_chkdiv_noalloc(const void* u, const void* v, const void* w, int32_t elem_bytes, int32_t nx, int32_t ny,
+                            int32_t nz, double lx, double ly, double lz, double* out, void* stream);
 int32_t icls_chkdt(double* dtmax);                       /* chkdt.f90:11-105            */
 int32_t icls_max_vel_at_center(double* vmax);            /* misc.f90:610-636            */
 /* spurious_currents (misc.f90:802-843): the static-drop capillary number
