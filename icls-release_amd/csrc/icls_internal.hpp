@@ -335,7 +335,7 @@ int mlp_samples(Ctx& c, long n, unsigned long long seed, float r_min, float r_ma
                 float* t, float* fd);
 int launch_mlp_reset_list(Ctx& c);
 int launch_mlp_reset_total(Ctx& c);
-int launch_mlp_curv(Ctx& c, const double* phi, double* kappa);
+int launch_mlp_curv(Ctx& c, const double* phi, const double* kappa_fd, double* kappa_out);
 // momentum / correction
 int launch_momentum(Ctx& c);
 int launch_momentum_rhs_only(Ctx& c);

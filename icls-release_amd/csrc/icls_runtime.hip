@@ -1008,17 +1008,15 @@ static int gfm_impl(Ctx& c, bool fuse_fillps, bool store_pjump) {
     c.p_in_pnew = false;   // fillps writes p
     const bool learned = c.learned && c.mlp.ready;
     if (learned) TRY(launch_mlp_reset_total(c));
-    // get_curvature (with the learned closure on the under-resolved cells), re_curvature,
-    // and for the level sets after the first their jumps added inside their windows
+    // get_curvature, re_curvature, and for the level sets after the first their jumps
+    // added inside their windows.  In the learned mode the closure takes the under-
+    // resolved cells of the smoothed field: their curvature is the cell's own least-
+    // squares value plus the network's correction toward the 4x finer grid's.
     auto curvature = [&](int l) -> int {
-        if (learned) {
-            TRY(launch_mlp_reset_list(c));
-            TRY(launch_curvature(c, c.lvset[l], c.curv, nullptr, nullptr, true));
-            TRY(launch_mlp_curv(c, c.lvset[l], c.curv));
-        } else {
-            TRY(launch_curvature(c, c.lvset[l], c.curv, nullptr, nullptr));
-        }
+        if (learned) TRY(launch_mlp_reset_list(c));
+        TRY(launch_curvature(c, c.lvset[l], c.curv, nullptr, nullptr, learned));
         TRY(launch_recurvature(c, c.lvset[l], c.curv, c.curv2));
+        if (learned) TRY(launch_mlp_curv(c, c.lvset[l], c.curv, c.curv2));
         return l > 0 ? launch_gfm(c, l, c.curv2, false, true) : ICLS_OK;
     };
     // the first level set's jumps are written over the whole grid (they clear the

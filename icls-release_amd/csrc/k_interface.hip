@@ -39,9 +39,9 @@ __device__ __forceinline__ T lsq_row(const T* __restrict__ A, int r, const T* v)
 // quadratic of init_matrix to the 27 values, kappa = -div(grad phi/|grad phi|) with
 // the 0.9 gradient filter and the +-1/dz and 1e-9 limits; zero elsewhere.  The
 // z-ghost layers of kappa are never written and stay zero (interface.f90:49).
-// With a cell list (the learned mode) the interface cells whose curvature is
-// under-resolved -- |phi| <= 4 dz, |grad phi| >= 0.9 and |kappa| dz >= gate -- are
-// appended to it for the learned closure (k_mlp_curv).
+// With a cell list (the learned mode) the cells of the re_curvature shell whose
+// curvature is under-resolved -- |phi| < 2 dz, |grad phi| >= 0.9 and |kappa| dz >= gate
+// -- are appended to it for the learned closure (k_mlp_curv).
 template <class T>
 __global__ void k_curvature(Grid g, T NB2, const T* __restrict__ A, const T* __restrict__ phi, T* __restrict__ kappa,
                             T* __restrict__ nm, const double* __restrict__ flag, int* __restrict__ list,
@@ -56,7 +56,7 @@ __global__ void k_curvature(Grid g, T NB2, const T* __restrict__ A, const T* __r
         T v[27];
         gather27(g, phi, i0, j0, k, v);
         cv = curv_lsq(A, v, max_kappa, m);
-        if (list && fabs(phi[q]) <= T(4.) * dz && m >= T(0.9) && fabs(cv) * dz >= gate) {
+        if (list && fabs(phi[q]) < T(2.) * dz && m >= T(0.9) && fabs(cv) * dz >= gate) {
             const int slot = atomicAdd(count, 1);
             if (slot < cap) list[slot] = (int)(i0 + g.sy * j0 + g.sz * (long)(k - 1));
         }
@@ -82,7 +82,8 @@ int launch_curvature(Ctx& c, const double* phi, double* kappa, double* normal_m,
 }
 
 // re_curvature (interface.f90:186-218): for |phi| < 2 dz the constant term of the
-// least-squares fit to the 27 curvature values; zero elsewhere.
+// least-squares fit to the 27 curvature values; zero elsewhere.  This smoothed value is
+// what the GFM jumps interpolate to the interface.
 template <class T>
 __global__ void k_recurvature(Grid g, const T* __restrict__ A, const T* __restrict__ phi, const T* __restrict__ kappa,
                               T* __restrict__ kappa2) {

@@ -317,6 +317,30 @@ __device__ __forceinline__ double urand(unsigned long long& s) { return (double)
 // With gate > 0 a sample is drawn again (up to 64 times) until the coarse cell is one
 // the closure is consulted on: |grad phi| >= 0.9 in the least-squares fit and
 // |kappa_coarse| dz >= gate.
+// The closure's inputs from the 27-point stencil v (x fastest, offsets -1 .. 1 in each
+// direction) and the stencil's least-squares gradient (gx, gy, gz):
+//   f[q], q != 13  (v[q] - v[13] - g . r_q) / dz, the stencil less the value and the slope
+//                  at the centre, which leaves its quadratic part: numbers of the size of
+//                  kappa dz that bfloat16 holds to three digits (the raw |v| / dz, up to 6,
+//                  would lose the second differences the curvature lives in to rounding);
+//   f[13]          v[13] / dz, where the interface lies;
+//   f[27..29]      the unit normal g / |g|.
+__device__ inline void mlp_features(const double* v, double gx, double gy, double gz, double dx, double dy, double dz,
+                                    float* f) {
+    int q = 0;
+    for (int c = -1; c <= 1; c++)
+        for (int b = -1; b <= 1; b++)
+            for (int a = -1; a <= 1; a++, q++)
+                f[q] = (float)((v[q] - v[13] - (gx * (a * dx) + gy * (b * dy) + gz * (c * dz))) / dz);
+    f[13] = (float)(v[13] / dz);
+    const double gn = sqrt(gx * gx + gy * gy + gz * gz);
+    f[27] = (float)(gx / gn);
+    f[28] = (float)(gy / gn);
+    f[29] = (float)(gz / gn);
+    f[30] = 0.f;
+    f[31] = 0.f;
+}
+
 __global__ void k_mlp_samples(long n, unsigned long long seed, double r_min, double r_max, double gate, double dx,
                               double dy, double dz, const double* __restrict__ A, const double* __restrict__ Af,
                               uint16_t* __restrict__ x, float* __restrict__ t, float* __restrict__ fd) {
@@ -330,7 +354,8 @@ __global__ void k_mlp_samples(long n, unsigned long long seed, double r_min, dou
         const double rs = sqrt(fmax(0., 1. - zc * zc));
         const double nx = rs * cos(ang), ny = rs * sin(ang), nz = zc;
         sg = urand(s) < 0.5 ? 1. : -1.;
-        const double dlo = fmax(0.1 * R, R - 4. * dz), dhi = R + 4. * dz;
+        // a cell of the re_curvature shell |phi| < 2 dz, with half a cell of margin
+        const double dlo = fmax(0.1 * R, R - 2.5 * dz), dhi = R + 2.5 * dz;
         const double dist = dlo + (dhi - dlo) * urand(s);
         cx = -dist * nx;
         cy = -dist * ny;
@@ -356,14 +381,10 @@ __global__ void k_mlp_samples(long n, unsigned long long seed, double r_min, dou
         gy += A[2 + 10 * k] * v[k];
         gz += A[3 + 10 * k] * v[k];
     }
-    const double gn = sqrt(gx * gx + gy * gy + gz * gz);
+    float fe[MLP_NPAD];
+    mlp_features(v, gx, gy, gz, dx, dy, dz, fe);
     uint16_t* xr = x + i * MLP_NPAD;
-    for (int k = 0; k < 27; k++) xr[k] = f2bf((float)(v[k] / dz));
-    xr[27] = f2bf((float)(gx / gn));
-    xr[28] = f2bf((float)(gy / gn));
-    xr[29] = f2bf((float)(gz / gn));
-    xr[30] = 0;
-    xr[31] = 0;
+    for (int k = 0; k < MLP_NPAD; k++) xr[k] = f2bf(fe[k]);
     // the fine grid: spacing h/4, cells centred at +-h/8 around the origin
     const double fx = dx / 4., fy = dy / 4., fz = dz / 4.;
     double ksum = 0.;
@@ -384,13 +405,17 @@ __global__ void k_mlp_samples(long n, unsigned long long seed, double r_min, dou
 }
 
 // ---------------------------------------------------------------- inference
-// The listed cells (k_curvature): features from phi; the closure's kappa is the
-// finite-difference value plus the network's correction, accepted when the
-// correction is within tol / dz (the finite-difference value stays otherwise).
+// The listed cells (k_curvature): features from phi; the closure's kappa is the cell's
+// own finite-difference value (kfd_in, before re_curvature) plus the network's
+// correction, and it replaces the smoothed value in `kappa` when the correction is
+// within tol / dz (the smoothed finite-difference value stays otherwise).  The
+// smoothing averages the 27 neighbours' curvatures, which for a small drop includes
+// the clipped ones near its centre; the closure's value is the curvature of the level
+// surface through the cell, which is what the jump interpolation wants.
 template <class T>
 __global__ void __launch_bounds__(64)
 k_mlp_curv(Grid g, const int* __restrict__ list, const int* __restrict__ count, int cap, const T* __restrict__ A,
-           const T* __restrict__ phi, T* __restrict__ kappa, const uint16_t* __restrict__ w1b,
+           const T* __restrict__ phi, const T* __restrict__ kfd_in, T* __restrict__ kappa, const uint16_t* __restrict__ w1b,
            const uint16_t* __restrict__ w2b, const float* __restrict__ w, float tol, int* __restrict__ total) {
     __shared__ __align__(16) uint16_t xs[16 * MLP_NPAD];
     __shared__ __align__(16) uint16_t hs[16 * MLP_NH];
@@ -413,23 +438,20 @@ k_mlp_curv(Grid g, const int* __restrict__ list, const int* __restrict__ count, 
                 // strides, which differ from nx and ny inside a level-set window
                 const int i0 = (int)(c % g.sy), j0 = (int)((c / g.sy) % (g.sz / g.sy)), k = (int)(c / g.sz) + 1;
                 q = gidx(g, i0, j0, k);
-                T v[27];
+                double v[27];
                 int qq = 0;
                 for (int cc = -1; cc <= 1; cc++)
                     for (int b = -1; b <= 1; b++)
                         for (int a = -1; a <= 1; a++)
-                            v[qq++] = phi[gidx(g, wrap(i0 + a, g.nx), wrap(j0 + b, g.ny), k + cc)];
+                            v[qq++] = (double)phi[gidx(g, wrap(i0 + a, g.nx), wrap(j0 + b, g.ny), k + cc)];
                 double gx = 0., gy = 0., gz = 0.;
                 for (int kk = 0; kk < 27; kk++) {
-                    f[kk] = (float)((double)v[kk] / dz);
-                    gx += (double)A[1 + 10 * kk] * (double)v[kk];
-                    gy += (double)A[2 + 10 * kk] * (double)v[kk];
-                    gz += (double)A[3 + 10 * kk] * (double)v[kk];
+                    gx += (double)A[1 + 10 * kk] * v[kk];
+                    gy += (double)A[2 + 10 * kk] * v[kk];
+                    gz += (double)A[3 + 10 * kk] * v[kk];
                 }
-                const double gn = sqrt(gx * gx + gy * gy + gz * gz);
-                f[27] = (float)(gx / gn);
-                f[28] = (float)(gy / gn);
-                f[29] = (float)(gz / gn);
+                mlp_features(v, gx, gy, gz, g.dx, g.dy, dz, f);
+                if (!(sqrt(gx * gx + gy * gy + gz * gz) >= 0.9)) q = -1;   // keep the smoothed value
             }
 #pragma unroll
             for (int k = 0; k < MLP_NPAD; k++) xs[l * MLP_NPAD + k] = f2bf(f[k]);
@@ -443,7 +465,7 @@ k_mlp_curv(Grid g, const int* __restrict__ list, const int* __restrict__ count, 
             for (int r = 0; r < 4; r++) {
                 const long q = qs[4 * lg + r];
                 if (q < 0) continue;
-                const double kfd = (double)kappa[q];
+                const double kfd = (double)kfd_in[q];
                 double km = kfd + (double)o.y[r] / dz;
                 const double kmax = 1. / dz;
                 km = km < -kmax ? -kmax : (km > kmax ? kmax : km);
@@ -587,17 +609,18 @@ int launch_mlp_reset_list(Ctx& c) {
     return ICLS_OK;
 }
 
-int launch_mlp_curv(Ctx& c, const double* phi, double* kappa) {
+int launch_mlp_curv(Ctx& c, const double* phi, const double* kappa_fd, double* kappa_out) {
     ProfScope ps(c, "k_mlp_curv");
     Mlp& m = c.mlp;
     const int nb = 512;
     if (c.fp32)
         hipLaunchKernelGGL(k_mlp_curv<float>, dim3(nb), dim3(64), 0, c.stream, c.g, (const int*)m.list,
-                           (const int*)m.count, m.list_cap, as<float>(c.pinvA_f), as<float>(phi), as<float>(kappa),
-                           (const uint16_t*)m.w1b, (const uint16_t*)m.w2b, (const float*)m.w, m.tol, m.total);
+                           (const int*)m.count, m.list_cap, as<float>(c.pinvA_f), as<float>(phi),
+                           as<float>(kappa_fd), as<float>(kappa_out), (const uint16_t*)m.w1b, (const uint16_t*)m.w2b,
+                           (const float*)m.w, m.tol, m.total);
     else
         hipLaunchKernelGGL(k_mlp_curv<double>, dim3(nb), dim3(64), 0, c.stream, c.g, (const int*)m.list,
-                           (const int*)m.count, m.list_cap, (const double*)c.pinvA, phi, kappa,
+                           (const int*)m.count, m.list_cap, (const double*)c.pinvA, phi, kappa_fd, kappa_out,
                            (const uint16_t*)m.w1b, (const uint16_t*)m.w2b, (const float*)m.w, m.tol, m.total);
     HIP_OK(hipGetLastError());
     return ICLS_OK;
