@@ -20,6 +20,7 @@
 #include <cstdio>
 #include <map>
 #include <string>
+#include <cmath>
 #include <vector>
 
 #include "../../include/icls.h"
@@ -78,7 +79,7 @@ struct Mlp {
     std::vector<int*> list_s, count_s;  // one list per side stream (concurrent level-set windows)
     long adam_t = 0;
     bool ready = false;                 // weights set or trained
-    float gate = 0.4f;                  // |kappa| dz at which the closure is consulted
+    float gate = 0.25f;                 // |kappa| dz at which the closure is consulted
     float tol = 0.5f;                   // accepted |kappa_mlp - kappa_fd| dz
 };
 
@@ -267,6 +268,11 @@ int chkdiv_noalloc(const void* u, const void* v, const void* w, int elem_bytes, 
 int launch_tube(Ctx& c, const double* phi_in, double* phi_out);
 int launch_ls_extent(Ctx& c, const double* phi, int* ext);
 // the launch grid is a level-set window, not the whole grid
+// the ls_mass_correction trigger on |V/V0 - 1| (levelset.f90:392): 1e-5, or 10^-n with
+// mass_trigger_exp = n > 0
+inline double mass_trigger(const icls_params& P) {
+    return P.mass_trigger_exp > 0 ? std::pow(10., -P.mass_trigger_exp) : 1e-5;
+}
 inline bool in_window(const Ctx& c) { return c.g.nx != c.gfull.nx || c.g.ny != c.gfull.ny || c.g.nz != c.gfull.nz; }
 // the window of block b of a batched launch: its grid, the offset of its first cell and
 // of its first (i, j) column, its index t and the block's index inside the window

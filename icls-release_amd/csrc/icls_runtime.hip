@@ -200,6 +200,10 @@ static int check_params(const icls_params* p) {
     if (p->ls_adv_space != ICLS_ADV_HOUC5 && p->ls_adv_space != ICLS_ADV_WENO5) { set_error("unknown ls_adv_space"); return ICLS_ERR_PARAM; }
     if (p->mass_correction < 0 || p->mass_correction > 2) { set_error("mass_correction must be 0, 1 or 2"); return ICLS_ERR_PARAM; }
     if (p->reinit_step < 1 || p->inflate_step < 1) { set_error("reinit_step and inflate_step must be >= 1"); return ICLS_ERR_PARAM; }
+    if (p->mass_trigger_exp < 0 || p->mass_trigger_exp > 15) {
+        set_error("mass_trigger_exp must be in [0, 15] (got %d)", p->mass_trigger_exp);
+        return ICLS_ERR_PARAM;
+    }
     if (p->re_initialization && p->reinit_iter_max != 1) {
         set_error("reinit_iter_max other than 1 is not supported");
         return ICLS_ERR_UNSUPPORTED;
@@ -651,7 +655,7 @@ static int mass_correction_due(Ctx& c, int l, int istep, bool& due) {
     HIP_OK(hipEventSynchronize(c.ev_vol));
     double V = c.hvol[l], V0 = c.hvol[c.lmax + l];
     double massloss = (V - V0) / V0;
-    due = fabs(massloss) > 1e-5;
+    due = fabs(massloss) > mass_trigger(c.P);
     return ICLS_OK;
 }
 
@@ -1579,7 +1583,7 @@ int32_t icls_mlp_defaults(icls_mlp_train_opts* o) {
     o->batch = 1024;
     o->lr = 3e-3f;
     o->r_min = 1.5f;
-    o->r_max = 8.f;
+    o->r_max = 12.f;
     o->seed = 20261202ull;
     o->loss_first = o->loss_last = 0.f;
     return ICLS_OK;

@@ -294,13 +294,13 @@ int launch_ipot(Ctx& c, double* dtemp) {
 
 // ---------------------------------------------------------------- mass correction
 // ls_mass_correction trigger (levelset.f90:384-395): maxchk_glb = 0, then inflate
-// when mod(istep,inflate_step) == 0 and |(V - V0)/V0| > 1e-5.
-__global__ void k_infl_flag(int l, int lmax, int due, double* dscal) {
+// when mod(istep,inflate_step) == 0 and |(V - V0)/V0| > trig (1e-5 in the reference).
+__global__ void k_infl_flag(int l, int lmax, int due, double trig, double* dscal) {
     if (threadIdx.x != 0) return;
     dscal[DS_MAXCHK] = 0.;
     double V = dscal[DS_COUNT_FIXED + l], V0 = dscal[DS_COUNT_FIXED + lmax + l];
     double massloss = (V - V0) / V0;
-    dscal[DS_INFL_FLAG] = (due && fabs(massloss) > 1e-5) ? 1. : 0.;
+    dscal[DS_INFL_FLAG] = (due && fabs(massloss) > trig) ? 1. : 0.;
 }
 
 // partial sums of delta*kappa*|grad phi| (levelset.f90:499), of the Heaviside
@@ -393,7 +393,7 @@ __global__ void k_infl_center_vel(Grid g, const double* __restrict__ dscal, cons
 int launch_inflation_prepare(Ctx& c, int l, int istep) {
     ProfScope ps(c, "k_infl_flag");
     int due = (c.P.inflate_step > 0 && istep % c.P.inflate_step == 0) ? 1 : 0;
-    hipLaunchKernelGGL(k_infl_flag, dim3(1), dim3(64), 0, c.stream, l, c.lmax, due, c.dscal);
+    hipLaunchKernelGGL(k_infl_flag, dim3(1), dim3(64), 0, c.stream, l, c.lmax, due, mass_trigger(c.P), c.dscal);
     HIP_OK(hipGetLastError());
     return ICLS_OK;
 }

@@ -32,7 +32,8 @@ module icls_capi
      integer(c_int32_t) :: constPoi, iniu, mode, ioutchk
      real(c_double)     :: cfl
      integer(c_int32_t) :: ls_full
-     integer(c_int32_t) :: reserved(5)
+     integer(c_int32_t) :: mass_trigger_exp
+     integer(c_int32_t) :: reserved(4)
      real(c_double)     :: drop_xc(ICLS_MAX_DROPS), drop_yc(ICLS_MAX_DROPS), drop_zc(ICLS_MAX_DROPS), &
                            drop_r(ICLS_MAX_DROPS)
   end type icls_params

@@ -65,7 +65,7 @@ program ICLS
   p0%reinit_step = reinit_step; p0%reinit_iter_max = reinit_iter_max
   p0%mass_correction = mass_correction; p0%inflate_step = inflate_step; p0%ns_np = ns_np
   p0%wall_speed = wall_speed; p0%constPoi = constPoi; p0%iniu = iniu; p0%mode = mode; p0%ioutchk = ioutchk
-  p0%cfl = 0.; p0%ls_full = 0; p0%reserved = 0
+  p0%cfl = 0.; p0%ls_full = 0; p0%mass_trigger_exp = 0; p0%reserved = 0
   p0%drop_xc = drop_x; p0%drop_yc = drop_y; p0%drop_zc = drop_z; p0%drop_r = drop_r
   call icls_setup(p0)
 

@@ -40,7 +40,7 @@ MODES = {"fp64": P.MODE_FP64, "mixed": P.MODE_MIXED, "learned": P.MODE_MIXED_LEA
 FIELDS = {"phi": P.F_LVSET, "u": P.F_UNEW, "v": P.F_VNEW, "w": P.F_WNEW, "p": P.F_PNEW}
 
 
-def params_for(workload, n, mode_name, mass_correction):
+def params_for(workload, n, mode_name, mass_correction, trigger_exp=0):
     mode = MODES[mode_name]
     if workload == "bubble256":
         p = P.bubble256(n, mode=mode)
@@ -49,11 +49,12 @@ def params_for(workload, n, mode_name, mass_correction):
         p.mode = mode
         p.cfl = 0.3
     p.mass_correction = mass_correction
+    p.mass_trigger_exp = trigger_exp
     return p
 
 
-def run(workload, n, mode, steps, mass_correction):
-    p = params_for(workload, n, mode, mass_correction)
+def run(workload, n, mode, steps, mass_correction, trigger_exp=0):
+    p = params_for(workload, n, mode, mass_correction, trigger_exp)
     g = _lib.Solver(p)
     try:
         g.init_program()
@@ -100,16 +101,19 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--modes", default="mixed,learned")
     ap.add_argument("--mass-correction", type=int, default=2, choices=[0, 1, 2])
+    ap.add_argument("--mass-trigger-exp", type=int, default=0,
+                    help="n > 0: correct the volume when |V/V0 - 1| > 10^-n (the reference: 1e-5)")
     a = ap.parse_args()
     n = a.n or (512 if a.workload == "drop512" else 256)
-    ref, ref_fields = run(a.workload, n, "fp64", a.steps, a.mass_correction)
-    base = {"workload": a.workload, "grid": n, "steps": a.steps, "mass_correction": a.mass_correction}
+    ref, ref_fields = run(a.workload, n, "fp64", a.steps, a.mass_correction, a.mass_trigger_exp)
+    base = {"workload": a.workload, "grid": n, "steps": a.steps, "mass_correction": a.mass_correction,
+            "mass_trigger": 10.0 ** -a.mass_trigger_exp if a.mass_trigger_exp else 1e-5}
     print(json.dumps(dict(base, mode="fp64", ms_per_step=round(ref["ms_per_step"], 3), time=ref["time"],
                           drift=abs(ref["volume"] / ref["volume0"] - 1.0))), flush=True)
     for m in a.modes.split(","):
         if m == "fp64":
             continue
-        res, fields = run(a.workload, n, m, a.steps, a.mass_correction)
+        res, fields = run(a.workload, n, m, a.steps, a.mass_correction, a.mass_trigger_exp)
         linf, l2, band_dz, at, absmax = compare(ref_fields, fields, ref["dz"])
         del fields
         print(json.dumps(dict(base, mode=m, ms_per_step=round(res["ms_per_step"], 3), time=res["time"],

@@ -88,7 +88,8 @@ class IclsParams(ctypes.Structure):
         ("ioutchk", ctypes.c_int32),
         ("cfl", ctypes.c_double),
         ("ls_full", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 5),
+        ("mass_trigger_exp", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 4),
         ("drop_xc", ctypes.c_double * MAX_DROPS),
         ("drop_yc", ctypes.c_double * MAX_DROPS),
         ("drop_zc", ctypes.c_double * MAX_DROPS),

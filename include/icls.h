@@ -106,7 +106,11 @@ typedef struct icls_params {
                                         every 4 steps; the level-set arrays stay full-grid,
                                         values beyond the band outside the window are not
                                         updated); 1 = every level set over the whole grid */
-    int32_t reserved[5];
+    int32_t mass_trigger_exp;        /* the mass correction fires when |V/V0 - 1| > 1e-5
+                                        (levelset.f90:392); n > 0 lowers the trigger to
+                                        10^-n (an extension: the reference's 1e-5 leaves
+                                        the volume free to drift by that much)             */
+    int32_t reserved[4];
     double  drop_xc[ICLS_MAX_DROPS]; /* drop centres and radii, one level set each       */
     double  drop_yc[ICLS_MAX_DROPS];
     double  drop_zc[ICLS_MAX_DROPS];
@@ -262,12 +266,15 @@ int32_t icls_kernel_times(int32_t n, char* names /* n*64 */, double* total_ms, i
 /* ---- learned curvature closure ----
  * No counterpart in the reference: an MLP (30 -> 64 -> 64 -> 1, tanh, BF16 matrix
  * cores with FP32 accumulation) that, in ICLS_MODE_MIXED_LEARNED, replaces the
- * least-squares curvature of get_curvature (src/interface.f90:20-119) on
- * under-resolved interface cells: |phi| <= 4 dz and |kappa| dz >= gate.  The
- * finite-difference value is kept where the network departs from it by more than
- * tol / dz.
- *   input  (32): phi on the 3x3x3 stencil / dz (i fastest, then j, then k), the
- *                unit normal of the least-squares gradient, two zeros
+ * smoothed curvature of get_curvature + re_curvature (src/interface.f90:20-119,
+ * 186-218) on under-resolved interface cells -- |phi| < 2 dz, |grad phi| >= 0.9 and
+ * |kappa| dz >= gate (default 0.25) -- with the cell's own least-squares curvature plus
+ * the network's correction.  The smoothed finite-difference value is kept where the
+ * correction exceeds tol / dz (default 0.5).
+ *   input  (32): the 3x3x3 phi stencil (i fastest, then j, then k) less its centre
+ *                value and least-squares slope, over dz, with the centre value / dz
+ *                at position 13; the unit normal of the least-squares gradient; two
+ *                zeros
  *   output     : the correction to the least-squares kappa * dz
  *   weights    : ICLS_MLP_NPARAM floats, W1 (64 x 32, row-major), b1 (64),
  *                W2 (64 x 64), b2 (64), w3 (64), b3 (1)
@@ -281,7 +288,7 @@ typedef struct {
     int32_t epochs;       /* passes over them (default 60)                     */
     int32_t batch;        /* rows per Adam step, a multiple of 16 (1024)       */
     float lr;             /* initial Adam learning rate (default 3e-3)         */
-    float r_min, r_max;   /* sample radii in dz (default 1.5, 8)               */
+    float r_min, r_max;   /* sample radii in dz (default 1.5, 12)              */
     uint64_t seed;        /* sample generator seed                             */
     float loss_first;     /* out: mean loss of the first epoch                 */
     float loss_last;      /* out: mean loss of the last epoch                  */

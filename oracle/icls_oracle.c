@@ -803,7 +803,9 @@ int oracle_ls_mass_correction(oracle_state* S, int istep, int l) {
     S->maxchk_glb = 0.;
     if (istep % S->P.inflate_step == 0) {
         double massloss = (S->volume[l] - S->init_volume[l]) / S->init_volume[l];
-        if (fabs(massloss) > 1e-5) return oracle_inflation(S, l);
+        /* levelset.f90:392; mass_trigger_exp > 0 lowers the 1e-5 (an extension) */
+        const double trig = S->P.mass_trigger_exp > 0 ? pow(10., -S->P.mass_trigger_exp) : 1e-5;
+        if (fabs(massloss) > trig) return oracle_inflation(S, l);
     }
     return 0;
 }

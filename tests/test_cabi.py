@@ -41,9 +41,9 @@ def _c_layout():
 #include <stddef.h>
 #include "icls.h"
 int main(void) {
-  printf("%zu %zu %zu %zu %zu %zu %zu\n", sizeof(icls_params), offsetof(icls_params, g_unit),
-         offsetof(icls_params, wall_speed), offsetof(icls_params, cfl), offsetof(icls_params, reserved),
-         offsetof(icls_params, drop_xc), offsetof(icls_params, drop_r));
+  printf("%zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(icls_params), offsetof(icls_params, g_unit),
+         offsetof(icls_params, wall_speed), offsetof(icls_params, cfl), offsetof(icls_params, mass_trigger_exp),
+         offsetof(icls_params, reserved), offsetof(icls_params, drop_xc), offsetof(icls_params, drop_r));
   return 0;
 }
 """
@@ -57,8 +57,8 @@ int main(void) {
 
 def test_params_struct_layout_matches_c():
     S = P.IclsParams
-    ours = [ctypes.sizeof(S), S.g_unit.offset, S.wall_speed.offset, S.cfl.offset, S.reserved.offset,
-            S.drop_xc.offset, S.drop_r.offset]
+    ours = [ctypes.sizeof(S), S.g_unit.offset, S.wall_speed.offset, S.cfl.offset, S.mass_trigger_exp.offset,
+            S.reserved.offset, S.drop_xc.offset, S.drop_r.offset]
     assert ours == _c_layout()
 
 

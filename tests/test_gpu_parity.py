@@ -414,6 +414,32 @@ def test_inflation_parity(mc):
         g.close()
 
 
+def test_lowered_mass_trigger_parity():
+    """mass_trigger_exp = 7: a deficit of 3e-7, below the reference's 1e-5 trigger, fires
+    the normalised correction on both sides and not with the default trigger.  maxchk is
+    proportional to V - V0, so it agrees to about sqrt(N) eps / |V - V0| (4x that is the
+    bound, as for the windowed corrections)."""
+    for exp, fires in ((0, False), (7, True)):
+        p = small_drop(32, mass_correction=2)
+        p.mass_trigger_exp = exp
+        o, g = _pair(p)
+        try:
+            o.init_program()
+            g.init_program()
+            v, v0 = o.volume(0)
+            o.set_volume(0, v, v * (1 + 3e-7))
+            g.set_volume(0, v, v * (1 + 3e-7))
+            o.step(1, 0)
+            g.step(1, 0)
+            mo, mg = o.scalar(P.S_MAXCHK), g.scalar(P.S_MAXCHK)
+            assert (mo != 0.0) == fires and (mg != 0.0) == fires, (exp, mo, mg)
+            bound = 4 * np.sqrt(32 ** 3) * np.finfo(float).eps / (3e-7 * v)
+            assert abs(mg - mo) <= bound * abs(mo), (mg, mo, bound)
+            _compare(o, g, FIELDS, 1e-9 if fires else TOL, "trigger 10^-%d" % exp)
+        finally:
+            g.close()
+
+
 @pytest.mark.slow
 def test_drop64_100_steps_parity():
     """The CPU-runnable workload: 64^3 static drop, 100 AB2/FFT-projection steps

@@ -392,3 +392,18 @@ def test_step_leaves_the_fields_related_as_the_operator_order_implies():
     assert np.max(np.abs(o.get(P.F_WCC) - 0.5 * (w[1:-1, 1:-1, :-2] + w[1:-1, 1:-1, 1:-1]))) <= 1e-15
     rc, divtot, divmax, ijk = o.chkdiv(3)
     assert divmax < 1e-12
+
+
+def test_mass_trigger_threshold():
+    """ls_mass_correction fires when |V/V0 - 1| > 1e-5 (levelset.f90:386-395);
+    mass_trigger_exp = n lowers that to 10^-n (an extension)."""
+    fired = {}
+    for exp in (0, 6, 7):
+        p = P.drop64(16)
+        p.mass_trigger_exp = exp
+        o = O.Oracle(p)
+        o.init_program()
+        v, _ = o.volume(0)
+        o.set_volume(0, v, v * (1 + 3e-7))
+        fired[exp] = o.ls_mass_correction(1, 0)
+    assert fired == {0: 0, 6: 0, 7: 1}
