@@ -145,7 +145,7 @@ static void free_ctx(Ctx* c) {
                                c->u_l, c->v_l, c->w_l, c->dheav, c->ucc_l, c->vcc_l, c->wcc_l, c->normal_m,
                                c->ls_a, c->ls_b, c->ls_c, c->mask, c->thomas_d, c->dscal, c->dpart, c->pinvA,
                                c->xyrt, c->tw_x, c->tw_y, c->cos_tab, c->stash_b, c->dls, c->stash, c->scratch, c->pf, c->thomas_y,
-                               c->thomas_seed, c->pinvA_f, c->tw_x_f, c->tw_y_f};
+                               c->thomas_seed, c->pinvA_f, c->tw_x_f, c->tw_y_f, c->pf0};
     if (c->frhs != c->p) ptrs.push_back(c->frhs);   // in FP64 the right-hand side is p itself
     ptrs.push_back(c->jmask);
     ptrs.push_back(c->jmask_old);
@@ -268,6 +268,7 @@ static int build_ctx(const icls_params* prm) {
     if (C.fp32) {
         TRY(alloc_field(C, &C.frhs, sizeof(float)));
         TRY(alloc_field(C, &C.pf, sizeof(float)));
+        TRY(alloc_field(C, &C.pf0, sizeof(float)));
     } else {
         C.frhs = C.p;
     }

@@ -16,6 +16,7 @@
 namespace icls {
 
 static constexpr int BX = 64, BY = 4;
+static constexpr int MBY = 8;   // tile height of the momentum kernel
 static constexpr int KCH = 8;
 
 struct MomParams {
@@ -26,6 +27,7 @@ struct MomParams {
     int rho_equal;
     int mz;                     // layers per block (zchunk_for)
     int plazy, pprev, pfirst;   // the lazy pressure rotation (Ctx::p_lazy ..)
+    uint32_t fbytes;            // bytes of a field array (with 32-bit offsets: below 2^31)
 };
 
 // ab2 (ab2.f90:49-129) with mom_main (mom.f90:13-62), CNTR2_conv (mom.f90:67-130),
@@ -33,26 +35,29 @@ struct MomParams {
 // (mom.f90:779-851).  Face densities are the arithmetic means of update_materials
 // (misc.f90:151-172), formed here from rho.
 //
-// The kernel marches in z.  A block owns a 64 x 4 tile of (i,j) columns over m.mz
+// The kernel marches in z.  A block owns a 64 x 8 tile of (i,j) columns over m.mz
 // consecutive layers.  u, v, w, miu and rho -- the arrays read at neighbouring cells
-// -- pass through a ring of four LDS planes of the tile with a one-cell x/y halo, so
-// every element of them is fetched once per block.  Step k reads planes k-1 .. k+1;
-// right after the step's barrier the block issues the loads of plane k+2 and of the
-// cell's point data for step k+1, computes step k while they are in flight, and
-// stores plane k+2 into the fourth slot (that of plane k-2, which no thread reads any
-// more) before the one barrier of the step.  Tiles are taken in XCD order
-// (xcd_tile).  x and y wrap with a mask: itot and jtot are powers of two.
-constexpr int TW = BX + 2, TH = BY + 2, TP = TW * TH;   // tile plane with halo
-constexpr int NLD = (TP + BX * BY - 1) / (BX * BY);      // plane elements per thread
+// -- pass through rings of LDS planes of the tile with a one-cell x/y halo, so every
+// element of them is fetched once per block.  Step k reads planes k-1 .. k+1 of u, v,
+// w and miu and planes k, k+1 of rho; right after the step's barrier the block issues
+// the loads of plane k+2 and of the cell's point data for step k+1, computes step k
+// while they are in flight, and stores plane k+2 into the slot of a plane no thread
+// reads any more (k-2 in the four-slot ring, k-1 in rho's three-slot ring) before the
+// one barrier of the step.  The rings of the float instantiation (67 KB with the
+// pressure planes) let two blocks of 512 threads share a CU, four waves per SIMD; the
+// y halo adds a quarter to the planes read.  Tiles are taken in XCD order (xcd_tile).
+// x and y wrap with a mask: itot and jtot are powers of two.
+constexpr int TW = BX + 2, TH = MBY + 2, TP = TW * TH;   // tile plane with halo
+constexpr int NLD = (TP + BX * MBY - 1) / (BX * MBY);      // plane elements per thread
 // pressure planes: the tile with a +1 halo in x and y (the split-pressure term reads
-// the cell and its x+1, y+1 and z+1 neighbours)
-constexpr int PW = BX + 1, PP = PW * (BY + 1);
-constexpr int PLD = (PP + BX * BY - 1) / (BX * BY);
+// the cell and its x+1, y+1 neighbours from them, and its z+1 neighbour by a point load)
+constexpr int PW = BX + 1, PP = PW * (MBY + 1);
+constexpr int PLD = (PP + BX * MBY - 1) / (BX * MBY);
 
 // Element q of a field.  With O32 the index and the byte offset are 32-bit, so an access
 // is the field's base (uniform, in scalar registers) plus one 32-bit vector offset and
 // needs no 64-bit address arithmetic; the host launches O32 only when every field is
-// smaller than 4 GiB.
+// smaller than 2 GiB.
 template <bool O32, class P, class I>
 __device__ __forceinline__ P& el(P* b, I q) {
     if constexpr (O32) {
@@ -63,8 +68,26 @@ __device__ __forceinline__ P& el(P* b, I q) {
     }
 }
 
+// A buffer resource over the first nbytes bytes of a field (nbytes < 2^31).  A buffer
+// load at a byte offset of 2^31 or more is out of range: it returns zero and reads no
+// memory, which is how a masked-off face jump is read.
+constexpr uint32_t OFF_NONE = 0x80000000u;
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t field_rsrc(const void* p, uint32_t nbytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)nbytes, 0x00020000);
+}
+template <class T>
+__device__ __forceinline__ T bload(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    if constexpr (std::is_same<T, float>::value) {
+        return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
+    } else {
+        const auto d = __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0);
+        return __hiloint2double((int)d[1], (int)d[0]);
+    }
+}
+
 template <class T, int RHS_ONLY, bool P2, bool O32>
-__global__ void __launch_bounds__(BX * BY)
+__global__ void __launch_bounds__(BX * MBY)
+__attribute__((amdgpu_waves_per_eu(std::is_same<T, float>::value && O32 ? 4 : 1)))
 k_momentum_zm(Grid g, MomParams m, const double* __restrict__ dscal, const T* __restrict__ u, const T* __restrict__ v,
               const T* __restrict__ w, const T* __restrict__ miu, const T* __restrict__ rho, T* __restrict__ duold,
               T* __restrict__ dvold, T* __restrict__ dwold, const double* __restrict__ pnew,
@@ -72,17 +95,19 @@ k_momentum_zm(Grid g, MomParams m, const double* __restrict__ dscal, const T* __
               const T* __restrict__ p_z, const T* __restrict__ p_xold, const T* __restrict__ p_yold,
               const T* __restrict__ p_zold, T* __restrict__ dudt, T* __restrict__ dvdt, T* __restrict__ dwdt,
               const uint8_t* __restrict__ jm, const uint8_t* __restrict__ jmo, const T* __restrict__ zp) {
-    __shared__ T ring[5][4][TP];   // u, v, w, miu, rho x four planes
-    // pnew and pold x three planes: at step k the derive for layer k+1 reads planes k+1
-    // and k+2 while plane k+3 is stored into the slot of plane k
-    __shared__ double pring[2][3][RHS_ONLY ? 1 : PP];
+    __shared__ T ring[4][4][TP];   // u, v, w, miu x four planes
+    __shared__ T rring[3][TP];     // rho x three planes (read at layers k and k+1 only)
+    // pnew and pold x two planes: at step k the derive for layer k+1 reads plane k+1
+    // (the cell and its x+1, y+1 neighbours) while plane k+2 is stored into the slot of
+    // plane k; the z+1 neighbour is a point load of the cell's own column
+    __shared__ double pring[2][2][RHS_ONLY ? 1 : PP];
     using Ix = typename std::conditional<O32, uint32_t, long>::type;
     auto cell = [&](int i_, int j_, int k_) -> Ix { return (Ix)i_ + (Ix)g.sy * (Ix)j_ + (Ix)g.sz * (Ix)(k_ + 2); };
     const int MZ = m.mz;
-    const int ntx = (g.nx + BX - 1) / BX, nty = (g.ny + BY - 1) / BY, ntz = (g.nz + MZ - 1) / MZ;
+    const int ntx = (g.nx + BX - 1) / BX, nty = (g.ny + MBY - 1) / MBY, ntz = (g.nz + MZ - 1) / MZ;
     const int tile = xcd_tile(blockIdx.x, ntx * nty * ntz);
     const int bx = tile % ntx, by = (tile / ntx) % nty, bz = tile / (ntx * nty);
-    const int x0 = bx * BX, y0 = by * BY, kb = 1 + bz * MZ;
+    const int x0 = bx * BX, y0 = by * MBY, kb = 1 + bz * MZ;
     const int ke = kb + MZ - 1 < g.nz ? kb + MZ - 1 : g.nz;
     const int tx = threadIdx.x, ty = threadIdx.y, tid = tx + BX * ty;
     const int xm = g.nx - 1, ym = g.ny - 1;
@@ -91,7 +116,7 @@ k_momentum_zm(Grid g, MomParams m, const double* __restrict__ dscal, const T* __
     Ix poff[NLD];
 #pragma unroll
     for (int r = 0; r < NLD; r++) {
-        int e = tid + r * BX * BY;
+        int e = tid + r * BX * MBY;
         if (e >= TP) e = TP - 1;
         const int lx = e % TW - 1, ly = e / TW - 1;
         poff[r] = (Ix)((x0 + lx) & xm) + (Ix)g.sy * (Ix)((y0 + ly) & ym);
@@ -99,7 +124,7 @@ k_momentum_zm(Grid g, MomParams m, const double* __restrict__ dscal, const T* __
     Ix qoff[PLD];
 #pragma unroll
     for (int r = 0; r < PLD; r++) {
-        int e = tid + r * BX * BY;
+        int e = tid + r * BX * MBY;
         if (e >= PP) e = PP - 1;
         qoff[r] = (Ix)((x0 + e % PW) & xm) + (Ix)g.sy * (Ix)((y0 + e / PW) & ym);
     }
@@ -113,13 +138,13 @@ k_momentum_zm(Grid g, MomParams m, const double* __restrict__ dscal, const T* __
             pnx[1][r] = el<O32>(pold, base + qoff[r]);
         }
     };
-    auto pslot = [&](int k) { return (k - kb + 3) % 3; };   // k >= kb
+    auto pslot = [&](int k) { return (k - kb) & 1; };   // k >= kb
     auto pstash = [&](int k) {
         if (RHS_ONLY) return;
         const int sl = pslot(k);
 #pragma unroll
         for (int r = 0; r < PLD; r++) {
-            const int e = tid + r * BX * BY;
+            const int e = tid + r * BX * MBY;
             if (e < PP) {
                 pring[0][sl][e] = pnx[0][r];
                 pring[1][sl][e] = pnx[1][r];
@@ -135,32 +160,35 @@ k_momentum_zm(Grid g, MomParams m, const double* __restrict__ dscal, const T* __
 #pragma unroll
             for (int r = 0; r < NLD; r++) dst[f][r] = el<O32>(src[f], base + poff[r]);
     };
-    auto stash = [&](int slot, T (*from)[NLD]) {
-#pragma unroll
-        for (int f = 0; f < 5; f++)
-#pragma unroll
-            for (int r = 0; r < NLD; r++) {
-                const int e = tid + r * BX * BY;
-                if (e < TP) ring[f][slot][e] = from[f][r];
-            }
-    };
     auto slot_of = [&](int k) { return (k - kb + 4) & 3; };   // k >= kb - 1
+    auto rslot_of = [&](int k) { return (k - kb + 4) % 3; };  // k >= kb - 1
+    auto stash = [&](int k, T (*from)[NLD]) {
+        const int sl = slot_of(k), rs = rslot_of(k);
+#pragma unroll
+        for (int r = 0; r < NLD; r++) {
+            const int e = tid + r * BX * MBY;
+            if (e < TP) {
+#pragma unroll
+                for (int f = 0; f < 4; f++) ring[f][sl][e] = from[f][r];
+                rring[rs][e] = from[4][r];
+            }
+        }
+    };
     for (int k = kb - 1; k <= kb + 1; k++) {
         fetch(k, nxt);
-        stash(slot_of(k), nxt);
+        stash(k, nxt);
     }
-    // pressure planes kb .. kb + 2 (no further than ke + 1, the last one read)
-    for (int k = kb; k <= kb + 2 && k <= ke + 1; k++) {
+    // pressure planes kb and kb + 1 (no further than ke, the last one read)
+    for (int k = kb; k <= kb + 1 && k <= ke; k++) {
         pfetch(k);
         pstash(k);
     }
 
-    // A thread past the end of x or y (nx < 64 or ny < 4) takes the wrapped cell: its
+    // A thread past the end of x or y (nx < 64 or ny < 8) takes the wrapped cell: its
     // ring positions and point loads wrap the same way, so it computes and stores the
     // very values of the thread that owns the cell.  No branch guards the arithmetic
     // and the stores, which keeps the memory-counter waits of the loop exact.
     const int i = (x0 + tx) & xm, j = (y0 + ty) & ym;
-    const int ip = (i + 1) & xm, jp = (j + 1) & ym;
     // the cell's point data (AB2 history, pressures, face jumps), loaded one z step
     // ahead and turned into the values the step uses (du0 .. gpz) before it is loaded
     // again.  Every load of the loop is unconditional, its layer clamped to the last
@@ -168,12 +196,18 @@ k_momentum_zm(Grid g, MomParams m, const double* __restrict__ dscal, const T* __
     // copy, and the copy waits for every load in flight.
     // The face jumps are read only where the GFM masks say they may be nonzero; the
     // mask bytes travel one step further ahead (jn, jo: the layer fetch_point loads
-    // next), so that the jump loads do not wait for them.  A masked-off lane loads a
-    // zero of the zero plane zp instead (cached, the same words every layer): the load
-    // stays unconditional, with no branch and no select after it.
+    // next), so that the jump loads do not wait for them.  With 32-bit offsets a
+    // masked-off lane gives its buffer load an out-of-range offset, which returns zero
+    // without reading memory; with 64-bit offsets it loads a zero of the zero plane zp
+    // (cached, the same words every layer).  Either way the load stays unconditional,
+    // with no branch and no select after it.
     T pd[9];
-    double pp[8];
+    double pp[6], pz[2];
     unsigned jn = 7u, jo = 7u;
+    const uint32_t fb = m.fbytes;
+    const __amdgpu_buffer_rsrc_t rjx = field_rsrc(p_x, fb), rjxo = field_rsrc(p_xold, fb);
+    const __amdgpu_buffer_rsrc_t rjy = field_rsrc(p_y, fb), rjyo = field_rsrc(p_yold, fb);
+    const __amdgpu_buffer_rsrc_t rjz = field_rsrc(p_z, fb), rjzo = field_rsrc(p_zold, fb);
     auto fetch_mask = [&](int k) {
         if (RHS_ONLY) return;
         const Ix q = cell(i, j, k);
@@ -182,19 +216,31 @@ k_momentum_zm(Grid g, MomParams m, const double* __restrict__ dscal, const T* __
     };
     auto fetch_point = [&](int k) {
         if (RHS_ONLY) return;
-        const Ix q = cell(i, j, k), qx = cell(ip, j, k), qy = cell(i, jp, k), qz = q + (Ix)g.sz;
+        const Ix q = cell(i, j, k);
         pd[0] = el<O32>(duold, q); pd[1] = el<O32>(dvold, q); pd[2] = el<O32>(dwold, q);
-        const T* z0 = zp + (i + g.sy * (long)j);
-        pd[3] = *((jn & 1u) ? &el<O32>(p_x, q) : z0);
-        pd[4] = *((jo & 1u) ? &el<O32>(p_xold, q) : z0);
-        pd[5] = *((jn & 2u) ? &el<O32>(p_y, q) : z0);
-        pd[6] = *((jo & 2u) ? &el<O32>(p_yold, q) : z0);
-        pd[7] = *((jn & 4u) ? &el<O32>(p_z, q) : z0);
-        pd[8] = *((jo & 4u) ? &el<O32>(p_zold, q) : z0);
-        (void)qx; (void)qy; (void)qz;
+        // the pressures of the z+1 neighbour, for the split-pressure term of layer k
+        pz[0] = el<O32>(pnew, q + (Ix)g.sz);
+        pz[1] = el<O32>(pold, q + (Ix)g.sz);
+        if constexpr (O32) {
+            const uint32_t b = (uint32_t)q * (uint32_t)sizeof(T);
+            pd[3] = bload<T>(rjx, (jn & 1u) ? b : OFF_NONE);
+            pd[4] = bload<T>(rjxo, (jo & 1u) ? b : OFF_NONE);
+            pd[5] = bload<T>(rjy, (jn & 2u) ? b : OFF_NONE);
+            pd[6] = bload<T>(rjyo, (jo & 2u) ? b : OFF_NONE);
+            pd[7] = bload<T>(rjz, (jn & 4u) ? b : OFF_NONE);
+            pd[8] = bload<T>(rjzo, (jo & 4u) ? b : OFF_NONE);
+        } else {
+            const T* z0 = zp + (i + g.sy * (long)j);
+            pd[3] = *((jn & 1u) ? &el<O32>(p_x, q) : z0);
+            pd[4] = *((jo & 1u) ? &el<O32>(p_xold, q) : z0);
+            pd[5] = *((jn & 2u) ? &el<O32>(p_y, q) : z0);
+            pd[6] = *((jo & 2u) ? &el<O32>(p_yold, q) : z0);
+            pd[7] = *((jn & 4u) ? &el<O32>(p_z, q) : z0);
+            pd[8] = *((jo & 4u) ? &el<O32>(p_zold, q) : z0);
+        }
     };
     T du0 = T(0.), dv0 = T(0.), dw0 = T(0.);
-    double gpx = 0., gpy = 0., gpz = 0.;
+    T gpx = T(0.), gpy = T(0.), gpz = T(0.);   // formed in double, kept in T
     const double exn = m.ex_n, exo = m.ex_o;
     const T exn_ = T(m.ex_n), exo_ = T(m.ex_o);
     // pnew and pold from the raw buffers of a lazy rotation, with the operations of
@@ -212,12 +258,9 @@ k_momentum_zm(Grid g, MomParams m, const double* __restrict__ dscal, const T* __
         {
             const double* N0 = pring[0][pslot(L)];
             const double* O0 = pring[1][pslot(L)];
-            const double* N1 = pring[0][pslot(L + 1)];
-            const double* O1 = pring[1][pslot(L + 1)];
             pp[0] = N0[cp]; pp[1] = O0[cp];
             pp[2] = N0[cp + 1]; pp[3] = O0[cp + 1];
             pp[4] = N0[cp + PW]; pp[5] = O0[cp + PW];
-            pp[6] = N1[cp]; pp[7] = O1[cp];
         }
         du0 = pd[0]; dv0 = pd[1]; dw0 = pd[2];
         // split pressure (ab2.f90:79-101): p_hat = 2 p^n - p^(n-1), p_hat_j = 2 p_j - p_j,old
@@ -230,7 +273,7 @@ k_momentum_zm(Grid g, MomParams m, const double* __restrict__ dscal, const T* __
             const double phq = exn * pn_of(pp[0]) - exo * po_of(pp[1]);
             const double phx = exn * pn_of(pp[2]) - exo * po_of(pp[3]);
             const double phy = exn * pn_of(pp[4]) - exo * po_of(pp[5]);
-            const double phz = exn * pn_of(pp[6]) - exo * po_of(pp[7]);
+            const double phz = exn * pn_of(pz[0]) - exo * po_of(pz[1]);
             gpx = phx - (double)pjx - phq;
             gpy = phy - (double)pjy - phq;
             gpz = phz - (double)pjz - phq;
@@ -238,9 +281,9 @@ k_momentum_zm(Grid g, MomParams m, const double* __restrict__ dscal, const T* __
             // float fields use only the differences p_hat(neighbour) - p_hat(cell), still
             // formed in double; the lazy rotation's offsets are the same at every cell and
             // cancel in them, so the raw buffers are differenced directly
-            gpx = (exn * (pp[2] - pp[0]) - exo * (pp[3] - pp[1])) - (double)pjx;
-            gpy = (exn * (pp[4] - pp[0]) - exo * (pp[5] - pp[1])) - (double)pjy;
-            gpz = (exn * (pp[6] - pp[0]) - exo * (pp[7] - pp[1])) - (double)pjz;
+            gpx = T((exn * (pp[2] - pp[0]) - exo * (pp[3] - pp[1])) - (double)pjx);
+            gpy = T((exn * (pp[4] - pp[0]) - exo * (pp[5] - pp[1])) - (double)pjy);
+            gpz = T((exn * (pz[0] - pp[0]) - exo * (pz[1] - pp[1])) - (double)pjz);
         }
     };
     fetch_mask(kb);
@@ -248,7 +291,7 @@ k_momentum_zm(Grid g, MomParams m, const double* __restrict__ dscal, const T* __
     __syncthreads();
     derive(kb);
     fetch_mask(min(kb + 1, ke));
-    // the first step stores plane kb + 3 into the slot of plane kb, read just above
+    // the first step stores pressure plane kb + 2 into the slot of plane kb, read just above
     if (!RHS_ONLY) __syncthreads();
     const Sp<T, P2> sx{T(g.dx), T(g.dxi)}, sy{T(g.dy), T(g.dyi)}, sz{T(g.dz), T(g.dzi)};
     const T qt = T(0.25), two = T(2.), one = T(1.);
@@ -259,15 +302,15 @@ k_momentum_zm(Grid g, MomParams m, const double* __restrict__ dscal, const T* __
     for (int k = kb; k <= ke; k++) {
         const Ix q = cell(i, j, k);
         fetch(min(k + 2, ke + 1), nxt);
-        pfetch(min(k + 3, ke + 1));
+        pfetch(min(k + 2, ke));
         fetch_point(min(k + 1, ke));
         fetch_mask(min(k + 2, ke));
         const T* U[3] = {ring[0][slot_of(k - 1)], ring[0][slot_of(k)], ring[0][slot_of(k + 1)]};
         const T* V[3] = {ring[1][slot_of(k - 1)], ring[1][slot_of(k)], ring[1][slot_of(k + 1)]};
         const T* W[3] = {ring[2][slot_of(k - 1)], ring[2][slot_of(k)], ring[2][slot_of(k + 1)]};
         const T* MU[3] = {ring[3][slot_of(k - 1)], ring[3][slot_of(k)], ring[3][slot_of(k + 1)]};
-        const T* RH = ring[4][slot_of(k)];
-        const T* RHP = ring[4][slot_of(k + 1)];
+        const T* RH = rring[rslot_of(k)];
+        const T* RHP = rring[rslot_of(k + 1)];
         // a(dx, dy, dz) at offset (dx, dy) in plane k + dz
 #define AT(A, dx_, dy_, dz_) A[(dz_) + 1][c0 + (dy_) * TW + (dx_)]
         {
@@ -391,9 +434,9 @@ k_momentum_zm(Grid g, MomParams m, const double* __restrict__ dscal, const T* __
                 T ws = wq + dt * (c15 * dwn - c05 * dw0);
                 el<O32>(duold, q) = dun; el<O32>(dvold, q) = dvn; el<O32>(dwold, q) = dwn;
                 // ---- split-pressure terms (mom.f90:793,818,843)
-                T mpx = -(qdiv(one, rho_u) - irho0) * T(g.dxi) * T(gpx);
-                T mpy = -(qdiv(one, rho_v) - irho0) * T(g.dyi) * T(gpy);
-                T mpz = -(qdiv(one, rho_w) - irho0) * T(g.dzi) * T(gpz);
+                T mpx = -(qdiv(one, rho_u) - irho0) * T(g.dxi) * gpx;
+                T mpy = -(qdiv(one, rho_v) - irho0) * T(g.dyi) * gpy;
+                T mpz = -(qdiv(one, rho_w) - irho0) * T(g.dzi) * gpz;
                 el<O32>(dudt, q) = us + dt * mpx;
                 el<O32>(dvdt, q) = vs + dt * mpy - qdiv(T(m.forceytot) * dt, rho_v);
                 el<O32>(dwdt, q) = ws + dt * mpz;
@@ -401,20 +444,20 @@ k_momentum_zm(Grid g, MomParams m, const double* __restrict__ dscal, const T* __
         }
 #undef AT
         derive(min(k + 1, ke));
-        stash(slot_of(k + 2), nxt);
-        if (k + 3 <= ke + 1) pstash(k + 3);
+        stash(k + 2, nxt);
+        if (k + 2 <= ke) pstash(k + 2);
         __syncthreads();
     }
 }
 
 template <class T>
 static void momentum_dispatch(Ctx& c, MomParams m, int rhs_only) {
-    dim3 blk(BX, BY, 1);
-    const int txy = ((c.g.nx + BX - 1) / BX) * ((c.g.ny + BY - 1) / BY);
+    dim3 blk(BX, MBY, 1);
+    const int txy = ((c.g.nx + BX - 1) / BX) * ((c.g.ny + MBY - 1) / MBY);
     int ntiles = 0;
 #define MOM(R, P, O)                                                                                               \
     do {                                                                                                           \
-        m.mz = zchunk_for(c, (const void*)k_momentum_zm<T, R, P, O>, BX * BY, txy, 1, 2);                        \
+        m.mz = zchunk_for(c, (const void*)k_momentum_zm<T, R, P, O>, BX * MBY, txy, 1, 2);                        \
         ntiles = txy * ((c.g.nz + m.mz - 1) / m.mz);                                                               \
         hipLaunchKernelGGL((k_momentum_zm<T, R, P, O>), dim3(ntiles), blk, 0, c.stream, c.g, m,                     \
                            (const double*)c.dscal, as<T>(c.unew), as<T>(c.vnew), as<T>(c.wnew), as<T>(c.miu),      \
@@ -423,9 +466,11 @@ static void momentum_dispatch(Ctx& c, MomParams m, int rhs_only) {
                            as<T>(c.p_yold), as<T>(c.p_zold), as<T>(c.dudt), as<T>(c.dvdt), as<T>(c.dwdt),    \
                            (const uint8_t*)c.jmask, (const uint8_t*)c.jmask_old, as<T>(c.zplane));                 \
     } while (0)
-    // 32-bit offsets when the largest field (a double pressure) is below 4 GiB; the
-    // right-hand-side-only launch (once, at init) keeps 64-bit offsets
-    const bool o32 = (unsigned long)c.nfield * sizeof(double) < (1ul << 32);
+    // 32-bit offsets when the largest field (a double pressure) is below 2 GiB, so that
+    // every offset, and the out-of-range offset 2^31 of a masked-off jump, fit in 32
+    // bits; the right-hand-side-only launch (once, at init) keeps 64-bit offsets
+    const bool o32 = (unsigned long)c.nfield * sizeof(double) < (1ul << 31);
+    m.fbytes = o32 ? (uint32_t)(c.nfield * sizeof(T)) : 0u;
     if (c.g.pow2) {
         if (rhs_only) MOM(1, true, false);
         else if (o32) MOM(0, true, true);

@@ -28,20 +28,22 @@ template <class C> __device__ __forceinline__ cx<C> cadd(cx<C> a, cx<C> b) { ret
 template <class C> __device__ __forceinline__ cx<C> csub(cx<C> a, cx<C> b) { return {a.re - b.re, a.im - b.im}; }
 template <class C> __device__ __forceinline__ cx<C> cconj(cx<C> a) { return {a.re, -a.im}; }
 
+// out[q] = v, or with ACC out[q] = b0[q] + v: the refinement pass adds its correction
+// to the float solution b0 of the first pass
 template <class TO, class C, int ACC>
-__device__ __forceinline__ void put(TO* out, long q, C v) {
-    if (ACC) out[q] = out[q] + (TO)v;
+__device__ __forceinline__ void put(TO* out, long q, C v, const float* __restrict__ b0) {
+    if (ACC) out[q] = (TO)b0[q] + (TO)v;
     else out[q] = (TO)v;
 }
 
 // Residual of the discrete Poisson problem the transforms and the Thomas solve
 // invert exactly (periodic 3-point second differences in x and y, the Neumann
 // tridiagonal rows of initsolver.f90:36-49 in z): rhs - L p at one cell, in double.
-template <class T>
-__device__ __forceinline__ double residual_at(const Grid& g, const T* __restrict__ rhs, const double* __restrict__ p,
+template <class T, class P>
+__device__ __forceinline__ double residual_at(const Grid& g, const T* __restrict__ rhs, const P* __restrict__ p,
                                               int i0, int j0, int k) {
     const long q = gidx(g, i0, j0, k);
-    const double pc = p[q];
+    const double pc = (double)p[q];
     const double ax = g.dxi * g.dxi, ay = g.dyi * g.dyi, az = g.dzi * g.dzi;
     double lx = (p[gidx(g, wrap(i0 + 1, g.nx), j0, k)] - 2. * pc + p[gidx(g, wrap(i0 - 1, g.nx), j0, k)]) * ax;
     double ly = (p[gidx(g, i0, wrap(j0 + 1, g.ny), k)] - 2. * pc + p[gidx(g, i0, wrap(j0 - 1, g.ny), k)]) * ay;
@@ -77,7 +79,8 @@ __device__ __forceinline__ void stockham(cx<C>* x, cx<C>* y, int M, int lane, in
 // DIR 1: lines along y (stride nx), blocks of LPB consecutive i at one k.
 // The input is C (the compute type), the output TO; ACC adds into the output.
 template <int DIR, int INV, class C, class TO, int ACC>
-__global__ void k_rfft(Grid g, int N, int LPB, int TPL, const C* __restrict__ twg, const C* in, TO* out) {
+__global__ void k_rfft(Grid g, int N, int LPB, int TPL, const C* __restrict__ twg, const C* in, TO* out,
+                       const float* __restrict__ b0) {
     extern __shared__ unsigned char smem_raw[];
     cx<C>* smem = (cx<C>*)smem_raw;
     const int M = N / 2;
@@ -156,23 +159,24 @@ __global__ void k_rfft(Grid g, int N, int LPB, int TPL, const C* __restrict__ tw
                 cx<C> X = cadd(ev, cmul(w, od));
                 v = (n & 1) ? X.re : X.im;
             }
-            put<TO, C, ACC>(out, gq(lnn, n), scale * v);
+            put<TO, C, ACC>(out, gq(lnn, n), scale * v, b0);
         }
     } else {
         for (int e = tid; e < LPB * N; e += nthr) {
             int lnn, n;
             split(e, lnn, n);
             cx<C> z = outb[(long)lnn * M + (n >> 1)];
-            put<TO, C, ACC>(out, gq(lnn, n), scale * ((n & 1) ? z.im : z.re));
+            put<TO, C, ACC>(out, gq(lnn, n), scale * ((n & 1) ? z.im : z.re), b0);
         }
     }
 }
 
 // The same transform with the line length N and the lines per block LPB fixed at
 // compile time: every index split is a shift and every stage loop unrolls.  With
-// RES (forward x only) the input is not read from in but formed as the residual
-// rhs - L p of the double solution p, so the refinement pass starts without a
-// residual array.
+// RES (forward x only) the input is not read from in but formed in double as the
+// residual rhs - L b0 of the first pass's float solution b0, so the refinement pass
+// starts without a residual array.  With ACC (inverse x only) the output is b0 plus
+// the transform.
 template <int N, int LPB>
 struct RfftShape {
     static constexpr int M = N / 2;
@@ -182,7 +186,7 @@ struct RfftShape {
 
 template <int DIR, int INV, int N, int LPB, class C, class TO, int ACC, int RES>
 __global__ void __launch_bounds__((RfftShape<N, LPB>::NTHR))
-k_rfft_n(Grid g, const C* __restrict__ twg, const C* in, TO* out, const double* __restrict__ pres) {
+k_rfft_n(Grid g, const C* __restrict__ twg, const C* in, TO* out, const float* __restrict__ b0) {
     static_assert(!RES || (DIR == 0 && !INV), "the residual feeds the forward x transform");
     constexpr int M = N / 2;
     constexpr int TPL = RfftShape<N, LPB>::TPL;
@@ -207,7 +211,7 @@ k_rfft_n(Grid g, const C* __restrict__ twg, const C* in, TO* out, const double* 
             int ln, n;
             split(e, ln, n);
             C v;
-            if (RES) v = (C)residual_at(g, in, pres, n, line0 + ln, k);
+            if (RES) v = (C)residual_at(g, in, b0, n, line0 + ln, k);
             else v = in[gq(ln, n)];
             cx<C>* z = bufa + ln * M + (n >> 1);
             if (n & 1) z->im = v; else z->re = v;
@@ -276,7 +280,7 @@ k_rfft_n(Grid g, const C* __restrict__ twg, const C* in, TO* out, const double* 
                 cx<C> X = cadd(ev, cmul(w, od));
                 v = (n & 1) ? X.re : X.im;
             }
-            put<TO, C, ACC>(out, gq(lnn, n), scale * v);
+            put<TO, C, ACC>(out, gq(lnn, n), scale * v, b0);
         }
     } else {
 #pragma unroll 4
@@ -284,7 +288,7 @@ k_rfft_n(Grid g, const C* __restrict__ twg, const C* in, TO* out, const double* 
             int lnn, n;
             split(e, lnn, n);
             cx<C> z = outb[lnn * M + (n >> 1)];
-            put<TO, C, ACC>(out, gq(lnn, n), scale * ((n & 1) ? z.im : z.re));
+            put<TO, C, ACC>(out, gq(lnn, n), scale * ((n & 1) ? z.im : z.re), b0);
         }
     }
 }
@@ -397,15 +401,16 @@ __device__ __forceinline__ void cfft128(cx<float>* v, cx<float> (*u)[8], int s, 
 }  // namespace r16
 
 template <class TO, int ACC>
-__device__ __forceinline__ void put2(TO* out, long q, float a, float b) {
-    put<TO, float, ACC>(out, q, a);
-    put<TO, float, ACC>(out, q + 1, b);
+__device__ __forceinline__ void put2(TO* out, long q, float a, float b, const float* __restrict__ b0) {
+    put<TO, float, ACC>(out, q, a, b0);
+    put<TO, float, ACC>(out, q + 1, b, b0);
 }
 
 // along y: 512 threads, 64 lines (i) x 8 slots; LDS 64 KB exchange + twiddles
 template <int INV, class TO, int ACC>
 __global__ void __launch_bounds__(512) k_rfft256_y(Grid g, const float* __restrict__ twg, const float* in, TO* out) {
     using namespace r16;
+    static_assert(!ACC, "only the x transform adds into its output");
     __shared__ cx<float> xb[M * 64];
     __shared__ cx<float> tw[M];
     const int lane = threadIdx.x & 63, s = threadIdx.x >> 6;
@@ -448,8 +453,8 @@ __global__ void __launch_bounds__(512) k_rfft256_y(Grid g, const float* __restri
             for (int k2 = 0; k2 < 8; k2++) {
                 const int m = 2 * s + h + 16 * k2;
                 const cx<float> z = u[h][2 * (k2 % 4) + k2 / 4];
-                put<TO, float, ACC>(out, row(2 * m), z.re * scale);
-                put<TO, float, ACC>(out, row(2 * m + 1), z.im * scale);
+                put<TO, float, ACC>(out, row(2 * m), z.re * scale, nullptr);
+                put<TO, float, ACC>(out, row(2 * m + 1), z.im * scale, nullptr);
             }
         return;
     }
@@ -465,21 +470,21 @@ __global__ void __launch_bounds__(512) k_rfft256_y(Grid g, const float* __restri
             const int kk = 2 * s + h + 16 * k2;
             const cx<float> a = u[h][2 * (k2 % 4) + k2 / 4];
             if (kk == 0) {
-                put<TO, float, ACC>(out, row(0), (a.re + a.im) * scale);
-                put<TO, float, ACC>(out, row(N - 1), (a.re - a.im) * scale);
+                put<TO, float, ACC>(out, row(0), (a.re + a.im) * scale, nullptr);
+                put<TO, float, ACC>(out, row(N - 1), (a.re - a.im) * scale, nullptr);
             } else {
                 const cx<float> F = fwd_split(tw, kk, a, xb[(M - kk) * 64 + lane]);
-                put<TO, float, ACC>(out, row(2 * kk - 1), F.re * scale);
-                put<TO, float, ACC>(out, row(2 * kk), F.im * scale);
+                put<TO, float, ACC>(out, row(2 * kk - 1), F.re * scale, nullptr);
+                put<TO, float, ACC>(out, row(2 * kk), F.im * scale, nullptr);
             }
         }
 }
 
 // along x: 256 threads, 32 lines (j) x 8 slots; with RES (forward) the input is
-// rhs - L pres formed on the fly
+// rhs - L b0 formed on the fly
 template <int INV, class TO, int ACC, int RES>
 __global__ void __launch_bounds__(256) k_rfft256_x(Grid g, const float* __restrict__ twg, const float* in, TO* out,
-                                                  const double* __restrict__ pres) {
+                                                  const float* __restrict__ b0) {
     using namespace r16;
     static_assert(!RES || !INV, "the residual feeds the forward transform");
     static_assert(INV || (!ACC && sizeof(TO) == 4), "the forward transform writes the float work array");
@@ -499,7 +504,7 @@ __global__ void __launch_bounds__(256) k_rfft256_x(Grid g, const float* __restri
         // the residual line goes through LDS so that its stencils are not all in flight
         float* fl = reinterpret_cast<float*>(xl);
 #pragma unroll 2
-        for (int x = s; x < N; x += 8) fl[x] = (float)residual_at(g, in, pres, x, j, k);
+        for (int x = s; x < N; x += 8) fl[x] = (float)residual_at(g, in, b0, x, j, k);
         __syncthreads();
 #pragma unroll
         for (int r = 0; r < 16; r++) v[r] = xl[s + 8 * r];
@@ -538,8 +543,8 @@ __global__ void __launch_bounds__(256) k_rfft256_x(Grid g, const float* __restri
             if (sizeof(TO) == 4 && !ACC) {
                 *reinterpret_cast<float4*>(out + q) = make_float4(z0.re * scale, z0.im * scale, z1.re * scale, z1.im * scale);
             } else {
-                put2<TO, ACC>(out, q, z0.re * scale, z0.im * scale);
-                put2<TO, ACC>(out, q + 2, z1.re * scale, z1.im * scale);
+                put2<TO, ACC>(out, q, z0.re * scale, z0.im * scale, b0);
+                put2<TO, ACC>(out, q + 2, z1.re * scale, z1.im * scale, b0);
             }
         }
         return;
@@ -589,13 +594,13 @@ __global__ void __launch_bounds__(256) k_rfft256_x(Grid g, const float* __restri
 
 // the register kernels cover float lines of 256 with 64 | nx (y) or 32 | ny (x)
 template <int DIR, int INV, class C, class TO, int ACC, int RES>
-static bool try_rfft256(Ctx& c, const C* tw, const C* in, TO* out, const double* pres) {
+static bool try_rfft256(Ctx& c, const C* tw, const C* in, TO* out, const float* b0) {
     if constexpr (!std::is_same<C, float>::value) return false;
     else {
         const int N = DIR == 0 ? c.g.nx : c.g.ny;
         if (N != 256) return false;
         if (DIR == 1) {
-            if constexpr (RES) return false;
+            if constexpr (RES || ACC) return false;
             else {
                 if (c.g.nx % 64) return false;
                 hipLaunchKernelGGL((k_rfft256_y<INV, TO, ACC>), dim3((c.g.nx / 64) * c.g.nz), dim3(512), 0, c.stream,
@@ -605,41 +610,41 @@ static bool try_rfft256(Ctx& c, const C* tw, const C* in, TO* out, const double*
         }
         if (c.g.ny % 32) return false;
         hipLaunchKernelGGL((k_rfft256_x<INV, TO, ACC, RES>), dim3((c.g.ny / 32) * c.g.nz), dim3(256), 0, c.stream,
-                           c.g, tw, in, out, pres);
+                           c.g, tw, in, out, b0);
         return true;
     }
 }
 
 template <int DIR, int INV, int N, int LPB, class C, class TO, int ACC, int RES>
-static bool try_rfft_n(Ctx& c, const C* tw, const C* in, TO* out, int n, int nlines, const double* pres) {
+static bool try_rfft_n(Ctx& c, const C* tw, const C* in, TO* out, int n, int nlines, const float* b0) {
     if (n != N || nlines % LPB != 0) return false;
     hipLaunchKernelGGL((k_rfft_n<DIR, INV, N, LPB, C, TO, ACC, RES>), dim3((nlines / LPB) * c.g.nz),
-                       dim3(RfftShape<N, LPB>::NTHR), 0, c.stream, c.g, tw, in, out, pres);
+                       dim3(RfftShape<N, LPB>::NTHR), 0, c.stream, c.g, tw, in, out, b0);
     return true;
 }
 
 // lines per block: along x 4; along y enough lines for 64-byte row segments
 template <int DIR, int INV, class C, class TO, int ACC, int RES>
-static bool rfft_fixed(Ctx& c, const C* tw, const C* in, TO* out, int n, int nlines, const double* pres) {
+static bool rfft_fixed(Ctx& c, const C* tw, const C* in, TO* out, int n, int nlines, const float* b0) {
     constexpr int L = DIR == 0 ? 4 : (sizeof(C) == 4 ? 16 : 8);
     constexpr int L512 = L > 8 ? 8 : 4, L1024 = L > 8 ? 4 : 2;
-    return try_rfft_n<DIR, INV, 32, L, C, TO, ACC, RES>(c, tw, in, out, n, nlines, pres) ||
-           try_rfft_n<DIR, INV, 64, L, C, TO, ACC, RES>(c, tw, in, out, n, nlines, pres) ||
-           try_rfft_n<DIR, INV, 128, L, C, TO, ACC, RES>(c, tw, in, out, n, nlines, pres) ||
-           try_rfft_n<DIR, INV, 256, L, C, TO, ACC, RES>(c, tw, in, out, n, nlines, pres) ||
-           try_rfft_n<DIR, INV, 512, L512, C, TO, ACC, RES>(c, tw, in, out, n, nlines, pres) ||
-           try_rfft_n<DIR, INV, 1024, L1024, C, TO, ACC, RES>(c, tw, in, out, n, nlines, pres);
+    return try_rfft_n<DIR, INV, 32, L, C, TO, ACC, RES>(c, tw, in, out, n, nlines, b0) ||
+           try_rfft_n<DIR, INV, 64, L, C, TO, ACC, RES>(c, tw, in, out, n, nlines, b0) ||
+           try_rfft_n<DIR, INV, 128, L, C, TO, ACC, RES>(c, tw, in, out, n, nlines, b0) ||
+           try_rfft_n<DIR, INV, 256, L, C, TO, ACC, RES>(c, tw, in, out, n, nlines, b0) ||
+           try_rfft_n<DIR, INV, 512, L512, C, TO, ACC, RES>(c, tw, in, out, n, nlines, b0) ||
+           try_rfft_n<DIR, INV, 1024, L1024, C, TO, ACC, RES>(c, tw, in, out, n, nlines, b0);
 }
 
 // Transform of every line along DIR.  With RES the forward x transform takes
-// rhs - L pres as its input (in = rhs); false means no fixed-size kernel covers the
+// rhs - L b0 as its input (in = rhs); false means no fixed-size kernel covers the
 // shape and the caller forms the residual itself.
 template <int DIR, int INV, class C, class TO, int ACC, int RES = 0>
-static int rfft_launch(Ctx& c, const C* tw, const C* in, TO* out, const double* pres = nullptr) {
+static int rfft_launch(Ctx& c, const C* tw, const C* in, TO* out, const float* b0 = nullptr) {
     const int N = DIR == 0 ? c.g.nx : c.g.ny;
     const int nlines = DIR == 0 ? c.g.ny : c.g.nx;
-    if (try_rfft256<DIR, INV, C, TO, ACC, RES>(c, tw, in, out, pres) ||
-        rfft_fixed<DIR, INV, C, TO, ACC, RES>(c, tw, in, out, N, nlines, pres)) {
+    if (try_rfft256<DIR, INV, C, TO, ACC, RES>(c, tw, in, out, b0) ||
+        rfft_fixed<DIR, INV, C, TO, ACC, RES>(c, tw, in, out, N, nlines, b0)) {
         HIP_OK(hipGetLastError());
         return ICLS_OK;
     }
@@ -657,7 +662,7 @@ static int rfft_launch(Ctx& c, const C* tw, const C* in, TO* out, const double* 
     if (nthr < 64) nthr = 64;
     size_t shm = (size_t)(2L * LPB * M + M) * sizeof(cx<C>);
     dim3 grd((nlines / LPB) * c.g.nz);
-    hipLaunchKernelGGL((k_rfft<DIR, INV, C, TO, ACC>), grd, dim3(nthr), shm, c.stream, c.g, N, LPB, TPL, tw, in, out);
+    hipLaunchKernelGGL((k_rfft<DIR, INV, C, TO, ACC>), grd, dim3(nthr), shm, c.stream, c.g, N, LPB, TPL, tw, in, out, b0);
     HIP_OK(hipGetLastError());
     return ICLS_OK;
 }
@@ -943,7 +948,7 @@ static int thomas_launch(Ctx& c, TS* f, double* y) {
 
 // the residual as an array, for shapes the fused forward transform does not cover
 template <class T>
-__global__ void k_residual(Grid g, const T* __restrict__ rhs, const double* __restrict__ p, T* __restrict__ r) {
+__global__ void k_residual(Grid g, const T* __restrict__ rhs, const float* __restrict__ p, T* __restrict__ r) {
     int i0 = blockIdx.x * 64 + threadIdx.x, j0 = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z + 1;
     if (i0 >= g.nx || j0 >= g.ny) return;
     r[gidx(g, i0, j0, k)] = (T)residual_at(g, rhs, p, i0, j0, k);
@@ -975,16 +980,18 @@ static int solve_fp64(Ctx& c) {
     return rfft_launch<0, 1, double, double, 0>(c, twx, p, p);
 }
 
-// one float-transform solve, result written to (ACC = 0) or added into (ACC = 1) the
-// double pressure: of L x = rhs (ACC = 0), or of L x = rhs - L p (ACC = 1), the
-// residual formed inside the first transform where a fixed-size kernel covers the
-// line length and by k_residual otherwise
+// One float-transform solve.  ACC = 0: of L p0 = rhs, the solution written as float to
+// p0 (the transforms produce float values, so nothing is lost against a double
+// store).  ACC = 1: of L e = rhs - L p0, the residual formed in double inside the
+// first transform where a fixed-size kernel covers the line length and by k_residual
+// otherwise, and p = p0 + e written as double.
 template <int ACC>
 static int solve_fp32_pass(Ctx& c) {
     const float* twx = as<float>(c.tw_x_f);
     const float* twy = as<float>(c.tw_y_f);
     const float* rhs = as<float>(c.frhs);
     float* w = as<float>(c.pf);
+    float* p0 = as<float>(c.pf0);
     int rc;
     if (!ACC) {
         ProfScope ps(c, "k_rfft_x_fwd");
@@ -992,11 +999,11 @@ static int solve_fp32_pass(Ctx& c) {
         if (rc) return rc;
     } else {
         ProfScope ps(c, "k_rfft_x_fwd_res");
-        rc = rfft_launch<0, 0, float, float, 0, 1>(c, twx, rhs, w, (const double*)c.p);
+        rc = rfft_launch<0, 0, float, float, 0, 1>(c, twx, rhs, w, p0);
         if (rc < 0) {
             dim3 blk(64, 4, 1);
             hipLaunchKernelGGL(k_residual<float>, grid3(c.g, c.g.nz, blk), blk, 0, c.stream, c.g, rhs,
-                               (const double*)c.p, w);
+                               (const float*)p0, w);
             HIP_OK(hipGetLastError());
             rc = rfft_launch<0, 0, float, float, 0>(c, twx, w, w);
         }
@@ -1015,7 +1022,8 @@ static int solve_fp32_pass(Ctx& c) {
         if (rc) return rc;
     }
     ProfScope ps(c, "k_rfft_x_inv");
-    return rfft_launch<0, 1, float, double, ACC>(c, twx, w, c.p);
+    if (!ACC) return rfft_launch<0, 1, float, float, 0>(c, twx, w, p0);
+    return rfft_launch<0, 1, float, double, 1>(c, twx, w, c.p, p0);
 }
 
 static int solve_mixed(Ctx& c) {
