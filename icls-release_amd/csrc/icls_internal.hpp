@@ -136,6 +136,7 @@ struct Ctx {
     int device = 0;
     int ncu = 0;                          // compute units of the device
     std::map<const void*, int> zchunk;    // z-chunk length per z-marching kernel
+    std::map<const void*, int> resident;  // resident blocks per CU per looping kernel
     bool initialised = false;
 
     // field class (T)

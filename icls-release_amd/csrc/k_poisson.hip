@@ -828,82 +828,105 @@ constexpr int TMAXSEG = 16 * (64 / TCPW);   // segments per block at 1024 thread
 
 template <class TS, int S>
 __global__ void __launch_bounds__(1024, (TCPW == 32 && S <= 8 ? 8 : 4))
-k_thomas_seg(Grid g, const double* __restrict__ xyrt, const double* __restrict__ dseed, TS* p) {
+k_thomas_seg(Grid g, const double* __restrict__ xyrt, const double* __restrict__ dseed, TS* p, int ngrp) {
     __shared__ double cv[TMAXSEG][TCPW], cf[TMAXSEG][TCPW];   // segment carries: value, coefficient
     const int lane = threadIdx.x % TCPW, s = threadIdx.x / TCPW;
     const int P = blockDim.x / TCPW;
     const long ncol = (long)g.nx * g.ny;
-    const long c = (long)blockIdx.x * TCPW + lane;
-    const bool act = c < ncol;
     ThomasRow<S> R;
     R.a = g.dzi * g.dzi;
     R.cc = g.dzi * g.dzi;
     R.bmid = -(R.a + R.cc);
     R.b1 = R.bmid + R.a;
     R.bN = R.bmid + R.cc;
-    R.lam = act ? xyrt[c] : 0.;
     R.n = g.nz;
     const int k0 = s * S + 1;
-    const long q0 = gidx(g, 0, 0, k0) + c;
-    double y[S], z[S];
+    const long qk = gidx(g, 0, 0, k0);
+    // A block takes the column groups blockIdx.x, + gridDim.x, ...  With PF the
+    // right-hand side, the mode's shift and the multiplier seed of the next group are
+    // loaded before the current group is eliminated, so the loads are in flight during
+    // the arithmetic (where the registers allow it: float storage, 16 layers).
+    constexpr bool PF = sizeof(TS) == 4 && S == 16;
+    TS rn[S];
+    double lam_n = 0., seed_n = 0.;
+    auto fetch = [&](int grp) {
+        const long c = (long)grp * TCPW + lane;
+        const bool act = c < ncol;
 #pragma unroll
-    for (int t = 0; t < S; t++) y[t] = act ? (double)p[q0 + (long)t * g.sz] : 0.;
-    double dprev = (s > 0 && act) ? dseed[(long)(s - 1) * ncol + c] : 0.;
+        for (int t = 0; t < S; t++) rn[t] = act ? p[qk + c + (long)t * g.sz] : TS(0.);
+        lam_n = act ? xyrt[c] : 0.;
+        seed_n = (s > 0 && act) ? dseed[(long)(s - 1) * ncol + c] : 0.;
+    };
+    int grp = blockIdx.x;
+    if (PF && grp < ngrp) fetch(grp);
+    for (; grp < ngrp; grp += gridDim.x) {
+        if (!PF) fetch(grp);
+        const long c = (long)grp * TCPW + lane;
+        const bool act = c < ncol;
+        const long q0 = qk + c;
+        double y[S], z[S];
 #pragma unroll
-    for (int t = 0; t < S; t++) z[t] = R.piv(k0 + t, dprev);
-    // forward: exit value with 0 entering, and the coefficient of the entering value
-    {
-        double v = 0., f = 1.;
+        for (int t = 0; t < S; t++) y[t] = (double)rn[t];
+        R.lam = lam_n;
+        double dprev = seed_n;
+        if (PF && grp + (int)gridDim.x < ngrp) fetch(grp + gridDim.x);
+#pragma unroll
+        for (int t = 0; t < S; t++) z[t] = R.piv(k0 + t, dprev);
+        // forward: exit value with 0 entering, and the coefficient of the entering value
+        {
+            double v = 0., f = 1.;
+#pragma unroll
+            for (int t = 0; t < S; t++) {
+                const int k = k0 + t;
+                v = R.fwd(k, y[t], z[t], v);
+                if (k == 1) f = 0.;
+                else if (k < R.n) f = -R.a * z[t] * f;
+                else f = z[t] != 0. ? -R.a * f / z[t] : 0.;
+            }
+            cv[s][lane] = v;
+            cf[s][lane] = f;
+        }
+        __syncthreads();
+        double yin = 0.;
+        for (int t = 0; t < s; t++) yin = cv[t][lane] + cf[t][lane] * yin;
 #pragma unroll
         for (int t = 0; t < S; t++) {
-            const int k = k0 + t;
-            v = R.fwd(k, y[t], z[t], v);
-            if (k == 1) f = 0.;
-            else if (k < R.n) f = -R.a * z[t] * f;
-            else f = z[t] != 0. ? -R.a * f / z[t] : 0.;
+            yin = R.fwd(k0 + t, y[t], z[t], yin);
+            y[t] = yin;
         }
-        cv[s][lane] = v;
-        cf[s][lane] = f;
-    }
-    __syncthreads();
-    double yin = 0.;
-    for (int t = 0; t < s; t++) yin = cv[t][lane] + cf[t][lane] * yin;
+        __syncthreads();   // the carries are read before they are replaced
+        // back substitution: value at the segment's first layer with 0 above it, and
+        // the coefficient of the value above
+        {
+            double v = 0., f = 1.;
 #pragma unroll
-    for (int t = 0; t < S; t++) {
-        yin = R.fwd(k0 + t, y[t], z[t], yin);
-        y[t] = yin;
-    }
-    __syncthreads();   // the carries are read before they are replaced
-    // back substitution: value at the segment's first layer with 0 above it, and the
-    // coefficient of the value above
-    {
-        double v = 0., f = 1.;
+            for (int t = S - 1; t >= 0; t--) {
+                const int k = k0 + t;
+                if (k == R.n) { v = y[t]; f = 0.; }
+                else {
+                    const double d = R.cc * z[t];
+                    v = y[t] - d * v;
+                    f = -d * f;
+                }
+            }
+            cv[s][lane] = v;
+            cf[s][lane] = f;
+        }
+        __syncthreads();
+        double pa = 0.;
+        for (int t = P - 1; t > s; t--) pa = cv[t][lane] + cf[t][lane] * pa;
 #pragma unroll
         for (int t = S - 1; t >= 0; t--) {
             const int k = k0 + t;
-            if (k == R.n) { v = y[t]; f = 0.; }
-            else {
-                const double d = R.cc * z[t];
-                v = y[t] - d * v;
-                f = -d * f;
-            }
+            if (k == R.n) pa = y[t];
+            else pa = y[t] - (R.cc * z[t]) * pa;
+            y[t] = pa;
         }
-        cv[s][lane] = v;
-        cf[s][lane] = f;
-    }
-    __syncthreads();
-    double pa = 0.;
-    for (int t = P - 1; t > s; t--) pa = cv[t][lane] + cf[t][lane] * pa;
+        if (act)
 #pragma unroll
-    for (int t = S - 1; t >= 0; t--) {
-        const int k = k0 + t;
-        if (k == R.n) pa = y[t];
-        else pa = y[t] - (R.cc * z[t]) * pa;
-        y[t] = pa;
+            for (int t = 0; t < S; t++) p[q0 + (long)t * g.sz] = (TS)y[t];
+        __syncthreads();   // the carries are read before the next group writes them
     }
-    if (act)
-#pragma unroll
-        for (int t = 0; t < S; t++) p[q0 + (long)t * g.sz] = (TS)y[t];
 }
 
 // Segment length of the partitioned solve for nz layers, 0 where the serial kernel
@@ -923,19 +946,37 @@ int launch_thomas_seed(Ctx& c) {
     return ICLS_OK;
 }
 
+// blocks of a looping kernel: as many as are resident at once, no more than there is
+// work for (two rounds of resident blocks measured the same as one)
+static unsigned looping_grid(Ctx& c, const void* fn, int threads, long nwork) {
+    auto it = c.resident.find(fn);
+    int occ;
+    if (it != c.resident.end()) occ = it->second;
+    else {
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, threads, 0) != hipSuccess || occ < 1) occ = 1;
+        c.resident[fn] = occ;
+    }
+    const long nb = (long)occ * (c.ncu > 0 ? c.ncu : 1);
+    return (unsigned)(nb < nwork ? nb : nwork);
+}
+
 template <class TS>
 static int thomas_launch(Ctx& c, TS* f, double* y) {
     ProfScope ps(c, "k_thomas");
     if (c.thomas_seg) {
         const long ncol = (long)c.g.nx * c.g.ny;
         const int P = c.g.nz / c.thomas_seg;
-        dim3 grd((unsigned)((ncol + TCPW - 1) / TCPW)), blk(TCPW * P);
-        if (c.thomas_seg == 16)
-            hipLaunchKernelGGL((k_thomas_seg<TS, 16>), grd, blk, 0, c.stream, c.g, (const double*)c.xyrt,
-                               (const double*)c.thomas_seed, f);
-        else
-            hipLaunchKernelGGL((k_thomas_seg<TS, 8>), grd, blk, 0, c.stream, c.g, (const double*)c.xyrt,
-                               (const double*)c.thomas_seed, f);
+        const int ngrp = (int)((ncol + TCPW - 1) / TCPW);
+        dim3 blk(TCPW * P);
+        if (c.thomas_seg == 16) {
+            const void* fn = (const void*)k_thomas_seg<TS, 16>;
+            hipLaunchKernelGGL((k_thomas_seg<TS, 16>), dim3(looping_grid(c, fn, blk.x, ngrp)), blk, 0, c.stream, c.g,
+                               (const double*)c.xyrt, (const double*)c.thomas_seed, f, ngrp);
+        } else {
+            const void* fn = (const void*)k_thomas_seg<TS, 8>;
+            hipLaunchKernelGGL((k_thomas_seg<TS, 8>), dim3(looping_grid(c, fn, blk.x, ngrp)), blk, 0, c.stream, c.g,
+                               (const double*)c.xyrt, (const double*)c.thomas_seed, f, ngrp);
+        }
         HIP_OK(hipGetLastError());
         return ICLS_OK;
     }
