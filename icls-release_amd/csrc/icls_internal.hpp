@@ -323,8 +323,9 @@ int launch_recurvature(Ctx& c, const double* phi, const double* kappa, double* k
 int launch_gfm(Ctx& c, int l, const double* kappa2, bool fuse_fillps, bool store_pjump);
 int launch_fillps(Ctx& c);
 int launch_add_pjump(Ctx& c);
-// poisson: solve L p = frhs into p (solver2d; in the mixed modes with one refinement)
-int launch_solve(Ctx& c);
+// poisson: solve L p = frhs into p (solver2d; in the mixed modes with one refinement,
+// which with warm starts from the previous time step's solution where one is held)
+int launch_solve(Ctx& c, bool warm);
 // solver2d_cos: p -> solution with y rule 1 Walls, 2 InOut, 3 OutOut, in double in every mode
 int launch_solve_cos(Ctx& c, int bcy);
 int launch_pressure_materialize(Ctx& c);

@@ -1054,15 +1054,16 @@ int32_t icls_add_pjump(void) {
     return launch_add_pjump(c);
 }
 
-// solver2d on the right-hand side frhs (p itself in FP64) into p
-static int solver_impl(Ctx& c) {
+// solver2d on the right-hand side frhs (p itself in FP64) into p; warm inside the time
+// step (the mixed refinement may start from the previous step's solution)
+static int solver_impl(Ctx& c, bool warm) {
     c.p_in_pnew = false;
-    return launch_solve(c);
+    return launch_solve(c, warm);
 }
 
 int32_t icls_solver2d(void) {
     CTX_OR_FAIL();
-    return solver_impl(c);
+    return solver_impl(c, false);
 }
 
 int32_t icls_solver2d_cos(int32_t bc_y) {
@@ -1541,7 +1542,7 @@ int32_t icls_step(int32_t istep, int32_t begin) {
     TRY(launch_momentum(c));                               // main.f90:130
     TRY(launch_bc_uvw(c, c.dudt, c.dvdt, c.dwdt));
     TRY(gfm_impl(c, true, false));                         // main.f90:139-143
-    TRY(solver_impl(c));                                   // main.f90:149
+    TRY(solver_impl(c, true));                             // main.f90:149
     TRY(launch_bc_p(c, c.p));                              // main.f90:155
     TRY(launch_correc(c));                                 // main.f90:159-160
     TRY(launch_bc_uvw(c, c.unew, c.vnew, c.wnew));

@@ -29,9 +29,9 @@ template <class C> __device__ __forceinline__ cx<C> csub(cx<C> a, cx<C> b) { ret
 template <class C> __device__ __forceinline__ cx<C> cconj(cx<C> a) { return {a.re, -a.im}; }
 
 // out[q] = v, or with ACC out[q] = b0[q] + v: the refinement pass adds its correction
-// to the float solution b0 of the first pass
-template <class TO, class C, int ACC>
-__device__ __forceinline__ void put(TO* out, long q, C v, const float* __restrict__ b0) {
+// to its starting solution b0
+template <class TO, class C, int ACC, class B>
+__device__ __forceinline__ void put(TO* out, long q, C v, const B* __restrict__ b0) {
     if (ACC) out[q] = (TO)b0[q] + (TO)v;
     else out[q] = (TO)v;
 }
@@ -39,19 +39,28 @@ __device__ __forceinline__ void put(TO* out, long q, C v, const float* __restric
 // Residual of the discrete Poisson problem the transforms and the Thomas solve
 // invert exactly (periodic 3-point second differences in x and y, the Neumann
 // tridiagonal rows of initsolver.f90:36-49 in z): rhs - L p at one cell, in double.
+// From the cell's value pc, its x, y and z neighbours (zp unused at k = nz, zm at
+// k = 1) and the right-hand side r.
+__device__ __forceinline__ double residual_of(const Grid& g, int k, double r, double pc, double xp, double xm,
+                                              double yp, double ym, double zp, double zm) {
+    const double ax = g.dxi * g.dxi, ay = g.dyi * g.dyi, az = g.dzi * g.dzi;
+    double lx = (xp - 2. * pc + xm) * ax;
+    double ly = (yp - 2. * pc + ym) * ay;
+    double lz;
+    if (k == 1) lz = (zp - pc) * az;
+    else if (k == g.nz) lz = (zm - pc) * az;
+    else lz = (zp - 2. * pc + zm) * az;
+    return r - (lx + ly + lz);
+}
 template <class T, class P>
 __device__ __forceinline__ double residual_at(const Grid& g, const T* __restrict__ rhs, const P* __restrict__ p,
                                               int i0, int j0, int k) {
     const long q = gidx(g, i0, j0, k);
-    const double pc = (double)p[q];
-    const double ax = g.dxi * g.dxi, ay = g.dyi * g.dyi, az = g.dzi * g.dzi;
-    double lx = (p[gidx(g, wrap(i0 + 1, g.nx), j0, k)] - 2. * pc + p[gidx(g, wrap(i0 - 1, g.nx), j0, k)]) * ax;
-    double ly = (p[gidx(g, i0, wrap(j0 + 1, g.ny), k)] - 2. * pc + p[gidx(g, i0, wrap(j0 - 1, g.ny), k)]) * ay;
-    double lz;
-    if (k == 1) lz = (p[gidx(g, i0, j0, 2)] - pc) * az;
-    else if (k == g.nz) lz = (p[gidx(g, i0, j0, k - 1)] - pc) * az;
-    else lz = (p[gidx(g, i0, j0, k + 1)] - 2. * pc + p[gidx(g, i0, j0, k - 1)]) * az;
-    return (double)rhs[q] - (lx + ly + lz);
+    const double zp = k < g.nz || k == 1 ? (double)p[q + g.sz] : 0.;
+    const double zm = k > 1 ? (double)p[q - g.sz] : 0.;
+    return residual_of(g, k, (double)rhs[q], (double)p[q], (double)p[gidx(g, wrap(i0 + 1, g.nx), j0, k)],
+                       (double)p[gidx(g, wrap(i0 - 1, g.nx), j0, k)], (double)p[gidx(g, i0, wrap(j0 + 1, g.ny), k)],
+                       (double)p[gidx(g, i0, wrap(j0 - 1, g.ny), k)], zp, zm);
 }
 
 // In-LDS Stockham radix-2 FFT of M points for one line; TPL threads per line.
@@ -78,9 +87,9 @@ __device__ __forceinline__ void stockham(cx<C>* x, cx<C>* y, int M, int lane, in
 // DIR 0: lines along x (contiguous), blocks of LPB consecutive j at one k.
 // DIR 1: lines along y (stride nx), blocks of LPB consecutive i at one k.
 // The input is C (the compute type), the output TO; ACC adds into the output.
-template <int DIR, int INV, class C, class TO, int ACC>
+template <int DIR, int INV, class C, class TO, int ACC, class B>
 __global__ void k_rfft(Grid g, int N, int LPB, int TPL, const C* __restrict__ twg, const C* in, TO* out,
-                       const float* __restrict__ b0) {
+                       const B* __restrict__ b0) {
     extern __shared__ unsigned char smem_raw[];
     cx<C>* smem = (cx<C>*)smem_raw;
     const int M = N / 2;
@@ -184,9 +193,9 @@ struct RfftShape {
     static constexpr int NTHR = LPB * TPL < 64 ? 64 : LPB * TPL;
 };
 
-template <int DIR, int INV, int N, int LPB, class C, class TO, int ACC, int RES>
+template <int DIR, int INV, int N, int LPB, class C, class TO, int ACC, int RES, class B>
 __global__ void __launch_bounds__((RfftShape<N, LPB>::NTHR))
-k_rfft_n(Grid g, const C* __restrict__ twg, const C* in, TO* out, const float* __restrict__ b0) {
+k_rfft_n(Grid g, const C* __restrict__ twg, const C* in, TO* out, const B* __restrict__ b0) {
     static_assert(!RES || (DIR == 0 && !INV), "the residual feeds the forward x transform");
     constexpr int M = N / 2;
     constexpr int TPL = RfftShape<N, LPB>::TPL;
@@ -400,8 +409,8 @@ __device__ __forceinline__ void cfft128(cx<float>* v, cx<float> (*u)[8], int s, 
 }
 }  // namespace r16
 
-template <class TO, int ACC>
-__device__ __forceinline__ void put2(TO* out, long q, float a, float b, const float* __restrict__ b0) {
+template <class TO, int ACC, class B>
+__device__ __forceinline__ void put2(TO* out, long q, float a, float b, const B* __restrict__ b0) {
     put<TO, float, ACC>(out, q, a, b0);
     put<TO, float, ACC>(out, q + 1, b, b0);
 }
@@ -453,8 +462,8 @@ __global__ void __launch_bounds__(512) k_rfft256_y(Grid g, const float* __restri
             for (int k2 = 0; k2 < 8; k2++) {
                 const int m = 2 * s + h + 16 * k2;
                 const cx<float> z = u[h][2 * (k2 % 4) + k2 / 4];
-                put<TO, float, ACC>(out, row(2 * m), z.re * scale, nullptr);
-                put<TO, float, ACC>(out, row(2 * m + 1), z.im * scale, nullptr);
+                put<TO, float, ACC>(out, row(2 * m), z.re * scale, (const float*)nullptr);
+                put<TO, float, ACC>(out, row(2 * m + 1), z.im * scale, (const float*)nullptr);
             }
         return;
     }
@@ -470,21 +479,21 @@ __global__ void __launch_bounds__(512) k_rfft256_y(Grid g, const float* __restri
             const int kk = 2 * s + h + 16 * k2;
             const cx<float> a = u[h][2 * (k2 % 4) + k2 / 4];
             if (kk == 0) {
-                put<TO, float, ACC>(out, row(0), (a.re + a.im) * scale, nullptr);
-                put<TO, float, ACC>(out, row(N - 1), (a.re - a.im) * scale, nullptr);
+                put<TO, float, ACC>(out, row(0), (a.re + a.im) * scale, (const float*)nullptr);
+                put<TO, float, ACC>(out, row(N - 1), (a.re - a.im) * scale, (const float*)nullptr);
             } else {
                 const cx<float> F = fwd_split(tw, kk, a, xb[(M - kk) * 64 + lane]);
-                put<TO, float, ACC>(out, row(2 * kk - 1), F.re * scale, nullptr);
-                put<TO, float, ACC>(out, row(2 * kk), F.im * scale, nullptr);
+                put<TO, float, ACC>(out, row(2 * kk - 1), F.re * scale, (const float*)nullptr);
+                put<TO, float, ACC>(out, row(2 * kk), F.im * scale, (const float*)nullptr);
             }
         }
 }
 
 // along x: 256 threads, 32 lines (j) x 8 slots; with RES (forward) the input is
 // rhs - L b0 formed on the fly
-template <int INV, class TO, int ACC, int RES>
+template <int INV, class TO, int ACC, int RES, class B>
 __global__ void __launch_bounds__(256) k_rfft256_x(Grid g, const float* __restrict__ twg, const float* in, TO* out,
-                                                  const float* __restrict__ b0) {
+                                                  const B* __restrict__ b0) {
     using namespace r16;
     static_assert(!RES || !INV, "the residual feeds the forward transform");
     static_assert(INV || (!ACC && sizeof(TO) == 4), "the forward transform writes the float work array");
@@ -501,10 +510,22 @@ __global__ void __launch_bounds__(256) k_rfft256_x(Grid g, const float* __restri
     cx<float>* xl = xb + line * LS;
     cx<float> v[16];
     if (RES) {
-        // the residual line goes through LDS so that its stencils are not all in flight
+        // the residual line goes through LDS so that its stencils are not all in flight;
+        // the neighbour lines are addressed from their starts
         float* fl = reinterpret_cast<float*>(xl);
-#pragma unroll 2
-        for (int x = s; x < N; x += 8) fl[x] = (float)residual_at(g, in, b0, x, j, k);
+        const B* pc = b0 + q0;
+        const B* pyp = b0 + gidx(g, 0, wrap(j + 1, g.ny), k);
+        const B* pym = b0 + gidx(g, 0, wrap(j - 1, g.ny), k);
+        const bool zup = k < g.nz || k == 1, zdn = k > 1;
+        const B* pzp = pc + (zup ? g.sz : 0);
+        const B* pzm = pc - (zdn ? g.sz : 0);
+        const float* rl = in + q0;
+#pragma unroll 4
+        for (int x = s; x < N; x += 8) {
+            const double zp = (double)pzp[x], zm = (double)pzm[x];
+            fl[x] = (float)residual_of(g, k, (double)rl[x], (double)pc[x], (double)pc[(x + 1) & (N - 1)],
+                                       (double)pc[(x - 1) & (N - 1)], (double)pyp[x], (double)pym[x], zp, zm);
+        }
         __syncthreads();
 #pragma unroll
         for (int r = 0; r < 16; r++) v[r] = xl[s + 8 * r];
@@ -540,8 +561,23 @@ __global__ void __launch_bounds__(256) k_rfft256_x(Grid g, const float* __restri
         for (int k2 = 0; k2 < 8; k2++) {
             const cx<float> z0 = u[0][2 * (k2 % 4) + k2 / 4], z1 = u[1][2 * (k2 % 4) + k2 / 4];
             const long q = q0 + 4 * s + 32 * k2;
-            if (sizeof(TO) == 4 && !ACC) {
+            if constexpr (sizeof(TO) == 4 && !ACC) {
                 *reinterpret_cast<float4*>(out + q) = make_float4(z0.re * scale, z0.im * scale, z1.re * scale, z1.im * scale);
+            } else if constexpr (sizeof(TO) == 8 && ACC) {
+                // q is a multiple of 4: the base and the output move as 16-byte vectors
+                double b[4];
+                if constexpr (std::is_same<B, float>::value) {
+                    const float4 t = *reinterpret_cast<const float4*>(b0 + q);
+                    b[0] = t.x; b[1] = t.y; b[2] = t.z; b[3] = t.w;
+                } else {
+                    const double2 t0 = *reinterpret_cast<const double2*>(b0 + q);
+                    const double2 t1 = *reinterpret_cast<const double2*>(b0 + q + 2);
+                    b[0] = t0.x; b[1] = t0.y; b[2] = t1.x; b[3] = t1.y;
+                }
+                *reinterpret_cast<double2*>(out + q) =
+                    make_double2(b[0] + (double)(z0.re * scale), b[1] + (double)(z0.im * scale));
+                *reinterpret_cast<double2*>(out + q + 2) =
+                    make_double2(b[2] + (double)(z1.re * scale), b[3] + (double)(z1.im * scale));
             } else {
                 put2<TO, ACC>(out, q, z0.re * scale, z0.im * scale, b0);
                 put2<TO, ACC>(out, q + 2, z1.re * scale, z1.im * scale, b0);
@@ -593,8 +629,8 @@ __global__ void __launch_bounds__(256) k_rfft256_x(Grid g, const float* __restri
 }
 
 // the register kernels cover float lines of 256 with 64 | nx (y) or 32 | ny (x)
-template <int DIR, int INV, class C, class TO, int ACC, int RES>
-static bool try_rfft256(Ctx& c, const C* tw, const C* in, TO* out, const float* b0) {
+template <int DIR, int INV, class C, class TO, int ACC, int RES, class B>
+static bool try_rfft256(Ctx& c, const C* tw, const C* in, TO* out, const B* b0) {
     if constexpr (!std::is_same<C, float>::value) return false;
     else {
         const int N = DIR == 0 ? c.g.nx : c.g.ny;
@@ -609,42 +645,42 @@ static bool try_rfft256(Ctx& c, const C* tw, const C* in, TO* out, const float* 
             }
         }
         if (c.g.ny % 32) return false;
-        hipLaunchKernelGGL((k_rfft256_x<INV, TO, ACC, RES>), dim3((c.g.ny / 32) * c.g.nz), dim3(256), 0, c.stream,
+        hipLaunchKernelGGL((k_rfft256_x<INV, TO, ACC, RES, B>), dim3((c.g.ny / 32) * c.g.nz), dim3(256), 0, c.stream,
                            c.g, tw, in, out, b0);
         return true;
     }
 }
 
-template <int DIR, int INV, int N, int LPB, class C, class TO, int ACC, int RES>
-static bool try_rfft_n(Ctx& c, const C* tw, const C* in, TO* out, int n, int nlines, const float* b0) {
+template <int DIR, int INV, int N, int LPB, class C, class TO, int ACC, int RES, class B>
+static bool try_rfft_n(Ctx& c, const C* tw, const C* in, TO* out, int n, int nlines, const B* b0) {
     if (n != N || nlines % LPB != 0) return false;
-    hipLaunchKernelGGL((k_rfft_n<DIR, INV, N, LPB, C, TO, ACC, RES>), dim3((nlines / LPB) * c.g.nz),
+    hipLaunchKernelGGL((k_rfft_n<DIR, INV, N, LPB, C, TO, ACC, RES, B>), dim3((nlines / LPB) * c.g.nz),
                        dim3(RfftShape<N, LPB>::NTHR), 0, c.stream, c.g, tw, in, out, b0);
     return true;
 }
 
 // lines per block: along x 4; along y enough lines for 64-byte row segments
-template <int DIR, int INV, class C, class TO, int ACC, int RES>
-static bool rfft_fixed(Ctx& c, const C* tw, const C* in, TO* out, int n, int nlines, const float* b0) {
+template <int DIR, int INV, class C, class TO, int ACC, int RES, class B>
+static bool rfft_fixed(Ctx& c, const C* tw, const C* in, TO* out, int n, int nlines, const B* b0) {
     constexpr int L = DIR == 0 ? 4 : (sizeof(C) == 4 ? 16 : 8);
     constexpr int L512 = L > 8 ? 8 : 4, L1024 = L > 8 ? 4 : 2;
-    return try_rfft_n<DIR, INV, 32, L, C, TO, ACC, RES>(c, tw, in, out, n, nlines, b0) ||
-           try_rfft_n<DIR, INV, 64, L, C, TO, ACC, RES>(c, tw, in, out, n, nlines, b0) ||
-           try_rfft_n<DIR, INV, 128, L, C, TO, ACC, RES>(c, tw, in, out, n, nlines, b0) ||
-           try_rfft_n<DIR, INV, 256, L, C, TO, ACC, RES>(c, tw, in, out, n, nlines, b0) ||
-           try_rfft_n<DIR, INV, 512, L512, C, TO, ACC, RES>(c, tw, in, out, n, nlines, b0) ||
-           try_rfft_n<DIR, INV, 1024, L1024, C, TO, ACC, RES>(c, tw, in, out, n, nlines, b0);
+    return try_rfft_n<DIR, INV, 32, L, C, TO, ACC, RES, B>(c, tw, in, out, n, nlines, b0) ||
+           try_rfft_n<DIR, INV, 64, L, C, TO, ACC, RES, B>(c, tw, in, out, n, nlines, b0) ||
+           try_rfft_n<DIR, INV, 128, L, C, TO, ACC, RES, B>(c, tw, in, out, n, nlines, b0) ||
+           try_rfft_n<DIR, INV, 256, L, C, TO, ACC, RES, B>(c, tw, in, out, n, nlines, b0) ||
+           try_rfft_n<DIR, INV, 512, L512, C, TO, ACC, RES, B>(c, tw, in, out, n, nlines, b0) ||
+           try_rfft_n<DIR, INV, 1024, L1024, C, TO, ACC, RES, B>(c, tw, in, out, n, nlines, b0);
 }
 
 // Transform of every line along DIR.  With RES the forward x transform takes
 // rhs - L b0 as its input (in = rhs); false means no fixed-size kernel covers the
 // shape and the caller forms the residual itself.
-template <int DIR, int INV, class C, class TO, int ACC, int RES = 0>
-static int rfft_launch(Ctx& c, const C* tw, const C* in, TO* out, const float* b0 = nullptr) {
+template <int DIR, int INV, class C, class TO, int ACC, int RES = 0, class B = float>
+static int rfft_launch(Ctx& c, const C* tw, const C* in, TO* out, const B* b0 = nullptr) {
     const int N = DIR == 0 ? c.g.nx : c.g.ny;
     const int nlines = DIR == 0 ? c.g.ny : c.g.nx;
-    if (try_rfft256<DIR, INV, C, TO, ACC, RES>(c, tw, in, out, b0) ||
-        rfft_fixed<DIR, INV, C, TO, ACC, RES>(c, tw, in, out, N, nlines, b0)) {
+    if (try_rfft256<DIR, INV, C, TO, ACC, RES, B>(c, tw, in, out, b0) ||
+        rfft_fixed<DIR, INV, C, TO, ACC, RES, B>(c, tw, in, out, N, nlines, b0)) {
         HIP_OK(hipGetLastError());
         return ICLS_OK;
     }
@@ -662,7 +698,7 @@ static int rfft_launch(Ctx& c, const C* tw, const C* in, TO* out, const float* b
     if (nthr < 64) nthr = 64;
     size_t shm = (size_t)(2L * LPB * M + M) * sizeof(cx<C>);
     dim3 grd((nlines / LPB) * c.g.nz);
-    hipLaunchKernelGGL((k_rfft<DIR, INV, C, TO, ACC>), grd, dim3(nthr), shm, c.stream, c.g, N, LPB, TPL, tw, in, out, b0);
+    hipLaunchKernelGGL((k_rfft<DIR, INV, C, TO, ACC, B>), grd, dim3(nthr), shm, c.stream, c.g, N, LPB, TPL, tw, in, out, b0);
     HIP_OK(hipGetLastError());
     return ICLS_OK;
 }
@@ -988,8 +1024,8 @@ static int thomas_launch(Ctx& c, TS* f, double* y) {
 }
 
 // the residual as an array, for shapes the fused forward transform does not cover
-template <class T>
-__global__ void k_residual(Grid g, const T* __restrict__ rhs, const float* __restrict__ p, T* __restrict__ r) {
+template <class T, class B>
+__global__ void k_residual(Grid g, const T* __restrict__ rhs, const B* __restrict__ p, T* __restrict__ r) {
     int i0 = blockIdx.x * 64 + threadIdx.x, j0 = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z + 1;
     if (i0 >= g.nx || j0 >= g.ny) return;
     r[gidx(g, i0, j0, k)] = (T)residual_at(g, rhs, p, i0, j0, k);
@@ -1023,16 +1059,16 @@ static int solve_fp64(Ctx& c) {
 
 // One float-transform solve.  ACC = 0: of L p0 = rhs, the solution written as float to
 // p0 (the transforms produce float values, so nothing is lost against a double
-// store).  ACC = 1: of L e = rhs - L p0, the residual formed in double inside the
+// store).  ACC = 1: of L e = rhs - L b0, the residual formed in double inside the
 // first transform where a fixed-size kernel covers the line length and by k_residual
-// otherwise, and p = p0 + e written as double.
-template <int ACC>
-static int solve_fp32_pass(Ctx& c) {
+// otherwise, and p = b0 + e written as double; b0 is the float solution of an ACC = 0
+// pass or the double solution of the previous time step.
+template <int ACC, class B = float>
+static int solve_fp32_pass(Ctx& c, const B* b0) {
     const float* twx = as<float>(c.tw_x_f);
     const float* twy = as<float>(c.tw_y_f);
     const float* rhs = as<float>(c.frhs);
     float* w = as<float>(c.pf);
-    float* p0 = as<float>(c.pf0);
     int rc;
     if (!ACC) {
         ProfScope ps(c, "k_rfft_x_fwd");
@@ -1040,11 +1076,10 @@ static int solve_fp32_pass(Ctx& c) {
         if (rc) return rc;
     } else {
         ProfScope ps(c, "k_rfft_x_fwd_res");
-        rc = rfft_launch<0, 0, float, float, 0, 1>(c, twx, rhs, w, p0);
+        rc = rfft_launch<0, 0, float, float, 0, 1, B>(c, twx, rhs, w, b0);
         if (rc < 0) {
             dim3 blk(64, 4, 1);
-            hipLaunchKernelGGL(k_residual<float>, grid3(c.g, c.g.nz, blk), blk, 0, c.stream, c.g, rhs,
-                               (const float*)p0, w);
+            hipLaunchKernelGGL((k_residual<float, B>), grid3(c.g, c.g.nz, blk), blk, 0, c.stream, c.g, rhs, b0, w);
             HIP_OK(hipGetLastError());
             rc = rfft_launch<0, 0, float, float, 0>(c, twx, w, w);
         }
@@ -1063,17 +1098,27 @@ static int solve_fp32_pass(Ctx& c) {
         if (rc) return rc;
     }
     ProfScope ps(c, "k_rfft_x_inv");
-    if (!ACC) return rfft_launch<0, 1, float, float, 0>(c, twx, w, p0);
-    return rfft_launch<0, 1, float, double, 1>(c, twx, w, c.p, p0);
+    if (!ACC) return rfft_launch<0, 1, float, float, 0>(c, twx, w, as<float>(c.pf0));
+    return rfft_launch<0, 1, float, double, 1, 0, B>(c, twx, w, c.p, b0);
 }
 
-static int solve_mixed(Ctx& c) {
-    int rc = solve_fp32_pass<0>(c);
+// One step of iterative refinement.  Its starting solution is, where there is one, the
+// raw solution of the previous time step (guess): one pass then solves for the
+// change of the pressure over the step, whose float error is that of the change.
+// Otherwise (the first step, after a restart or an upload of the pressures, and for
+// a right-hand side given through the API) a float solve provides it first.
+static int solve_mixed(Ctx& c, const double* guess) {
+    if (guess) return solve_fp32_pass<1, double>(c, guess);
+    int rc = solve_fp32_pass<0>(c, (const float*)nullptr);
     if (rc) return rc;
-    return solve_fp32_pass<1>(c);
+    return solve_fp32_pass<1>(c, (const float*)as<float>(c.pf0));
 }
 
-int launch_solve(Ctx& c) { return c.fp32 ? solve_mixed(c) : solve_fp64(c); }
+int launch_solve(Ctx& c, bool warm) {
+    if (!c.fp32) return solve_fp64(c);
+    // under a lazy rotation the pnew buffer holds the previous step's raw solution
+    return solve_mixed(c, warm && c.p_lazy ? (const double*)c.pnew : nullptr);
+}
 
 // ---------------------------------------------------------------- solver2d_cos
 // solver.f90:202-317 with initsolver.f90:84-160: real FFT along x, the cosine transform
