@@ -30,8 +30,10 @@ One JSON line on stdout:
                 launch time (HIP events on the library stream over the timed region)
                 vs the 8 TB/s HBM peak; traffic from rocprofv3 FETCH_SIZE + WRITE_SIZE
                 where a committed measurement for that kernel exists under profiles/
-  cpu_baseline= the CPU oracle (C restatement of the reference, FP64, 1 core) on one
-                step of the same workload
+  cpu_baseline= the CPU oracle (C restatement of the reference, FP64) on the host's
+                cores (its OpenMP build, results identical to the serial one) over
+                steps of the same workload; "cpu_baseline_1core" the serial build on
+                one step
 """
 import argparse
 import json
@@ -52,11 +54,21 @@ DESCR = {"fp64": "plain FP64",
          "learned": "FP32 fields with FP64 pressure/Thomas/residual/volumes + learned curvature closure"}
 
 
-def cpu_baseline(p, nsteps=1):
-    """The oracle on one step of the same workload (after an untimed init)."""
+def omp_threads():
+    """Threads the OpenMP runtime of the oracle's parallel build uses."""
+    import ctypes
+    try:
+        return int(ctypes.CDLL("libgomp.so.1").omp_get_max_threads())
+    except OSError:
+        return os.cpu_count() or 1
+
+
+def cpu_baseline(p, nsteps=1, omp=False):
+    """The oracle on nsteps steps of the same workload (after an untimed init); with
+    omp its OpenMP build on the host's cores."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
-    o = pyoracle.Oracle(p)
+    o = pyoracle.Oracle(p, omp=omp)
     o.init_program()
     t0 = time.perf_counter()
     for s in range(1, nsteps + 1):
@@ -64,9 +76,11 @@ def cpu_baseline(p, nsteps=1):
     el = time.perf_counter() - t0
     o.close()
     ncell = p.itot * p.jtot * p.ktot
-    return {"value": ncell * nsteps / el / 1e6, "unit": "Mcell-steps/s", "cores": 1, "kind": "port",
-            "sample": "%d step(s) of the %dx%dx%d workload (FP64) after an untimed init, %.1f s" %
-                      (nsteps, p.itot, p.jtot, p.ktot, el)}
+    cores = omp_threads() if omp else 1
+    return {"value": ncell * nsteps / el / 1e6, "unit": "Mcell-steps/s", "cores": cores, "kind": "port",
+            "sample": "%d step(s) of the %dx%dx%d workload (FP64) after an untimed init, %.1f s%s" %
+                      (nsteps, p.itot, p.jtot, p.ktot, el,
+                       ", OpenMP build on %d threads" % cores if omp else ", serial build")}
 
 
 def traffic_for(kernel, workload, fp32):
@@ -210,7 +224,8 @@ def main():
             "speedup_vs_fp64": round(dl["value"] / df["value"], 3)}
     if not a.no_cpu:
         q = P.bubble256(a.n, mode=P.MODE_FP64)
-        line["cpu_baseline"] = cpu_baseline(q)
+        line["cpu_baseline"] = cpu_baseline(q, nsteps=3, omp=True)
+        line["cpu_baseline_1core"] = cpu_baseline(q)
     print(json.dumps(line))
 
 

@@ -28,6 +28,20 @@
 
 #define PI_F77 3.1415926535897932   /* PIMACHNEW, libs/fft/fft.f:1877-1889 */
 
+/* The per-cell and per-line loops carry OMP_FOR.  Built with -fopenmp
+ * (liboracle_omp.so) their iterations run on all host cores; every iteration writes
+ * its own outputs, so the results are those of the serial build bit for bit.  The
+ * reductions marked OMP_FOR_RED are integer sums, minima and maxima, which no order
+ * changes; the floating-point sums stay serial, in their order. */
+#ifdef _OPENMP
+#define OMP_FOR _Pragma("omp parallel for schedule(static)")
+#define OMP_PRAGMA_(x) _Pragma(#x)
+#define OMP_FOR_RED(r) OMP_PRAGMA_(omp parallel for schedule(static) reduction(r))
+#else
+#define OMP_FOR
+#define OMP_FOR_RED(r)
+#endif
+
 struct oracle_state {
     icls_params P;
     int imax, jmax, kmax, i1, j1, k1, lmax;
@@ -460,12 +474,14 @@ static void initsolver(oracle_state* S) {
 int oracle_tube_construction(oracle_state* S, int l) {
     double* phi = S->lvset[l];
     const double NB2 = S->NB2;
+    OMP_FOR
     for (long q = 0; q < S->nls; q++) {
         if (phi[q] > NB2) phi[q] = NB2;
         else if (phi[q] < -NB2) phi[q] = -NB2;
     }
     int cnt = 0;
     int* mask = (int*)malloc(sizeof(int) * S->ni);
+    OMP_FOR_RED(+ : cnt)
     for (int k = 1; k <= S->kmax; k++)
         for (int j = 1; j <= S->jmax; j++)
             for (int i = 1; i <= S->imax; i++) {
@@ -508,6 +524,7 @@ static void houc5_nb(oracle_state* S, const double* vu, const double* vv, const 
     const double dx = S->dx, dy = S->dy, dz = S->dz;
     const int kmax = S->kmax;
     memset(dfdt, 0, sizeof(double) * S->ni);
+    OMP_FOR
     for (int cnt = 0; cnt < S->nb_cnt; cnt++) {
         int i = S->nb_i[cnt], j = S->nb_j[cnt], k = S->nb_k[cnt];
         long c = II(i, j, k);
@@ -580,6 +597,7 @@ static void weno5_nb(oracle_state* S, const double* vu, const double* vv, const 
     const double dx = S->dx, dz = S->dz;
     const int kmax = S->kmax;
     memset(dfdt, 0, sizeof(double) * S->ni);
+    OMP_FOR
     for (int cnt = 0; cnt < S->nb_cnt; cnt++) {
         int i = S->nb_i[cnt], j = S->nb_j[cnt], k = S->nb_k[cnt];
         long c = II(i, j, k);
@@ -617,6 +635,7 @@ static void adv_rhs(oracle_state* S, int scheme, const double* vu, const double*
 /* RK_nb (levelset.f90:342-367) */
 static void rk_nb(oracle_state* S, double* out, const double* in, double factor, const double* dfdt) {
     memcpy(out, in, sizeof(double) * S->nls);
+    OMP_FOR
     for (int cnt = 0; cnt < S->nb_cnt; cnt++) {
         int i = S->nb_i[cnt], j = S->nb_j[cnt], k = S->nb_k[cnt];
         out[LS(i, j, k)] = in[LS(i, j, k)] + factor * S->dt * dfdt[II(i, j, k)];
@@ -636,9 +655,11 @@ static void rk3_advect(oracle_state* S, int scheme, const double* vu, const doub
     adv_rhs(S, scheme, vu, vv, vw, phi_0, dfdt);
     rk_nb(S, phi_1, phi_0, 1., dfdt);
     adv_rhs(S, scheme, vu, vv, vw, phi_1, dfdt);
+    OMP_FOR
     for (long q = 0; q < n; q++) tmp[q] = 3. / 4. * phi_0[q] + 1. / 4. * phi_1[q];
     rk_nb(S, phi_2, tmp, 1. / 4., dfdt);
     adv_rhs(S, scheme, vu, vv, vw, phi_2, dfdt);
+    OMP_FOR
     for (long q = 0; q < n; q++) tmp[q] = 1. / 3. * phi_0[q] + 2. / 3. * phi_2[q];
     rk_nb(S, phi_io, tmp, 2. / 3., dfdt);  /* phi_3 -> lvset */
     free(phi_1); free(phi_2); free(tmp); free(dfdt);
@@ -666,9 +687,12 @@ static void gather27(oracle_state* S, const double* f, int is_ls, int i, int j, 
 /* get_curvature (interface.f90:20-119) on an LS array into a C array */
 static void get_curvature_arr(oracle_state* S, const double* phi, double* curvature) {
     const double dz = S->dz;
+    OMP_FOR
     for (long q = 0; q < S->ni; q++) { S->nrm_m[q] = 1.; S->nrm_x[q] = 0.; S->nrm_y[q] = 0.; S->nrm_z[q] = 0.; }
+    OMP_FOR
     for (long q = 0; q < S->nc; q++) curvature[q] = 0.;
     const double max_kappa = 1. / dz;
+    OMP_FOR
     for (int k = 1; k <= S->kmax; k++)
         for (int j = 1; j <= S->jmax; j++)
             for (int i = 1; i <= S->imax; i++) {
@@ -706,7 +730,9 @@ static void get_curvature_arr(oracle_state* S, const double* phi, double* curvat
 
 /* re_curvature (interface.f90:186-218) */
 static void re_curvature_arr(oracle_state* S, const double* phi, const double* curv, double* re_curv) {
+    OMP_FOR
     for (long q = 0; q < S->nc; q++) re_curv[q] = 0.;
+    OMP_FOR
     for (int k = 1; k <= S->kmax; k++)
         for (int j = 1; j <= S->jmax; j++)
             for (int i = 1; i <= S->imax; i++)
@@ -744,9 +770,11 @@ int oracle_inflation(oracle_state* S, int l) {
     double* phi = S->lvset[l];
     get_curvature_arr(S, phi, kappa);
     boundc(S, kappa);
+    OMP_FOR
     for (int k = 1; k <= S->kmax; k++)
         for (int j = 1; j <= S->jmax; j++)
             for (int i = 1; i <= S->imax; i++) delta[II(i, j, k)] = oracle_dirac_trig(phi[LS(i, j, k)], s1);
+    OMP_FOR
     for (int k = 0; k <= S->k1; k++)
         for (int j = 0; j <= S->j1; j++)
             for (int i = 0; i <= S->i1; i++) dheav[CC(i, j, k)] = oracle_heaviside_trig(phi[LS(i, j, k)], s1);
@@ -777,6 +805,7 @@ int oracle_inflation(oracle_state* S, int l) {
     temp = loss_rate / area;
     S->maxchk_glb = temp / s1;
     boundp_arr(S, dheav);
+    OMP_FOR
     for (int k = 1; k <= S->kmax; k++)
         for (int j = 1; j <= S->jmax; j++)
             for (int i = 1; i <= S->imax; i++) {
@@ -785,6 +814,7 @@ int oracle_inflation(oracle_state* S, int l) {
                 S->w_l[CC(i, j, k)] = -temp * (dheav[CC(i, j, k + 1)] - dheav[CC(i, j, k)]) / S->dz;
             }
     bounduvw_arr(S, S->u_l, S->v_l, S->w_l);
+    OMP_FOR
     for (int k = 1; k <= S->kmax; k++)
         for (int j = 1; j <= S->jmax; j++)
             for (int i = 1; i <= S->imax; i++) {
@@ -815,7 +845,9 @@ int oracle_ls_mass_correction(oracle_state* S, int istep, int l) {
 static void weno5_reinit(oracle_state* S, const double* sgn, const double* phi, double* RHS) {
     const double dx = S->dx, dz = S->dz, lim = S->NB2 + dz;
     const int kmax = S->kmax;
+    OMP_FOR
     for (long q = 0; q < S->nls; q++) RHS[q] = 0.;
+    OMP_FOR
     for (int k = 1; k <= S->kmax; k++)
         for (int j = 1; j <= S->jmax; j++)
             for (int i = 1; i <= S->imax; i++) {
@@ -848,6 +880,7 @@ static void weno5_reinit(oracle_state* S, const double* sgn, const double* phi, 
 
 static void reinit_sgn(oracle_state* S, const double* phi_0, double* sgn) {
     const double dz = S->dz;
+    OMP_FOR
     for (int k = 1; k <= S->kmax; k++)
         for (int j = 1; j <= S->jmax; j++)
             for (int i = 1; i <= S->imax; i++) {
@@ -884,15 +917,19 @@ int oracle_ls_reinit(oracle_state* S, int l, int maxiter, int* iter_out, double*
     double residual = 0.;
     for (int it = 1; it <= maxiter; it++) {
         weno5_reinit(S, sgn, phi_0, RHS);
+        OMP_FOR
         for (long q = 0; q < n; q++) phi_n1[q] = phi_0[q] + dtau * RHS[q];
         boundsls(S, phi_n1);
         weno5_reinit(S, sgn, phi_n1, RHS);
+        OMP_FOR
         for (long q = 0; q < n; q++) phi_n2[q] = 3. / 4. * phi_0[q] + 1. / 4. * (phi_n1[q] + dtau * RHS[q]);
         boundsls(S, phi_n2);
         weno5_reinit(S, sgn, phi_n2, RHS);
+        OMP_FOR
         for (long q = 0; q < n; q++) phi_new[q] = 1. / 3. * phi_0[q] + 2. / 3. * (phi_n2[q] + dtau * RHS[q]);
         boundsls(S, phi_new);
         residual = 0.;
+        OMP_FOR_RED(max : residual)
         for (long q = 0; q < n; q++) { double d = fabs(phi_new[q] - phi_0[q]); if (d > residual) residual = d; }
         if (residual < pow(S->dz, 5)) { iter = it; break; }
         iter = maxiter;
@@ -909,6 +946,7 @@ int oracle_ls_reinit(oracle_state* S, int l, int maxiter, int* iter_out, double*
 int oracle_update_materials(oracle_state* S) {
     const double eps = 1.5 * S->dz;
     double* h_rho = zalloc(S->nc);
+    OMP_FOR
     for (int k = 0; k <= S->k1; k++)
         for (int j = 0; j <= S->j1; j++)
             for (int i = 0; i <= S->i1; i++) {
@@ -916,16 +954,20 @@ int oracle_update_materials(oracle_state* S) {
                 for (int l = 1; l < S->lmax; l++) s = fmin(s, S->lvset[l][LS(i, j, k)]);
                 S->slset[CC(i, j, k)] = s;
             }
-    for (int l = 0; l < S->lmax; l++)
+    for (int l = 0; l < S->lmax; l++) {
+        OMP_FOR
         for (int k = 0; k <= S->k1; k++)
             for (int j = 0; j <= S->j1; j++)
                 for (int i = 0; i <= S->i1; i++)
                     S->hnew_mls[l][CC(i, j, k)] = oracle_heaviside_trig(S->lvset[l][LS(i, j, k)], eps);
+    }
+    OMP_FOR
     for (long q = 0; q < S->nc; q++) {
         double h = S->hnew_mls[0][q];
         for (int l = 1; l < S->lmax; l++) h = fmin(h, S->hnew_mls[l][q]);
         S->jiemian[q] = h;
     }
+    OMP_FOR
     for (long q = 0; q < S->nc; q++) S->miu[q] = S->P.miu1 * S->jiemian[q] + S->P.miu2 * (1. - S->jiemian[q]);
     for (int l = 0; l < S->lmax; l++) {
         double s = 0.;
@@ -934,15 +976,19 @@ int oracle_update_materials(oracle_state* S) {
             for (int j = 1; j <= S->jmax; j++)
                 for (int i = 1; i <= S->imax; i++) s = s + h[CC(i, j, k)];
         S->volume[l] = 1. - s / (1. * S->imax * S->jmax * S->kmax);
+        OMP_FOR
         for (long q = 0; q < S->nc; q++) h_rho[q] = (l == 0) ? h[q] : fmin(h_rho[q], h[q]);
     }
+    OMP_FOR
     for (long q = 0; q < S->nc; q++) S->rho[q] = S->P.rho1 * h_rho[q] + S->P.rho2 * (1. - h_rho[q]);
     double sv = 0.;
     for (int l = 0; l < S->lmax; l++) sv = sv + S->volume[l];
     S->rho_avr = S->P.rho1 * (1. - sv) + S->P.rho2 * sv;
     if (S->P.rho1 == S->P.rho2) {
+        OMP_FOR
         for (long q = 0; q < S->ni; q++) { S->rho_u[q] = S->P.rho1; S->rho_v[q] = S->P.rho1; S->rho_w[q] = S->P.rho1; }
     } else {
+        OMP_FOR
         for (int k = 1; k <= S->kmax; k++)
             for (int j = 1; j <= S->jmax; j++)
                 for (int i = 1; i <= S->imax; i++) {
@@ -978,6 +1024,7 @@ int oracle_ls_main(oracle_state* S, int istep) {
 static void cntr2_conv(oracle_state* S, double* RCu, double* RCv, double* RCw) {
     const double *u = S->unew, *v = S->vnew, *w = S->wnew;
     const double dx = S->dx, dy = S->dy, dz = S->dz;
+    OMP_FOR
     for (int k = 1; k <= S->kmax; k++)
         for (int j = 1; j <= S->jmax; j++)
             for (int i = 1; i <= S->imax; i++) {
@@ -1011,6 +1058,7 @@ static void cntr2_conv(oracle_state* S, double* RCu, double* RCv, double* RCw) {
 static void delta_diff(oracle_state* S, double* RDu, double* RDv, double* RDw) {
     const double *u = S->unew, *v = S->vnew, *w = S->wnew, *miu = S->miu;
     const double dx = S->dx, dy = S->dy, dz = S->dz;
+    OMP_FOR
     for (int k = 1; k <= S->kmax; k++)
         for (int j = 1; j <= S->jmax; j++)
             for (int i = 1; i <= S->imax; i++) {
@@ -1088,6 +1136,7 @@ int oracle_mom_main(oracle_state* S, double* RHSu, double* RHSv, double* RHSw) {
     cntr2_conv(S, RCu, RCv, RCw);
     delta_diff(S, RDu, RDv, RDw);
     const double iFr = S->P.iFr, *g = S->P.g_unit;
+    OMP_FOR
     for (long c = 0; c < S->ni; c++) {
         double RGu = iFr * (1. - S->rho_avr / S->rho_u[c]) * g[0];
         double RGv = iFr * (1. - S->rho_avr / S->rho_v[c]) * g[1];
@@ -1133,6 +1182,7 @@ int oracle_ab2(oracle_state* S) {
     const double abn = cf[0], abo = cf[1], exn = cf[2], exo = cf[3];
     S->dt_old = dt;
     oracle_mom_main(S, dunew, dvnew, dwnew);
+    OMP_FOR
     for (int k = 1; k <= S->kmax; k++)
         for (int j = 1; j <= S->jmax; j++)
             for (int i = 1; i <= S->imax; i++) {
@@ -1144,6 +1194,7 @@ int oracle_ab2(oracle_state* S) {
     memcpy(S->duold, dunew, sizeof(double) * ni);
     memcpy(S->dvold, dvnew, sizeof(double) * ni);
     memcpy(S->dwold, dwnew, sizeof(double) * ni);
+    OMP_FOR
     for (long q = 0; q < nc; q++) {
         pjx[q] = exn * S->p_x[q] - exo * S->p_xold[q];
         pjy[q] = exn * S->p_y[q] - exo * S->p_yold[q];
@@ -1151,6 +1202,7 @@ int oracle_ab2(oracle_state* S) {
         phat[q] = exn * S->pnew[q] - exo * S->pold[q];
     }
     const double rho0 = S->rho0;
+    OMP_FOR
     for (int k = 1; k <= S->kmax; k++)
         for (int j = 1; j <= S->jmax; j++)
             for (int i = 1; i <= S->imax; i++) {
@@ -1160,6 +1212,7 @@ int oracle_ab2(oracle_state* S) {
                 dwnew[c] = -(1. / S->rho_w[c] - 1. / rho0) * S->dzi * (phat[CC(i, j, k + 1)] - pjz[CC(i, j, k)] - phat[CC(i, j, k)]);
             }
     double forceytot = S->P.constPoi ? -12. * S->visc / (S->lz * S->lz) : 0.;
+    OMP_FOR
     for (int k = 1; k <= S->kmax; k++)
         for (int j = 1; j <= S->jmax; j++)
             for (int i = 1; i <= S->imax; i++) {
@@ -1177,6 +1230,7 @@ int oracle_ab2(oracle_state* S) {
 /* fillps (fillps.f90:13-57) into the main-program array p */
 int oracle_fillps(oracle_state* S) {
     const double coef = S->rho0 / S->dt;
+    OMP_FOR
     for (int k = 1; k <= S->kmax; k++)
         for (int j = 1; j <= S->jmax; j++)
             for (int i = 1; i <= S->imax; i++) {
@@ -1199,6 +1253,7 @@ static void ipot(oracle_state* S) {
             if (n == m) continue;
             double d1 = 1e3;
             const double *pm = S->lvset[m], *pn = S->lvset[n];
+            OMP_FOR_RED(min : d1)
             for (long q = 0; q < S->nls; q++) {
                 double sp = 1e3;
                 if (pm[q] > -1. * dz && pm[q] < 1. * dz && pn[q] < cutoff) sp = pm[q] + pn[q];
@@ -1222,6 +1277,7 @@ int oracle_gfm4p(oracle_state* S) {
     memcpy(S->p_xold, S->p_x, sizeof(double) * S->nc);
     memcpy(S->p_yold, S->p_y, sizeof(double) * S->nc);
     memcpy(S->p_zold, S->p_z, sizeof(double) * S->nc);
+    OMP_FOR
     for (long q = 0; q < S->nc; q++) { S->p_jump[q] = 0.; S->p_x[q] = 0.; S->p_y[q] = 0.; S->p_z[q] = 0.; }
     double* re_curv = zalloc(S->nc);
     for (int l = 0; l < S->lmax; l++) {
@@ -1232,6 +1288,7 @@ int oracle_gfm4p(oracle_state* S) {
         memcpy(S->curv_cmn, re_curv, sizeof(double) * S->nc);
         boundc(S, S->curv_cmn);
         const double* cvf = S->curv_cmn;
+        OMP_FOR
         for (int k = 1; k <= S->kmax; k++)
             for (int j = 1; j <= S->jmax; j++)
                 for (int i = 1; i <= S->imax; i++) {
@@ -1288,6 +1345,7 @@ int oracle_gfm4p(oracle_state* S) {
 }
 
 int oracle_add_pjump(oracle_state* S) {
+    OMP_FOR
     for (long q = 0; q < S->nc; q++) S->p[q] = S->p[q] + S->p_jump[q];
     return 0;
 }
@@ -1403,72 +1461,82 @@ int oracle_solver2d_cos(oracle_state* S, int bcy) {
 
 int oracle_solver2d(oracle_state* S) {
     int imax = S->imax, jmax = S->jmax, kmax = S->kmax;
-    double* line = (double*)malloc(sizeof(double) * (imax > jmax ? (imax > kmax ? imax : kmax) : (jmax > kmax ? jmax : kmax)));
-    for (int k = 1; k <= kmax; k++)
+    OMP_FOR
+    for (int k = 1; k <= kmax; k++) {
+        double* line = (double*)malloc(sizeof(double) * imax);
         for (int j = 1; j <= jmax; j++) {
             for (int i = 1; i <= imax; i++) line[i - 1] = S->p[CC(i, j, k)];
             oracle_rfft_forward(imax, line);
             for (int i = 1; i <= imax; i++) S->p[CC(i, j, k)] = line[i - 1];
         }
-    for (int k = 1; k <= kmax; k++)
+        free(line);
+    }
+    OMP_FOR
+    for (int k = 1; k <= kmax; k++) {
+        double* line = (double*)malloc(sizeof(double) * jmax);
         for (int i = 1; i <= imax; i++) {
             for (int j = 1; j <= jmax; j++) line[j - 1] = S->p[CC(i, j, k)];
             oracle_rfft_forward(jmax, line);
             for (int j = 1; j <= jmax; j++) S->p[CC(i, j, k)] = line[j - 1];
         }
+        free(line);
+    }
     double* d = zalloc((long)imax * jmax * kmax);
 #define PZ(i, j, k) S->p[CC(i, j, k)]
 #define DD(i, j, k) d[II(i, j, k)]
     const double *a = S->a, *b = S->b, *c = S->c;
+    /* solver.f90:119-160, each (i, j) column through its own sweeps */
+    OMP_FOR
     for (int j = 1; j <= jmax; j++)
         for (int i = 1; i <= imax; i++) {
-            double xy = S->xyrt[(i - 1) + (long)imax * (j - 1)];
+            const double xy = S->xyrt[(i - 1) + (long)imax * (j - 1)];
             double z = 1. / (b[0] + xy);
             DD(i, j, 1) = c[0] * z;
             PZ(i, j, 1) = PZ(i, j, 1) * z;
-        }
-    for (int k = 2; k <= kmax - 1; k++)
-        for (int j = 1; j <= jmax; j++)
-            for (int i = 1; i <= imax; i++) {
-                double xy = S->xyrt[(i - 1) + (long)imax * (j - 1)];
+            for (int k = 2; k <= kmax - 1; k++) {
                 double bb = b[k - 1] + xy;
-                double z = 1. / (bb - a[k - 1] * DD(i, j, k - 1));
+                z = 1. / (bb - a[k - 1] * DD(i, j, k - 1));
                 DD(i, j, k) = c[k - 1] * z;
                 PZ(i, j, k) = (PZ(i, j, k) - a[k - 1] * PZ(i, j, k - 1)) * z;
             }
-    for (int j = 1; j <= jmax; j++)
-        for (int i = 1; i <= imax; i++) {
-            double xy = S->xyrt[(i - 1) + (long)imax * (j - 1)];
-            double bb = b[kmax - 1] + xy;
-            double z = bb - a[kmax - 1] * DD(i, j, kmax - 1);
-            if (z != 0.) PZ(i, j, kmax) = (PZ(i, j, kmax) - a[kmax - 1] * PZ(i, j, kmax - 1)) / z;
-            else PZ(i, j, kmax) = 0.;
+            {
+                double bb = b[kmax - 1] + xy;
+                z = bb - a[kmax - 1] * DD(i, j, kmax - 1);
+                if (z != 0.) PZ(i, j, kmax) = (PZ(i, j, kmax) - a[kmax - 1] * PZ(i, j, kmax - 1)) / z;
+                else PZ(i, j, kmax) = 0.;
+            }
+            for (int k = kmax - 1; k >= 1; k--) PZ(i, j, k) = PZ(i, j, k) - DD(i, j, k) * PZ(i, j, k + 1);
         }
-    for (int k = kmax - 1; k >= 1; k--)
-        for (int j = 1; j <= jmax; j++)
-            for (int i = 1; i <= imax; i++) PZ(i, j, k) = PZ(i, j, k) - DD(i, j, k) * PZ(i, j, k + 1);
 #undef PZ
 #undef DD
     free(d);
-    for (int k = 1; k <= kmax; k++)
+    OMP_FOR
+    for (int k = 1; k <= kmax; k++) {
+        double* line = (double*)malloc(sizeof(double) * jmax);
         for (int i = 1; i <= imax; i++) {
             for (int j = 1; j <= jmax; j++) line[j - 1] = S->p[CC(i, j, k)];
             oracle_rfft_backward(jmax, line);
             for (int j = 1; j <= jmax; j++) S->p[CC(i, j, k)] = line[j - 1];
         }
-    for (int k = 1; k <= kmax; k++)
+        free(line);
+    }
+    OMP_FOR
+    for (int k = 1; k <= kmax; k++) {
+        double* line = (double*)malloc(sizeof(double) * imax);
         for (int j = 1; j <= jmax; j++) {
             for (int i = 1; i <= imax; i++) line[i - 1] = S->p[CC(i, j, k)];
             oracle_rfft_backward(imax, line);
             for (int i = 1; i <= imax; i++) S->p[CC(i, j, k)] = line[i - 1];
         }
-    free(line);
+        free(line);
+    }
     return 0;
 }
 
 /* correc (correc.f90:15-44) */
 int oracle_correc(oracle_state* S) {
     const double *p = S->p, dt = S->dt, rho0 = S->rho0;
+    OMP_FOR
     for (int k = 1; k <= S->kmax; k++)
         for (int j = 1; j <= S->jmax; j++)
             for (int i = 1; i <= S->imax; i++) {
@@ -1541,15 +1609,18 @@ int oracle_update_pressure(oracle_state* S, int first_step) {
         norm = norm + s;
     }
     norm = norm / (1. * S->imax * S->jmax);
+    OMP_FOR
     for (long q = 0; q < S->nc; q++) S->pnew[q] = S->pnew[q] - norm;
-    if (first_step) for (long q = 0; q < S->nc; q++) S->pold[q] = S->pold[q] - norm;
-    else for (long q = 0; q < S->nc; q++) S->pold[q] = S->pold[q] + S->norm_old - norm;
+    const double norm_old = S->norm_old;
+    OMP_FOR
+    for (long q = 0; q < S->nc; q++) S->pold[q] = first_step ? S->pold[q] - norm : S->pold[q] + norm_old - norm;
     S->norm_old = norm;
     return 0;
 }
 
 /* average_vel_at_center (misc.f90:587-604) */
 int oracle_average_vel_at_center(oracle_state* S) {
+    OMP_FOR
     for (int k = 1; k <= S->kmax; k++)
         for (int j = 1; j <= S->jmax; j++)
             for (int i = 1; i <= S->imax; i++) {

@@ -1,5 +1,8 @@
 """ctypes binding of the CPU oracle (liboracle.so) -- TEST INFRASTRUCTURE ONLY.
 
+liboracle_omp.so is the same source built with OpenMP (omp=True): its per-cell and
+per-line loops run on the host's cores with results identical bit for bit.
+
 Loaded by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg, as the
 checker.  The product library (libicls.so) never imports this module.
 
@@ -18,13 +21,12 @@ ROOT = os.path.dirname(HERE)
 sys.path.insert(0, os.path.join(ROOT, "icls-release_amd"))
 from iclsamd import params as P  # noqa: E402
 
-_LIB = None
+_LIBS = {}
 
 
-def lib():
-    global _LIB
-    if _LIB is None:
-        path = os.path.join(HERE, "liboracle.so")
+def lib(omp=False):
+    if omp not in _LIBS:
+        path = os.path.join(HERE, "liboracle_omp.so" if omp else "liboracle.so")
         if not os.path.exists(path):
             raise RuntimeError("oracle not built: run `make -C oracle`")
         L = ctypes.CDLL(path)
@@ -89,8 +91,8 @@ def lib():
         L.oracle_icurvature.argtypes = [i32, dp, dp]
         L.oracle_icurvature.restype = dbl
         L.oracle_thomas.argtypes = [i32, dbl, dbl, dp]
-        _LIB = L
-    return _LIB
+        _LIBS[omp] = L
+    return _LIBS[omp]
 
 
 def _dp(a):
@@ -100,9 +102,9 @@ def _dp(a):
 class Oracle:
     """One oracle state for one parameter block."""
 
-    def __init__(self, p):
+    def __init__(self, p, omp=False):
         self.p = p.copy()
-        self.L = lib()
+        self.L = lib(omp)
         self.h = self.L.oracle_create(ctypes.byref(self.p))
         if not self.h:
             raise ValueError("oracle_create rejected the parameter block")
