@@ -73,10 +73,10 @@ struct Mlp {
     float* grad = nullptr;              // reduced gradient (MLP_NPARAM) and loss
     double* pinvA_fine = nullptr;       // least-squares matrix at the 4x finer spacing
     int* list = nullptr;                // under-resolved interface cells of one level set
+    int* list_t = nullptr;              // batched windows: the window of each listed cell
     int* count = nullptr;
     int list_cap = 0;
     int* total = nullptr;               // cells listed over all level sets in the last gfm pass
-    std::vector<int*> list_s, count_s;  // one list per side stream (concurrent level-set windows)
     long adam_t = 0;
     bool ready = false;                 // weights set or trained
     float gate = 0.25f;                 // |kappa| dz at which the closure is consulted
@@ -117,15 +117,10 @@ struct Ctx {
     int* dwin = nullptr;                  // device copy of the windows, 6 ints per level set
     int* hwin = nullptr;                  // pinned staging of it
     // Windows of one colour do not overlap (ghost layers included): their level sets run
-    // at once on the side streams, sharing the stage buffers.
+    // in one launch per kernel (WinBatch).
     std::vector<int> colour;              // per level set
     int ncolour = 0;
-    std::vector<hipStream_t> lstream;     // side streams (default priority)
-    std::vector<hipEvent_t> lev;          // one per side stream, for the joins
-    hipEvent_t fork_ev = nullptr;
     int res_slot = 0;                     // dscal slot of the re-init residual (DS_REINIT_RES)
-    int res_slot0 = 0;                    // first of the side streams' residual slots
-    std::vector<double*> stash_s;         // the re-init stash planes of each side stream
     int lmax = 1;
     long nfield = 0;       // elements per device 3D field
     bool fp32 = false;     // field arrays hold float (mixed modes)
@@ -217,7 +212,6 @@ inline const T* as(const double* p) { return reinterpret_cast<const T*>(p); }
 
 // ---- error handling
 void set_error(const char* fmt, ...);
-constexpr int LSTREAMS = 4;               // side streams for concurrent level-set windows
 
 // Layers per block for a z-marching kernel `fn` launched with `threads` threads per block
 // over tiles_xy (x, y) tiles: a multiple of `mult` chosen so that the launch comes as
@@ -299,6 +293,52 @@ template <class P>
 __device__ __forceinline__ P* win_ptr(P* shared, const WinBatch& wb, const WinPick& w) {
     return (shared ? shared : (P*)wb.ls[w.t]) + w.off;
 }
+// The level sets ls (windows of one colour, which do not overlap) in launches of at
+// most WB_MAX windows, each launch holding windows with the same key(window); blocks
+// (window) is the window's block count.  launch(batch, key, members) issues a launch.
+template <class KeyFn, class BlocksFn, class LaunchFn>
+inline int for_each_batch(Ctx& c, const std::vector<int>& ls, KeyFn key, BlocksFn blocks, LaunchFn launch) {
+    std::vector<int> keys;
+    for (int l : ls) {
+        const int k = key(c.win[l]);
+        bool seen = false;
+        for (int q : keys) seen = seen || q == k;
+        if (!seen) keys.push_back(k);
+    }
+    for (int k : keys) {
+        WinBatch wb;
+        std::vector<int> members;
+        auto flush = [&]() -> int {
+            if (wb.n == 0) return ICLS_OK;
+            int rc = launch(wb, k, members);
+            wb = WinBatch();
+            members.clear();
+            return rc;
+        };
+        for (int l : ls) {
+            const LsWindow& w = c.win[l];
+            if (key(w) != k) continue;
+            const long nb = blocks(w);
+            if (nb <= 0 || (long)wb.first[wb.n] + nb > (1L << 30)) {
+                set_error("window batch: bad block count");
+                return ICLS_ERR_STATE;
+            }
+            const int t = wb.n;
+            wb.ox[t] = w.ox; wb.oy[t] = w.oy; wb.oz[t] = w.oz;
+            wb.wx[t] = w.wx; wb.wy[t] = w.wy; wb.wz[t] = w.wz;
+            wb.ls[t] = c.lvset[l];
+            wb.lidx[t] = l;
+            wb.first[t + 1] = wb.first[t] + (int)nb;
+            wb.n = t + 1;
+            members.push_back(l);
+            if (wb.n == WB_MAX) { int rc = flush(); if (rc) return rc; }
+        }
+        int rc = flush();
+        if (rc) return rc;
+    }
+    return ICLS_OK;
+}
+
 // one colour's windows (level sets ls), batched: tube + the three advection stages, and
 // the three re-initialisation stages; the re-init residual of level set l goes to
 // dscal[res_b0 + l], and that of level set lmax - 1 also to DS_REINIT_RES
@@ -345,6 +385,12 @@ int mlp_samples(Ctx& c, long n, unsigned long long seed, float r_min, float r_ma
 int launch_mlp_reset_list(Ctx& c);
 int launch_mlp_reset_total(Ctx& c);
 int launch_mlp_curv(Ctx& c, const double* phi, const double* kappa_fd, double* kappa_out);
+// the closure over the cells listed by a batched curvature launch (whole-grid indices,
+// each with its window in the batch wb)
+int launch_mlp_curv_batch(Ctx& c, const WinBatch& wb, const double* kappa_fd, double* kappa_out);
+// get_curvature, re_curvature, the closure (learned) and the added GFM jumps of the
+// level sets ls, windows of one colour, in batched launches
+int launch_curvature_jumps_batch(Ctx& c, const std::vector<int>& ls, bool learned);
 // momentum / correction
 int launch_momentum(Ctx& c);
 int launch_momentum_rhs_only(Ctx& c);

@@ -6,7 +6,6 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstring>
-#include <functional>
 #include <map>
 #include <string>
 #include <vector>
@@ -158,10 +157,6 @@ static void free_ctx(Ctx* c) {
     if (c->hext) hipHostFree(c->hext);
     if (c->dwin) hipFree(c->dwin);
     if (c->hwin) hipHostFree(c->hwin);
-    for (auto* p : c->stash_s) if (p) hipFree(p);
-    for (auto e : c->lev) if (e) hipEventDestroy(e);
-    if (c->fork_ev) hipEventDestroy(c->fork_ev);
-    for (auto st : c->lstream) if (st) { hipStreamSynchronize(st); hipStreamDestroy(st); }
     if (c->ev_vol) hipEventDestroy(c->ev_vol);
     if (c->stream) hipStreamDestroy(c->stream);
     mlp_free(*c);
@@ -282,12 +277,11 @@ static int build_ctx(const icls_params* prm) {
     HIP_OK(hipMalloc((void**)&C.zplane, sizeof(double) * g.nx * g.ny));
     HIP_OK(hipMemset(C.zplane, 0, sizeof(double) * g.nx * g.ny));
     HIP_OK(hipMalloc((void**)&C.stash, sizeof(double) * 2 * g.sz));
-    // volumes, initial volumes and scratch sums per level set, then the side streams'
-    // re-init residual slots
-    const int nscal = DS_COUNT_FIXED + 3 * C.lmax + 16 + LSTREAMS + C.lmax + 2;
+    // volumes, initial volumes and scratch sums per level set, then the batched
+    // re-init residual slot of each level set and two saved scalars
+    const int nscal = DS_COUNT_FIXED + 3 * C.lmax + 16 + C.lmax + 2;
     C.res_slot = DS_REINIT_RES;
-    C.res_slot0 = DS_COUNT_FIXED + 3 * C.lmax + 16;
-    C.res_b0 = C.res_slot0 + LSTREAMS;
+    C.res_b0 = DS_COUNT_FIXED + 3 * C.lmax + 16;
     C.save0 = C.res_b0 + C.lmax;
     HIP_OK(hipMalloc((void**)&C.dscal, sizeof(double) * nscal));
     HIP_OK(hipMemset(C.dscal, 0, sizeof(double) * nscal));
@@ -302,15 +296,6 @@ static int build_ctx(const icls_params* prm) {
         HIP_OK(hipHostMalloc((void**)&C.hext, sizeof(int) * 6 * C.lmax));
         HIP_OK(hipMalloc((void**)&C.dwin, sizeof(int) * 6 * C.lmax));
         HIP_OK(hipHostMalloc((void**)&C.hwin, sizeof(int) * 6 * C.lmax));
-        C.lstream.assign(LSTREAMS, nullptr);
-        C.lev.assign(LSTREAMS, nullptr);
-        for (int s = 0; s < LSTREAMS; s++) {
-            HIP_OK(hipStreamCreateWithFlags(&C.lstream[s], hipStreamNonBlocking));
-            HIP_OK(hipEventCreateWithFlags(&C.lev[s], hipEventDisableTiming));
-        }
-        HIP_OK(hipEventCreateWithFlags(&C.fork_ev, hipEventDisableTiming));
-        C.stash_s.assign(LSTREAMS, nullptr);
-        for (int s = 0; s < LSTREAMS; s++) HIP_OK(hipMalloc((void**)&C.stash_s[s], sizeof(double) * 2 * g.sz));
         HIP_OK(hipMalloc((void**)&C.stash_b, (size_t)C.esz * 2 * g.sz * WB_MAX));
     }
     // reduction partials: enough for lmax per-block sums of any sweep
@@ -679,15 +664,11 @@ struct WindowScope {
     Ctx& c;
     bool on;
     int l;
-    int s;                     // side stream, or -1 for the library stream
     Grid g{};
     double* saved[21];
     double* ls = nullptr;
     uint8_t* mask = nullptr;
     double* stash = nullptr;
-    hipStream_t stream0 = nullptr;
-    int res0 = 0;
-    int *list0 = nullptr, *count0 = nullptr;
     double** fld(int i) {
         double** f[21] = {&c.unew, &c.vnew, &c.wnew, &c.curv, &c.curv2, &c.ucc_l, &c.vcc_l, &c.wcc_l, &c.u_l,
                           &c.v_l, &c.w_l, &c.dheav, &c.normal_m, &c.ls_a, &c.ls_b, &c.ls_c, &c.p_jump, &c.p_x,
@@ -695,7 +676,7 @@ struct WindowScope {
         return f[i];
     }
     static double* shift(double* p, long off, size_t esz) { return (double*)((char*)p + off * (long)esz); }
-    WindowScope(Ctx& c_, int l_, int s_ = -1) : c(c_), on(!c_.win.empty()), l(l_), s(on ? s_ : -1) {
+    WindowScope(Ctx& c_, int l_) : c(c_), on(!c_.win.empty()), l(l_) {
         if (!on) return;
         const LsWindow& w = c.win[l];
         g = c.g;
@@ -704,14 +685,7 @@ struct WindowScope {
         for (int i = 0; i < 21; i++) { saved[i] = *fld(i); *fld(i) = shift(saved[i], off, c.esz); }
         ls = c.lvset[l]; c.lvset[l] = shift(ls, off, c.esz);
         mask = c.mask; c.mask = mask + off;
-        stash = c.stash; c.stash = shift(s >= 0 ? c.stash_s[s] : stash, poff, c.esz);
-        if (s >= 0) {
-            // the side stream, its residual slot and its closure cell list
-            stream0 = c.stream; c.stream = c.lstream[s];
-            res0 = c.res_slot; c.res_slot = c.res_slot0 + s;
-            list0 = c.mlp.list; count0 = c.mlp.count;
-            if (!c.mlp.list_s.empty()) { c.mlp.list = c.mlp.list_s[s]; c.mlp.count = c.mlp.count_s[s]; }
-        }
+        stash = c.stash; c.stash = shift(stash, poff, c.esz);
     }
     ~WindowScope() {
         if (!on) return;
@@ -720,54 +694,8 @@ struct WindowScope {
         c.lvset[l] = ls;
         c.mask = mask;
         c.stash = stash;
-        if (s >= 0) {
-            c.stream = stream0;
-            c.res_slot = res0;
-            c.mlp.list = list0; c.mlp.count = count0;
-        }
     }
 };
-
-// the side streams wait for the library stream's work so far
-static int fork_streams(Ctx& c) {
-    HIP_OK(hipEventRecord(c.fork_ev, c.stream));
-    for (auto st : c.lstream) HIP_OK(hipStreamWaitEvent(st, c.fork_ev, 0));
-    return ICLS_OK;
-}
-
-// the library stream waits for the side streams' work so far
-static int join_streams(Ctx& c) {
-    for (size_t s = 0; s < c.lstream.size(); s++) {
-        HIP_OK(hipEventRecord(c.lev[s], c.lstream[s]));
-        HIP_OK(hipStreamWaitEvent(c.stream, c.lev[s], 0));
-    }
-    return ICLS_OK;
-}
-
-// fn(l) for every level set (except `skip`) inside its window: colour by colour, the
-// level sets of a colour spread over the side streams, the colours one after another.
-// With last_residual the re-init residual of the last level set (the reference reports
-// the last ls_reinit's, levelset.f90:831) is copied to its slot after its colour.
-static int run_colours(Ctx& c, const std::function<int(int)>& fn, bool last_residual, int skip = -1) {
-    for (int col = 0; col < c.ncolour; col++) {
-        TRY(fork_streams(c));
-        int idx = 0, last_s = -1;
-        for (int l = 0; l < c.lmax; l++) {
-            if (c.colour[l] != col || l == skip) continue;
-            const int s = idx++ % (int)c.lstream.size();
-            {
-                WindowScope ws(c, l, s);
-                TRY(fn(l));
-            }
-            if (l == c.lmax - 1) last_s = s;
-        }
-        TRY(join_streams(c));
-        if (last_residual && last_s >= 0)
-            HIP_OK(hipMemcpyAsync(c.dscal + DS_REINIT_RES, c.dscal + c.res_slot0 + last_s, sizeof(double),
-                                  hipMemcpyDeviceToDevice, c.stream));
-    }
-    return ICLS_OK;
-}
 
 // smallest power of two >= v
 static int pow2_at_least(int v) { int p = 1; while (p < v) p <<= 1; return p; }
@@ -1034,8 +962,12 @@ static int gfm_impl(Ctx& c, bool fuse_fillps, bool store_pjump) {
     if (c.win.empty()) {
         for (int l = 1; l < c.lmax; l++) TRY(curvature(l));
     } else {
-        // windows of one colour do not overlap: their jumps are added at once
-        TRY(run_colours(c, curvature, false, 0));
+        // windows of one colour do not overlap: their curvatures, closures and added
+        // jumps run in one launch per kernel, colour after colour
+        std::vector<std::vector<int>> by_colour(c.ncolour);
+        for (int l = 1; l < c.lmax; l++) by_colour[c.colour[l]].push_back(l);
+        for (const auto& ls : by_colour)
+            if (!ls.empty()) TRY(launch_curvature_jumps_batch(c, ls, learned));
     }
     if (fuse_fillps && c.lmax > 1) {
         TRY(launch_fillps(c));

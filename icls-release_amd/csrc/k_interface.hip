@@ -42,13 +42,14 @@ __device__ __forceinline__ T lsq_row(const T* __restrict__ A, int r, const T* v)
 // With a cell list (the learned mode) the cells of the re_curvature shell whose
 // curvature is under-resolved -- |phi| < 2 dz, |grad phi| >= 0.9 and |kappa| dz >= gate
 // -- are appended to it for the learned closure (k_mlp_curv).
+// A listed cell is entered as gofs + i0 + sy j0 + sz (k - 1): gofs is 0 for a launch on
+// one grid or window (the index is the window's own), and the window's offset in the
+// whole grid for a batched launch, which also enters the window t in list_t.
 template <class T>
-__global__ void k_curvature(Grid g, T NB2, const T* __restrict__ A, const T* __restrict__ phi, T* __restrict__ kappa,
-                            T* __restrict__ nm, const double* __restrict__ flag, int* __restrict__ list,
-                            int* __restrict__ count, int cap, T gate) {
-    if (flag && flag[0] == 0.) return;
-    int i0 = blockIdx.x * BX + threadIdx.x, j0 = blockIdx.y * BY + threadIdx.y, k = blockIdx.z + 1;
-    if (i0 >= g.nx || j0 >= g.ny) return;
+__device__ __forceinline__ void curvature_at(const Grid& g, T NB2, const T* __restrict__ A, const T* __restrict__ phi,
+                                             T* __restrict__ kappa, T* __restrict__ nm, int* __restrict__ list,
+                                             int* __restrict__ list_t, int t, long gofs, int* __restrict__ count,
+                                             int cap, T gate, int i0, int j0, int k) {
     long q = gidx(g, i0, j0, k);
     const T dz = T(g.dz), max_kappa = T(1.) / dz;
     T cv = T(0.), m = T(1.);
@@ -58,11 +59,57 @@ __global__ void k_curvature(Grid g, T NB2, const T* __restrict__ A, const T* __r
         cv = curv_lsq(A, v, max_kappa, m);
         if (list && fabs(phi[q]) < T(2.) * dz && m >= T(0.9) && fabs(cv) * dz >= gate) {
             const int slot = atomicAdd(count, 1);
-            if (slot < cap) list[slot] = (int)(i0 + g.sy * j0 + g.sz * (long)(k - 1));
+            if (slot < cap) {
+                list[slot] = (int)(gofs + i0 + g.sy * j0 + g.sz * (long)(k - 1));
+                if (list_t) list_t[slot] = t;
+            }
         }
     }
     kappa[q] = cv;
     if (nm) nm[q] = m;
+}
+
+template <class T>
+__global__ void k_curvature(Grid g, T NB2, const T* __restrict__ A, const T* __restrict__ phi, T* __restrict__ kappa,
+                            T* __restrict__ nm, const double* __restrict__ flag, int* __restrict__ list,
+                            int* __restrict__ count, int cap, T gate) {
+    if (flag && flag[0] == 0.) return;
+    int i0 = blockIdx.x * BX + threadIdx.x, j0 = blockIdx.y * BY + threadIdx.y, k = blockIdx.z + 1;
+    if (i0 >= g.nx || j0 >= g.ny) return;
+    curvature_at<T>(g, NB2, A, phi, kappa, nm, list, nullptr, 0, 0, count, cap, gate, i0, j0, k);
+}
+
+// ---------------------------------------------------------------- batched windows
+// A launch over the windows of a batch: 64 x 4 tiles, one layer each, window t owning
+// blocks wb.first[t] .. wb.first[t+1]-1; the cell's window coordinates, and the window.
+struct WinCell {
+    WinPick w;
+    int i0, j0, k;
+    bool in;
+};
+__device__ __forceinline__ WinCell win_cell(const Grid& g, const WinBatch& wb) {
+    WinCell r;
+    r.w = win_pick(g, wb, blockIdx.x);
+    const int ntx = (r.w.g.nx + BX - 1) / BX, nty = (r.w.g.ny + BY - 1) / BY;
+    const int b = r.w.bid;
+    r.i0 = (b % ntx) * BX + threadIdx.x;
+    r.j0 = ((b / ntx) % nty) * BY + threadIdx.y;
+    r.k = b / (ntx * nty) + 1;
+    r.in = r.i0 < r.w.g.nx && r.j0 < r.w.g.ny;
+    return r;
+}
+static long win_blocks(const LsWindow& w) { return (long)((w.wx + BX - 1) / BX) * ((w.wy + BY - 1) / BY) * w.wz; }
+
+template <class T>
+__global__ void k_curvature_batch(Grid g, WinBatch wb, T NB2, const T* __restrict__ A, T* __restrict__ kappa,
+                                  int* __restrict__ list, int* __restrict__ list_t, int* __restrict__ count, int cap,
+                                  T gate) {
+    const WinCell wc = win_cell(g, wb);
+    if (!wc.in) return;
+    const int t = wc.w.t;
+    const long gofs = (long)wb.ox[t] + g.sy * (long)wb.oy[t] + g.sz * (long)wb.oz[t];
+    curvature_at<T>(wc.w.g, NB2, A, (const T*)wb.ls[t] + wc.w.off, kappa + wc.w.off, (T*)nullptr, list, list_t, t,
+                    gofs, count, cap, gate, wc.i0, wc.j0, wc.k);
 }
 
 int launch_curvature(Ctx& c, const double* phi, double* kappa, double* normal_m, const double* flag, bool listed) {
@@ -85,10 +132,9 @@ int launch_curvature(Ctx& c, const double* phi, double* kappa, double* normal_m,
 // least-squares fit to the 27 curvature values; zero elsewhere.  This smoothed value is
 // what the GFM jumps interpolate to the interface.
 template <class T>
-__global__ void k_recurvature(Grid g, const T* __restrict__ A, const T* __restrict__ phi, const T* __restrict__ kappa,
-                              T* __restrict__ kappa2) {
-    int i0 = blockIdx.x * BX + threadIdx.x, j0 = blockIdx.y * BY + threadIdx.y, k = blockIdx.z + 1;
-    if (i0 >= g.nx || j0 >= g.ny) return;
+__device__ __forceinline__ void recurvature_at(const Grid& g, const T* __restrict__ A, const T* __restrict__ phi,
+                                               const T* __restrict__ kappa, T* __restrict__ kappa2, int i0, int j0,
+                                               int k) {
     long q = gidx(g, i0, j0, k);
     T r = T(0.);
     if (fabs(phi[q]) < T(2.) * T(g.dz)) {
@@ -97,6 +143,23 @@ __global__ void k_recurvature(Grid g, const T* __restrict__ A, const T* __restri
         r = lsq_row(A, 0, v);
     }
     kappa2[q] = r;
+}
+
+template <class T>
+__global__ void k_recurvature_batch(Grid g, WinBatch wb, const T* __restrict__ A, const T* __restrict__ kappa,
+                                    T* __restrict__ kappa2) {
+    const WinCell wc = win_cell(g, wb);
+    if (!wc.in) return;
+    recurvature_at<T>(wc.w.g, A, (const T*)wb.ls[wc.w.t] + wc.w.off, kappa + wc.w.off, kappa2 + wc.w.off, wc.i0,
+                      wc.j0, wc.k);
+}
+
+template <class T>
+__global__ void k_recurvature(Grid g, const T* __restrict__ A, const T* __restrict__ phi, const T* __restrict__ kappa,
+                              T* __restrict__ kappa2) {
+    int i0 = blockIdx.x * BX + threadIdx.x, j0 = blockIdx.y * BY + threadIdx.y, k = blockIdx.z + 1;
+    if (i0 >= g.nx || j0 >= g.ny) return;
+    recurvature_at<T>(g, A, phi, kappa, kappa2, i0, j0, k);
 }
 
 int launch_recurvature(Ctx& c, const double* phi, const double* kappa, double* kappa2) {
@@ -197,6 +260,21 @@ __global__ void k_gfm(Grid g, T nondi, T coef, const T* __restrict__ ls, const T
     if (MODE == 2) rhs[q] = fillps_rhs<T>(g, coef, du, dv, dw, i0, j0, k) + pjs;
 }
 
+// MODE 1 over the windows of a batch: each level set's jumps added inside its shell
+template <class T>
+__global__ void k_gfm_batch(Grid g, WinBatch wb, T nondi, const T* __restrict__ cvf, T* __restrict__ pjump,
+                            T* __restrict__ px, T* __restrict__ py, T* __restrict__ pz) {
+    const WinCell wc = win_cell(g, wb);
+    if (!wc.in) return;
+    const long o = wc.w.off;
+    const long q = gidx(wc.w.g, wc.i0, wc.j0, wc.k) + o;
+    T pjs = T(0.), pxs = T(0.), pys = T(0.), pzs = T(0.);
+    if (gfm_cell<T>(wc.w.g, nondi, (const T*)wb.ls[wc.w.t] + o, cvf + o, wc.i0, wc.j0, wc.k, pjs, pxs, pys, pzs)) {
+        pjump[q] = pjump[q] + pjs;
+        px[q] = px[q] + pxs; py[q] = py[q] + pys; pz[q] = pz[q] + pzs;
+    }
+}
+
 template <class T>
 static void gfm_dispatch(Ctx& c, int l, const double* kappa2, int mode, bool store_pjump) {
     dim3 blk(BX, BY, 1);
@@ -226,6 +304,51 @@ int launch_gfm(Ctx& c, int l, const double* kappa2, bool fuse_fillps, bool store
     else gfm_dispatch<double>(c, l, kappa2, mode, store_pjump);
     HIP_OK(hipGetLastError());
     return ICLS_OK;
+}
+
+// get_curvature, re_curvature, the learned closure (learned) and the added jumps of the
+// level sets ls -- windows of one colour, which do not overlap -- one launch per kernel
+// for up to WB_MAX windows.  Per cell the operations are those of the launches on one
+// window at a time; the closure's cell list holds whole-grid indices and their windows.
+template <class T>
+static int curvature_jumps_batch_t(Ctx& c, const std::vector<int>& ls, bool learned) {
+    auto key = [](const LsWindow&) { return 0; };
+    auto blocks = [](const LsWindow& w) { return win_blocks(w); };
+    const T nondi = (T)(1. / c.P.We);
+    const T* A = as<T>(c.fp32 ? c.pinvA_f : c.pinvA);
+    return for_each_batch(c, ls, key, blocks, [&](const WinBatch& wb, int, const std::vector<int>&) -> int {
+        const dim3 grd((unsigned)wb.first[wb.n]), blk(BX, BY, 1);
+        if (learned) {
+            int rc = launch_mlp_reset_list(c);
+            if (rc) return rc;
+        }
+        {
+            ProfScope ps(c, "k_curvature");
+            hipLaunchKernelGGL(k_curvature_batch<T>, grd, blk, 0, c.stream, c.g, wb, (T)c.NB2, A, as<T>(c.curv),
+                               learned ? c.mlp.list : nullptr, learned ? c.mlp.list_t : nullptr, c.mlp.count,
+                               c.mlp.list_cap, (T)c.mlp.gate);
+            HIP_OK(hipGetLastError());
+        }
+        {
+            ProfScope ps(c, "k_recurvature");
+            hipLaunchKernelGGL(k_recurvature_batch<T>, grd, blk, 0, c.stream, c.g, wb, A, (const T*)as<T>(c.curv),
+                               as<T>(c.curv2));
+            HIP_OK(hipGetLastError());
+        }
+        if (learned) {
+            int rc = launch_mlp_curv_batch(c, wb, c.curv, c.curv2);
+            if (rc) return rc;
+        }
+        ProfScope ps(c, "k_gfm");
+        hipLaunchKernelGGL(k_gfm_batch<T>, grd, blk, 0, c.stream, c.g, wb, nondi, (const T*)as<T>(c.curv2),
+                           as<T>(c.p_jump), as<T>(c.p_x), as<T>(c.p_y), as<T>(c.p_z));
+        HIP_OK(hipGetLastError());
+        return ICLS_OK;
+    });
+}
+
+int launch_curvature_jumps_batch(Ctx& c, const std::vector<int>& ls, bool learned) {
+    return c.fp32 ? curvature_jumps_batch_t<float>(c, ls, learned) : curvature_jumps_batch_t<double>(c, ls, learned);
 }
 
 // every stored cell: jumps whose mask bit is clear become zero, every bit is set

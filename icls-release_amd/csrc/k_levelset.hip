@@ -756,48 +756,6 @@ static Grid win_grid(const Ctx& c, const LsWindow& w) {
     g.nx = w.wx; g.ny = w.wy; g.nz = w.wz;
     return g;
 }
-template <class KeyFn, class BlocksFn, class LaunchFn>
-static int for_each_batch(Ctx& c, const std::vector<int>& ls, KeyFn key, BlocksFn blocks, LaunchFn launch) {
-    std::vector<int> keys;
-    for (int l : ls) {
-        const int k = key(c.win[l]);
-        bool seen = false;
-        for (int q : keys) seen = seen || q == k;
-        if (!seen) keys.push_back(k);
-    }
-    for (int k : keys) {
-        WinBatch wb;
-        std::vector<int> members;
-        auto flush = [&]() -> int {
-            if (wb.n == 0) return ICLS_OK;
-            int rc = launch(wb, k, members);
-            wb = WinBatch();
-            members.clear();
-            return rc;
-        };
-        for (int l : ls) {
-            const LsWindow& w = c.win[l];
-            if (key(w) != k) continue;
-            const long nb = blocks(w);
-            if (nb <= 0 || (long)wb.first[wb.n] + nb > (1L << 30)) {
-                set_error("window batch: bad block count");
-                return ICLS_ERR_STATE;
-            }
-            const int t = wb.n;
-            wb.ox[t] = w.ox; wb.oy[t] = w.oy; wb.oz[t] = w.oz;
-            wb.wx[t] = w.wx; wb.wy[t] = w.wy; wb.wz[t] = w.wz;
-            wb.ls[t] = c.lvset[l];
-            wb.lidx[t] = l;
-            wb.first[t + 1] = wb.first[t] + (int)nb;
-            wb.n = t + 1;
-            members.push_back(l);
-            if (wb.n == WB_MAX) { int rc = flush(); if (rc) return rc; }
-        }
-        int rc = flush();
-        if (rc) return rc;
-    }
-    return ICLS_OK;
-}
 
 // tube and the three advection stages of ls_main for the level sets ls, whose windows
 // do not overlap: the launches of advect() in icls_runtime, one per colour instead of

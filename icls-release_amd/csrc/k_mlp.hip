@@ -412,10 +412,13 @@ __global__ void k_mlp_samples(long n, unsigned long long seed, double r_min, dou
 // smoothing averages the 27 neighbours' curvatures, which for a small drop includes
 // the clipped ones near its centre; the closure's value is the curvature of the level
 // surface through the cell, which is what the jump interpolation wants.
-template <class T>
+// BATCH: the cells of a batched curvature launch, whole-grid indices (g is the whole
+// grid) with the window of each in list_t, whose level set is wb.ls[window].
+template <class T, bool BATCH>
 __global__ void __launch_bounds__(64)
-k_mlp_curv(Grid g, const int* __restrict__ list, const int* __restrict__ count, int cap, const T* __restrict__ A,
-           const T* __restrict__ phi, const T* __restrict__ kfd_in, T* __restrict__ kappa, const uint16_t* __restrict__ w1b,
+k_mlp_curv(Grid g, const int* __restrict__ list, const int* __restrict__ list_t, WinBatch wb,
+           const int* __restrict__ count, int cap, const T* __restrict__ A, const T* __restrict__ phi,
+           const T* __restrict__ kfd_in, T* __restrict__ kappa, const uint16_t* __restrict__ w1b,
            const uint16_t* __restrict__ w2b, const float* __restrict__ w, float tol, int* __restrict__ total) {
     __shared__ __align__(16) uint16_t xs[16 * MLP_NPAD];
     __shared__ __align__(16) uint16_t hs[16 * MLP_NH];
@@ -434,6 +437,8 @@ k_mlp_curv(Grid g, const int* __restrict__ list, const int* __restrict__ count, 
             for (int k = 0; k < MLP_NPAD; k++) f[k] = 0.f;
             if (e < cnt) {
                 const int c = list[e];
+                const T* ph = phi;
+                if constexpr (BATCH) ph = (const T*)wb.ls[list_t[e]];
                 // the list holds i0 + sy j0 + sz (k - 1) (k_curvature): decode with the
                 // strides, which differ from nx and ny inside a level-set window
                 const int i0 = (int)(c % g.sy), j0 = (int)((c / g.sy) % (g.sz / g.sy)), k = (int)(c / g.sz) + 1;
@@ -443,7 +448,7 @@ k_mlp_curv(Grid g, const int* __restrict__ list, const int* __restrict__ count, 
                 for (int cc = -1; cc <= 1; cc++)
                     for (int b = -1; b <= 1; b++)
                         for (int a = -1; a <= 1; a++)
-                            v[qq++] = (double)phi[gidx(g, wrap(i0 + a, g.nx), wrap(j0 + b, g.ny), k + cc)];
+                            v[qq++] = (double)ph[gidx(g, wrap(i0 + a, g.nx), wrap(j0 + b, g.ny), k + cc)];
                 double gx = 0., gy = 0., gz = 0.;
                 for (int kk = 0; kk < 27; kk++) {
                     gx += (double)A[1 + 10 * kk] * v[kk];
@@ -500,18 +505,11 @@ int mlp_alloc(Ctx& c) {
     HIP_OK(hipMemcpy(m.pinvA_fine, af.data(), sizeof(double) * 270, hipMemcpyHostToDevice));
     m.list_cap = (int)std::min<long>(c.g.sz * (long)c.g.nz, 1L << 22);
     HIP_OK(hipMalloc((void**)&m.list, sizeof(int) * m.list_cap));
+    HIP_OK(hipMalloc((void**)&m.list_t, sizeof(int) * m.list_cap));
     HIP_OK(hipMalloc((void**)&m.count, sizeof(int)));
     HIP_OK(hipMemset(m.count, 0, sizeof(int)));
     HIP_OK(hipMalloc((void**)&m.total, sizeof(int)));
     HIP_OK(hipMemset(m.total, 0, sizeof(int)));
-    for (size_t s = 0; s < c.lstream.size(); s++) {   // concurrent level-set windows
-        int *l = nullptr, *n = nullptr;
-        HIP_OK(hipMalloc((void**)&l, sizeof(int) * m.list_cap));
-        HIP_OK(hipMalloc((void**)&n, sizeof(int)));
-        HIP_OK(hipMemset(n, 0, sizeof(int)));
-        m.list_s.push_back(l);
-        m.count_s.push_back(n);
-    }
     // deterministic initial weights: uniform(-1, 1) / sqrt(fan_in), zero biases and
     // zero padding columns
     std::vector<float> w(MLP_NPARAM, 0.f);
@@ -532,11 +530,9 @@ int mlp_alloc(Ctx& c) {
 
 void mlp_free(Ctx& c) {
     Mlp& m = c.mlp;
-    void* ptrs[] = {m.w, m.adam_m, m.adam_v, m.w1b, m.w2b, m.w2tb, m.gpart, m.grad, m.pinvA_fine, m.list, m.count,
-                    m.total};
+    void* ptrs[] = {m.w, m.adam_m, m.adam_v, m.w1b, m.w2b, m.w2tb, m.gpart, m.grad, m.pinvA_fine, m.list, m.list_t,
+                    m.count, m.total};
     for (void* p : ptrs) if (p) hipFree(p);
-    for (int* p : m.list_s) if (p) hipFree(p);
-    for (int* p : m.count_s) if (p) hipFree(p);
     m = Mlp();
 }
 
@@ -609,21 +605,35 @@ int launch_mlp_reset_list(Ctx& c) {
     return ICLS_OK;
 }
 
-int launch_mlp_curv(Ctx& c, const double* phi, const double* kappa_fd, double* kappa_out) {
+template <bool BATCH>
+static int mlp_curv_launch(Ctx& c, const WinBatch& wb, const double* phi, const double* kappa_fd, double* kappa_out) {
     ProfScope ps(c, "k_mlp_curv");
     Mlp& m = c.mlp;
     const int nb = 512;
     if (c.fp32)
-        hipLaunchKernelGGL(k_mlp_curv<float>, dim3(nb), dim3(64), 0, c.stream, c.g, (const int*)m.list,
-                           (const int*)m.count, m.list_cap, as<float>(c.pinvA_f), as<float>(phi),
-                           as<float>(kappa_fd), as<float>(kappa_out), (const uint16_t*)m.w1b, (const uint16_t*)m.w2b,
-                           (const float*)m.w, m.tol, m.total);
+        hipLaunchKernelGGL((k_mlp_curv<float, BATCH>), dim3(nb), dim3(64), 0, c.stream, c.g, (const int*)m.list,
+                           (const int*)m.list_t, wb, (const int*)m.count, m.list_cap, as<float>(c.pinvA_f),
+                           as<float>(phi), as<float>(kappa_fd), as<float>(kappa_out), (const uint16_t*)m.w1b,
+                           (const uint16_t*)m.w2b, (const float*)m.w, m.tol, m.total);
     else
-        hipLaunchKernelGGL(k_mlp_curv<double>, dim3(nb), dim3(64), 0, c.stream, c.g, (const int*)m.list,
-                           (const int*)m.count, m.list_cap, (const double*)c.pinvA, phi, kappa_fd, kappa_out,
-                           (const uint16_t*)m.w1b, (const uint16_t*)m.w2b, (const float*)m.w, m.tol, m.total);
+        hipLaunchKernelGGL((k_mlp_curv<double, BATCH>), dim3(nb), dim3(64), 0, c.stream, c.g, (const int*)m.list,
+                           (const int*)m.list_t, wb, (const int*)m.count, m.list_cap, (const double*)c.pinvA, phi,
+                           kappa_fd, kappa_out, (const uint16_t*)m.w1b, (const uint16_t*)m.w2b, (const float*)m.w,
+                           m.tol, m.total);
     HIP_OK(hipGetLastError());
     return ICLS_OK;
+}
+
+int launch_mlp_curv(Ctx& c, const double* phi, const double* kappa_fd, double* kappa_out) {
+    return mlp_curv_launch<false>(c, WinBatch(), phi, kappa_fd, kappa_out);
+}
+
+int launch_mlp_curv_batch(Ctx& c, const WinBatch& wb, const double* kappa_fd, double* kappa_out) {
+    if (!c.mlp.list_t) {
+        set_error("closure cell list without window entries");
+        return ICLS_ERR_STATE;
+    }
+    return mlp_curv_launch<true>(c, wb, nullptr, kappa_fd, kappa_out);
 }
 
 int launch_mlp_reset_total(Ctx& c) {
