@@ -29,7 +29,9 @@ One JSON line on stdout:
   roofline    = that run's dominant kernel: algorithmic bytes per launch / its mean
                 launch time (HIP events on the library stream over the timed region)
                 vs the 8 TB/s HBM peak; traffic from rocprofv3 FETCH_SIZE + WRITE_SIZE
-                where a committed measurement for that kernel exists under profiles/
+                where a committed measurement for that kernel exists under profiles/;
+                in the many-drop leg the dominant kernel that sweeps the whole grid
+                (the level-set kernels there cover windows, whose bytes vary)
   cpu_baseline= the CPU oracle (C restatement of the reference, FP64) on the host's
                 cores (its OpenMP build, results identical to the serial one) over
                 steps of the same workload; "cpu_baseline_1core" the serial build on
@@ -106,6 +108,13 @@ def drops_params(n, ndrops, mode):
     return p
 
 
+# scopes whose launches cover level-set windows, not the whole grid, when every drop
+# is stepped in its own window: their bytes per launch depend on the windows, so the
+# many-drop leg names the dominant whole-grid kernel instead
+WINDOWED = {"k_tube", "k_adv_stage1", "k_adv_stage23", "k_reinit_stage1", "k_reinit_stage23", "k_curvature",
+            "k_recurvature", "k_gfm", "k_mlp_curv"}
+
+
 def run_mode(mode, n, steps, warmup, show_kernels, drops=0):
     """Time `steps` steps of the workload in one precision mode; returns a dict."""
     from iclsamd import _lib
@@ -135,7 +144,7 @@ def run_mode(mode, n, steps, warmup, show_kernels, drops=0):
             istep += 1
             g.step(istep, 0)
         kt = g.kernel_times()
-        known = {k: v for k, v in kt.items() if k in kbytes}
+        known = {k: v for k, v in kt.items() if k in kbytes and not (drops and k in WINDOWED)}
         dominant = max(known, key=lambda k: known[k][0])
         if show_kernels:
             tot = sum(v[0] for v in kt.values())
