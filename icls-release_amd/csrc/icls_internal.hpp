@@ -254,6 +254,15 @@ __device__ __forceinline__ int xcd_tile(int b, int n) {
     const int x = b % NUM_XCD, i = b / NUM_XCD;
     return x * per + (x < rem ? x : rem) + i;
 }
+// The same with the tile ids dealt to the XCDs in groups of G: XCD x takes groups x,
+// x + 8, x + 16, ... so that every XCD holds a share of each region of the grid, which
+// balances kernels whose work sits in a few regions (the narrow band).  A count that is
+// not a multiple of 8 G keeps the contiguous ranges.
+__device__ __forceinline__ int xcd_tile_dealt(int b, int n, int G) {
+    if (G <= 0 || n % (NUM_XCD * G) != 0) return xcd_tile(b, n);
+    const int x = b % NUM_XCD, i = b / NUM_XCD;
+    return ((i / G) * NUM_XCD + x) * G + i % G;
+}
 
 // ---- launchers (defined in the k_*.hip files); each returns ICLS_OK or an error.
 // Field-class pointers are reinterpreted as the mode's element type.

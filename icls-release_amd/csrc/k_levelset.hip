@@ -17,6 +17,13 @@
 
 namespace icls {
 
+// Tube and re-init tiles go to the XCDs in groups of 8 (xcd_tile_dealt): the band, where
+// their work is, lies in a few z chunks, which contiguous ranges would give to two or
+// three XCDs.  At 256^3 (MI355X) groups of 2, 4, 8 and 16 ran the float re-init stage
+// 2/3 in 92.7, 88.1, 87.8 and 88.0 us against 98.5 with contiguous ranges (FP64: 170
+// against 229); the tube was unchanged.
+constexpr int XCD_DEAL = 8;
+
 static constexpr int BX = 64, BY = 4;
 
 // ---------------------------------------------------------------- tube_construction
@@ -42,7 +49,7 @@ __device__ __forceinline__ void tube_zm_body(const Grid& g, int bid, int TZ, T N
                                              T* __restrict__ out, uint8_t* __restrict__ mask) {
     __shared__ T ring[TNS][TTP];
     const int ntx = (g.nx + BX - 1) / BX, nty = (g.ny + BY - 1) / BY, ntz = (g.nz + TZ - 1) / TZ;
-    const int tile = xcd_tile(bid, ntx * nty * ntz);
+    const int tile = xcd_tile_dealt(bid, ntx * nty * ntz, XCD_DEAL);
     const int bx = tile % ntx, by = (tile / ntx) % nty, bz = tile / (ntx * nty);
     const int x0 = bx * BX, y0 = by * BY, kb = 1 + bz * TZ;
     const int ke = kb + TZ - 1 < g.nz ? kb + TZ - 1 : g.nz;
@@ -439,7 +446,7 @@ __device__ __forceinline__ void reinit_zm_body(const Grid& g, int bid, int RZ, T
     __shared__ T wv[RNS][BX * BY / 64];
     __shared__ double smem[16];
     const int ntx = (g.nx + BX - 1) / BX, nty = (g.ny + RTY - 1) / RTY, ntz = (g.nz + RZ - 1) / RZ;
-    const int tile = xcd_tile(bid, ntx * nty * ntz);
+    const int tile = xcd_tile_dealt(bid, ntx * nty * ntz, XCD_DEAL);
     const int bx = tile % ntx, by = (tile / ntx) % nty, bz = tile / (ntx * nty);
     const int x0 = bx * BX, y0 = by * RTY, kb = 1 + bz * RZ;
     const int ke = kb + RZ - 1 < g.nz ? kb + RZ - 1 : g.nz;
@@ -695,8 +702,8 @@ static void reinit_dispatch_pd(Ctx& c, int rz, int stage, const double* phi0, co
 // 24 and 32 layers ran stages 1 / 2-3 in 0.122 / 0.138, 0.130 / 0.140, 0.114 / 0.121,
 // 0.139 / 0.151 and 0.098 / 0.105 ms (FP32), and chunks filling whole rounds of
 // resident blocks (52 layers) in 0.131 / 0.137 ms: far-field blocks end early (the
-// uniform-window path), so block count matters more than rounds, and 8 chunks put one
-// z slab of tiles on each XCD.
+// uniform-window path), so block count matters more than rounds.  With the tiles dealt
+// to the XCDs in groups (XCD_DEAL), chunks of 8 and 16 layers were again slower than 32.
 // Inside a level-set window (a few tens of thousands of cells) 8-layer chunks: four
 // times the blocks of 32-layer chunks, and a quarter of each block's serial march.
 constexpr int REINIT_RZ = 32, REINIT_RZ_WINDOW = 8;
