@@ -150,7 +150,8 @@ struct Ctx {
     double *ls_a = nullptr, *ls_b = nullptr, *ls_c = nullptr;  // RK / re-init stage buffers
     double* frhs = nullptr;               // Poisson right-hand side (the array p itself in FP64)
     double* pf = nullptr;                 // mixed: float work array of the transforms
-    double* pf0 = nullptr;                // mixed: float solution of the first Poisson pass
+    double* pf0 = nullptr;                // mixed: float starting solution of the refinement pass
+    bool pf0_step = false;                // pf0 holds the time step's last solution rounded to float
     // pressure class (double)
     double *pnew = nullptr, *pold = nullptr, *p = nullptr;
     // Lazy pressure rotation (main.f90:164-183).  After a step the pnew buffer holds

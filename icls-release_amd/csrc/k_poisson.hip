@@ -29,11 +29,17 @@ template <class C> __device__ __forceinline__ cx<C> csub(cx<C> a, cx<C> b) { ret
 template <class C> __device__ __forceinline__ cx<C> cconj(cx<C> a) { return {a.re, -a.im}; }
 
 // out[q] = v, or with ACC out[q] = b0[q] + v: the refinement pass adds its correction
-// to its starting solution b0
+// to its float starting solution b0, which then takes the new solution rounded to float
+// (the next step's starting solution)
 template <class TO, class C, int ACC, class B>
-__device__ __forceinline__ void put(TO* out, long q, C v, const B* __restrict__ b0) {
-    if (ACC) out[q] = (TO)b0[q] + (TO)v;
-    else out[q] = (TO)v;
+__device__ __forceinline__ void put(TO* out, long q, C v, B* __restrict__ b0) {
+    if (ACC) {
+        const TO r = (TO)b0[q] + (TO)v;
+        out[q] = r;
+        b0[q] = (B)r;
+    } else {
+        out[q] = (TO)v;
+    }
 }
 
 // Residual of the discrete Poisson problem the transforms and the Thomas solve
@@ -89,7 +95,7 @@ __device__ __forceinline__ void stockham(cx<C>* x, cx<C>* y, int M, int lane, in
 // The input is C (the compute type), the output TO; ACC adds into the output.
 template <int DIR, int INV, class C, class TO, int ACC, class B>
 __global__ void k_rfft(Grid g, int N, int LPB, int TPL, const C* __restrict__ twg, const C* in, TO* out,
-                       const B* __restrict__ b0) {
+                       B* __restrict__ b0) {
     extern __shared__ unsigned char smem_raw[];
     cx<C>* smem = (cx<C>*)smem_raw;
     const int M = N / 2;
@@ -195,7 +201,7 @@ struct RfftShape {
 
 template <int DIR, int INV, int N, int LPB, class C, class TO, int ACC, int RES, class B>
 __global__ void __launch_bounds__((RfftShape<N, LPB>::NTHR))
-k_rfft_n(Grid g, const C* __restrict__ twg, const C* in, TO* out, const B* __restrict__ b0) {
+k_rfft_n(Grid g, const C* __restrict__ twg, const C* in, TO* out, B* __restrict__ b0) {
     static_assert(!RES || (DIR == 0 && !INV), "the residual feeds the forward x transform");
     constexpr int M = N / 2;
     constexpr int TPL = RfftShape<N, LPB>::TPL;
@@ -410,7 +416,7 @@ __device__ __forceinline__ void cfft128(cx<float>* v, cx<float> (*u)[8], int s, 
 }  // namespace r16
 
 template <class TO, int ACC, class B>
-__device__ __forceinline__ void put2(TO* out, long q, float a, float b, const B* __restrict__ b0) {
+__device__ __forceinline__ void put2(TO* out, long q, float a, float b, B* __restrict__ b0) {
     put<TO, float, ACC>(out, q, a, b0);
     put<TO, float, ACC>(out, q + 1, b, b0);
 }
@@ -462,8 +468,8 @@ __global__ void __launch_bounds__(512) k_rfft256_y(Grid g, const float* __restri
             for (int k2 = 0; k2 < 8; k2++) {
                 const int m = 2 * s + h + 16 * k2;
                 const cx<float> z = u[h][2 * (k2 % 4) + k2 / 4];
-                put<TO, float, ACC>(out, row(2 * m), z.re * scale, (const float*)nullptr);
-                put<TO, float, ACC>(out, row(2 * m + 1), z.im * scale, (const float*)nullptr);
+                put<TO, float, ACC>(out, row(2 * m), z.re * scale, (float*)nullptr);
+                put<TO, float, ACC>(out, row(2 * m + 1), z.im * scale, (float*)nullptr);
             }
         return;
     }
@@ -479,12 +485,12 @@ __global__ void __launch_bounds__(512) k_rfft256_y(Grid g, const float* __restri
             const int kk = 2 * s + h + 16 * k2;
             const cx<float> a = u[h][2 * (k2 % 4) + k2 / 4];
             if (kk == 0) {
-                put<TO, float, ACC>(out, row(0), (a.re + a.im) * scale, (const float*)nullptr);
-                put<TO, float, ACC>(out, row(N - 1), (a.re - a.im) * scale, (const float*)nullptr);
+                put<TO, float, ACC>(out, row(0), (a.re + a.im) * scale, (float*)nullptr);
+                put<TO, float, ACC>(out, row(N - 1), (a.re - a.im) * scale, (float*)nullptr);
             } else {
                 const cx<float> F = fwd_split(tw, kk, a, xb[(M - kk) * 64 + lane]);
-                put<TO, float, ACC>(out, row(2 * kk - 1), F.re * scale, (const float*)nullptr);
-                put<TO, float, ACC>(out, row(2 * kk), F.im * scale, (const float*)nullptr);
+                put<TO, float, ACC>(out, row(2 * kk - 1), F.re * scale, (float*)nullptr);
+                put<TO, float, ACC>(out, row(2 * kk), F.im * scale, (float*)nullptr);
             }
         }
 }
@@ -493,7 +499,7 @@ __global__ void __launch_bounds__(512) k_rfft256_y(Grid g, const float* __restri
 // rhs - L b0 formed on the fly
 template <int INV, class TO, int ACC, int RES, class B>
 __global__ void __launch_bounds__(256) k_rfft256_x(Grid g, const float* __restrict__ twg, const float* in, TO* out,
-                                                  const B* __restrict__ b0) {
+                                                  B* __restrict__ b0) {
     using namespace r16;
     static_assert(!RES || !INV, "the residual feeds the forward transform");
     static_assert(INV || (!ACC && sizeof(TO) == 4), "the forward transform writes the float work array");
@@ -565,19 +571,13 @@ __global__ void __launch_bounds__(256) k_rfft256_x(Grid g, const float* __restri
                 *reinterpret_cast<float4*>(out + q) = make_float4(z0.re * scale, z0.im * scale, z1.re * scale, z1.im * scale);
             } else if constexpr (sizeof(TO) == 8 && ACC) {
                 // q is a multiple of 4: the base and the output move as 16-byte vectors
-                double b[4];
-                if constexpr (std::is_same<B, float>::value) {
-                    const float4 t = *reinterpret_cast<const float4*>(b0 + q);
-                    b[0] = t.x; b[1] = t.y; b[2] = t.z; b[3] = t.w;
-                } else {
-                    const double2 t0 = *reinterpret_cast<const double2*>(b0 + q);
-                    const double2 t1 = *reinterpret_cast<const double2*>(b0 + q + 2);
-                    b[0] = t0.x; b[1] = t0.y; b[2] = t1.x; b[3] = t1.y;
-                }
-                *reinterpret_cast<double2*>(out + q) =
-                    make_double2(b[0] + (double)(z0.re * scale), b[1] + (double)(z0.im * scale));
-                *reinterpret_cast<double2*>(out + q + 2) =
-                    make_double2(b[2] + (double)(z1.re * scale), b[3] + (double)(z1.im * scale));
+                static_assert(std::is_same<B, float>::value, "the starting solution is float");
+                const float4 t = *reinterpret_cast<const float4*>(b0 + q);
+                const double r0 = (double)t.x + (double)(z0.re * scale), r1 = (double)t.y + (double)(z0.im * scale);
+                const double r2 = (double)t.z + (double)(z1.re * scale), r3 = (double)t.w + (double)(z1.im * scale);
+                *reinterpret_cast<double2*>(out + q) = make_double2(r0, r1);
+                *reinterpret_cast<double2*>(out + q + 2) = make_double2(r2, r3);
+                *reinterpret_cast<float4*>(b0 + q) = make_float4((float)r0, (float)r1, (float)r2, (float)r3);
             } else {
                 put2<TO, ACC>(out, q, z0.re * scale, z0.im * scale, b0);
                 put2<TO, ACC>(out, q + 2, z1.re * scale, z1.im * scale, b0);
@@ -630,7 +630,7 @@ __global__ void __launch_bounds__(256) k_rfft256_x(Grid g, const float* __restri
 
 // the register kernels cover float lines of 256 with 64 | nx (y) or 32 | ny (x)
 template <int DIR, int INV, class C, class TO, int ACC, int RES, class B>
-static bool try_rfft256(Ctx& c, const C* tw, const C* in, TO* out, const B* b0) {
+static bool try_rfft256(Ctx& c, const C* tw, const C* in, TO* out, B* b0) {
     if constexpr (!std::is_same<C, float>::value) return false;
     else {
         const int N = DIR == 0 ? c.g.nx : c.g.ny;
@@ -652,7 +652,7 @@ static bool try_rfft256(Ctx& c, const C* tw, const C* in, TO* out, const B* b0) 
 }
 
 template <int DIR, int INV, int N, int LPB, class C, class TO, int ACC, int RES, class B>
-static bool try_rfft_n(Ctx& c, const C* tw, const C* in, TO* out, int n, int nlines, const B* b0) {
+static bool try_rfft_n(Ctx& c, const C* tw, const C* in, TO* out, int n, int nlines, B* b0) {
     if (n != N || nlines % LPB != 0) return false;
     hipLaunchKernelGGL((k_rfft_n<DIR, INV, N, LPB, C, TO, ACC, RES, B>), dim3((nlines / LPB) * c.g.nz),
                        dim3(RfftShape<N, LPB>::NTHR), 0, c.stream, c.g, tw, in, out, b0);
@@ -661,7 +661,7 @@ static bool try_rfft_n(Ctx& c, const C* tw, const C* in, TO* out, int n, int nli
 
 // lines per block: along x 4; along y enough lines for 64-byte row segments
 template <int DIR, int INV, class C, class TO, int ACC, int RES, class B>
-static bool rfft_fixed(Ctx& c, const C* tw, const C* in, TO* out, int n, int nlines, const B* b0) {
+static bool rfft_fixed(Ctx& c, const C* tw, const C* in, TO* out, int n, int nlines, B* b0) {
     constexpr int L = DIR == 0 ? 4 : (sizeof(C) == 4 ? 16 : 8);
     constexpr int L512 = L > 8 ? 8 : 4, L1024 = L > 8 ? 4 : 2;
     return try_rfft_n<DIR, INV, 32, L, C, TO, ACC, RES, B>(c, tw, in, out, n, nlines, b0) ||
@@ -676,7 +676,7 @@ static bool rfft_fixed(Ctx& c, const C* tw, const C* in, TO* out, int n, int nli
 // rhs - L b0 as its input (in = rhs); false means no fixed-size kernel covers the
 // shape and the caller forms the residual itself.
 template <int DIR, int INV, class C, class TO, int ACC, int RES = 0, class B = float>
-static int rfft_launch(Ctx& c, const C* tw, const C* in, TO* out, const B* b0 = nullptr) {
+static int rfft_launch(Ctx& c, const C* tw, const C* in, TO* out, B* b0 = nullptr) {
     const int N = DIR == 0 ? c.g.nx : c.g.ny;
     const int nlines = DIR == 0 ? c.g.ny : c.g.nx;
     if (try_rfft256<DIR, INV, C, TO, ACC, RES, B>(c, tw, in, out, b0) ||
@@ -1059,16 +1059,17 @@ static int solve_fp64(Ctx& c) {
 
 // One float-transform solve.  ACC = 0: of L p0 = rhs, the solution written as float to
 // p0 (the transforms produce float values, so nothing is lost against a double
-// store).  ACC = 1: of L e = rhs - L b0, the residual formed in double inside the
-// first transform where a fixed-size kernel covers the line length and by k_residual
-// otherwise, and p = b0 + e written as double; b0 is the float solution of an ACC = 0
-// pass or the double solution of the previous time step.
-template <int ACC, class B = float>
-static int solve_fp32_pass(Ctx& c, const B* b0) {
+// store).  ACC = 1: of L e = rhs - L p0 for the float starting solution p0, the
+// residual formed in double inside the first transform where a fixed-size kernel
+// covers the line length and by k_residual otherwise; the last transform writes
+// p = p0 + e as double and p rounded to float into p0.
+template <int ACC>
+static int solve_fp32_pass(Ctx& c) {
     const float* twx = as<float>(c.tw_x_f);
     const float* twy = as<float>(c.tw_y_f);
     const float* rhs = as<float>(c.frhs);
     float* w = as<float>(c.pf);
+    float* p0 = as<float>(c.pf0);
     int rc;
     if (!ACC) {
         ProfScope ps(c, "k_rfft_x_fwd");
@@ -1076,10 +1077,11 @@ static int solve_fp32_pass(Ctx& c, const B* b0) {
         if (rc) return rc;
     } else {
         ProfScope ps(c, "k_rfft_x_fwd_res");
-        rc = rfft_launch<0, 0, float, float, 0, 1, B>(c, twx, rhs, w, b0);
+        rc = rfft_launch<0, 0, float, float, 0, 1>(c, twx, rhs, w, p0);
         if (rc < 0) {
             dim3 blk(64, 4, 1);
-            hipLaunchKernelGGL((k_residual<float, B>), grid3(c.g, c.g.nz, blk), blk, 0, c.stream, c.g, rhs, b0, w);
+            hipLaunchKernelGGL((k_residual<float, float>), grid3(c.g, c.g.nz, blk), blk, 0, c.stream, c.g, rhs,
+                               (const float*)p0, w);
             HIP_OK(hipGetLastError());
             rc = rfft_launch<0, 0, float, float, 0>(c, twx, w, w);
         }
@@ -1098,26 +1100,33 @@ static int solve_fp32_pass(Ctx& c, const B* b0) {
         if (rc) return rc;
     }
     ProfScope ps(c, "k_rfft_x_inv");
-    if (!ACC) return rfft_launch<0, 1, float, float, 0>(c, twx, w, as<float>(c.pf0));
-    return rfft_launch<0, 1, float, double, 1, 0, B>(c, twx, w, c.p, b0);
+    if (!ACC) return rfft_launch<0, 1, float, float, 0>(c, twx, w, p0);
+    return rfft_launch<0, 1, float, double, 1>(c, twx, w, c.p, p0);
 }
 
-// One step of iterative refinement.  Its starting solution is, where there is one, the
-// raw solution of the previous time step (guess): one pass then solves for the
-// change of the pressure over the step, whose float error is that of the change.
-// Otherwise (the first step, after a restart or an upload of the pressures, and for
-// a right-hand side given through the API) a float solve provides it first.
-static int solve_mixed(Ctx& c, const double* guess) {
-    if (guess) return solve_fp32_pass<1, double>(c, guess);
-    int rc = solve_fp32_pass<0>(c, (const float*)nullptr);
-    if (rc) return rc;
-    return solve_fp32_pass<1>(c, (const float*)as<float>(c.pf0));
+// One step of iterative refinement from the float starting solution p0 (pf0).  Inside
+// the time step p0 is, where there is one, the previous step's solution rounded to
+// float, which the last refinement left in pf0: one pass then solves for the change of
+// the pressure over the step (and the float rounding of the start), whose float error
+// is that of the change.  Otherwise (the first step, after a restart or an upload of
+// the pressures, after a solve through the API, and for every API solve) a float solve
+// provides p0 first.
+static int solve_mixed(Ctx& c, bool from_previous) {
+    if (!from_previous) {
+        int rc = solve_fp32_pass<0>(c);
+        if (rc) return rc;
+    }
+    return solve_fp32_pass<1>(c);
 }
 
 int launch_solve(Ctx& c, bool warm) {
     if (!c.fp32) return solve_fp64(c);
-    // under a lazy rotation the pnew buffer holds the previous step's raw solution
-    return solve_mixed(c, warm && c.p_lazy ? (const double*)c.pnew : nullptr);
+    // under a lazy rotation the pnew buffer holds the raw solution of the last solve,
+    // and pf0 that solution rounded to float when that solve was the time step's
+    const bool from_previous = warm && c.p_lazy && c.pf0_step;
+    const int rc = solve_mixed(c, from_previous);
+    c.pf0_step = warm && rc == ICLS_OK;
+    return rc;
 }
 
 // ---------------------------------------------------------------- solver2d_cos

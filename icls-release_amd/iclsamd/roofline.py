@@ -38,15 +38,15 @@ KERNEL_ARRAYS = {
 
 # the Poisson passes differ by mode: FP64 works in place on p (P class); the mixed
 # passes read and write the float work array.  Inside a mixed time step the one
-# refinement pass starts from the previous step's double solution p0: its forward x
-# transform forms rhs - L p0 itself (R rhs, p0; W w: 16) and its inverse x transform
-# writes p0 + e (R w, p0; W p: 20).  Where no previous solution is held (the first
-# step, an API solve) a float pass comes first: its forward x transform reads rhs and
-# writes w (8), its inverse writes the float solution (8), and the refinement then
-# reads that float solution (12 and 16); those launches are not in the steady-state
-# step.  k_residual runs only for line lengths without a fixed-size transform.
+# refinement pass starts from the previous step's solution rounded to float, p0: its
+# forward x transform forms rhs - L p0 itself (R rhs, p0; W w: 12) and its inverse x
+# transform writes p0 + e as double and, rounded, back into p0 (R w, p0; W p, p0: 20).
+# Where no previous solution is held (the first step, an API solve) a float pass comes
+# first: its forward x transform reads rhs and writes w (8) and its inverse writes p0
+# (8); those launches are not in the steady-state step.  k_residual runs only for line
+# lengths without a fixed-size transform.
 _POISSON_FP64 = {"k_rfft_x_fwd": 16, "k_rfft_y_fwd": 16, "k_rfft_y_inv": 16, "k_rfft_x_inv": 16, "k_thomas": 16}
-_POISSON_MIXED = {"k_rfft_x_fwd": 8, "k_rfft_x_fwd_res": 16, "k_rfft_y_fwd": 8, "k_rfft_y_inv": 8,
+_POISSON_MIXED = {"k_rfft_x_fwd": 8, "k_rfft_x_fwd_res": 12, "k_rfft_y_fwd": 8, "k_rfft_y_inv": 8,
                   "k_rfft_x_inv": 20, "k_thomas": 8}
 
 
