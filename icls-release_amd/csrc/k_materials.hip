@@ -345,13 +345,22 @@ __global__ void k_infl_sums(Grid g, const double* __restrict__ flag, const T* __
 // restores the loss.  The reference's normalisation (mass_correction = 1) restores
 // about int(delta^2) dphi = 3/(4 eps) = 1/(2 dz) times the loss.
 // hoff: cells of the grid outside the swept window (Heaviside 1 there), 0 for a
-// whole-grid sweep.
-__global__ void k_infl_scalars(int l, int lmax, int norm, double ncell, double hoff, double dx, double dy,
-                               double dz, double lxyz, double inflate_dt, double s1,
-                               const double* __restrict__ sums, double* dscal) {
-    if (threadIdx.x != 0 || dscal[DS_INFL_FLAG] == 0.) return;
-    double area = (norm ? sums[2] : sums[0]) * dx * dy * dz;
-    double vol = 1. - (sums[1] + hoff) / ncell;
+// whole-grid sweep.  The three sums of k_infl_sums are each folded as k_reduce_final
+// folds one array (the same order, the same results), then one thread forms the
+// scalars.
+__global__ void __launch_bounds__(RED_FINAL_THREADS)
+k_infl_final(const double* __restrict__ part, int nb, int l, int lmax, int norm, double ncell, double hoff, double dx,
+             double dy, double dz, double lxyz, double inflate_dt, double s1, double* __restrict__ sums,
+             double* dscal) {
+    if (dscal[DS_INFL_FLAG] == 0.) return;
+    __shared__ double smem[16];
+    double r[3];
+#pragma unroll
+    for (int t = 0; t < 3; t++) r[t] = block_reduce(fold_partials<OpSum>(part + (long)t * nb, nb, 0.), OpSum(), smem);
+    if (threadIdx.x != 0) return;
+    for (int t = 0; t < 3; t++) sums[t] = r[t];
+    double area = (norm ? r[2] : r[0]) * dx * dy * dz;
+    double vol = 1. - (r[1] + hoff) / ncell;
     dscal[DS_COUNT_FIXED + l] = vol;
     double loss_rate = lxyz * (vol - dscal[DS_COUNT_FIXED + lmax + l]) / inflate_dt;
     double temp = loss_rate / area;
@@ -360,7 +369,9 @@ __global__ void k_infl_scalars(int l, int lmax, int norm, double ncell, double h
     dscal[DS_MAXCHK] = temp / s1;
 }
 
-// u_l = -temp*(dheav(i+1)-dheav(i))/dx etc. (levelset.f90:527-535) after boundp(dheav)
+// u_l = -temp*(dheav(i+1)-dheav(i))/dx etc. (levelset.f90:527-535).  boundp(dheav) is
+// read in place: dheav(k1) = dheav(kmax) at the top layer, whose w_l is then zero, the
+// value bounduvw gives it (bound.f90:26-47).
 template <class T>
 __global__ void k_infl_face_vel(Grid g, const double* __restrict__ dscal, const T* __restrict__ dheav,
                                 T* __restrict__ ul, T* __restrict__ vl, T* __restrict__ wl) {
@@ -372,10 +383,11 @@ __global__ void k_infl_face_vel(Grid g, const double* __restrict__ dscal, const 
     T h = dheav[q];
     ul[q] = -temp * (dheav[gidx(g, wrap(i0 + 1, g.nx), j0, k)] - h) / T(g.dx);
     vl[q] = -temp * (dheav[gidx(g, i0, wrap(j0 + 1, g.ny), k)] - h) / T(g.dy);
-    wl[q] = -temp * (dheav[gidx(g, i0, j0, k + 1)] - h) / T(g.dz);
+    wl[q] = k < g.nz ? -temp * (dheav[gidx(g, i0, j0, k + 1)] - h) / T(g.dz) : T(0.);
 }
 
-// u_lc = (u_l(i-1)+u_l(i))/2*kappa (levelset.f90:541-549)
+// u_lc = (u_l(i-1)+u_l(i))/2*kappa (levelset.f90:541-549); below the first layer w_l is
+// the wall value zero that bounduvw gives it
 template <class T>
 __global__ void k_infl_center_vel(Grid g, const double* __restrict__ dscal, const T* __restrict__ kappa,
                                   const T* __restrict__ ul, const T* __restrict__ vl, const T* __restrict__ wl,
@@ -387,7 +399,7 @@ __global__ void k_infl_center_vel(Grid g, const double* __restrict__ dscal, cons
     T kap = kappa[q];
     uc[q] = (ul[gidx(g, wrap(i0 - 1, g.nx), j0, k)] + ul[q]) / T(2.) * kap;
     vc[q] = (vl[gidx(g, i0, wrap(j0 - 1, g.ny), k)] + vl[q]) / T(2.) * kap;
-    wc[q] = (wl[gidx(g, i0, j0, k - 1)] + wl[q]) / T(2.) * kap;
+    wc[q] = ((k > 1 ? wl[gidx(g, i0, j0, k - 1)] : T(0.)) + wl[q]) / T(2.) * kap;
 }
 
 int launch_inflation_prepare(Ctx& c, int l, int istep) {
@@ -412,26 +424,16 @@ static int inflation_velocity(Ctx& c, int l) {
     hipLaunchKernelGGL(k_infl_sums<T>, grd, blk, 0, c.stream, c.g, flag, as<T>(c.lvset[l]), as<T>(c.curv2),
                        as<T>(c.normal_m), (T)s1, (T)c.pi, as<T>(c.dheav), c.dpart, nb);
     double* sums = c.dscal + DS_COUNT_FIXED + 2 * c.lmax;
-    hipLaunchKernelGGL(k_reduce_final_pred<OpSum>, dim3(1), dim3(RED_FINAL_THREADS), 0, c.stream, (const double*)c.dpart, nb, 0.,
-                       sums, flag);
-    hipLaunchKernelGGL(k_reduce_final_pred<OpSum>, dim3(1), dim3(RED_FINAL_THREADS), 0, c.stream, (const double*)(c.dpart + nb), nb,
-                       0., sums + 1, flag);
-    hipLaunchKernelGGL(k_reduce_final_pred<OpSum>, dim3(1), dim3(RED_FINAL_THREADS), 0, c.stream,
-                       (const double*)(c.dpart + 2 * nb), nb, 0., sums + 2, flag);
     double ncell = 1. * c.ncell_all;
     double hoff = (double)(c.ncell_all - (long)c.g.nx * c.g.ny * c.g.nz);
     double lxyz = c.P.lx * c.P.ly * c.P.lz;
     double inflate_dt = c.P.inflate_step * c.dt;
-    hipLaunchKernelGGL(k_infl_scalars, dim3(1), dim3(64), 0, c.stream, l, c.lmax, c.P.mass_correction == 2 ? 1 : 0,
-                       ncell, hoff, c.g.dx, c.g.dy, c.g.dz, lxyz, inflate_dt, s1, (const double*)sums, c.dscal);
-    HIP_OK(hipGetLastError());
-    int rc = launch_bc_p_t_pred(c, c.dheav, flag);
-    if (rc) return rc;
+    hipLaunchKernelGGL(k_infl_final, dim3(1), dim3(RED_FINAL_THREADS), 0, c.stream, (const double*)c.dpart, nb, l,
+                       c.lmax, c.P.mass_correction == 2 ? 1 : 0, ncell, hoff, c.g.dx, c.g.dy, c.g.dz, lxyz,
+                       inflate_dt, s1, sums, c.dscal);
     hipLaunchKernelGGL(k_infl_face_vel<T>, grid3(c.g, c.g.nz, blk), blk, 0, c.stream, c.g, (const double*)c.dscal,
                        as<T>(c.dheav), as<T>(c.u_l), as<T>(c.v_l), as<T>(c.w_l));
     HIP_OK(hipGetLastError());
-    rc = launch_bc_uvw_pred(c, c.u_l, c.v_l, c.w_l, flag);
-    if (rc) return rc;
     hipLaunchKernelGGL(k_infl_center_vel<T>, grid3(c.g, c.g.nz, blk), blk, 0, c.stream, c.g, (const double*)c.dscal,
                        as<T>(c.curv2), as<T>(c.u_l), as<T>(c.v_l), as<T>(c.w_l), as<T>(c.ucc_l), as<T>(c.vcc_l),
                        as<T>(c.wcc_l));
