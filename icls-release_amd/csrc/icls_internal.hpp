@@ -379,6 +379,8 @@ int launch_solve(Ctx& c, bool warm);
 // solver2d_cos: p -> solution with y rule 1 Walls, 2 InOut, 3 OutOut, in double in every mode
 int launch_solve_cos(Ctx& c, int bcy);
 int launch_pressure_materialize(Ctx& c);
+// pressure norm, lazy rotation and NaN check of a time step's end (main.f90:164-185, 207)
+int launch_step_end(Ctx& c, int first_step);
 int launch_jump_densify(Ctx& c, bool old);   // zero the unmasked jumps, mask every cell
 int launch_pressure_release_p(Ctx& c);   // the work array p takes the raw solution back
 int thomas_seg_len(int nz);
@@ -393,7 +395,7 @@ int mlp_adam(Ctx& c, float lr);                                            // af
 int mlp_samples(Ctx& c, long n, unsigned long long seed, float r_min, float r_max, float gate, uint16_t* x_bf16,
                 float* t, float* fd);
 int launch_mlp_reset_list(Ctx& c);
-int launch_mlp_reset_total(Ctx& c);
+int launch_mlp_reset_list_total(Ctx& c);   // both counters in one launch
 int launch_mlp_curv(Ctx& c, const double* phi, const double* kappa_fd, double* kappa_out);
 // the closure over the cells listed by a batched curvature launch (whole-grid indices,
 // each with its window in the batch wb)

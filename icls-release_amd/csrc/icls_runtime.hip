@@ -940,13 +940,14 @@ static int gfm_impl(Ctx& c, bool fuse_fillps, bool store_pjump) {
     rotate_jumps(c);
     c.p_in_pnew = false;   // fillps writes p
     const bool learned = c.learned && c.mlp.ready;
-    if (learned) TRY(launch_mlp_reset_total(c));
+
     // get_curvature, re_curvature, and for the level sets after the first their jumps
     // added inside their windows.  In the learned mode the closure takes the under-
     // resolved cells of the smoothed field: their curvature is the cell's own least-
     // squares value plus the network's correction toward the 4x finer grid's.
     auto curvature = [&](int l) -> int {
-        if (learned) TRY(launch_mlp_reset_list(c));
+        // the first level set's call also clears the pass's total of listed cells
+        if (learned) TRY(l == 0 ? launch_mlp_reset_list_total(c) : launch_mlp_reset_list(c));
         TRY(launch_curvature(c, c.lvset[l], c.curv, nullptr, nullptr, learned));
         TRY(launch_recurvature(c, c.lvset[l], c.curv, c.curv2));
         if (learned) TRY(launch_mlp_curv(c, c.lvset[l], c.curv, c.curv2));
@@ -1478,11 +1479,10 @@ int32_t icls_step(int32_t istep, int32_t begin) {
     TRY(launch_bc_p(c, c.p));                              // main.f90:155
     TRY(launch_correc(c));                                 // main.f90:159-160
     TRY(launch_bc_uvw(c, c.unew, c.vnew, c.wnew));
-    TRY(launch_pressure_norm(c));                          // main.f90:164-185
-    TRY(launch_pressure_rotate(c, istep == begin + 1));
+    // main.f90:164-185 (norm and rotation) and main.f90:207 (NaN, a sticky flag);
     // main.f90:190: the cell-centre velocities are averaged from unew/vnew/wnew where
     // they are consumed (the next step's advection), and on download
-    TRY(launch_nan_check(c));                              // main.f90:207 (sticky flag)
+    TRY(launch_step_end(c, istep == begin + 1));
     if (c.P.cfl > 0. && istep % c.P.ioutchk == 0) TRY(adapt_dt(c, true));   // main.f90:214
     return ICLS_OK;
 }

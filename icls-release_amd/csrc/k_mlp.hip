@@ -483,6 +483,9 @@ k_mlp_curv(Grid g, const int* __restrict__ list, const int* __restrict__ list_t,
 __global__ void k_zero_int(int* p) {
     if (threadIdx.x == 0) p[0] = 0;
 }
+__global__ void k_zero_int2(int* p, int* q) {
+    if (threadIdx.x == 0) { p[0] = 0; q[0] = 0; }
+}
 
 // ---------------------------------------------------------------- host side
 void pseudo_inverse(double dx, double dy, double dz, double* out);
@@ -636,10 +639,11 @@ int launch_mlp_curv_batch(Ctx& c, const WinBatch& wb, const double* kappa_fd, do
     return mlp_curv_launch<true>(c, wb, nullptr, kappa_fd, kappa_out);
 }
 
-int launch_mlp_reset_total(Ctx& c) {
-    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, c.stream, c.mlp.total);
+int launch_mlp_reset_list_total(Ctx& c) {
+    hipLaunchKernelGGL(k_zero_int2, dim3(1), dim3(64), 0, c.stream, c.mlp.count, c.mlp.total);
     HIP_OK(hipGetLastError());
     return ICLS_OK;
 }
+
 
 }  // namespace icls

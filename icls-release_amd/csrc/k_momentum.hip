@@ -682,9 +682,7 @@ __global__ void k_pressure_lazy_scalars(double* dscal) {
     }
 }
 
-int launch_pressure_rotate(Ctx& c, int first_step) {
-    hipLaunchKernelGGL(k_pressure_lazy_scalars, dim3(1), dim3(64), 0, c.stream, c.dscal);
-    HIP_OK(hipGetLastError());
+static void rotate_buffers(Ctx& c, int first_step) {
     double* a = c.p;
     c.p = c.pold;
     c.pold = c.pnew;
@@ -694,6 +692,47 @@ int launch_pressure_rotate(Ctx& c, int first_step) {
     c.p_lazy = true;
     c.p_first = first_step != 0;
     c.p_in_pnew = true;
+}
+
+int launch_pressure_rotate(Ctx& c, int first_step) {
+    hipLaunchKernelGGL(k_pressure_lazy_scalars, dim3(1), dim3(64), 0, c.stream, c.dscal);
+    HIP_OK(hipGetLastError());
+    rotate_buffers(c, first_step);
+    return ICLS_OK;
+}
+
+// The end of a time step in one single-block launch after k_plane_sum: the norm as
+// k_norm_final folds it (the same order), the lazy rotation's scalars as
+// k_pressure_lazy_scalars sets them, and the NaN flag of k_nan_check on the new v.
+template <class T>
+__global__ void k_step_end(Grid g, const double* __restrict__ part, int n, double denom, double* dscal,
+                           const T* __restrict__ v) {
+    __shared__ double smem[16];
+    double acc = 0.;
+    for (int q = threadIdx.x; q < n; q += blockDim.x) acc += part[q];
+    const double r = block_reduce(acc, OpSum(), smem);
+    if (threadIdx.x == 0) {
+        const double norm = r / denom;
+        dscal[DS_NORM] = norm;
+        dscal[DS_PN_B] = dscal[DS_NORM_OLD];
+        dscal[DS_PN_A] = norm;
+        dscal[DS_NORM_OLD] = norm;
+        if (isnan(v[gidx(g, 0, 0, 1)])) dscal[DS_NAN_FLAG] = 1.;
+    }
+}
+
+int launch_step_end(Ctx& c, int first_step) {
+    ProfScope ps(c, "k_plane_sum");
+    const int nb = c.g.ny + 2;
+    hipLaunchKernelGGL(k_plane_sum, dim3(nb), dim3(256), 0, c.stream, c.g, (const double*)c.p, c.dpart);
+    if (c.fp32)
+        hipLaunchKernelGGL(k_step_end<float>, dim3(1), dim3(256), 0, c.stream, c.g, (const double*)c.dpart, nb,
+                           1. * c.g.nx * c.g.ny, c.dscal, as<float>(c.vnew));
+    else
+        hipLaunchKernelGGL(k_step_end<double>, dim3(1), dim3(256), 0, c.stream, c.g, (const double*)c.dpart, nb,
+                           1. * c.g.nx * c.g.ny, c.dscal, (const double*)c.vnew);
+    HIP_OK(hipGetLastError());
+    rotate_buffers(c, first_step);
     return ICLS_OK;
 }
 
