@@ -84,8 +84,9 @@ struct Mlp {
 };
 
 // The part of the grid one level set is stepped on: cells ox .. ox+wx-1 in x (and
-// likewise y), layers oz+1 .. oz+wz in z.  wx and wy are powers of two, and either the
-// window does not cross the periodic boundary or it spans the whole grid.
+// likewise y), layers oz+1 .. oz+wz in z.  wx is a power of two, wy a multiple of 8,
+// and either the window does not cross the periodic boundary or it spans the whole
+// grid.
 struct LsWindow {
     int ox = 0, oy = 0, oz = 0, wx = 0, wy = 0, wz = 0;
 };

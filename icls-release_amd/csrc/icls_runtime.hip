@@ -701,11 +701,13 @@ struct WindowScope {
 static int pow2_at_least(int v) { int p = 1; while (p < v) p <<= 1; return p; }
 
 // One periodic axis: cover global cells lo .. hi (lo may be < 0 or hi >= n when the
-// drop crosses the boundary) with a power-of-two range that does not cross it, else
-// with the whole axis.
-static void window_axis_periodic(int lo, int hi, int n, int& o, int& w) {
+// drop crosses the boundary) with a range that does not cross it, else with the whole
+// axis.  x ranges are powers of two (the z-marching kernels wrap x with a mask, and
+// their tiles are 64 cells wide); y ranges are multiples of 8, which on drops of radius
+// 3 .. 20 cells cut the cells of the windows by a quarter against powers of two.
+static void window_axis_periodic(int lo, int hi, int n, bool pow2_range, int& o, int& w) {
     const int need = hi - lo + 1;
-    w = pow2_at_least(need);
+    w = pow2_range ? pow2_at_least(need) : (need + 7) / 8 * 8;
     if (lo < 0 || hi > n - 1 || w >= n) { o = 0; w = n; return; }
     o = lo - (w - need) / 2;
     if (o < 0) o = 0;
@@ -758,9 +760,9 @@ static int update_windows(Ctx& c) {
         }
         LsWindow n;
         if (w.wx == G.nx && (lo[0] < m || hi[0] > G.nx - 1 - m)) { n.ox = 0; n.wx = G.nx; }
-        else window_axis_periodic(w.ox + lo[0] - m, w.ox + hi[0] + m, G.nx, n.ox, n.wx);
+        else window_axis_periodic(w.ox + lo[0] - m, w.ox + hi[0] + m, G.nx, true, n.ox, n.wx);
         if (w.wy == G.ny && (lo[1] < m || hi[1] > G.ny - 1 - m)) { n.oy = 0; n.wy = G.ny; }
-        else window_axis_periodic(w.oy + lo[1] - m, w.oy + hi[1] + m, G.ny, n.oy, n.wy);
+        else window_axis_periodic(w.oy + lo[1] - m, w.oy + hi[1] + m, G.ny, false, n.oy, n.wy);
         window_axis_z(w.oz + lo[2] - m, w.oz + hi[2] + m, G.nz, n.oz, n.wz);
         w = n;
     }

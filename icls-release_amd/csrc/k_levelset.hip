@@ -54,16 +54,16 @@ __device__ __forceinline__ void tube_zm_body(const Grid& g, int bid, int TZ, T N
     const int x0 = bx * BX, y0 = by * BY, kb = 1 + bz * TZ;
     const int ke = kb + TZ - 1 < g.nz ? kb + TZ - 1 : g.nz;
     const int tx = threadIdx.x, ty = threadIdx.y, tid = tx + BX * ty;
-    const int xm = g.nx - 1, ym = g.ny - 1;
+    const int xm = g.nx - 1;   // nx is a power of two; ny need not be (a window's)
     // a thread past the end of x or y takes the wrapped cell (same values, same stores)
-    const int i = (x0 + tx) & xm, j = (y0 + ty) & ym;
+    const int i = (x0 + tx) & xm, j = wrap(y0 + ty, g.ny);
     long poff[TNLD];
 #pragma unroll
     for (int r = 0; r < TNLD; r++) {
         int e = tid + r * BX * BY;
         if (e >= TTP) e = TTP - 1;
         const int lx = e % TTW - TH, ly = e / TTW - TH;
-        poff[r] = (long)((x0 + lx) & xm) + g.sy * (long)((y0 + ly) & ym);
+        poff[r] = (long)((x0 + lx) & xm) + g.sy * (long)wrap(y0 + ly, g.ny);
     }
     T nxt[TPD][TNLD];
     auto fetch = [&](int k, T* dst) {
@@ -452,14 +452,14 @@ __device__ __forceinline__ void reinit_zm_body(const Grid& g, int bid, int RZ, T
     const int ke = kb + RZ - 1 < g.nz ? kb + RZ - 1 : g.nz;
     const int tx = threadIdx.x, ty = threadIdx.y, tid = tx + BX * ty;
     if (STAGE == 1 && tile == 0 && tid == 0) res[0] = 0.;
-    const int xm = g.nx - 1, ym = g.ny - 1;
+    const int xm = g.nx - 1;   // nx is a power of two; ny need not be (a window's)
     long poff[RNLD];
 #pragma unroll
     for (int r = 0; r < RNLD; r++) {
         int e = tid + r * BX * BY;
         if (e >= RTP) e = RTP - 1;
         const int lx = e % RTW - RH, ly = e / RTW - RH;
-        poff[r] = (long)((x0 + lx) & xm) + g.sy * (long)((y0 + ly) & ym);
+        poff[r] = (long)((x0 + lx) & xm) + g.sy * (long)wrap(y0 + ly, g.ny);
     }
     // planes travel global -> registers -> LDS; the register queue holds RPD planes so
     // that a plane's loads are issued RPD steps before the step that stores it
@@ -520,7 +520,7 @@ __device__ __forceinline__ void reinit_zm_body(const Grid& g, int bid, int RZ, T
     int jr[RR], cr[RR];
 #pragma unroll
     for (int rr = 0; rr < RR; rr++) {
-        jr[rr] = (y0 + ty + rr * BY) & ym;
+        jr[rr] = wrap(y0 + ty + rr * BY, g.ny);
         cr[rr] = (ty + rr * BY + RH) * RTW + tx + RH;
     }
     const T dz = T(g.dz), dtau = T(0.5) * dz;
