@@ -36,13 +36,14 @@ static constexpr int BX = 64, BY = 4;
 // the linear extrapolation of the clamped layers next to them.
 //
 // The construction marches in z over a 64 x 4 tile: the input passes through a
-// ring of five LDS planes (k-2 .. k+2) with a two-cell x/y halo, so the 13-point scan
-// reads LDS and every input element is fetched once per block.  Chunks of TZ layers;
+// ring of eight LDS planes with a two-cell x/y halo (the five planes k-2 .. k+2 of
+// step k, and free slots for the next planes, so a step needs one barrier), the
+// 13-point scan reads LDS and every input element is fetched once per block.  Chunks of TZ layers;
 // the first and last chunk also write the ghost layers.  TPD planes are loaded ahead
 // (unconditional loads, clamped to the chunk, as in the re-initialisation kernel).
 // Chunks of TZ_GRID layers on the whole grid, of TZ_WINDOW inside a level-set window.
 constexpr int TZ_GRID = 32, TZ_WINDOW = 8, TH = 2, TTW = BX + 2 * TH, TTH = BY + 2 * TH, TTP = TTW * TTH;
-constexpr int TNLD = (TTP + BX * BY - 1) / (BX * BY), TNS = 2 * TH + 1;
+constexpr int TNLD = (TTP + BX * BY - 1) / (BX * BY), TNS = 8;   // ring slots: 5 planes of a step and the next
 
 template <class T, int TPD>
 __device__ __forceinline__ void tube_zm_body(const Grid& g, int bid, int TZ, T NB2, const T* __restrict__ in,
@@ -71,7 +72,7 @@ __device__ __forceinline__ void tube_zm_body(const Grid& g, int bid, int TZ, T N
 #pragma unroll
         for (int r = 0; r < TNLD; r++) dst[r] = in[base + poff[r]];
     };
-    auto slot_of = [&](int k) { return (k - kb + 2 * TNS) % TNS; };
+    auto slot_of = [&](int k) { return (k - kb + 2 * TNS) & (TNS - 1); };
     auto stash_plane = [&](int k, const T* src) {
         T* dst = ring[slot_of(k)];
 #pragma unroll
@@ -114,7 +115,8 @@ __device__ __forceinline__ void tube_zm_body(const Grid& g, int bid, int TZ, T N
             const long q = gidx(g, i, j, k);
             out[q] = clampv(v);
             mask[q] = m ? 1 : 0;
-            __syncthreads();
+            // no barrier here: the next step stores plane k + 3 into the slot of plane
+            // k - 5, which no thread reads after the barrier above
         }
     }
     if (bz == ntz - 1) {
@@ -398,9 +400,11 @@ int launch_adv_stage(Ctx& c, int scheme, int stage, const double* phi0, const do
 // k_bc_ls.  Stage 1 resets the residual.  Stage 3 may run in place (out == phi0).
 //
 // The kernel marches in z over a 64 x 4 tile of columns.  phi_s passes through
-// a ring of seven LDS planes (k-3 .. k+3) of the tile with a three-cell x/y halo, so
-// the 19-point WENO stencils read LDS and each phi_s element is fetched once per
-// block; the next plane is loaded into registers while the current one is computed.
+// a ring of eight LDS planes of the tile with a three-cell x/y halo: the seven planes
+// k-3 .. k+3 of step k and a free slot, which takes plane k+4 at the start of step
+// k+1 while slower wavefronts may still read k-3, so a step needs one barrier.  The
+// 19-point WENO stencils read LDS and each phi_s element is fetched once per block;
+// the next planes are loaded into registers while the current one is computed.
 // Uniform windows: when a plane is stored, the block also records whether all its
 // values are one value.  Where the seven planes of a step are one value v (the
 // clamped far field, most of the domain) every stencil is flat, all six WENO5
@@ -420,7 +424,7 @@ constexpr int RR = ICLS_REINIT_RR;                       // rows per thread
 constexpr int RTY = BY * RR;                             // tile rows
 constexpr int RTW = BX + 2 * RH, RTH = RTY + 2 * RH, RTP = RTW * RTH;
 constexpr int RNLD = (RTP + BX * BY - 1) / (BX * BY);
-constexpr int RNS = 2 * RH + 1;                           // planes in the ring
+constexpr int RNS = 8;                                    // ring slots: the 7 planes of a step and the next
 constexpr int RPD_MAX = ICLS_REINIT_PD;                   // planes in flight ahead of the ring
 
 template <class T, int STAGE, bool P2>
@@ -469,7 +473,7 @@ __device__ __forceinline__ void reinit_zm_body(const Grid& g, int bid, int RZ, T
 #pragma unroll
         for (int r = 0; r < RNLD; r++) dst[r] = phis[base + poff[r]];
     };
-    auto slot_of = [&](int k) { return (k - kb + 2 * RNS) % RNS; };
+    auto slot_of = [&](int k) { return (k - kb + 2 * RNS) & (RNS - 1); };
     auto stash_plane = [&](int k, const T* src) {
         const int sl = slot_of(k);
         T* dst = ring[sl];
@@ -580,7 +584,7 @@ __device__ __forceinline__ void reinit_zm_body(const Grid& g, int bid, int RZ, T
                 const T p0 = p0s[rr][tid];
                 const long q = gidx(g, i, j, k);
                 const T* P0 = ring[slot_of(k)];
-                const bool flat = run >= RNS;   // planes k - 3 .. k + 3 are all rval
+                const bool flat = run >= 2 * RH + 1;   // planes k - 3 .. k + 3 are all rval
                 const T ps = flat ? rval : P0[c0];
                 T rhs = T(0.);
                 if (flat) {
@@ -650,7 +654,8 @@ __device__ __forceinline__ void reinit_zm_body(const Grid& g, int bid, int RZ, T
                     }
                 }
             }
-            __syncthreads();
+            // no barrier here: the next step stores plane k + 4 into the slot of plane
+            // k - 4, which no thread reads after the barrier above
         }
     }
     if (bz == ntz - 1)
