@@ -89,9 +89,13 @@ __device__ __forceinline__ void tube_zm_body(const Grid& g, int bid, int TZ, T N
         out[gidx(g, i, j, -1)] = clampv(col(-1));
         out[gidx(g, i, j, 0)] = T(2.) * clampv(col(1)) - clampv(col(2));
     }
-    for (int k = kb - TH; k < kb + TH; k++) {
-        fetch(k, nxt[0]);
-        stash_plane(k, nxt[0]);
+    {
+        // the four planes before the first step: all loads in flight before the first store
+        T pro[2 * TH][TNLD];
+#pragma unroll
+        for (int t = 0; t < 2 * TH; t++) fetch(kb - TH + t, pro[t]);
+#pragma unroll
+        for (int t = 0; t < 2 * TH; t++) stash_plane(kb - TH + t, pro[t]);
     }
 #pragma unroll
     for (int u = 0; u < TPD; u++) fetch(min(kb + u, ke) + TH, nxt[u]);
@@ -508,9 +512,13 @@ __device__ __forceinline__ void reinit_zm_body(const Grid& g, int bid, int RZ, T
         else if (ok) { run = 1; rval = v; }
         else run = 0;
     };
-    for (int k = kb - RH; k < kb + RH; k++) {
-        fetch(k, nxt[0]);
-        stash_plane(k, nxt[0]);
+    {
+        // the six planes before the first step: all loads in flight before the first store
+        T pro[2 * RH][RNLD];
+#pragma unroll
+        for (int t = 0; t < 2 * RH; t++) fetch(kb - RH + t, pro[t]);
+#pragma unroll
+        for (int t = 0; t < 2 * RH; t++) stash_plane(kb - RH + t, pro[t]);
     }
     __syncthreads();
     for (int k = kb - RH; k < kb + RH; k++) push(k);

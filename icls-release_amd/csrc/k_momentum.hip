@@ -174,9 +174,19 @@ k_momentum_zm(Grid g, MomParams m, const double* __restrict__ dscal, const T* __
             }
         }
     };
-    for (int k = kb - 1; k <= kb + 1; k++) {
-        fetch(k, nxt);
-        stash(k, nxt);
+    if constexpr (std::is_same<T, float>::value) {
+        // the three planes before the first step: all loads in flight before the first
+        // store (in FP64 the extra registers cost more than the overlap gains)
+        T pro[3][5][NLD];
+#pragma unroll
+        for (int t = 0; t < 3; t++) fetch(kb - 1 + t, pro[t]);
+#pragma unroll
+        for (int t = 0; t < 3; t++) stash(kb - 1 + t, pro[t]);
+    } else {
+        for (int k = kb - 1; k <= kb + 1; k++) {
+            fetch(k, nxt);
+            stash(k, nxt);
+        }
     }
     // pressure planes kb and kb + 1 (no further than ke, the last one read)
     for (int k = kb; k <= kb + 1 && k <= ke; k++) {
