@@ -495,6 +495,10 @@ __global__ void __launch_bounds__(512) k_rfft256_y(Grid g, const float* __restri
         }
 }
 
+struct cx4 {
+    float v[4];
+};
+
 // along x: 256 threads, 32 lines (j) x 8 slots; with RES (forward) the input is
 // rhs - L b0 formed on the fly
 template <int INV, class TO, int ACC, int RES, class B>
@@ -517,20 +521,36 @@ __global__ void __launch_bounds__(256) k_rfft256_x(Grid g, const float* __restri
     cx<float> v[16];
     if (RES) {
         // the residual line goes through LDS so that its stencils are not all in flight;
-        // the neighbour lines are addressed from their starts
+        // the neighbour lines are addressed from their starts.  A thread forms four
+        // consecutive cells from 16-byte loads of the line and of its four neighbour
+        // lines (and two loads for the x neighbours of the four), so a wavefront's loads
+        // cover whole 128-byte rows.
+        static_assert(std::is_same<B, float>::value, "the starting solution is float");
         float* fl = reinterpret_cast<float*>(xl);
-        const B* pc = b0 + q0;
-        const B* pyp = b0 + gidx(g, 0, wrap(j + 1, g.ny), k);
-        const B* pym = b0 + gidx(g, 0, wrap(j - 1, g.ny), k);
+        const float* pc = b0 + q0;
+        const float* pyp = b0 + gidx(g, 0, wrap(j + 1, g.ny), k);
+        const float* pym = b0 + gidx(g, 0, wrap(j - 1, g.ny), k);
         const bool zup = k < g.nz || k == 1, zdn = k > 1;
-        const B* pzp = pc + (zup ? g.sz : 0);
-        const B* pzm = pc - (zdn ? g.sz : 0);
+        const float* pzp = pc + (zup ? g.sz : 0);
+        const float* pzm = pc - (zdn ? g.sz : 0);
         const float* rl = in + q0;
-#pragma unroll 4
-        for (int x = s; x < N; x += 8) {
-            const double zp = (double)pzp[x], zm = (double)pzm[x];
-            fl[x] = (float)residual_of(g, k, (double)rl[x], (double)pc[x], (double)pc[(x + 1) & (N - 1)],
-                                       (double)pc[(x - 1) & (N - 1)], (double)pyp[x], (double)pym[x], zp, zm);
+        auto ld4 = [](const float* a) {
+            const float4 t = *reinterpret_cast<const float4*>(a);
+            return cx4{t.x, t.y, t.z, t.w};
+        };
+#pragma unroll 2
+        for (int it = 0; it < N / 32; it++) {
+            const int x = 4 * s + 32 * it;
+            const cx4 r4 = ld4(rl + x), c4 = ld4(pc + x), yp4 = ld4(pyp + x), ym4 = ld4(pym + x);
+            const cx4 zp4 = ld4(pzp + x), zm4 = ld4(pzm + x);
+            const float cl = pc[(x - 1) & (N - 1)], cr = pc[(x + 4) & (N - 1)];
+            const float cc[6] = {cl, c4.v[0], c4.v[1], c4.v[2], c4.v[3], cr};
+            float o4[4];
+#pragma unroll
+            for (int t = 0; t < 4; t++)
+                o4[t] = (float)residual_of(g, k, (double)r4.v[t], (double)cc[t + 1], (double)cc[t + 2], (double)cc[t],
+                                           (double)yp4.v[t], (double)ym4.v[t], (double)zp4.v[t], (double)zm4.v[t]);
+            *reinterpret_cast<float4*>(fl + x) = make_float4(o4[0], o4[1], o4[2], o4[3]);
         }
         __syncthreads();
 #pragma unroll
