@@ -719,10 +719,14 @@ static void reinit_dispatch_pd(Ctx& c, int rz, int stage, const double* phi0, co
 // to the XCDs in groups (XCD_DEAL), chunks of 8 and 16 layers were again slower than 32.
 // Inside a level-set window (a few tens of thousands of cells) 8-layer chunks: four
 // times the blocks of 32-layer chunks, and a quarter of each block's serial march.
-constexpr int REINIT_RZ = 32, REINIT_RZ_WINDOW = 8;
+// With one barrier per step and the prologue loads in flight together, 40, 52 and 64
+// layers ran the FP32 stages 2-3 in 0.172, 0.203 and 0.197 ms against 0.169 ms for 32,
+// and the FP64 ones in 0.298, 0.329 and 0.345 ms against 0.321 ms: FP64 (three resident
+// blocks per CU instead of five) takes 40.
+constexpr int REINIT_RZ = 32, REINIT_RZ_FP64 = 40, REINIT_RZ_WINDOW = 8;
 template <class T>
 static int reinit_rz(Ctx& c) {
-    const int rz = in_window(c) ? REINIT_RZ_WINDOW : REINIT_RZ;
+    const int rz = in_window(c) ? REINIT_RZ_WINDOW : (sizeof(T) == 4 ? REINIT_RZ : REINIT_RZ_FP64);
     return c.g.nz < rz ? c.g.nz : rz;
 }
 
@@ -855,7 +859,7 @@ static int reinit_batch_t(Ctx& c, const std::vector<int>& ls) {
     dim3 blk(BX, BY, 1);
     const T NB2 = (T)c.NB2;
     auto rz_of = [&](const LsWindow& w) {
-        const int rz = win_full(c, w) ? REINIT_RZ : REINIT_RZ_BATCH;
+        const int rz = win_full(c, w) ? (sizeof(T) == 4 ? REINIT_RZ : REINIT_RZ_FP64) : REINIT_RZ_BATCH;
         return w.wz < rz ? w.wz : rz;
     };
     // key: rz * 4 + 2 deep + wall
