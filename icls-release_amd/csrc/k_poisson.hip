@@ -648,6 +648,269 @@ __global__ void __launch_bounds__(256) k_rfft256_x(Grid g, const float* __restri
     }
 }
 
+// ---------------------------------------------------------------- register transform, 512
+// Float lines of length N = 512: the 256-point complex FFT of z_m = x_2m + i x_2m+1
+// over 16 slots, point m = s + 16 r in register r of slot s.  A 16-point DFT over r
+// in registers and the twiddles W256^(s k1); one LDS exchange after which slot s holds
+// the 16 values of k1 = s from all slots; a second 16-point DFT gives X_(s + 16 k2).
+// A further exchange pairs X_k with X_256-k for the even/odd split (inverse: the split
+// comes first, from global memory).  Along y a block is 16 waves over 64 consecutive i
+// (lane = line, wave = slot); along x 4 waves over 16 lines, 16 lanes per line.
+namespace r512 {
+constexpr int N = 512, M = 256;
+
+// exp(+2 pi i t / N) for 0 <= t < N from the half table
+__device__ __forceinline__ cx<float> twf(const cx<float>* tw, int t) {
+    return t < M ? tw[t] : cx<float>{-tw[t - M].re, -tw[t - M].im};
+}
+
+// the complex FFT from "v[r] = z_(s + 16 r)" to "v[4 (k2 % 4) + k2 / 4] = X_(s + 16 k2)";
+// xch(k1, t) addresses the exchange slot of (k1, slot t) of this thread's line
+template <int SG, class XCH>
+__device__ __forceinline__ void cfft256(cx<float>* v, int s, const cx<float>* tw, XCH xch) {
+    r16::dft16<SG>(v);
+#pragma unroll
+    for (int k1 = 0; k1 < 16; k1++) {
+        cx<float> val = v[4 * (k1 % 4) + k1 / 4];
+        if (k1) {
+            cx<float> w = twf(tw, 2 * s * k1);
+            if (SG < 0) w.im = -w.im;
+            val = cmul(val, w);
+        }
+        *xch(k1, s) = val;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < 16; t++) v[t] = *xch(s, t);
+    r16::dft16<SG>(v);
+    __syncthreads();   // the exchange buffer is free again
+}
+}  // namespace r512
+
+// along y: 1024 threads, 64 lines (i) x 16 slots; LDS 128 KB exchange + twiddles
+template <int INV, class TO, int ACC>
+__global__ void __launch_bounds__(1024) k_rfft512_y(Grid g, const float* __restrict__ twg, const float* in, TO* out) {
+    using namespace r512;
+    static_assert(!ACC, "only the x transform adds into its output");
+    __shared__ cx<float> xb[M * 64];
+    __shared__ cx<float> tw[M];
+    const int lane = threadIdx.x & 63, s = threadIdx.x >> 6;
+    const int nbl = g.nx / 64;
+    const int tile = xcd_tile(blockIdx.x, nbl * g.nz);
+    const int k = tile / nbl + 1, i = (tile % nbl) * 64 + lane;
+    for (int t = threadIdx.x; t < M; t += 1024) tw[t] = {twg[2 * t], twg[2 * t + 1]};
+    const float scale = (float)sqrt(1. / (double)N);
+    auto row = [&](int n) -> long { return gidx(g, i, n, k); };
+    cx<float> v[16];
+    if (!INV) {
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            const int m = s + 16 * r;
+            v[r] = {in[row(2 * m)], in[row(2 * m + 1)]};
+        }
+        __syncthreads();
+    } else {
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            const int kk = s + 16 * r;
+            cx<float> X, Y;
+            if (kk == 0) { X = {in[row(0)], 0.f}; Y = {in[row(N - 1)], 0.f}; }
+            else {
+                X = {in[row(2 * kk - 1)], in[row(2 * kk)]};
+                Y = {in[row(2 * (M - kk) - 1)], in[row(2 * (M - kk))]};
+            }
+            v[r] = r16::inv_split(tw, kk, X, Y);
+        }
+    }
+    auto xch = [&](int k1, int t) { return &xb[(k1 * 16 + t) * 64 + lane]; };
+    if (!INV) cfft256<-1>(v, s, tw, xch);
+    else cfft256<1>(v, s, tw, xch);
+    if (INV) {
+#pragma unroll
+        for (int k2 = 0; k2 < 16; k2++) {
+            const int m = s + 16 * k2;
+            const cx<float> z = v[4 * (k2 % 4) + k2 / 4];
+            put<TO, float, ACC>(out, row(2 * m), z.re * scale, (float*)nullptr);
+            put<TO, float, ACC>(out, row(2 * m + 1), z.im * scale, (float*)nullptr);
+        }
+        return;
+    }
+#pragma unroll
+    for (int k2 = 0; k2 < 16; k2++) xb[(s + 16 * k2) * 64 + lane] = v[4 * (k2 % 4) + k2 / 4];
+    __syncthreads();
+#pragma unroll
+    for (int k2 = 0; k2 < 16; k2++) {
+        const int kk = s + 16 * k2;
+        const cx<float> a = v[4 * (k2 % 4) + k2 / 4];
+        if (kk == 0) {
+            put<TO, float, ACC>(out, row(0), (a.re + a.im) * scale, (float*)nullptr);
+            put<TO, float, ACC>(out, row(N - 1), (a.re - a.im) * scale, (float*)nullptr);
+        } else {
+            const cx<float> F = r16::fwd_split(tw, kk, a, xb[(M - kk) * 64 + lane]);
+            put<TO, float, ACC>(out, row(2 * kk - 1), F.re * scale, (float*)nullptr);
+            put<TO, float, ACC>(out, row(2 * kk), F.im * scale, (float*)nullptr);
+        }
+    }
+}
+
+// along x: 256 threads, 16 lines (j) x 16 slots; with RES (forward) the input is
+// rhs - L b0 formed on the fly, as in the 256-point kernel
+template <int INV, class TO, int ACC, int RES, class B>
+__global__ void __launch_bounds__(256) k_rfft512_x(Grid g, const float* __restrict__ twg, const float* in, TO* out,
+                                                  B* __restrict__ b0) {
+    using namespace r512;
+    static_assert(!RES || !INV, "the residual feeds the forward transform");
+    static_assert(INV || (!ACC && sizeof(TO) == 4), "the forward transform writes the float work array");
+    constexpr int LS = 16 * 17 + 4;   // line buffer (complex): the exchange at stride 17, padded
+    __shared__ cx<float> xb[16 * LS];
+    __shared__ cx<float> tw[M];
+    const int lane = threadIdx.x & 63, s = lane & 15, line = (threadIdx.x >> 6) * 4 + (lane >> 4);
+    const int nbl = g.ny / 16;
+    const int tile = xcd_tile(blockIdx.x, nbl * g.nz);
+    const int k = tile / nbl + 1, j = (tile % nbl) * 16 + line;
+    for (int t = threadIdx.x; t < M; t += 256) tw[t] = {twg[2 * t], twg[2 * t + 1]};
+    const float scale = (float)sqrt(1. / (double)N);
+    const long q0 = gidx(g, 0, j, k);
+    cx<float>* xl = xb + line * LS;
+    cx<float> v[16];
+    if (RES) {
+        // four consecutive cells per thread from 16-byte loads of the line and of its
+        // four neighbour lines, staged through LDS
+        static_assert(std::is_same<B, float>::value, "the starting solution is float");
+        float* fl = reinterpret_cast<float*>(xl);
+        const float* pc = b0 + q0;
+        const float* pyp = b0 + gidx(g, 0, wrap(j + 1, g.ny), k);
+        const float* pym = b0 + gidx(g, 0, wrap(j - 1, g.ny), k);
+        const bool zup = k < g.nz || k == 1, zdn = k > 1;
+        const float* pzp = pc + (zup ? g.sz : 0);
+        const float* pzm = pc - (zdn ? g.sz : 0);
+        const float* rl = in + q0;
+        auto ld4 = [](const float* a) {
+            const float4 t = *reinterpret_cast<const float4*>(a);
+            return cx4{t.x, t.y, t.z, t.w};
+        };
+#pragma unroll 2
+        for (int it = 0; it < N / 64; it++) {
+            const int x = 4 * s + 64 * it;
+            const cx4 r4 = ld4(rl + x), c4 = ld4(pc + x), yp4 = ld4(pyp + x), ym4 = ld4(pym + x);
+            const cx4 zp4 = ld4(pzp + x), zm4 = ld4(pzm + x);
+            const float cl = pc[(x - 1) & (N - 1)], cr = pc[(x + 4) & (N - 1)];
+            const float cc[6] = {cl, c4.v[0], c4.v[1], c4.v[2], c4.v[3], cr};
+            float o4[4];
+#pragma unroll
+            for (int t = 0; t < 4; t++)
+                o4[t] = (float)residual_of(g, k, (double)r4.v[t], (double)cc[t + 1], (double)cc[t + 2], (double)cc[t],
+                                           (double)yp4.v[t], (double)ym4.v[t], (double)zp4.v[t], (double)zm4.v[t]);
+            *reinterpret_cast<float4*>(fl + x) = make_float4(o4[0], o4[1], o4[2], o4[3]);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < 16; r++) v[r] = xl[s + 16 * r];
+        __syncthreads();
+    } else if (!INV) {
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            const float2 a = *reinterpret_cast<const float2*>(in + q0 + 2 * (s + 16 * r));
+            v[r] = {a.x, a.y};
+        }
+        __syncthreads();
+    } else {
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            const int kk = s + 16 * r;
+            cx<float> X, Y;
+            if (kk == 0) { X = {in[q0], 0.f}; Y = {in[q0 + N - 1], 0.f}; }
+            else {
+                X = {in[q0 + 2 * kk - 1], in[q0 + 2 * kk]};
+                Y = {in[q0 + 2 * (M - kk) - 1], in[q0 + 2 * (M - kk)]};
+            }
+            v[r] = r16::inv_split(tw, kk, X, Y);
+        }
+    }
+    auto xch = [&](int k1, int t) { return &xl[k1 * 17 + t]; };
+    if (!INV) cfft256<-1>(v, s, tw, xch);
+    else cfft256<1>(v, s, tw, xch);
+    if (INV) {
+        // slot s holds z_m for m = s + 16 k2: x = 2 m and 2 m + 1
+#pragma unroll
+        for (int k2 = 0; k2 < 16; k2++) {
+            const cx<float> z = v[4 * (k2 % 4) + k2 / 4];
+            const long q = q0 + 2 * (s + 16 * k2);
+            if constexpr (sizeof(TO) == 4 && !ACC) {
+                *reinterpret_cast<float2*>(out + q) = make_float2(z.re * scale, z.im * scale);
+            } else if constexpr (sizeof(TO) == 8 && ACC) {
+                static_assert(std::is_same<B, float>::value, "the starting solution is float");
+                const float2 t = *reinterpret_cast<const float2*>(b0 + q);
+                const double r0 = (double)t.x + (double)(z.re * scale), r1 = (double)t.y + (double)(z.im * scale);
+                *reinterpret_cast<double2*>(out + q) = make_double2(r0, r1);
+                *reinterpret_cast<float2*>(b0 + q) = make_float2((float)r0, (float)r1);
+            } else {
+                put2<TO, ACC>(out, q, z.re * scale, z.im * scale, b0);
+            }
+        }
+        return;
+    }
+#pragma unroll
+    for (int k2 = 0; k2 < 16; k2++) xl[s + 16 * k2] = v[4 * (k2 % 4) + k2 / 4];
+    __syncthreads();
+    float o[16][2];
+#pragma unroll
+    for (int k2 = 0; k2 < 16; k2++) {
+        const int kk = s + 16 * k2;
+        const cx<float> a = v[4 * (k2 % 4) + k2 / 4];
+        if (kk == 0) {
+            o[k2][0] = (a.re + a.im) * scale;   // x = 0
+            o[k2][1] = (a.re - a.im) * scale;   // x = N - 1
+        } else {
+            const cx<float> F = r16::fwd_split(tw, kk, a, xl[M - kk]);
+            o[k2][0] = F.re * scale;            // x = 2 kk - 1
+            o[k2][1] = F.im * scale;            // x = 2 kk
+        }
+    }
+    __syncthreads();
+    // stage the line in LDS (floats) for aligned, coalesced stores
+    float* fl = reinterpret_cast<float*>(xl);
+#pragma unroll
+    for (int k2 = 0; k2 < 16; k2++) {
+        const int kk = s + 16 * k2;
+        if (kk == 0) { fl[0] = o[k2][0]; fl[N - 1] = o[k2][1]; }
+        else { fl[2 * kk - 1] = o[k2][0]; fl[2 * kk] = o[k2][1]; }
+    }
+    __syncthreads();
+    // 16 lines x 128 float4
+#pragma unroll
+    for (int it = 0; it < 8; it++) {
+        const int e = threadIdx.x + 256 * it, ln = e >> 7, c4 = e & 127;
+        const float* src = reinterpret_cast<const float*>(xb + ln * LS) + 4 * c4;
+        const long q = gidx(g, 4 * c4, (tile % nbl) * 16 + ln, k);
+        *reinterpret_cast<float4*>(out + q) = make_float4(src[0], src[1], src[2], src[3]);
+    }
+}
+
+template <int DIR, int INV, class C, class TO, int ACC, int RES, class B>
+static bool try_rfft512(Ctx& c, const C* tw, const C* in, TO* out, B* b0) {
+    if constexpr (!std::is_same<C, float>::value) return false;
+    else {
+        const int N = DIR == 0 ? c.g.nx : c.g.ny;
+        if (N != 512) return false;
+        if (DIR == 1) {
+            if constexpr (RES || ACC) return false;
+            else {
+                if (c.g.nx % 64) return false;
+                hipLaunchKernelGGL((k_rfft512_y<INV, TO, ACC>), dim3((c.g.nx / 64) * c.g.nz), dim3(1024), 0, c.stream,
+                                   c.g, tw, in, out);
+                return true;
+            }
+        }
+        if (c.g.ny % 16) return false;
+        hipLaunchKernelGGL((k_rfft512_x<INV, TO, ACC, RES, B>), dim3((c.g.ny / 16) * c.g.nz), dim3(256), 0, c.stream,
+                           c.g, tw, in, out, b0);
+        return true;
+    }
+}
+
 // the register kernels cover float lines of 256 with 64 | nx (y) or 32 | ny (x)
 template <int DIR, int INV, class C, class TO, int ACC, int RES, class B>
 static bool try_rfft256(Ctx& c, const C* tw, const C* in, TO* out, B* b0) {
@@ -700,6 +963,7 @@ static int rfft_launch(Ctx& c, const C* tw, const C* in, TO* out, B* b0 = nullpt
     const int N = DIR == 0 ? c.g.nx : c.g.ny;
     const int nlines = DIR == 0 ? c.g.ny : c.g.nx;
     if (try_rfft256<DIR, INV, C, TO, ACC, RES, B>(c, tw, in, out, b0) ||
+        try_rfft512<DIR, INV, C, TO, ACC, RES, B>(c, tw, in, out, b0) ||
         rfft_fixed<DIR, INV, C, TO, ACC, RES, B>(c, tw, in, out, N, nlines, b0)) {
         HIP_OK(hipGetLastError());
         return ICLS_OK;

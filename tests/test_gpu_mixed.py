@@ -34,7 +34,7 @@ def _mixed(p):
     return q
 
 
-@pytest.mark.parametrize("n,nz", [(256, 16), (64, 32), (16, 8)])
+@pytest.mark.parametrize("n,nz", [(512, 8), (256, 16), (64, 32), (16, 8)])
 def test_mixed_solver2d_refined_matches_oracle(n, nz):
     import pyoracle
     p = _mixed(small_drop(n, nz=nz))
