@@ -61,6 +61,37 @@ def test_mixed_solver2d_refined_matches_oracle(n, nz):
         g.close()
 
 
+@pytest.mark.parametrize("nx,ny,nz", [(512, 64, 8), (64, 512, 8), (256, 32, 8), (32, 256, 8)])
+def test_mixed_solver2d_rectangular_matches_oracle(nx, ny, nz):
+    """Lines of 256 or 512 along one axis and of another length along the other: the
+    register transforms on one axis, the LDS transform on the other (and the y register
+    kernels' fallback where nx is not a multiple of 64)."""
+    import pyoracle
+    p = _mixed(small_drop(max(nx, ny), nz=nz))
+    p.itot, p.jtot = nx, ny
+    o = pyoracle.Oracle(p)
+    g = _solver(p)
+    try:
+        o.init_program()
+        g.init_program()
+        rng = np.random.default_rng(nx + 3 * ny + nz)
+        f = np.zeros(P.field_dims(p, P.F_P), order="F")
+        r = rng.standard_normal((nx, ny, nz))
+        r -= r.mean()
+        f[1:-1, 1:-1, 1:-1] = r.astype(np.float32).astype(np.float64)
+        o.set(P.F_P, f)
+        g.upload(P.F_P, f)
+        o.solver2d()
+        g.solver2d()
+        a = g.download(P.F_P)[1:-1, 1:-1, 1:-1]
+        b = o.get(P.F_P)[1:-1, 1:-1, 1:-1]
+        e = rel_linf(a, b)
+        print("mixed solver2d %dx%dx%d: rel Linf vs oracle %.3e" % (nx, ny, nz, e))
+        assert e <= 1e-9
+    finally:
+        g.close()
+
+
 FIELDS = [P.F_LVSET, P.F_UNEW, P.F_VNEW, P.F_WNEW, P.F_PNEW]
 TOLS = {P.F_LVSET: 1e-5, P.F_UNEW: 1e-3, P.F_VNEW: 1e-3, P.F_WNEW: 1e-3, P.F_PNEW: 1e-3}
 
