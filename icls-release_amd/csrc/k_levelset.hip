@@ -720,9 +720,9 @@ static void reinit_dispatch_pd(Ctx& c, int rz, int stage, const double* phi0, co
 // Inside a level-set window (a few tens of thousands of cells) 8-layer chunks: four
 // times the blocks of 32-layer chunks, and a quarter of each block's serial march.
 // With one barrier per step and the prologue loads in flight together, 40, 52 and 64
-// layers ran the FP32 stages 2-3 in 0.172, 0.203 and 0.197 ms against 0.169 ms for 32,
-// and the FP64 ones in 0.298, 0.329 and 0.345 ms against 0.321 ms: FP64 (three resident
-// blocks per CU instead of five) takes 40.
+// layers ran the two FP32 stage 2-3 launches of a step in 0.172, 0.203 and 0.197 ms
+// together against 0.169 ms for 32, and the FP64 ones in 0.298, 0.329 and 0.345 ms
+// against 0.321 ms: FP64 (three resident blocks per CU instead of five) takes 40.
 constexpr int REINIT_RZ = 32, REINIT_RZ_FP64 = 40, REINIT_RZ_WINDOW = 8;
 template <class T>
 static int reinit_rz(Ctx& c) {
