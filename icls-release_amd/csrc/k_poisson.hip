@@ -321,19 +321,23 @@ k_rfft_n(Grid g, const C* __restrict__ twg, const C* in, TO* out, B* __restrict_
 namespace r16 {
 constexpr int N = 256, M = 128;
 
-__device__ __forceinline__ cx<float> w16(int j, int sg) {
-    constexpr float C1 = 0.92387953251128674f, S1 = 0.38268343236508978f, C2 = 0.70710678118654752f;
-    const float co[16] = {1.f, C1, C2, S1, 0.f, -S1, -C2, -C1, -1.f, -C1, -C2, -S1, 0.f, S1, C2, C1};
-    const float si[16] = {0.f, S1, C2, C1, 1.f, C1, C2, S1, 0.f, -S1, -C2, -C1, -1.f, -C1, -C2, -S1};
+template <class T>
+__device__ __forceinline__ cx<T> w16(int j, int sg) {
+    // cos and sin of pi/8 and pi/4, in the element type
+    constexpr T C1 = std::is_same<T, float>::value ? T(0.92387953251128674f) : T(0.92387953251128674);
+    constexpr T S1 = std::is_same<T, float>::value ? T(0.38268343236508978f) : T(0.38268343236508978);
+    constexpr T C2 = std::is_same<T, float>::value ? T(0.70710678118654752f) : T(0.70710678118654752);
+    const T co[16] = {T(1.), C1, C2, S1, T(0.), -S1, -C2, -C1, T(-1.), -C1, -C2, -S1, T(0.), S1, C2, C1};
+    const T si[16] = {T(0.), S1, C2, C1, T(1.), C1, C2, S1, T(0.), -S1, -C2, -C1, T(-1.), -C1, -C2, -S1};
     j &= 15;
     return {co[j], sg > 0 ? si[j] : -si[j]};
 }
 
 // 4-point DFT in place, exp(SG 2 pi i / 4) = SG i
-template <int SG>
-__device__ __forceinline__ void dft4(cx<float>& x0, cx<float>& x1, cx<float>& x2, cx<float>& x3) {
-    const cx<float> t0 = cadd(x0, x2), t1 = csub(x0, x2), t2 = cadd(x1, x3), t3 = csub(x1, x3);
-    const cx<float> jt3 = SG > 0 ? cx<float>{-t3.im, t3.re} : cx<float>{t3.im, -t3.re};
+template <int SG, class T>
+__device__ __forceinline__ void dft4(cx<T>& x0, cx<T>& x1, cx<T>& x2, cx<T>& x3) {
+    const cx<T> t0 = cadd(x0, x2), t1 = csub(x0, x2), t2 = cadd(x1, x3), t3 = csub(x1, x3);
+    const cx<T> jt3 = SG > 0 ? cx<T>{-t3.im, t3.re} : cx<T>{t3.im, -t3.re};
     x0 = cadd(t0, t2);
     x1 = cadd(t1, jt3);
     x2 = csub(t0, t2);
@@ -341,64 +345,67 @@ __device__ __forceinline__ void dft4(cx<float>& x0, cx<float>& x1, cx<float>& x2
 }
 
 // 16-point DFT of v[0..15] in place (4 x 4); X_k ends at v[4 (k % 4) + k / 4]
-template <int SG>
-__device__ __forceinline__ void dft16(cx<float>* v) {
+template <int SG, class T>
+__device__ __forceinline__ void dft16(cx<T>* v) {
 #pragma unroll
     for (int q = 0; q < 4; q++) dft4<SG>(v[q], v[q + 4], v[q + 8], v[q + 12]);
 #pragma unroll
     for (int q = 1; q < 4; q++)
 #pragma unroll
-        for (int ka = 1; ka < 4; ka++) v[q + 4 * ka] = cmul(v[q + 4 * ka], w16(q * ka, SG));
+        for (int ka = 1; ka < 4; ka++) v[q + 4 * ka] = cmul(v[q + 4 * ka], w16<T>(q * ka, SG));
 #pragma unroll
     for (int ka = 0; ka < 4; ka++) dft4<SG>(v[4 * ka], v[4 * ka + 1], v[4 * ka + 2], v[4 * ka + 3]);
 }
 
 // 8-point DFT of u[0..7] in place (4 x 2); X_k ends at u[2 (k % 4) + k / 4]
-template <int SG>
-__device__ __forceinline__ void dft8(cx<float>* u) {
+template <int SG, class T>
+__device__ __forceinline__ void dft8(cx<T>* u) {
     dft4<SG>(u[0], u[2], u[4], u[6]);
     dft4<SG>(u[1], u[3], u[5], u[7]);
-    u[3] = cmul(u[3], w16(2, SG));
-    u[5] = cmul(u[5], w16(4, SG));
-    u[7] = cmul(u[7], w16(6, SG));
+    u[3] = cmul(u[3], w16<T>(2, SG));
+    u[5] = cmul(u[5], w16<T>(4, SG));
+    u[7] = cmul(u[7], w16<T>(6, SG));
 #pragma unroll
     for (int ka = 0; ka < 4; ka++) {
-        const cx<float> a = u[2 * ka], b = u[2 * ka + 1];
+        const cx<T> a = u[2 * ka], b = u[2 * ka + 1];
         u[2 * ka] = cadd(a, b);
         u[2 * ka + 1] = csub(a, b);
     }
 }
 
 // exp(+2 pi i t / N) for 0 <= t < N from the half table
-__device__ __forceinline__ cx<float> twf(const cx<float>* tw, int t) {
-    return t < M ? tw[t] : cx<float>{-tw[t - M].re, -tw[t - M].im};
+template <class T>
+__device__ __forceinline__ cx<T> twf(const cx<T>* tw, int t) {
+    return t < M ? tw[t] : cx<T>{-tw[t - M].re, -tw[t - M].im};
 }
 
 // Z_k of the inverse split from X_k and X_(M-k) of the half-complex line
-__device__ __forceinline__ cx<float> inv_split(const cx<float>* tw, int kk, cx<float> X, cx<float> Y) {
-    const cx<float> cy = cconj(Y);
-    const cx<float> t = cmul(tw[kk], csub(X, cy));
-    return cadd(cadd(X, cy), cx<float>{-t.im, t.re});
+template <class T>
+__device__ __forceinline__ cx<T> inv_split(const cx<T>* tw, int kk, cx<T> X, cx<T> Y) {
+    const cx<T> cy = cconj(Y);
+    const cx<T> t = cmul(tw[kk], csub(X, cy));
+    return cadd(cadd(X, cy), cx<T>{-t.im, t.re});
 }
 
 // F_k of the forward split from X_k and X_(M-k) (k >= 1)
-__device__ __forceinline__ cx<float> fwd_split(const cx<float>* tw, int kk, cx<float> a, cx<float> y) {
-    const cx<float> b = cconj(y);
-    const cx<float> ev = {(a.re + b.re) * 0.5f, (a.im + b.im) * 0.5f};
-    const cx<float> od = {(a.im - b.im) * 0.5f, -(a.re - b.re) * 0.5f};
+template <class T>
+__device__ __forceinline__ cx<T> fwd_split(const cx<T>* tw, int kk, cx<T> a, cx<T> y) {
+    const cx<T> b = cconj(y);
+    const cx<T> ev = {(a.re + b.re) * T(0.5), (a.im + b.im) * T(0.5)};
+    const cx<T> od = {(a.im - b.im) * T(0.5), -(a.re - b.re) * T(0.5)};
     return cadd(ev, cmul(cconj(tw[kk]), od));
 }
 
 // the complex FFT from "v[r] = z_(s + 8 r)" to "u[h][2 (k2 % 4) + k2 / 4] = X_(2s+h+16k2)";
 // xch(k1, t) addresses the exchange slot of (k1, slot t) of this thread's line
-template <int SG, class XCH>
-__device__ __forceinline__ void cfft128(cx<float>* v, cx<float> (*u)[8], int s, const cx<float>* tw, XCH xch) {
+template <int SG, class T, class XCH>
+__device__ __forceinline__ void cfft128(cx<T>* v, cx<T> (*u)[8], int s, const cx<T>* tw, XCH xch) {
     dft16<SG>(v);
 #pragma unroll
     for (int k1 = 0; k1 < 16; k1++) {
-        cx<float> val = v[4 * (k1 % 4) + k1 / 4];
+        cx<T> val = v[4 * (k1 % 4) + k1 / 4];
         if (k1) {
-            cx<float> w = twf(tw, 2 * s * k1);
+            cx<T> w = twf(tw, 2 * s * k1);
             if (SG < 0) w.im = -w.im;
             val = cmul(val, w);
         }
@@ -421,21 +428,22 @@ __device__ __forceinline__ void put2(TO* out, long q, float a, float b, B* __res
     put<TO, float, ACC>(out, q + 1, b, b0);
 }
 
-// along y: 512 threads, 64 lines (i) x 8 slots; LDS 64 KB exchange + twiddles
-template <int INV, class TO, int ACC>
-__global__ void __launch_bounds__(512) k_rfft256_y(Grid g, const float* __restrict__ twg, const float* in, TO* out) {
+// along y: 512 threads, 64 lines (i) x 8 slots; LDS 64 KB (float) or 128 KB (double)
+// exchange + twiddles
+template <int INV, class C, class TO, int ACC>
+__global__ void __launch_bounds__(512) k_rfft256_y(Grid g, const C* __restrict__ twg, const C* in, TO* out) {
     using namespace r16;
     static_assert(!ACC, "only the x transform adds into its output");
-    __shared__ cx<float> xb[M * 64];
-    __shared__ cx<float> tw[M];
+    __shared__ cx<C> xb[M * 64];
+    __shared__ cx<C> tw[M];
     const int lane = threadIdx.x & 63, s = threadIdx.x >> 6;
     const int nbl = g.nx / 64;
     const int tile = xcd_tile(blockIdx.x, nbl * g.nz);
     const int k = tile / nbl + 1, i = (tile % nbl) * 64 + lane;
     for (int t = threadIdx.x; t < M; t += 512) tw[t] = {twg[2 * t], twg[2 * t + 1]};
-    const float scale = 0.0625f;   // sqrt(1/256)
+    const C scale = C(0.0625);   // sqrt(1/256)
     auto row = [&](int n) -> long { return gidx(g, i, n, k); };
-    cx<float> v[16];
+    cx<C> v[16];
     if (!INV) {
 #pragma unroll
         for (int r = 0; r < 16; r++) {
@@ -448,8 +456,8 @@ __global__ void __launch_bounds__(512) k_rfft256_y(Grid g, const float* __restri
 #pragma unroll
         for (int r = 0; r < 16; r++) {
             const int kk = s + 8 * r;
-            cx<float> X, Y;
-            if (kk == 0) { X = {in[row(0)], 0.f}; Y = {in[row(N - 1)], 0.f}; }
+            cx<C> X, Y;
+            if (kk == 0) { X = {in[row(0)], C(0.)}; Y = {in[row(N - 1)], C(0.)}; }
             else {
                 X = {in[row(2 * kk - 1)], in[row(2 * kk)]};
                 Y = {in[row(2 * (M - kk) - 1)], in[row(2 * (M - kk))]};
@@ -457,7 +465,7 @@ __global__ void __launch_bounds__(512) k_rfft256_y(Grid g, const float* __restri
             v[r] = inv_split(tw, kk, X, Y);
         }
     }
-    cx<float> u[2][8];
+    cx<C> u[2][8];
     auto xch = [&](int k1, int t) { return &xb[(k1 * 8 + t) * 64 + lane]; };
     if (!INV) cfft128<-1>(v, u, s, tw, xch);
     else cfft128<1>(v, u, s, tw, xch);
@@ -467,9 +475,9 @@ __global__ void __launch_bounds__(512) k_rfft256_y(Grid g, const float* __restri
 #pragma unroll
             for (int k2 = 0; k2 < 8; k2++) {
                 const int m = 2 * s + h + 16 * k2;
-                const cx<float> z = u[h][2 * (k2 % 4) + k2 / 4];
-                put<TO, float, ACC>(out, row(2 * m), z.re * scale, (float*)nullptr);
-                put<TO, float, ACC>(out, row(2 * m + 1), z.im * scale, (float*)nullptr);
+                const cx<C> z = u[h][2 * (k2 % 4) + k2 / 4];
+                put<TO, C, ACC>(out, row(2 * m), z.re * scale, (C*)nullptr);
+                put<TO, C, ACC>(out, row(2 * m + 1), z.im * scale, (C*)nullptr);
             }
         return;
     }
@@ -483,14 +491,14 @@ __global__ void __launch_bounds__(512) k_rfft256_y(Grid g, const float* __restri
 #pragma unroll
         for (int k2 = 0; k2 < 8; k2++) {
             const int kk = 2 * s + h + 16 * k2;
-            const cx<float> a = u[h][2 * (k2 % 4) + k2 / 4];
+            const cx<C> a = u[h][2 * (k2 % 4) + k2 / 4];
             if (kk == 0) {
-                put<TO, float, ACC>(out, row(0), (a.re + a.im) * scale, (float*)nullptr);
-                put<TO, float, ACC>(out, row(N - 1), (a.re - a.im) * scale, (float*)nullptr);
+                put<TO, C, ACC>(out, row(0), (a.re + a.im) * scale, (C*)nullptr);
+                put<TO, C, ACC>(out, row(N - 1), (a.re - a.im) * scale, (C*)nullptr);
             } else {
-                const cx<float> F = fwd_split(tw, kk, a, xb[(M - kk) * 64 + lane]);
-                put<TO, float, ACC>(out, row(2 * kk - 1), F.re * scale, (float*)nullptr);
-                put<TO, float, ACC>(out, row(2 * kk), F.im * scale, (float*)nullptr);
+                const cx<C> F = fwd_split(tw, kk, a, xb[(M - kk) * 64 + lane]);
+                put<TO, C, ACC>(out, row(2 * kk - 1), F.re * scale, (C*)nullptr);
+                put<TO, C, ACC>(out, row(2 * kk), F.im * scale, (C*)nullptr);
             }
         }
 }
@@ -500,26 +508,29 @@ struct cx4 {
 };
 
 // along x: 256 threads, 32 lines (j) x 8 slots; with RES (forward) the input is
-// rhs - L b0 formed on the fly
-template <int INV, class TO, int ACC, int RES, class B>
-__global__ void __launch_bounds__(256) k_rfft256_x(Grid g, const float* __restrict__ twg, const float* in, TO* out,
+// rhs - L b0 formed on the fly.  C = double: the FP64 transforms in place on p
+// (neither RES nor ACC).
+template <int INV, class C, class TO, int ACC, int RES, class B>
+__global__ void __launch_bounds__(256) k_rfft256_x(Grid g, const C* __restrict__ twg, const C* in, TO* out,
                                                   B* __restrict__ b0) {
     using namespace r16;
     static_assert(!RES || !INV, "the residual feeds the forward transform");
-    static_assert(INV || (!ACC && sizeof(TO) == 4), "the forward transform writes the float work array");
+    static_assert(INV || (!ACC && sizeof(TO) == sizeof(C)), "the forward transform writes the work array");
+    static_assert(std::is_same<C, float>::value || (!RES && !ACC && std::is_same<TO, C>::value),
+                  "the FP64 transforms neither form the residual nor accumulate");
     constexpr int LS = 16 * 8 + 8;    // exchange stride of a line (complex)
-    __shared__ cx<float> xb[32 * LS];
-    __shared__ cx<float> tw[M];
+    __shared__ cx<C> xb[32 * LS];
+    __shared__ cx<C> tw[M];
     const int lane = threadIdx.x & 63, s = lane & 7, line = (threadIdx.x >> 6) * 8 + (lane >> 3);
     const int nbl = g.ny / 32;
     const int tile = xcd_tile(blockIdx.x, nbl * g.nz);
     const int k = tile / nbl + 1, j = (tile % nbl) * 32 + line;
     for (int t = threadIdx.x; t < M; t += 256) tw[t] = {twg[2 * t], twg[2 * t + 1]};
-    const float scale = 0.0625f;
+    const C scale = C(0.0625);
     const long q0 = gidx(g, 0, j, k);
-    cx<float>* xl = xb + line * LS;
-    cx<float> v[16];
-    if (RES) {
+    cx<C>* xl = xb + line * LS;
+    cx<C> v[16];
+    if constexpr (RES) {
         // the residual line goes through LDS so that its stencils are not all in flight;
         // the neighbour lines are addressed from their starts.  A thread forms four
         // consecutive cells from 16-byte loads of the line and of its four neighbour
@@ -559,8 +570,13 @@ __global__ void __launch_bounds__(256) k_rfft256_x(Grid g, const float* __restri
     } else if (!INV) {
 #pragma unroll
         for (int r = 0; r < 16; r++) {
-            const float2 a = *reinterpret_cast<const float2*>(in + q0 + 2 * (s + 8 * r));
-            v[r] = {a.x, a.y};
+            if constexpr (std::is_same<C, float>::value) {
+                const float2 a = *reinterpret_cast<const float2*>(in + q0 + 2 * (s + 8 * r));
+                v[r] = {a.x, a.y};
+            } else {
+                const double2 a = *reinterpret_cast<const double2*>(in + q0 + 2 * (s + 8 * r));
+                v[r] = {a.x, a.y};
+            }
         }
         __syncthreads();
     } else {
@@ -568,8 +584,8 @@ __global__ void __launch_bounds__(256) k_rfft256_x(Grid g, const float* __restri
 #pragma unroll
         for (int r = 0; r < 16; r++) {
             const int kk = s + 8 * r;
-            cx<float> X, Y;
-            if (kk == 0) { X = {in[q0], 0.f}; Y = {in[q0 + N - 1], 0.f}; }
+            cx<C> X, Y;
+            if (kk == 0) { X = {in[q0], C(0.)}; Y = {in[q0 + N - 1], C(0.)}; }
             else {
                 X = {in[q0 + 2 * kk - 1], in[q0 + 2 * kk]};
                 Y = {in[q0 + 2 * (M - kk) - 1], in[q0 + 2 * (M - kk)]};
@@ -577,7 +593,7 @@ __global__ void __launch_bounds__(256) k_rfft256_x(Grid g, const float* __restri
             v[r] = inv_split(tw, kk, X, Y);
         }
     }
-    cx<float> u[2][8];
+    cx<C> u[2][8];
     auto xch = [&](int k1, int t) { return &xl[k1 * 8 + t]; };
     if (!INV) cfft128<-1>(v, u, s, tw, xch);
     else cfft128<1>(v, u, s, tw, xch);
@@ -585,9 +601,12 @@ __global__ void __launch_bounds__(256) k_rfft256_x(Grid g, const float* __restri
         // x = 4 s + 32 k2 .. +3 holds z_(2s+16k2) and z_(2s+1+16k2)
 #pragma unroll
         for (int k2 = 0; k2 < 8; k2++) {
-            const cx<float> z0 = u[0][2 * (k2 % 4) + k2 / 4], z1 = u[1][2 * (k2 % 4) + k2 / 4];
+            const cx<C> z0 = u[0][2 * (k2 % 4) + k2 / 4], z1 = u[1][2 * (k2 % 4) + k2 / 4];
             const long q = q0 + 4 * s + 32 * k2;
-            if constexpr (sizeof(TO) == 4 && !ACC) {
+            if constexpr (std::is_same<C, double>::value) {
+                *reinterpret_cast<double2*>(out + q) = make_double2(z0.re * scale, z0.im * scale);
+                *reinterpret_cast<double2*>(out + q + 2) = make_double2(z1.re * scale, z1.im * scale);
+            } else if constexpr (sizeof(TO) == 4 && !ACC) {
                 *reinterpret_cast<float4*>(out + q) = make_float4(z0.re * scale, z0.im * scale, z1.re * scale, z1.im * scale);
             } else if constexpr (sizeof(TO) == 8 && ACC) {
                 // q is a multiple of 4: the base and the output move as 16-byte vectors
@@ -610,25 +629,25 @@ __global__ void __launch_bounds__(256) k_rfft256_x(Grid g, const float* __restri
 #pragma unroll
         for (int k2 = 0; k2 < 8; k2++) xl[(2 * s + h) + 16 * k2] = u[h][2 * (k2 % 4) + k2 / 4];
     __syncthreads();
-    float o[2][8][2];
+    C o[2][8][2];
 #pragma unroll
     for (int h = 0; h < 2; h++)
 #pragma unroll
         for (int k2 = 0; k2 < 8; k2++) {
             const int kk = 2 * s + h + 16 * k2;
-            const cx<float> a = u[h][2 * (k2 % 4) + k2 / 4];
+            const cx<C> a = u[h][2 * (k2 % 4) + k2 / 4];
             if (kk == 0) {
                 o[h][k2][0] = (a.re + a.im) * scale;   // x = 0
                 o[h][k2][1] = (a.re - a.im) * scale;   // x = N - 1
             } else {
-                const cx<float> F = fwd_split(tw, kk, a, xl[M - kk]);
+                const cx<C> F = fwd_split(tw, kk, a, xl[M - kk]);
                 o[h][k2][0] = F.re * scale;            // x = 2 kk - 1
                 o[h][k2][1] = F.im * scale;            // x = 2 kk
             }
         }
     __syncthreads();
-    // stage the line in LDS (floats) for aligned, coalesced stores
-    float* fl = reinterpret_cast<float*>(xl);
+    // stage the line in LDS (reals) for aligned, coalesced stores
+    C* fl = reinterpret_cast<C*>(xl);
 #pragma unroll
     for (int h = 0; h < 2; h++)
 #pragma unroll
@@ -638,13 +657,24 @@ __global__ void __launch_bounds__(256) k_rfft256_x(Grid g, const float* __restri
             else { fl[2 * kk - 1] = o[h][k2][0]; fl[2 * kk] = o[h][k2][1]; }
         }
     __syncthreads();
-    // 32 lines x 64 float4: thread t stores float4 t % 64 of lines t / 64 + 4 it
+    if constexpr (std::is_same<C, float>::value) {
+        // 32 lines x 64 float4: thread t stores float4 t % 64 of lines t / 64 + 4 it
 #pragma unroll
-    for (int it = 0; it < 8; it++) {
-        const int ln = (threadIdx.x >> 6) + 4 * it, c4 = threadIdx.x & 63;
-        const float* src = reinterpret_cast<const float*>(xb + ln * LS) + 4 * c4;
-        const long q = gidx(g, 4 * c4, (tile % nbl) * 32 + ln, k);
-        *reinterpret_cast<float4*>(out + q) = make_float4(src[0], src[1], src[2], src[3]);
+        for (int it = 0; it < 8; it++) {
+            const int ln = (threadIdx.x >> 6) + 4 * it, c4 = threadIdx.x & 63;
+            const float* src = reinterpret_cast<const float*>(xb + ln * LS) + 4 * c4;
+            const long q = gidx(g, 4 * c4, (tile % nbl) * 32 + ln, k);
+            *reinterpret_cast<float4*>(out + q) = make_float4(src[0], src[1], src[2], src[3]);
+        }
+    } else {
+        // 32 lines x 128 double2: thread t stores double2 t % 128 of lines t / 128 + 2 it
+#pragma unroll
+        for (int it = 0; it < 16; it++) {
+            const int ln = (threadIdx.x >> 7) + 2 * it, c2 = threadIdx.x & 127;
+            const double* src = reinterpret_cast<const double*>(xb + ln * LS) + 2 * c2;
+            const long q = gidx(g, 2 * c2, (tile % nbl) * 32 + ln, k);
+            *reinterpret_cast<double2*>(out + q) = make_double2(src[0], src[1]);
+        }
     }
 }
 
@@ -911,10 +941,13 @@ static bool try_rfft512(Ctx& c, const C* tw, const C* in, TO* out, B* b0) {
     }
 }
 
-// the register kernels cover float lines of 256 with 64 | nx (y) or 32 | ny (x)
+// the register kernels cover lines of 256 with 64 | nx (y) or 32 | ny (x): float, and
+// double for the FP64 transforms (no residual, no accumulation)
 template <int DIR, int INV, class C, class TO, int ACC, int RES, class B>
 static bool try_rfft256(Ctx& c, const C* tw, const C* in, TO* out, B* b0) {
-    if constexpr (!std::is_same<C, float>::value) return false;
+    constexpr bool F32 = std::is_same<C, float>::value;
+    constexpr bool F64 = std::is_same<C, double>::value && std::is_same<TO, double>::value && !RES && !ACC;
+    if constexpr (!F32 && !F64) return false;
     else {
         const int N = DIR == 0 ? c.g.nx : c.g.ny;
         if (N != 256) return false;
@@ -922,14 +955,14 @@ static bool try_rfft256(Ctx& c, const C* tw, const C* in, TO* out, B* b0) {
             if constexpr (RES || ACC) return false;
             else {
                 if (c.g.nx % 64) return false;
-                hipLaunchKernelGGL((k_rfft256_y<INV, TO, ACC>), dim3((c.g.nx / 64) * c.g.nz), dim3(512), 0, c.stream,
-                                   c.g, tw, in, out);
+                hipLaunchKernelGGL((k_rfft256_y<INV, C, TO, ACC>), dim3((c.g.nx / 64) * c.g.nz), dim3(512), 0,
+                                   c.stream, c.g, tw, in, out);
                 return true;
             }
         }
         if (c.g.ny % 32) return false;
-        hipLaunchKernelGGL((k_rfft256_x<INV, TO, ACC, RES, B>), dim3((c.g.ny / 32) * c.g.nz), dim3(256), 0, c.stream,
-                           c.g, tw, in, out, b0);
+        hipLaunchKernelGGL((k_rfft256_x<INV, C, TO, ACC, RES, B>), dim3((c.g.ny / 32) * c.g.nz), dim3(256), 0,
+                           c.stream, c.g, tw, in, out, b0);
         return true;
     }
 }
