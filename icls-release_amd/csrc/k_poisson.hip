@@ -580,18 +580,29 @@ __global__ void __launch_bounds__(256) k_rfft256_x(Grid g, const C* __restrict__
         }
         __syncthreads();
     } else {
+        // the half-complex line goes through LDS, loaded with 16-byte vectors: the split
+        // reads it at scattered positions
+        C* fl = reinterpret_cast<C*>(xl);
+        constexpr int VW = 16 / sizeof(C);   // reals per vector
+#pragma unroll
+        for (int it = 0; it < N / (8 * VW); it++) {
+            const int x = VW * s + 8 * VW * it;
+            if constexpr (VW == 4) *reinterpret_cast<float4*>(fl + x) = *reinterpret_cast<const float4*>(in + q0 + x);
+            else *reinterpret_cast<double2*>(fl + x) = *reinterpret_cast<const double2*>(in + q0 + x);
+        }
         __syncthreads();
 #pragma unroll
         for (int r = 0; r < 16; r++) {
             const int kk = s + 8 * r;
             cx<C> X, Y;
-            if (kk == 0) { X = {in[q0], C(0.)}; Y = {in[q0 + N - 1], C(0.)}; }
+            if (kk == 0) { X = {fl[0], C(0.)}; Y = {fl[N - 1], C(0.)}; }
             else {
-                X = {in[q0 + 2 * kk - 1], in[q0 + 2 * kk]};
-                Y = {in[q0 + 2 * (M - kk) - 1], in[q0 + 2 * (M - kk)]};
+                X = {fl[2 * kk - 1], fl[2 * kk]};
+                Y = {fl[2 * (M - kk) - 1], fl[2 * (M - kk)]};
             }
             v[r] = inv_split(tw, kk, X, Y);
         }
+        __syncthreads();   // the line buffer is the exchange buffer of the transform
     }
     cx<C> u[2][8];
     auto xch = [&](int k1, int t) { return &xl[k1 * 8 + t]; };
@@ -846,18 +857,26 @@ __global__ void __launch_bounds__(256) k_rfft512_x(Grid g, const float* __restri
         }
         __syncthreads();
     } else {
+        // the half-complex line through LDS, loaded with 16-byte vectors
+        float* fl = reinterpret_cast<float*>(xl);
+#pragma unroll
+        for (int it = 0; it < N / 64; it++) {
+            const int x = 4 * s + 64 * it;
+            *reinterpret_cast<float4*>(fl + x) = *reinterpret_cast<const float4*>(in + q0 + x);
+        }
         __syncthreads();
 #pragma unroll
         for (int r = 0; r < 16; r++) {
             const int kk = s + 16 * r;
             cx<float> X, Y;
-            if (kk == 0) { X = {in[q0], 0.f}; Y = {in[q0 + N - 1], 0.f}; }
+            if (kk == 0) { X = {fl[0], 0.f}; Y = {fl[N - 1], 0.f}; }
             else {
-                X = {in[q0 + 2 * kk - 1], in[q0 + 2 * kk]};
-                Y = {in[q0 + 2 * (M - kk) - 1], in[q0 + 2 * (M - kk)]};
+                X = {fl[2 * kk - 1], fl[2 * kk]};
+                Y = {fl[2 * (M - kk) - 1], fl[2 * (M - kk)]};
             }
             v[r] = r16::inv_split(tw, kk, X, Y);
         }
+        __syncthreads();   // the line buffer is the exchange buffer of the transform
     }
     auto xch = [&](int k1, int t) { return &xl[k1 * 17 + t]; };
     if (!INV) cfft256<-1>(v, s, tw, xch);
