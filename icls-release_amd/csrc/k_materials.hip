@@ -76,6 +76,57 @@ __global__ void k_materials(Grid g, int lmax, const T* const* __restrict__ ls, c
     }
 }
 
+// The same for one level set on the whole grid (nx a multiple of 256): four
+// consecutive cells of a row per thread, moved as 16-byte vectors.  Per cell the
+// operations are those of k_materials; the partial sums add the four cells of a layer
+// before the next layer.
+template <class T>
+__device__ __forceinline__ void ld4(const T* a, T* v) {
+    if constexpr (sizeof(T) == 4) {
+        const float4 t = *reinterpret_cast<const float4*>(a);
+        v[0] = t.x; v[1] = t.y; v[2] = t.z; v[3] = t.w;
+    } else {
+        const double2 t0 = *reinterpret_cast<const double2*>(a), t1 = *reinterpret_cast<const double2*>(a + 2);
+        v[0] = t0.x; v[1] = t0.y; v[2] = t1.x; v[3] = t1.y;
+    }
+}
+template <class T>
+__device__ __forceinline__ void st4(T* a, const T* v) {
+    if constexpr (sizeof(T) == 4) {
+        *reinterpret_cast<float4*>(a) = make_float4(v[0], v[1], v[2], v[3]);
+    } else {
+        *reinterpret_cast<double2*>(a) = make_double2(v[0], v[1]);
+        *reinterpret_cast<double2*>(a + 2) = make_double2(v[2], v[3]);
+    }
+}
+template <class T>
+__global__ void __launch_bounds__(BX * BY)
+k_materials1(Grid g, const T* __restrict__ phi, T eps, T pi, T miu1, T miu2, T rho1, T rho2, T* __restrict__ miu,
+             T* __restrict__ rho, double* __restrict__ part) {
+    __shared__ double smem[16];
+    const int i0 = 4 * (blockIdx.x * BX + threadIdx.x), j0 = blockIdx.y * BY + threadIdx.y;
+    const bool valid = j0 < g.ny;
+    double acc = 0.;
+    for (int kk = 0; kk < KCH; kk++) {
+        const int k = blockIdx.z * KCH + kk;   // k = 0 .. nz+1
+        if (!valid || k > g.nz + 1) continue;
+        const long q = gidx(g, i0, j0, k);
+        T p[4], mu[4], rh[4];
+        ld4(phi + q, p);
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            const T H = heaviside_trig<T>(p[t], eps, pi);
+            mu[t] = miu1 * H + miu2 * (T(1.) - H);
+            rh[t] = rho1 * H + rho2 * (T(1.) - H);
+            if (k >= 1 && k <= g.nz) acc += (double)H;
+        }
+        st4(miu + q, mu);
+        st4(rho + q, rh);
+    }
+    const double r = block_reduce(acc, OpSum(), smem);
+    if (threadIdx.x == 0 && threadIdx.y == 0) part[block_linear()] = r;
+}
+
 // The same with level-set windows.  The block first lists (in LDS) the windows its
 // cells can lie in; H is the minimum over those, and only their Heaviside sums are
 // reduced.  Every other level set is 1 on all the block's cells: its partial sum is
@@ -156,6 +207,9 @@ int launch_materials(Ctx& c) {
     ProfScope ps(c, "k_materials");
     dim3 blk(BX, BY, 1);
     dim3 grd = red_grid(c.g, 0, c.g.nz + 1);
+    // one level set on the whole grid: the vector kernel, a quarter of the x blocks
+    const bool vec = c.lmax == 1 && c.win.empty() && c.g.nx % (4 * BX) == 0;
+    if (vec) grd.x = c.g.nx / (4 * BX);
     int nb = red_nblocks(grd);
     if ((long)nb * c.lmax > c.npart_cap) { set_error("reduction scratch too small"); return ICLS_ERR_STATE; }
     const icls_params& P = c.P;
@@ -168,6 +222,14 @@ int launch_materials(Ctx& c) {
             hipLaunchKernelGGL(k_materials_win<double>, grd, blk, 0, c.stream, c.g, c.lmax, (const double* const*)c.dls,
                                (const int*)c.dwin, 1.5 * c.g.dz, c.pi, P.miu1, P.miu2, P.rho1, P.rho2, c.miu, c.rho,
                                c.dpart, nb);
+    } else if (vec) {
+        if (c.fp32)
+            hipLaunchKernelGGL(k_materials1<float>, grd, blk, 0, c.stream, c.g, as<float>(c.lvset[0]),
+                               (float)(1.5 * c.g.dz), (float)c.pi, (float)P.miu1, (float)P.miu2, (float)P.rho1,
+                               (float)P.rho2, as<float>(c.miu), as<float>(c.rho), c.dpart);
+        else
+            hipLaunchKernelGGL(k_materials1<double>, grd, blk, 0, c.stream, c.g, (const double*)c.lvset[0],
+                               1.5 * c.g.dz, c.pi, P.miu1, P.miu2, P.rho1, P.rho2, c.miu, c.rho, c.dpart);
     } else if (c.fp32)
         hipLaunchKernelGGL(k_materials<float>, grd, blk, 0, c.stream, c.g, c.lmax, (const float* const*)c.dls,
                            (const int*)(c.win.empty() ? nullptr : c.dwin), (float)(1.5 * c.g.dz), (float)c.pi, (float)P.miu1, (float)P.miu2, (float)P.rho1,
