@@ -245,6 +245,27 @@ inline dim3 grid3(const Grid& g, int kcount, dim3 blk) {
     return dim3((g.nx + blk.x - 1) / blk.x, (g.ny + blk.y - 1) / blk.y, kcount);
 }
 
+// four consecutive elements (16-byte aligned) as one or two 16-byte vector accesses
+template <class T>
+__device__ __forceinline__ void ld4(const T* a, T* v) {
+    if constexpr (sizeof(T) == 4) {
+        const float4 t = *reinterpret_cast<const float4*>(a);
+        v[0] = t.x; v[1] = t.y; v[2] = t.z; v[3] = t.w;
+    } else {
+        const double2 t0 = *reinterpret_cast<const double2*>(a), t1 = *reinterpret_cast<const double2*>(a + 2);
+        v[0] = t0.x; v[1] = t0.y; v[2] = t1.x; v[3] = t1.y;
+    }
+}
+template <class T>
+__device__ __forceinline__ void st4(T* a, const T* v) {
+    if constexpr (sizeof(T) == 4) {
+        *reinterpret_cast<float4*>(a) = make_float4(v[0], v[1], v[2], v[3]);
+    } else {
+        *reinterpret_cast<double2*>(a) = make_double2(v[0], v[1]);
+        *reinterpret_cast<double2*>(a + 2) = make_double2(v[2], v[3]);
+    }
+}
+
 // XCD-aware block order.  The hardware hands consecutive workgroup ids round-robin to
 // the 8 XCDs, each with its own L2.  Kernels that share halos between neighbouring
 // blocks launch a 1D grid and take their tile from xcd_tile(): XCD x then works

@@ -5,6 +5,8 @@
 // T is the element type of the field arrays; the right-hand side the Poisson solve
 // reads (c.frhs) has the same type.  In the FP64 mode c.frhs is the pressure array p
 // itself, as in the reference.
+#include <initializer_list>
+
 #include "icls_internal.hpp"
 #include "icls_stencil.hpp"
 
@@ -45,26 +47,35 @@ __device__ __forceinline__ T lsq_row(const T* __restrict__ A, int r, const T* v)
 // A listed cell is entered as gofs + i0 + sy j0 + sz (k - 1): gofs is 0 for a launch on
 // one grid or window (the index is the window's own), and the window's offset in the
 // whole grid for a batched launch, which also enters the window t in list_t.
+// the curvature of a band cell (phi value ph) and its gradient filter m
+template <class T>
+__device__ __forceinline__ T curvature_band(const Grid& g, const T* __restrict__ A, const T* __restrict__ phi, T ph,
+                                            T& m, int* __restrict__ list, int* __restrict__ list_t, int t, long gofs,
+                                            int* __restrict__ count, int cap, T gate, int i0, int j0, int k) {
+    const T dz = T(g.dz), max_kappa = T(1.) / dz;
+    T v[27];
+    gather27(g, phi, i0, j0, k, v);
+    const T cv = curv_lsq(A, v, max_kappa, m);
+    if (list && fabs(ph) < T(2.) * dz && m >= T(0.9) && fabs(cv) * dz >= gate) {
+        const int slot = atomicAdd(count, 1);
+        if (slot < cap) {
+            list[slot] = (int)(gofs + i0 + g.sy * j0 + g.sz * (long)(k - 1));
+            if (list_t) list_t[slot] = t;
+        }
+    }
+    return cv;
+}
+
 template <class T>
 __device__ __forceinline__ void curvature_at(const Grid& g, T NB2, const T* __restrict__ A, const T* __restrict__ phi,
                                              T* __restrict__ kappa, T* __restrict__ nm, int* __restrict__ list,
                                              int* __restrict__ list_t, int t, long gofs, int* __restrict__ count,
                                              int cap, T gate, int i0, int j0, int k) {
     long q = gidx(g, i0, j0, k);
-    const T dz = T(g.dz), max_kappa = T(1.) / dz;
+    const T dz = T(g.dz), ph = phi[q];
     T cv = T(0.), m = T(1.);
-    if (fabs(phi[q]) <= NB2 - T(1.) * dz) {
-        T v[27];
-        gather27(g, phi, i0, j0, k, v);
-        cv = curv_lsq(A, v, max_kappa, m);
-        if (list && fabs(phi[q]) < T(2.) * dz && m >= T(0.9) && fabs(cv) * dz >= gate) {
-            const int slot = atomicAdd(count, 1);
-            if (slot < cap) {
-                list[slot] = (int)(gofs + i0 + g.sy * j0 + g.sz * (long)(k - 1));
-                if (list_t) list_t[slot] = t;
-            }
-        }
-    }
+    if (fabs(ph) <= NB2 - T(1.) * dz)
+        cv = curvature_band<T>(g, A, phi, ph, m, list, list_t, t, gofs, count, cap, gate, i0, j0, k);
     kappa[q] = cv;
     if (nm) nm[q] = m;
 }
@@ -77,6 +88,31 @@ __global__ void k_curvature(Grid g, T NB2, const T* __restrict__ A, const T* __r
     int i0 = blockIdx.x * BX + threadIdx.x, j0 = blockIdx.y * BY + threadIdx.y, k = blockIdx.z + 1;
     if (i0 >= g.nx || j0 >= g.ny) return;
     curvature_at<T>(g, NB2, A, phi, kappa, nm, list, nullptr, 0, 0, count, cap, gate, i0, j0, k);
+}
+
+// the same with four consecutive cells of a row per thread (nx a multiple of 256):
+// phi, kappa and the normal magnitude move as 16-byte vectors
+template <class T>
+__global__ void __launch_bounds__(BX * BY)
+k_curvature4(Grid g, T NB2, const T* __restrict__ A, const T* __restrict__ phi, T* __restrict__ kappa,
+             T* __restrict__ nm, const double* __restrict__ flag, int* __restrict__ list, int* __restrict__ count,
+             int cap, T gate) {
+    if (flag && flag[0] == 0.) return;
+    const int i0 = 4 * (blockIdx.x * BX + threadIdx.x), j0 = blockIdx.y * BY + threadIdx.y, k = blockIdx.z + 1;
+    if (j0 >= g.ny) return;
+    const long q = gidx(g, i0, j0, k);
+    const T dz = T(g.dz);
+    T ph[4], cv[4], m[4];
+    ld4(phi + q, ph);
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        cv[t] = T(0.);
+        m[t] = T(1.);
+        if (fabs(ph[t]) <= NB2 - T(1.) * dz)
+            cv[t] = curvature_band<T>(g, A, phi, ph[t], m[t], list, nullptr, 0, 0, count, cap, gate, i0 + t, j0, k);
+    }
+    st4(kappa + q, cv);
+    if (nm) st4(nm + q, m);
 }
 
 // ---------------------------------------------------------------- batched windows
@@ -112,12 +148,32 @@ __global__ void k_curvature_batch(Grid g, WinBatch wb, T NB2, const T* __restric
                     gofs, count, cap, gate, wc.i0, wc.j0, wc.k);
 }
 
+// the four-cells-per-thread kernels: whole-grid launches (inside a level-set window
+// the band fills most cells, and one cell per thread measured faster there), rows of a
+// multiple of 256 cells, and the vector arrays 16-byte aligned at every row
+static bool vec4_ok(const Ctx& c, std::initializer_list<const void*> ptrs) {
+    const Grid& g = c.g;
+    if (in_window(c) || g.nx % (4 * BX) != 0 || g.sy % 4 != 0 || g.sz % 4 != 0) return false;
+    for (const void* p : ptrs)
+        if (p && reinterpret_cast<uintptr_t>(p) % 16 != 0) return false;
+    return true;
+}
+
 int launch_curvature(Ctx& c, const double* phi, double* kappa, double* normal_m, const double* flag, bool listed) {
     ProfScope ps(c, "k_curvature");
     dim3 blk(BX, BY, 1);
     dim3 grd = grid3(c.g, c.g.nz, blk);
     int* list = listed ? c.mlp.list : nullptr;
-    if (c.fp32)
+    if (vec4_ok(c, {phi, kappa, normal_m})) {
+        grd.x = c.g.nx / (4 * BX);
+        if (c.fp32)
+            hipLaunchKernelGGL(k_curvature4<float>, grd, blk, 0, c.stream, c.g, (float)c.NB2, as<float>(c.pinvA_f),
+                               as<float>(phi), as<float>(kappa), as<float>(normal_m), flag, list, c.mlp.count,
+                               c.mlp.list_cap, c.mlp.gate);
+        else
+            hipLaunchKernelGGL(k_curvature4<double>, grd, blk, 0, c.stream, c.g, c.NB2, (const double*)c.pinvA, phi,
+                               kappa, normal_m, flag, list, c.mlp.count, c.mlp.list_cap, (double)c.mlp.gate);
+    } else if (c.fp32)
         hipLaunchKernelGGL(k_curvature<float>, grd, blk, 0, c.stream, c.g, (float)c.NB2, as<float>(c.pinvA_f),
                            as<float>(phi), as<float>(kappa), as<float>(normal_m), flag, list, c.mlp.count,
                            c.mlp.list_cap, c.mlp.gate);
@@ -162,11 +218,39 @@ __global__ void k_recurvature(Grid g, const T* __restrict__ A, const T* __restri
     recurvature_at<T>(g, A, phi, kappa, kappa2, i0, j0, k);
 }
 
+// the same with four consecutive cells of a row per thread (nx a multiple of 256)
+template <class T>
+__global__ void __launch_bounds__(BX * BY)
+k_recurvature4(Grid g, const T* __restrict__ A, const T* __restrict__ phi, const T* __restrict__ kappa,
+               T* __restrict__ kappa2) {
+    const int i0 = 4 * (blockIdx.x * BX + threadIdx.x), j0 = blockIdx.y * BY + threadIdx.y, k = blockIdx.z + 1;
+    if (j0 >= g.ny) return;
+    const long q = gidx(g, i0, j0, k);
+    T ph[4], r[4];
+    ld4(phi + q, ph);
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        r[t] = T(0.);
+        if (fabs(ph[t]) < T(2.) * T(g.dz)) {
+            T v[27];
+            gather27(g, kappa, i0 + t, j0, k, v);
+            r[t] = lsq_row(A, 0, v);
+        }
+    }
+    st4(kappa2 + q, r);
+}
+
 int launch_recurvature(Ctx& c, const double* phi, const double* kappa, double* kappa2) {
     ProfScope ps(c, "k_recurvature");
     dim3 blk(BX, BY, 1);
     dim3 grd = grid3(c.g, c.g.nz, blk);
-    if (c.fp32)
+    if (c.fp32 && vec4_ok(c, {phi, kappa2})) {
+        // float only: in FP64 the four cells' registers cost more occupancy than the
+        // vector accesses gain
+        grd.x = c.g.nx / (4 * BX);
+        hipLaunchKernelGGL(k_recurvature4<float>, grd, blk, 0, c.stream, c.g, as<float>(c.pinvA_f), as<float>(phi),
+                           as<float>(kappa), as<float>(kappa2));
+    } else if (c.fp32)
         hipLaunchKernelGGL(k_recurvature<float>, grd, blk, 0, c.stream, c.g, as<float>(c.pinvA_f), as<float>(phi),
                            as<float>(kappa), as<float>(kappa2));
     else

@@ -81,25 +81,6 @@ __global__ void k_materials(Grid g, int lmax, const T* const* __restrict__ ls, c
 // operations are those of k_materials; the partial sums add the four cells of a layer
 // before the next layer.
 template <class T>
-__device__ __forceinline__ void ld4(const T* a, T* v) {
-    if constexpr (sizeof(T) == 4) {
-        const float4 t = *reinterpret_cast<const float4*>(a);
-        v[0] = t.x; v[1] = t.y; v[2] = t.z; v[3] = t.w;
-    } else {
-        const double2 t0 = *reinterpret_cast<const double2*>(a), t1 = *reinterpret_cast<const double2*>(a + 2);
-        v[0] = t0.x; v[1] = t0.y; v[2] = t1.x; v[3] = t1.y;
-    }
-}
-template <class T>
-__device__ __forceinline__ void st4(T* a, const T* v) {
-    if constexpr (sizeof(T) == 4) {
-        *reinterpret_cast<float4*>(a) = make_float4(v[0], v[1], v[2], v[3]);
-    } else {
-        *reinterpret_cast<double2*>(a) = make_double2(v[0], v[1]);
-        *reinterpret_cast<double2*>(a + 2) = make_double2(v[2], v[3]);
-    }
-}
-template <class T>
 __global__ void __launch_bounds__(BX * BY)
 k_materials1(Grid g, const T* __restrict__ phi, T eps, T pi, T miu1, T miu2, T rho1, T rho2, T* __restrict__ miu,
              T* __restrict__ rho, double* __restrict__ part) {
