@@ -331,6 +331,66 @@ __device__ __forceinline__ void adv_stage_body(const Grid& g, int bx, int by, in
     out[gidx(g, i0, j0, k)] = r;
 }
 
+// The same stage with four consecutive cells of a row per thread on the whole grid (nx
+// a multiple of 256): phi0, phi_s, the mask and the result move as vectors; a band cell
+// takes the stencil of adv_stage_body, with the same operations.
+template <class T, int SCHEME, int STAGE, bool P2, bool VELF>
+__global__ void __launch_bounds__(BX * BY)
+k_adv_stage4(Grid g, T dt, const T* phi0, const T* __restrict__ phis, T* out, const uint8_t* __restrict__ mask,
+             const T* __restrict__ vu, const T* __restrict__ vv, const T* __restrict__ vw,
+             const double* __restrict__ flag) {
+    if (flag && flag[0] == 0.) return;
+    const int i0 = 4 * (blockIdx.x * BX + threadIdx.x), j0 = blockIdx.y * BY + threadIdx.y;
+    if (j0 >= g.ny) return;
+    const int k = (int)blockIdx.z - 2;
+    T factor, comb[4], p0[4], ps[4] = {T(0.), T(0.), T(0.), T(0.)};
+    auto comb_of = [&](T a, T b) -> T {
+        if (STAGE == 1) return a;
+        if (STAGE == 2) return T(3.) / T(4.) * a + T(1.) / T(4.) * b;
+        return T(1.) / T(3.) * a + T(2.) / T(3.) * b;
+    };
+    if (STAGE == 1) factor = T(1.);
+    else if (STAGE == 2) factor = T(1.) / T(4.);
+    else factor = T(2.) / T(3.);
+    auto dfdt_at = [&](int ii, int kk, long qq) -> T {
+        T uc, vc, wc;
+        if (VELF) {
+            uc = (vu[gidx(g, wrap(ii - 1, g.nx), j0, kk)] + vu[qq]) / T(2.);
+            vc = (vv[gidx(g, ii, wrap(j0 - 1, g.ny), kk)] + vv[qq]) / T(2.);
+            wc = (vw[gidx(g, ii, j0, kk - 1)] + vw[qq]) / T(2.);
+        } else {
+            uc = vu[qq]; vc = vv[qq]; wc = vw[qq];
+        }
+        return adv_rhs<T, SCHEME, P2>(g, phis, ii, j0, kk, uc, vc, wc);
+    };
+    // the four cells of layer kk, as adv_stage_body's value()
+    auto values = [&](int kk, T* r) {
+        const long qq = gidx(g, i0, j0, kk);
+        ld4(phi0 + qq, p0);
+        if (STAGE > 1) ld4(phis + qq, ps);
+        uchar4 m = {0, 0, 0, 0};
+        if (kk >= 1 && kk <= g.nz) m = *reinterpret_cast<const uchar4*>(mask + qq);
+        const unsigned char mm[4] = {m.x, m.y, m.z, m.w};
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            comb[t] = comb_of(p0[t], ps[t]);
+            r[t] = comb[t];
+            if (mm[t]) r[t] = comb[t] + factor * dt * dfdt_at(i0 + t, kk, qq + t);
+        }
+    };
+    T r[4];
+    if (k == 0 || k == g.nz + 1) {
+        T a[4], b[4];
+        values(k == 0 ? 1 : g.nz, a);
+        values(k == 0 ? 2 : g.nz - 1, b);
+#pragma unroll
+        for (int t = 0; t < 4; t++) r[t] = T(2.) * a[t] - b[t];
+    } else {
+        values(k, r);
+    }
+    st4(out + gidx(g, i0, j0, k), r);
+}
+
 template <class T, int SCHEME, int STAGE, bool P2, bool VELF>
 __global__ void k_adv_stage(Grid g, T dt, const T* phi0, const T* __restrict__ phis, T* out,
                             const uint8_t* __restrict__ mask, const T* __restrict__ vu, const T* __restrict__ vv,
@@ -358,9 +418,23 @@ static void adv_dispatch(Ctx& c, int scheme, int stage, const double* phi0, cons
                          const double* vu, const double* vv, const double* vw, int velf, const double* flag) {
     dim3 blk(BX, BY, 1);
     dim3 grd = grid3(c.g, c.g.nz + 6, blk);
-#define ADV1(S, ST, P, V)                                                                                         \
-    hipLaunchKernelGGL((k_adv_stage<T, S, ST, P, V>), grd, blk, 0, c.stream, c.g, (T)c.dt, as<T>(phi0), as<T>(phis), \
-                       as<T>(out), (const uint8_t*)c.mask, as<T>(vu), as<T>(vv), as<T>(vw), flag)
+    // four cells per thread on the whole grid (the arrays of a level-set window start
+    // where the window does, not on a vector boundary)
+    auto al = [](const void* p) { return reinterpret_cast<uintptr_t>(p) % 16 == 0; };
+    const bool vec = !in_window(c) && c.g.nx % (4 * BX) == 0 && c.g.sy % 4 == 0 && al(phi0) && al(phis) &&
+                     al(out) && reinterpret_cast<uintptr_t>(c.mask) % 4 == 0;
+    if (vec) grd.x = c.g.nx / (4 * BX);
+#define ADV1(S, ST, P, V)                                                                                          \
+    do {                                                                                                           \
+        if (vec)                                                                                                   \
+            hipLaunchKernelGGL((k_adv_stage4<T, S, ST, P, V>), grd, blk, 0, c.stream, c.g, (T)c.dt, as<T>(phi0),  \
+                               as<T>(phis), as<T>(out), (const uint8_t*)c.mask, as<T>(vu), as<T>(vv), as<T>(vw),  \
+                               flag);                                                                              \
+        else                                                                                                       \
+            hipLaunchKernelGGL((k_adv_stage<T, S, ST, P, V>), grd, blk, 0, c.stream, c.g, (T)c.dt, as<T>(phi0),   \
+                               as<T>(phis), as<T>(out), (const uint8_t*)c.mask, as<T>(vu), as<T>(vv), as<T>(vw),  \
+                               flag);                                                                              \
+    } while (0)
 #define ADV(S, ST)                                                             \
     if (scheme == S && stage == ST) {                                          \
         if (c.g.pow2) {                                                        \
