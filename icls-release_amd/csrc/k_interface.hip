@@ -344,6 +344,49 @@ __global__ void k_gfm(Grid g, T nondi, T coef, const T* __restrict__ ls, const T
     if (MODE == 2) rhs[q] = fillps_rhs<T>(g, coef, du, dv, dw, i0, j0, k) + pjs;
 }
 
+// MODE 2 with four consecutive cells of a row per thread on the whole grid (nx a
+// multiple of 256): the mask, p_jump, the momentum results and the right-hand side move
+// as vectors; per cell the operations of k_gfm and fillps_rhs
+template <class T>
+__global__ void __launch_bounds__(BX * BY)
+k_gfm4(Grid g, T nondi, T coef, const T* __restrict__ ls, const T* __restrict__ cvf, T* __restrict__ pjump,
+       T* __restrict__ px, T* __restrict__ py, T* __restrict__ pz, const T* __restrict__ du,
+       const T* __restrict__ dv, const T* __restrict__ dw, T* __restrict__ rhs, uint8_t* __restrict__ jm) {
+    const int i0 = 4 * (blockIdx.x * BX + threadIdx.x), j0 = blockIdx.y * BY + threadIdx.y, k = blockIdx.z + 1;
+    if (j0 >= g.ny) return;
+    const long q = gidx(g, i0, j0, k);
+    T pj[4];
+    unsigned bits[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        T pjs = T(0.), pxs = T(0.), pys = T(0.), pzs = T(0.);
+        gfm_cell<T>(g, nondi, ls, cvf, i0 + t, j0, k, pjs, pxs, pys, pzs);
+        bits[t] = (pxs != T(0.) ? 1u : 0u) | (pys != T(0.) ? 2u : 0u) | (pzs != T(0.) ? 4u : 0u);
+        if (bits[t] & 1u) px[q + t] = pxs;
+        if (bits[t] & 2u) py[q + t] = pys;
+        if (bits[t] & 4u) pz[q + t] = pzs;
+        pj[t] = pjs;
+    }
+    *reinterpret_cast<uchar4*>(jm + q) =
+        make_uchar4((uint8_t)bits[0], (uint8_t)bits[1], (uint8_t)bits[2], (uint8_t)bits[3]);
+    if (pjump) st4(pjump + q, pj);
+    T duc[4], dvc[4], dvm[4], dwc[4], dwm[4], r[4];
+    ld4(du + q, duc);
+    ld4(dv + q, dvc);
+    ld4(dv + gidx(g, i0, wrap(j0 - 1, g.ny), k), dvm);
+    ld4(dw + q, dwc);
+    ld4(dw + q - g.sz, dwm);
+    const T dum = du[gidx(g, wrap(i0 - 1, g.nx), j0, k)];
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        const T vel_x = (duc[t] - (t ? duc[t - 1] : dum)) * T(g.dxi);
+        const T vel_y = (dvc[t] - dvm[t]) * T(g.dyi);
+        const T vel_z = (dwc[t] - dwm[t]) * T(g.dzi);
+        r[t] = coef * (vel_x + vel_y + vel_z) + pj[t];
+    }
+    st4(rhs + q, r);
+}
+
 // MODE 1 over the windows of a batch: each level set's jumps added inside its shell
 template <class T>
 __global__ void k_gfm_batch(Grid g, WinBatch wb, T nondi, const T* __restrict__ cvf, T* __restrict__ pjump,
@@ -367,7 +410,13 @@ static void gfm_dispatch(Ctx& c, int l, const double* kappa2, int mode, bool sto
     const T* ls = as<T>(c.lvset[l]);
     const T* k2 = as<T>(kappa2);
     T *px = as<T>(c.p_x), *py = as<T>(c.p_y), *pz = as<T>(c.p_z);
-    if (mode == 2)
+    T* pj = store_pjump ? as<T>(c.p_jump) : (T*)nullptr;
+    // float only: in FP64 the four cells' registers cost more than the vectors gain
+    if (mode == 2 && c.fp32 && vec4_ok(c, {ls, pj, c.dudt, c.dvdt, c.dwdt, c.frhs})) {
+        grd.x = c.g.nx / (4 * BX);
+        hipLaunchKernelGGL(k_gfm4<T>, grd, blk, 0, c.stream, c.g, nondi, coef, ls, k2, pj, px, py, pz, as<T>(c.dudt),
+                           as<T>(c.dvdt), as<T>(c.dwdt), as<T>(c.frhs), c.jmask);
+    } else if (mode == 2)
         hipLaunchKernelGGL((k_gfm<T, 2>), grd, blk, 0, c.stream, c.g, nondi, coef, ls, k2,
                            store_pjump ? as<T>(c.p_jump) : (T*)nullptr, px, py, pz, as<T>(c.dudt), as<T>(c.dvdt),
                            as<T>(c.dwdt), as<T>(c.frhs), c.jmask);
