@@ -76,12 +76,12 @@ def step_bytes_per_cell(calls, mode_fp32=False):
 _TRAFFIC_KERNEL = {
     "k_momentum": "k_momentum_zm<{T}, 0, true, true>",   # power-of-two grid, 32-bit offsets
     "k_correc": "k_correc<{T}>",
-    "k_gfm": "k_gfm<{T}, 2>",
+    "k_gfm": ("k_gfm4<float>", "k_gfm<double, 2>"),     # four cells per thread in float only
     "k_tube": "k_tube_zm<{T}, 2>",
     "k_reinit_stage1": "k_reinit_zm<{T}, 1, true, 4, true>",
     "k_reinit_stage23": "k_reinit_zm<{T}, 3, true, 4, true>",
-    "k_materials": "k_materials<{T}>",
-    "k_curvature": "k_curvature<{T}>",
+    "k_materials": "k_materials1<{T}>",                  # one level set on the whole grid
+    "k_curvature": "k_curvature4<{T}>",
     "k_thomas": "k_thomas_seg<{T}, 16>",
 }
 
@@ -89,5 +89,9 @@ _TRAFFIC_KERNEL = {
 def traffic_key(scope, fp32):
     """The traffic.json key of the kernel a profiler scope times, or None."""
     k = _TRAFFIC_KERNEL.get(scope)
-    return None if k is None else k.format(T="float" if fp32 else "double")
+    if k is None:
+        return None
+    if isinstance(k, tuple):
+        return k[0] if fp32 else k[1]
+    return k.format(T="float" if fp32 else "double")
 
