@@ -701,20 +701,21 @@ namespace r512 {
 constexpr int N = 512, M = 256;
 
 // exp(+2 pi i t / N) for 0 <= t < N from the half table
-__device__ __forceinline__ cx<float> twf(const cx<float>* tw, int t) {
-    return t < M ? tw[t] : cx<float>{-tw[t - M].re, -tw[t - M].im};
+template <class T>
+__device__ __forceinline__ cx<T> twf(const cx<T>* tw, int t) {
+    return t < M ? tw[t] : cx<T>{-tw[t - M].re, -tw[t - M].im};
 }
 
 // the complex FFT from "v[r] = z_(s + 16 r)" to "v[4 (k2 % 4) + k2 / 4] = X_(s + 16 k2)";
 // xch(k1, t) addresses the exchange slot of (k1, slot t) of this thread's line
-template <int SG, class XCH>
-__device__ __forceinline__ void cfft256(cx<float>* v, int s, const cx<float>* tw, XCH xch) {
+template <int SG, class T, class XCH>
+__device__ __forceinline__ void cfft256(cx<T>* v, int s, const cx<T>* tw, XCH xch) {
     r16::dft16<SG>(v);
 #pragma unroll
     for (int k1 = 0; k1 < 16; k1++) {
-        cx<float> val = v[4 * (k1 % 4) + k1 / 4];
+        cx<T> val = v[4 * (k1 % 4) + k1 / 4];
         if (k1) {
-            cx<float> w = twf(tw, 2 * s * k1);
+            cx<T> w = twf(tw, 2 * s * k1);
             if (SG < 0) w.im = -w.im;
             val = cmul(val, w);
         }
@@ -796,26 +797,29 @@ __global__ void __launch_bounds__(1024) k_rfft512_y(Grid g, const float* __restr
 }
 
 // along x: 256 threads, 16 lines (j) x 16 slots; with RES (forward) the input is
-// rhs - L b0 formed on the fly, as in the 256-point kernel
-template <int INV, class TO, int ACC, int RES, class B>
-__global__ void __launch_bounds__(256) k_rfft512_x(Grid g, const float* __restrict__ twg, const float* in, TO* out,
+// rhs - L b0 formed on the fly, as in the 256-point kernel.  C = double: the FP64
+// transforms in place on p (neither RES nor ACC).
+template <int INV, class C, class TO, int ACC, int RES, class B>
+__global__ void __launch_bounds__(256) k_rfft512_x(Grid g, const C* __restrict__ twg, const C* in, TO* out,
                                                   B* __restrict__ b0) {
     using namespace r512;
     static_assert(!RES || !INV, "the residual feeds the forward transform");
-    static_assert(INV || (!ACC && sizeof(TO) == 4), "the forward transform writes the float work array");
+    static_assert(INV || (!ACC && sizeof(TO) == sizeof(C)), "the forward transform writes the work array");
+    static_assert(std::is_same<C, float>::value || (!RES && !ACC && std::is_same<TO, C>::value),
+                  "the FP64 transforms neither form the residual nor accumulate");
     constexpr int LS = 16 * 17 + 4;   // line buffer (complex): the exchange at stride 17, padded
-    __shared__ cx<float> xb[16 * LS];
-    __shared__ cx<float> tw[M];
+    __shared__ cx<C> xb[16 * LS];
+    __shared__ cx<C> tw[M];
     const int lane = threadIdx.x & 63, s = lane & 15, line = (threadIdx.x >> 6) * 4 + (lane >> 4);
     const int nbl = g.ny / 16;
     const int tile = xcd_tile(blockIdx.x, nbl * g.nz);
     const int k = tile / nbl + 1, j = (tile % nbl) * 16 + line;
     for (int t = threadIdx.x; t < M; t += 256) tw[t] = {twg[2 * t], twg[2 * t + 1]};
-    const float scale = (float)sqrt(1. / (double)N);
+    const C scale = (C)sqrt(1. / (double)N);
     const long q0 = gidx(g, 0, j, k);
-    cx<float>* xl = xb + line * LS;
-    cx<float> v[16];
-    if (RES) {
+    cx<C>* xl = xb + line * LS;
+    cx<C> v[16];
+    if constexpr (RES) {
         // four consecutive cells per thread from 16-byte loads of the line and of its
         // four neighbour lines, staged through LDS
         static_assert(std::is_same<B, float>::value, "the starting solution is float");
@@ -852,24 +856,31 @@ __global__ void __launch_bounds__(256) k_rfft512_x(Grid g, const float* __restri
     } else if (!INV) {
 #pragma unroll
         for (int r = 0; r < 16; r++) {
-            const float2 a = *reinterpret_cast<const float2*>(in + q0 + 2 * (s + 16 * r));
-            v[r] = {a.x, a.y};
+            if constexpr (std::is_same<C, float>::value) {
+                const float2 a = *reinterpret_cast<const float2*>(in + q0 + 2 * (s + 16 * r));
+                v[r] = {a.x, a.y};
+            } else {
+                const double2 a = *reinterpret_cast<const double2*>(in + q0 + 2 * (s + 16 * r));
+                v[r] = {a.x, a.y};
+            }
         }
         __syncthreads();
     } else {
         // the half-complex line through LDS, loaded with 16-byte vectors
-        float* fl = reinterpret_cast<float*>(xl);
+        C* fl = reinterpret_cast<C*>(xl);
+        constexpr int VW = 16 / sizeof(C);   // reals per vector
 #pragma unroll
-        for (int it = 0; it < N / 64; it++) {
-            const int x = 4 * s + 64 * it;
-            *reinterpret_cast<float4*>(fl + x) = *reinterpret_cast<const float4*>(in + q0 + x);
+        for (int it = 0; it < N / (16 * VW); it++) {
+            const int x = VW * s + 16 * VW * it;
+            if constexpr (VW == 4) *reinterpret_cast<float4*>(fl + x) = *reinterpret_cast<const float4*>(in + q0 + x);
+            else *reinterpret_cast<double2*>(fl + x) = *reinterpret_cast<const double2*>(in + q0 + x);
         }
         __syncthreads();
 #pragma unroll
         for (int r = 0; r < 16; r++) {
             const int kk = s + 16 * r;
-            cx<float> X, Y;
-            if (kk == 0) { X = {fl[0], 0.f}; Y = {fl[N - 1], 0.f}; }
+            cx<C> X, Y;
+            if (kk == 0) { X = {fl[0], C(0.)}; Y = {fl[N - 1], C(0.)}; }
             else {
                 X = {fl[2 * kk - 1], fl[2 * kk]};
                 Y = {fl[2 * (M - kk) - 1], fl[2 * (M - kk)]};
@@ -885,9 +896,11 @@ __global__ void __launch_bounds__(256) k_rfft512_x(Grid g, const float* __restri
         // slot s holds z_m for m = s + 16 k2: x = 2 m and 2 m + 1
 #pragma unroll
         for (int k2 = 0; k2 < 16; k2++) {
-            const cx<float> z = v[4 * (k2 % 4) + k2 / 4];
+            const cx<C> z = v[4 * (k2 % 4) + k2 / 4];
             const long q = q0 + 2 * (s + 16 * k2);
-            if constexpr (sizeof(TO) == 4 && !ACC) {
+            if constexpr (std::is_same<C, double>::value) {
+                *reinterpret_cast<double2*>(out + q) = make_double2(z.re * scale, z.im * scale);
+            } else if constexpr (sizeof(TO) == 4 && !ACC) {
                 *reinterpret_cast<float2*>(out + q) = make_float2(z.re * scale, z.im * scale);
             } else if constexpr (sizeof(TO) == 8 && ACC) {
                 static_assert(std::is_same<B, float>::value, "the starting solution is float");
@@ -904,23 +917,23 @@ __global__ void __launch_bounds__(256) k_rfft512_x(Grid g, const float* __restri
 #pragma unroll
     for (int k2 = 0; k2 < 16; k2++) xl[s + 16 * k2] = v[4 * (k2 % 4) + k2 / 4];
     __syncthreads();
-    float o[16][2];
+    C o[16][2];
 #pragma unroll
     for (int k2 = 0; k2 < 16; k2++) {
         const int kk = s + 16 * k2;
-        const cx<float> a = v[4 * (k2 % 4) + k2 / 4];
+        const cx<C> a = v[4 * (k2 % 4) + k2 / 4];
         if (kk == 0) {
             o[k2][0] = (a.re + a.im) * scale;   // x = 0
             o[k2][1] = (a.re - a.im) * scale;   // x = N - 1
         } else {
-            const cx<float> F = r16::fwd_split(tw, kk, a, xl[M - kk]);
+            const cx<C> F = r16::fwd_split(tw, kk, a, xl[M - kk]);
             o[k2][0] = F.re * scale;            // x = 2 kk - 1
             o[k2][1] = F.im * scale;            // x = 2 kk
         }
     }
     __syncthreads();
-    // stage the line in LDS (floats) for aligned, coalesced stores
-    float* fl = reinterpret_cast<float*>(xl);
+    // stage the line in LDS (reals) for aligned, coalesced stores
+    C* fl = reinterpret_cast<C*>(xl);
 #pragma unroll
     for (int k2 = 0; k2 < 16; k2++) {
         const int kk = s + 16 * k2;
@@ -928,24 +941,38 @@ __global__ void __launch_bounds__(256) k_rfft512_x(Grid g, const float* __restri
         else { fl[2 * kk - 1] = o[k2][0]; fl[2 * kk] = o[k2][1]; }
     }
     __syncthreads();
-    // 16 lines x 128 float4
+    if constexpr (std::is_same<C, float>::value) {
+        // 16 lines x 128 float4
 #pragma unroll
-    for (int it = 0; it < 8; it++) {
-        const int e = threadIdx.x + 256 * it, ln = e >> 7, c4 = e & 127;
-        const float* src = reinterpret_cast<const float*>(xb + ln * LS) + 4 * c4;
-        const long q = gidx(g, 4 * c4, (tile % nbl) * 16 + ln, k);
-        *reinterpret_cast<float4*>(out + q) = make_float4(src[0], src[1], src[2], src[3]);
+        for (int it = 0; it < 8; it++) {
+            const int e = threadIdx.x + 256 * it, ln = e >> 7, c4 = e & 127;
+            const float* src = reinterpret_cast<const float*>(xb + ln * LS) + 4 * c4;
+            const long q = gidx(g, 4 * c4, (tile % nbl) * 16 + ln, k);
+            *reinterpret_cast<float4*>(out + q) = make_float4(src[0], src[1], src[2], src[3]);
+        }
+    } else {
+        // 16 lines x 256 double2
+#pragma unroll
+        for (int it = 0; it < 16; it++) {
+            const int e = threadIdx.x + 256 * it, ln = e >> 8, c2 = e & 255;
+            const double* src = reinterpret_cast<const double*>(xb + ln * LS) + 2 * c2;
+            const long q = gidx(g, 2 * c2, (tile % nbl) * 16 + ln, k);
+            *reinterpret_cast<double2*>(out + q) = make_double2(src[0], src[1]);
+        }
     }
 }
 
 template <int DIR, int INV, class C, class TO, int ACC, int RES, class B>
 static bool try_rfft512(Ctx& c, const C* tw, const C* in, TO* out, B* b0) {
-    if constexpr (!std::is_same<C, float>::value) return false;
+    constexpr bool F32 = std::is_same<C, float>::value;
+    constexpr bool F64 = std::is_same<C, double>::value && std::is_same<TO, double>::value && !RES && !ACC;
+    if constexpr (!F32 && !F64) return false;
     else {
         const int N = DIR == 0 ? c.g.nx : c.g.ny;
         if (N != 512) return false;
         if (DIR == 1) {
-            if constexpr (RES || ACC) return false;
+            // along y the double exchange would need 256 KB of LDS: FP64 keeps k_rfft_n
+            if constexpr (RES || ACC || !F32) return false;
             else {
                 if (c.g.nx % 64) return false;
                 hipLaunchKernelGGL((k_rfft512_y<INV, TO, ACC>), dim3((c.g.nx / 64) * c.g.nz), dim3(1024), 0, c.stream,
@@ -954,8 +981,8 @@ static bool try_rfft512(Ctx& c, const C* tw, const C* in, TO* out, B* b0) {
             }
         }
         if (c.g.ny % 16) return false;
-        hipLaunchKernelGGL((k_rfft512_x<INV, TO, ACC, RES, B>), dim3((c.g.ny / 16) * c.g.nz), dim3(256), 0, c.stream,
-                           c.g, tw, in, out, b0);
+        hipLaunchKernelGGL((k_rfft512_x<INV, C, TO, ACC, RES, B>), dim3((c.g.ny / 16) * c.g.nz), dim3(256), 0,
+                           c.stream, c.g, tw, in, out, b0);
         return true;
     }
 }

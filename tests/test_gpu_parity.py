@@ -487,7 +487,7 @@ def test_fortran_driver_matches_oracle(tmp_path):
         assert rel_linf(a, b) <= TOL, P.FIELD_NAMES[f]
 
 
-@pytest.mark.parametrize("n,nz", [(256, 16), (64, 32), (16, 8)])
+@pytest.mark.parametrize("n,nz", [(512, 8), (256, 16), (64, 32), (16, 8)])
 def test_solver2d_parity(n, nz):
     """solver2d alone on a random zero-mean right-hand side: FFT x, FFT y, Thomas z,
     inverse FFTs (solver.f90:68-196), at the line lengths of the workloads."""
